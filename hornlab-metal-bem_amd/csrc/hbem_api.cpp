@@ -1,0 +1,932 @@
+// C ABI of libhbem (include/hbem.h): sessions, device workspaces and the
+// per-batch orchestration of assembly -> LU -> refinement -> field ->
+// reductions on one HIP stream per session.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hbem.h"
+#include "hbem_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HbemError : std::runtime_error {
+    int code;
+    HbemError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void fail(int code, const std::string &msg) { throw HbemError(code, msg); }
+
+template <typename F>
+int32_t guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        f();
+        return HBEM_OK;
+    } catch (const HbemError &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return HBEM_ENOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return HBEM_EINVAL;
+    }
+}
+
+void hip_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return;
+    const int code = (e == hipErrorOutOfMemory) ? HBEM_ENOMEM : HBEM_EHIP;
+    fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Growable device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t want) {
+        if (want <= bytes) return;
+        if (ptr) hip_check(hipFree(ptr), "hipFree");
+        ptr = nullptr;
+        bytes = 0;
+        hip_check(hipMalloc(&ptr, want), "hipMalloc");
+        bytes = want;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(ptr);
+    }
+};
+
+template <typename T>
+void upload(DevBuf &d, const std::vector<T> &h, hipStream_t s) {
+    d.reserve(std::max<size_t>(1, h.size() * sizeof(T)));
+    if (!h.empty()) {
+        hip_check(hipMemcpyAsync(d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync H2D");
+        // the host vector may be a temporary: finish the copy before returning
+        hip_check(hipStreamSynchronize(s), "upload sync");
+    }
+}
+
+int gfx950_device() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return -1;
+    for (int d = 0; d < count; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess) continue;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) return d;
+    }
+    return -1;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer() {
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+    }
+    ~Timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    void start(hipStream_t s) { hip_check(hipEventRecord(a, s), "hipEventRecord"); }
+    void stop(hipStream_t s) { hip_check(hipEventRecord(b, s), "hipEventRecord"); }
+    double seconds() {
+        hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        return ms * 1e-3;
+    }
+};
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct hbem_session {
+    std::mutex mu;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hbem::Plan plan;
+    hbem::DeviceGeometry dg{};
+    DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, pair_tt, pair_code, rule;
+    // per-call workspaces
+    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, q, kre, kim, fk, obs, field, tags,
+        tagout, norms, anorm, active, rowsplit, rcx;
+    // staging of host-input batches
+    DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag;
+    double setup_seconds = 0.0;
+    std::string device_name;
+    int cu_count = 0;
+
+    ~hbem_session() {
+        for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &pair_tt, &pair_code,
+                          &rule, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
+                          &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
+                          &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
+                          &io_p, &io_tag})
+            b->release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void set_device(const hbem_session *s) { hip_check(hipSetDevice(s->device), "hipSetDevice"); }
+
+// Interleave split (re, im) host arrays of length n (count rows of stride n).
+std::vector<float> interleave(const float *re, const float *im, size_t n) {
+    std::vector<float> out(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        out[2 * i] = re ? re[i] : 0.0f;
+        out[2 * i + 1] = im ? im[i] : 0.0f;
+    }
+    return out;
+}
+
+// Assembles B systems (frequencies) into s->A (column-major, lda, stride)
+// and s->rhs from device inputs: k_real/k_imag [B] (k_imag may be null for
+// real k) and interleaved complex Neumann rows q [B][T].  Optional timers
+// bracket the regular and the singular kernel on the session stream.
+void assemble_device(hbem_session *s, int B, const float *kre_d, const float *kim_d,
+                     const float *q_d, int64_t lda, int64_t a_stride, int64_t rhs_stride,
+                     Timer *t_reg, Timer *t_sing) {
+    const size_t a_floats = (size_t)B * a_stride * 2;
+    s->A.reserve(a_floats * sizeof(float));
+    s->rhs.reserve((size_t)B * rhs_stride * 2 * sizeof(float));
+    hbem::launch_fill_zero(s->A.as<float>(), a_floats, s->stream);
+    hbem::launch_fill_zero(s->rhs.as<float>(), (size_t)B * rhs_stride * 2, s->stream);
+    if (t_reg) t_reg->start(s->stream);
+    hbem::launch_assemble_regular(s->dg, B, kre_d, kim_d, q_d, s->A.as<float>(), lda, a_stride,
+                                  s->rhs.as<float>(), rhs_stride, s->stream);
+    if (t_reg) t_reg->stop(s->stream);
+    if (t_sing) t_sing->start(s->stream);
+    hbem::launch_assemble_singular(s->dg, B, kre_d, kim_d, q_d, s->A.as<float>(), lda, a_stride,
+                                   s->rhs.as<float>(), rhs_stride, s->stream);
+    if (t_sing) t_sing->stop(s->stream);
+    hip_check(hipGetLastError(), "assembly launch");
+}
+
+// Host-input twin of assemble_device (uploads k and split q rows).
+void assemble_batch(hbem_session *s, int B, const float *k_real_h, const float *k_imag_h,
+                    const float *q_re_h, const float *q_im_h, int64_t lda, int64_t a_stride,
+                    int64_t rhs_stride, bool complex_k) {
+    const int T = s->plan.n_triangles;
+    std::vector<float> kr(k_real_h, k_real_h + B);
+    upload(s->kre, kr, s->stream);
+    if (complex_k) {
+        std::vector<float> ki(k_imag_h, k_imag_h + B);
+        upload(s->kim, ki, s->stream);
+    }
+    std::vector<float> q = interleave(q_re_h, q_im_h, (size_t)B * T);
+    upload(s->q, q, s->stream);
+    assemble_device(s, B, s->kre.as<float>(), complex_k ? s->kim.as<float>() : nullptr,
+                    s->q.as<float>(), lda, a_stride, rhs_stride, nullptr, nullptr);
+}
+
+bool any_nonzero(const float *v, size_t n) {
+    if (!v) return false;
+    for (size_t i = 0; i < n; ++i)
+        if (v[i] != 0.0f) return true;
+    return false;
+}
+
+// LAPACK clacn2-style 1-norm estimate of ||A^{-1}||_1 for each system of a
+// factored batch (Higham's algorithm, as in cgecon).  Returns rcond per
+// system given anorm.
+void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, const std::vector<double> &anorm,
+                    std::vector<double> *rcond) {
+    const int B = w.B, n = w.n, N = w.N;
+    using cd = std::complex<double>;
+    std::vector<std::vector<cd>> X(B, std::vector<cd>(n));
+    std::vector<int> state(B, 0), iter(B, 0), jidx(B, 0);
+    std::vector<double> est(B, 0.0);
+    std::vector<double> altsgn(B, 1.0);
+    // state: 0 start(A^-1 of 1/n), 1 after first A^-H, 2 loop A^-1 of e_j,
+    // 3 loop A^-H, 4 alt A^-1, 5 done
+    for (int b = 0; b < B; ++b)
+        for (int i = 0; i < n; ++i) X[b][i] = cd(1.0 / n, 0.0);
+    std::vector<int> op(B);  // 0 = A^{-1}, 2 = A^{-H}, -1 none
+    for (int b = 0; b < B; ++b) op[b] = 0;
+    // own scratch: the solution in s->x must survive the estimate
+    s->rcx.reserve((size_t)B * N * 2 * sizeof(float));
+    std::vector<float> hx((size_t)B * N * 2);
+    const double safmin = 1.1754943508222875e-38;
+    auto norm_sign = [&](std::vector<cd> &v) {
+        for (auto &z : v) {
+            const double a = std::abs(z);
+            z = a > safmin ? z / a : cd(1.0, 0.0);
+        }
+    };
+    auto argmax = [&](const std::vector<cd> &v) {
+        int j = 0;
+        double m = -1.0;
+        for (int i = 0; i < (int)v.size(); ++i)
+            if (std::abs(v[i]) > m) {
+                m = std::abs(v[i]);
+                j = i;
+            }
+        return j;
+    };
+    auto l1 = [&](const std::vector<cd> &v) {
+        double acc = 0.0;
+        for (const auto &z : v) acc += std::abs(z);
+        return acc;
+    };
+    for (int round = 0; round < 32; ++round) {
+        bool any = false;
+        for (int trans : {0, 2}) {
+            bool need = false;
+            for (int b = 0; b < B; ++b) need |= (op[b] == trans);
+            if (!need) continue;
+            any = true;
+            std::fill(hx.begin(), hx.end(), 0.0f);
+            for (int b = 0; b < B; ++b)
+                if (op[b] == trans)
+                    for (int i = 0; i < n; ++i) {
+                        hx[((size_t)b * N + i) * 2] = (float)X[b][i].real();
+                        hx[((size_t)b * N + i) * 2 + 1] = (float)X[b][i].imag();
+                    }
+            hip_check(hipMemcpyAsync(s->rcx.ptr, hx.data(), hx.size() * sizeof(float),
+                                     hipMemcpyHostToDevice, s->stream),
+                      "rcond upload");
+            hbem::lu_solve(w, s->rcx.as<float>(), N, trans, s->stream);
+            hip_check(hipMemcpyAsync(hx.data(), s->rcx.ptr, hx.size() * sizeof(float),
+                                     hipMemcpyDeviceToHost, s->stream),
+                      "rcond download");
+            hip_check(hipStreamSynchronize(s->stream), "rcond sync");
+            for (int b = 0; b < B; ++b) {
+                if (op[b] != trans) continue;
+                for (int i = 0; i < n; ++i)
+                    X[b][i] = cd(hx[((size_t)b * N + i) * 2], hx[((size_t)b * N + i) * 2 + 1]);
+                op[b] = -1;
+                std::vector<cd> &v = X[b];
+                switch (state[b]) {
+                case 0:  // after A^{-1} (1/n)
+                    if (n == 1) {
+                        est[b] = std::abs(v[0]);
+                        state[b] = 5;
+                        break;
+                    }
+                    est[b] = l1(v);
+                    norm_sign(v);
+                    state[b] = 1;
+                    op[b] = 2;
+                    break;
+                case 1:  // after A^{-H}
+                    jidx[b] = argmax(v);
+                    iter[b] = 2;
+                    std::fill(v.begin(), v.end(), cd(0.0, 0.0));
+                    v[jidx[b]] = cd(1.0, 0.0);
+                    state[b] = 2;
+                    op[b] = 0;
+                    break;
+                case 2: {  // after A^{-1} e_j
+                    const double estold = est[b];
+                    est[b] = l1(v);
+                    if (est[b] <= estold) {
+                        est[b] = std::max(est[b], estold);
+                        state[b] = 4;
+                    } else {
+                        norm_sign(v);
+                        state[b] = 3;
+                        op[b] = 2;
+                        break;
+                    }
+                    // fallthrough to the alternating-sign vector
+                    for (int i = 0; i < n; ++i)
+                        v[i] = cd((i % 2 == 0 ? 1.0 : -1.0) *
+                                      (1.0 + (n > 1 ? (double)i / (n - 1) : 0.0)),
+                                  0.0);
+                    op[b] = 0;
+                    break;
+                }
+                case 3: {  // after A^{-H} in the loop
+                    const int jlast = jidx[b];
+                    jidx[b] = argmax(v);
+                    if (std::abs(v[jlast]) != std::abs(v[jidx[b]]) && iter[b] < 5) {
+                        iter[b]++;
+                        std::fill(v.begin(), v.end(), cd(0.0, 0.0));
+                        v[jidx[b]] = cd(1.0, 0.0);
+                        state[b] = 2;
+                        op[b] = 0;
+                    } else {
+                        for (int i = 0; i < n; ++i)
+                            v[i] = cd((i % 2 == 0 ? 1.0 : -1.0) *
+                                          (1.0 + (n > 1 ? (double)i / (n - 1) : 0.0)),
+                                      0.0);
+                        state[b] = 4;
+                        op[b] = 0;
+                    }
+                    break;
+                }
+                case 4: {  // after A^{-1} of the alternating vector
+                    const double temp = 2.0 * l1(v) / (3.0 * n);
+                    if (temp > est[b]) est[b] = temp;
+                    state[b] = 5;
+                    break;
+                }
+                default:
+                    break;
+                }
+            }
+        }
+        if (!any) break;
+    }
+    rcond->assign(B, 0.0);
+    for (int b = 0; b < B; ++b)
+        (*rcond)[b] = (anorm[b] > 0.0 && est[b] > 0.0) ? (1.0 / est[b]) / anorm[b] : 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hbem_abi_version(void) { return HBEM_ABI_VERSION; }
+
+const char *hbem_last_error(void) { return g_last_error.c_str(); }
+
+int32_t hbem_device_count(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return 0;
+    int n = 0;
+    for (int d = 0; d < count; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess) continue;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) ++n;
+    }
+    return n;
+}
+
+int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
+    return guarded([&] {
+        if (!geom || !out) fail(HBEM_EINVAL, "hbem_session_create: null argument");
+        *out = nullptr;
+        if (!geom->vertices_3xn || !geom->triangles_3xm || !geom->p1_local2global ||
+            !geom->areas || !geom->normals_3xm)
+            fail(HBEM_EINVAL, "hbem_session_create: geometry arrays must be non-null");
+        const double t0 = now_s();
+        const int dev = gfx950_device();
+        if (dev < 0) fail(HBEM_ENODEV, "no gfx950 (MI355X) device is visible");
+        auto s = std::make_unique<hbem_session>();
+        s->device = dev;
+        set_device(s.get());
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        s->device_name = prop.name;
+        s->cu_count = prop.multiProcessorCount;
+        hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+        const std::string err = hbem::build_plan(
+            geom->n_vertices, geom->n_triangles, geom->p1_dof_count, geom->symmetry,
+            geom->vertices_3xn, geom->triangles_3xm, geom->p1_local2global, geom->areas,
+            geom->normals_3xm, geom->physical_tags, &s->plan);
+        if (!err.empty()) fail(HBEM_EINVAL, "hbem_session_create: " + err);
+        const hbem::Plan &P = s->plan;
+        upload(s->tri_f_aos, P.tri_f_aos, s->stream);
+        upload(s->tri_i_aos, P.tri_i_aos, s->stream);
+        upload(s->tri_f_soa, P.tri_f_soa, s->stream);
+        upload(s->tri_i_soa, P.tri_i_soa, s->stream);
+        upload(s->pair_tt, P.pair_tt, s->stream);
+        upload(s->pair_code, P.pair_code, s->stream);
+        upload(s->rule, P.rule, s->stream);
+        hbem::DeviceGeometry &g = s->dg;
+        g.tri_f_aos = s->tri_f_aos.as<float>();
+        g.tri_i_aos = s->tri_i_aos.as<int32_t>();
+        g.tri_f_soa = s->tri_f_soa.as<float>();
+        g.tri_i_soa = s->tri_i_soa.as<int32_t>();
+        g.pair_tt = s->pair_tt.as<int32_t>();
+        g.pair_code = s->pair_code.as<int32_t>();
+        g.rule = s->rule.as<float>();
+        for (int i = 0; i < 3; ++i) {
+            g.rule_offset[i] = P.rule_offset[i];
+            g.rule_count[i] = P.rule_count[i];
+        }
+        g.n_pairs = (int)P.pair_code.size();
+        g.n_triangles = P.n_triangles;
+        g.n_dofs = P.n_dofs;
+        g.symmetry = P.symmetry;
+        g.n_masks = P.n_masks;
+        for (int i = 0; i < hbem::kMaxMasks; ++i) g.masks[i] = P.masks[i];
+        g.row_weight = P.row_weight;
+        hip_check(hipStreamSynchronize(s->stream), "session upload");
+        s->setup_seconds = now_s() - t0;
+        *out = s.release();
+    });
+}
+
+int32_t hbem_session_destroy(hbem_session *session) {
+    return guarded([&] {
+        if (!session) return;
+        (void)hipSetDevice(session->device);
+        (void)hipStreamSynchronize(session->stream);
+        delete session;
+    });
+}
+
+int32_t hbem_session_get_info(hbem_session *s, hbem_session_info *info) {
+    return guarded([&] {
+        if (!s || !info) fail(HBEM_EINVAL, "hbem_session_get_info: null argument");
+        std::memset(info, 0, sizeof(*info));
+        info->duffy_coincident = s->plan.kind_count[0];
+        info->duffy_edge = s->plan.kind_count[1];
+        info->duffy_vertex = s->plan.kind_count[2];
+        info->duffy_image_pairs = s->plan.image_pairs;
+        info->duffy_unique_slots = s->plan.unique_slots;
+        info->n_triangles = s->plan.n_triangles;
+        info->p1_dof_count = s->plan.n_dofs;
+        info->symmetry = s->plan.symmetry;
+        info->device_cu_count = s->cu_count;
+        info->setup_seconds = s->setup_seconds;
+        std::snprintf(info->device_name, sizeof(info->device_name), "%s", s->device_name.c_str());
+    });
+}
+
+int32_t hbem_assemble(hbem_session *s, float k_real, float k_imag, const float *q_re,
+                      const float *q_im, float *A_re, float *A_im, float *rhs_re, float *rhs_im) {
+    return guarded([&] {
+        if (!s) fail(HBEM_EINVAL, "hbem_assemble: null session");
+        if (!A_re || !A_im || !rhs_re || !rhs_im)
+            fail(HBEM_EINVAL, "hbem_assemble: output arrays must be non-null");
+        if (!std::isfinite(k_real) || !std::isfinite(k_imag) || k_imag < 0.0f)
+            fail(HBEM_EINVAL, "hbem_assemble: k must be finite with k_imag >= 0");
+        std::lock_guard<std::mutex> lock(s->mu);
+        set_device(s);
+        const int n = s->plan.n_dofs;
+        const int64_t lda = n, a_stride = (int64_t)n * n, rhs_stride = n;
+        assemble_batch(s, 1, &k_real, &k_imag, q_re, q_im, lda, a_stride, rhs_stride,
+                       k_imag != 0.0f);
+        s->rowsplit.reserve((size_t)2 * n * n * sizeof(float));
+        float *re_d = s->rowsplit.as<float>();
+        float *im_d = re_d + (size_t)n * n;
+        hbem::launch_colmajor_to_rowmajor_split(s->A.as<float>(), lda, a_stride, n, 1, re_d, im_d,
+                                                s->stream);
+        hip_check(hipMemcpyAsync(A_re, re_d, sizeof(float) * (size_t)n * n, hipMemcpyDeviceToHost,
+                                 s->stream),
+                  "A_re D2H");
+        hip_check(hipMemcpyAsync(A_im, im_d, sizeof(float) * (size_t)n * n, hipMemcpyDeviceToHost,
+                                 s->stream),
+                  "A_im D2H");
+        std::vector<float> rh((size_t)2 * n);
+        hip_check(hipMemcpyAsync(rh.data(), s->rhs.ptr, sizeof(float) * 2 * (size_t)n,
+                                 hipMemcpyDeviceToHost, s->stream),
+                  "rhs D2H");
+        hip_check(hipStreamSynchronize(s->stream), "hbem_assemble");
+        for (int i = 0; i < n; ++i) {
+            rhs_re[i] = rh[2 * i];
+            rhs_im[i] = rh[2 * i + 1];
+        }
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Factor + solve + optional refinement of the B systems resident in s->A
+// (column-major, order N, true order n) with right-hand sides s->rhs.
+// Solution (FP32, interleaved, stride N) ends in s->x.  Fills diagnostics.
+void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_options &opt,
+                    std::vector<hbem_case_diag> *diag, double *solve_seconds) {
+    hipStream_t st = s->stream;
+    const int64_t lda = N, a_stride = (int64_t)N * N;
+    const bool refine = opt.solve_mode == HBEM_SOLVE_F32_IR ||
+                        opt.solve_mode == HBEM_SOLVE_MIXED_F16 ||
+                        opt.solve_mode == HBEM_SOLVE_MIXED_BF16;
+    if (opt.solve_mode == HBEM_SOLVE_MIXED_F16 || opt.solve_mode == HBEM_SOLVE_MIXED_BF16)
+        fail(HBEM_EUNSUPPORTED, "mixed-precision LU modes are not built in this library version");
+    Timer t;
+    t.start(st);
+    hbem::launch_pad_identity(s->A.as<float>(), lda, a_stride, n, N, B, st);
+    s->anorm.reserve(sizeof(double) * B);
+    hbem::launch_norm1(s->A.as<float>(), lda, a_stride, n, B, s->anorm.as<double>(), st);
+    if (refine) {
+        s->A_orig.reserve(sizeof(float) * 2 * (size_t)B * a_stride);
+        hip_check(hipMemcpyAsync(s->A_orig.ptr, s->A.ptr, sizeof(float) * 2 * (size_t)B * a_stride,
+                                 hipMemcpyDeviceToDevice, st),
+                  "A copy");
+    }
+    s->ipiv.reserve(sizeof(int32_t) * (size_t)B * N);
+    s->info.reserve(sizeof(int32_t) * B);
+    hbem::LuWorkspace w{s->A.as<float>(), lda, a_stride, N, n, B, s->ipiv.as<int32_t>(),
+                        s->info.as<int32_t>()};
+    hbem::lu_factor(w, st);
+    s->x.reserve(sizeof(float) * 2 * (size_t)B * N);
+    hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)B * N,
+                             hipMemcpyDeviceToDevice, st),
+              "rhs copy");
+    hbem::lu_solve(w, s->x.as<float>(), N, 0, st);
+    std::vector<int32_t> info(B);
+    hip_check(hipMemcpyAsync(info.data(), s->info.ptr, sizeof(int32_t) * B, hipMemcpyDeviceToHost,
+                             st),
+              "info D2H");
+    std::vector<double> anorm(B);
+    hip_check(hipMemcpyAsync(anorm.data(), s->anorm.ptr, sizeof(double) * B,
+                             hipMemcpyDeviceToHost, st),
+              "anorm D2H");
+    hip_check(hipStreamSynchronize(st), "solve sync");
+    std::vector<int32_t> iters(B, 0);
+    std::vector<double> resid(B, -1.0);
+    if (refine && opt.refine_max_iterations > 0) {
+        s->x64.reserve(sizeof(double) * 2 * (size_t)B * N);
+        s->r64.reserve(sizeof(double) * 2 * (size_t)B * N);
+        s->xprev.reserve(sizeof(double) * 2 * (size_t)B * N);
+        s->norms.reserve(sizeof(double) * 2 * B);
+        s->active.reserve(sizeof(int32_t) * B);
+        hbem::launch_fill_zero_f64(s->x64.as<double>(), (size_t)2 * B * N, st);
+        hbem::launch_copy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, B, st);
+        std::vector<double> best(B, 0.0), nrm(2 * B);
+        std::vector<int32_t> active(B, 1);
+        auto residual = [&]() {
+            hbem::launch_residual_f64(s->A_orig.as<float>(), lda, a_stride, n, B,
+                                      s->rhs.as<float>(), N, s->x64.as<double>(), N,
+                                      s->r64.as<double>(), s->norms.as<double>(), st);
+            hip_check(hipMemcpyAsync(nrm.data(), s->norms.ptr, sizeof(double) * 2 * B,
+                                     hipMemcpyDeviceToHost, st),
+                      "norms D2H");
+            hip_check(hipStreamSynchronize(st), "residual sync");
+        };
+        residual();
+        for (int b = 0; b < B; ++b) {
+            best[b] = nrm[2 * b + 1] > 0.0 ? nrm[2 * b] / nrm[2 * b + 1] : nrm[2 * b];
+            if (best[b] <= opt.refine_tolerance) active[b] = 0;
+        }
+        for (int it = 0; it < opt.refine_max_iterations; ++it) {
+            bool any = false;
+            for (int b = 0; b < B; ++b) any |= active[b] != 0;
+            if (!any) break;
+            hip_check(hipMemcpyAsync(s->active.ptr, active.data(), sizeof(int32_t) * B,
+                                     hipMemcpyHostToDevice, st),
+                      "active H2D");
+            hip_check(hipMemcpyAsync(s->xprev.ptr, s->x64.ptr, sizeof(double) * 2 * (size_t)B * N,
+                                     hipMemcpyDeviceToDevice, st),
+                      "xprev copy");
+            hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * B * N, st);
+            hbem::launch_copy_f64_to_f32(s->r64.as<double>(), s->x.as<float>(), N, N, n, B, st);
+            hbem::lu_solve(w, s->x.as<float>(), N, 0, st);
+            hbem::launch_axpy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, B,
+                                         s->active.as<int32_t>(), st);
+            residual();
+            for (int b = 0; b < B; ++b) {
+                if (!active[b]) continue;
+                const double rel =
+                    nrm[2 * b + 1] > 0.0 ? nrm[2 * b] / nrm[2 * b + 1] : nrm[2 * b];
+                if (rel >= best[b]) {
+                    // stalled: keep the previous iterate (main.swift:7051-7058)
+                    hip_check(hipMemcpyAsync(s->x64.as<double>() + (size_t)b * N * 2,
+                                             s->xprev.as<double>() + (size_t)b * N * 2,
+                                             sizeof(double) * 2 * (size_t)N,
+                                             hipMemcpyDeviceToDevice, st),
+                              "revert");
+                    active[b] = 0;
+                    continue;
+                }
+                best[b] = rel;
+                iters[b]++;
+                if (rel <= opt.refine_tolerance) active[b] = 0;
+            }
+        }
+        hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * B * N, st);
+        hbem::launch_copy_f64_to_f32(s->x64.as<double>(), s->x.as<float>(), N, N, n, B, st);
+        for (int b = 0; b < B; ++b) resid[b] = best[b];
+    }
+    std::vector<double> rcond(B, -1.0);
+    if (opt.estimate_rcond) estimate_rcond(s, w, anorm, &rcond);
+    t.stop(st);
+    *solve_seconds = t.seconds();
+    diag->resize(B);
+    for (int b = 0; b < B; ++b) {
+        hbem_case_diag &d = (*diag)[b];
+        d.lapack_info = info[b];
+        d.refine_iterations = iters[b];
+        d.refine_residual_rel = resid[b];
+        d.dense_solve_condition_1norm = anorm[b];
+        d.dense_solve_rcond = rcond[b];
+    }
+}
+
+int batch_size_for(const hbem_session *s, int F, int N, const hbem_solve_options &opt) {
+    if (opt.batch_frequencies > 0) return std::min(F, opt.batch_frequencies);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)16 << 30;
+    const bool refine = opt.solve_mode != HBEM_SOLVE_F32;
+    const double per = (double)N * N * 8.0 * (refine ? 2.0 : 1.0) + (double)N * 64.0;
+    const double budget = std::min(0.5 * (double)free_b, 96.0 * (1ull << 30));
+    int b = (int)std::max(1.0, std::floor(budget / per));
+    (void)s;
+    return std::max(1, std::min({F, b, 64}));
+}
+
+
+void check_options(const hbem_solve_options &o) {
+    if (o.solve_mode < HBEM_SOLVE_F32 || o.solve_mode > HBEM_SOLVE_MIXED_BF16)
+        fail(HBEM_EINVAL, "solve_mode out of range");
+    if (o.refine_max_iterations < 0) fail(HBEM_EINVAL, "refine_max_iterations must be >= 0");
+    if (!(o.refine_tolerance >= 0.0)) fail(HBEM_EINVAL, "refine_tolerance must be >= 0");
+    if (o.n_tags < 0 || (o.n_tags > 0 && !o.tags)) fail(HBEM_EINVAL, "tags invalid");
+}
+
+// All-device batch pipeline.  Inputs: k_real/field_k [F], k_imag [F] or
+// null, q [F][T] interleaved complex, obs [3][n_obs].  Outputs: field
+// [F][n_obs] interleaved complex (may be null when n_obs == 0), p [F][n]
+// interleaved complex or null, tag_avg [F][1 + n_tags][2] double (impedance
+// tag first).  Frequencies are processed in resident chunks.
+void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
+                const float *fk_d, const float *q_d, int n_obs, const float *obs_d,
+                const hbem_solve_options &opts, float *field_d, float *p_d, double *tag_d,
+                std::vector<hbem_case_diag> *diag_out) {
+    hipStream_t st = s->stream;
+    const int n = s->plan.n_dofs;
+    const int T = s->plan.n_triangles;
+    const int N = (int)round_up(n, hbem::kLuPad);
+    const int64_t lda = N, a_stride = (int64_t)N * N;
+    const int Bmax = batch_size_for(s, F, N, opts);
+    std::vector<int32_t> tag_list;
+    tag_list.push_back(opts.impedance_tag);
+    for (int i = 0; i < opts.n_tags; ++i) tag_list.push_back(opts.tags[i]);
+    const int ntag = (int)tag_list.size();
+    upload(s->tags, tag_list, st);
+    diag_out->assign(F, hbem_case_diag{});
+    if (n_obs > 0) hbem::launch_fill_zero(field_d, (size_t)2 * F * n_obs, st);
+    for (int f0 = 0; f0 < F; f0 += Bmax) {
+        const int B = std::min(Bmax, F - f0);
+        Timer ta, treg, tsing, tf;
+        ta.start(st);
+        assemble_device(s, B, kre_d + f0, kim_d ? kim_d + f0 : nullptr, q_d + (size_t)f0 * T * 2,
+                        lda, a_stride, N, &treg, &tsing);
+        ta.stop(st);
+        std::vector<hbem_case_diag> dg;
+        double solve_s = 0.0;
+        solve_resident(s, B, n, N, opts, &dg, &solve_s);
+        tf.start(st);
+        if (n_obs > 0)
+            hbem::launch_field(s->dg, B, fk_d + f0, s->x.as<float>(), N, q_d + (size_t)f0 * T * 2,
+                               n_obs, obs_d, field_d + (size_t)f0 * n_obs * 2, st);
+        hbem::launch_tag_average(s->dg, B, s->x.as<float>(), N, ntag, s->tags.as<int32_t>(),
+                                 tag_d + (size_t)f0 * ntag * 2, st);
+        if (p_d)
+            hip_check(hipMemcpy2DAsync(p_d + (size_t)f0 * n * 2, sizeof(float) * 2 * n,
+                                       s->x.ptr, sizeof(float) * 2 * N, sizeof(float) * 2 * n, B,
+                                       hipMemcpyDeviceToDevice, st),
+                      "p copy");
+        hip_check(hipGetLastError(), "field launch");
+        tf.stop(st);
+        const double asm_s = ta.seconds(), reg_s = treg.seconds(), sing_s = tsing.seconds();
+        const double field_s = tf.seconds();
+        for (int b = 0; b < B; ++b) {
+            hbem_case_diag &d = (*diag_out)[f0 + b];
+            d = dg[b];
+            d.assembly_s = asm_s / B;
+            d.regular_assembly_s = reg_s / B;
+            d.singular_assembly_s = sing_s / B;
+            d.dense_solve_s = solve_s / B;
+            d.field_s = field_s / B;
+            d.batch_index = f0 / Bmax;
+            d.batch_size = B;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hbem_solve_batch(hbem_session *s, int32_t F, const float *k_real, const float *k_imag,
+                         const float *field_k, const float *q_re, const float *q_im, int32_t n_obs,
+                         const float *obs_3xn, const hbem_solve_options *opts, float *field_re,
+                         float *field_im, float *p_re, float *p_im, double *impedance,
+                         double *pavg, hbem_case_diag *diag) {
+    return guarded([&] {
+        if (!s || !opts) fail(HBEM_EINVAL, "hbem_solve_batch: null argument");
+        if (F <= 0) fail(HBEM_EINVAL, "hbem_solve_batch: n_freq must be positive");
+        if (!k_real || !field_k || !q_re || !q_im)
+            fail(HBEM_EINVAL, "hbem_solve_batch: k_real, field_k and q must be non-null");
+        if (n_obs < 0 || (n_obs > 0 && (!obs_3xn || !field_re || !field_im)))
+            fail(HBEM_EINVAL, "hbem_solve_batch: observation arrays invalid");
+        check_options(*opts);
+        for (int f = 0; f < F; ++f) {
+            if (!std::isfinite(k_real[f]) || !std::isfinite(field_k[f]))
+                fail(HBEM_EINVAL, "hbem_solve_batch: wavenumbers must be finite");
+            if (k_imag && (!std::isfinite(k_imag[f]) || k_imag[f] < 0.0f))
+                fail(HBEM_EINVAL, "hbem_solve_batch: k_imag must be finite and >= 0");
+        }
+        std::lock_guard<std::mutex> lock(s->mu);
+        set_device(s);
+        hipStream_t st = s->stream;
+        const int n = s->plan.n_dofs;
+        const int T = s->plan.n_triangles;
+        const bool complex_k = any_nonzero(k_imag, (size_t)F);
+        std::vector<float> kr(k_real, k_real + F), fk(field_k, field_k + F);
+        upload(s->io_k, kr, st);
+        upload(s->io_fk, fk, st);
+        if (complex_k) {
+            std::vector<float> ki(k_imag, k_imag + F);
+            upload(s->io_kim, ki, st);
+        }
+        upload(s->io_q, interleave(q_re, q_im, (size_t)F * T), st);
+        if (n_obs > 0) upload(s->io_obs, std::vector<float>(obs_3xn, obs_3xn + 3 * (size_t)n_obs), st);
+        const int ntag = 1 + opts->n_tags;
+        s->io_field.reserve(sizeof(float) * 2 * std::max<size_t>(1, (size_t)F * n_obs));
+        s->io_tag.reserve(sizeof(double) * 2 * (size_t)F * ntag);
+        const bool want_p = p_re && p_im;
+        if (want_p) s->io_p.reserve(sizeof(float) * 2 * (size_t)F * n);
+        std::vector<hbem_case_diag> dg;
+        solve_core(s, F, s->io_k.as<float>(), complex_k ? s->io_kim.as<float>() : nullptr,
+                   s->io_fk.as<float>(), s->io_q.as<float>(), n_obs, s->io_obs.as<float>(), *opts,
+                   s->io_field.as<float>(), want_p ? s->io_p.as<float>() : nullptr,
+                   s->io_tag.as<double>(), &dg);
+        std::vector<float> fh((size_t)2 * F * std::max(n_obs, 0));
+        if (n_obs > 0)
+            hip_check(hipMemcpyAsync(fh.data(), s->io_field.ptr, sizeof(float) * fh.size(),
+                                     hipMemcpyDeviceToHost, st),
+                      "field D2H");
+        std::vector<double> th((size_t)2 * F * ntag);
+        hip_check(hipMemcpyAsync(th.data(), s->io_tag.ptr, sizeof(double) * th.size(),
+                                 hipMemcpyDeviceToHost, st),
+                  "tags D2H");
+        std::vector<float> ph;
+        if (want_p) {
+            ph.resize((size_t)2 * F * n);
+            hip_check(hipMemcpyAsync(ph.data(), s->io_p.ptr, sizeof(float) * ph.size(),
+                                     hipMemcpyDeviceToHost, st),
+                      "p D2H");
+        }
+        hip_check(hipStreamSynchronize(st), "batch sync");
+        for (int f = 0; f < F; ++f) {
+            if (dg[f].lapack_info != 0 && !diag)
+                fail(HBEM_ESINGULAR, "LU found an exactly zero pivot (lapack_info=" +
+                                         std::to_string(dg[f].lapack_info) + ")");
+            for (int o = 0; o < n_obs; ++o) {
+                field_re[(size_t)f * n_obs + o] = fh[((size_t)f * n_obs + o) * 2];
+                field_im[(size_t)f * n_obs + o] = fh[((size_t)f * n_obs + o) * 2 + 1];
+            }
+            if (want_p)
+                for (int i = 0; i < n; ++i) {
+                    p_re[(size_t)f * n + i] = ph[((size_t)f * n + i) * 2];
+                    p_im[(size_t)f * n + i] = ph[((size_t)f * n + i) * 2 + 1];
+                }
+            if (impedance) {
+                impedance[2 * f] = th[((size_t)f * ntag) * 2];
+                impedance[2 * f + 1] = th[((size_t)f * ntag) * 2 + 1];
+            }
+            if (pavg)
+                for (int t = 0; t < opts->n_tags; ++t) {
+                    pavg[((size_t)f * opts->n_tags + t) * 2] = th[((size_t)f * ntag + 1 + t) * 2];
+                    pavg[((size_t)f * opts->n_tags + t) * 2 + 1] =
+                        th[((size_t)f * ntag + 1 + t) * 2 + 1];
+                }
+            if (diag) diag[f] = dg[f];
+        }
+    });
+}
+
+int32_t hbem_solve_batch_device(hbem_session *s, int32_t F, const float *k_real,
+                                const float *k_imag, const float *field_k, const float *q,
+                                int32_t n_obs, const float *obs_3xn,
+                                const hbem_solve_options *opts, float *field, float *p,
+                                double *tag_avg, hbem_case_diag *diag) {
+    return guarded([&] {
+        if (!s || !opts) fail(HBEM_EINVAL, "hbem_solve_batch_device: null argument");
+        if (F <= 0) fail(HBEM_EINVAL, "hbem_solve_batch_device: n_freq must be positive");
+        if (!k_real || !field_k || !q || !tag_avg)
+            fail(HBEM_EINVAL, "hbem_solve_batch_device: k_real, field_k, q, tag_avg required");
+        if (n_obs < 0 || (n_obs > 0 && (!obs_3xn || !field)))
+            fail(HBEM_EINVAL, "hbem_solve_batch_device: observation arrays invalid");
+        check_options(*opts);
+        std::lock_guard<std::mutex> lock(s->mu);
+        set_device(s);
+        std::vector<hbem_case_diag> dg;
+        solve_core(s, F, k_real, k_imag, field_k, q, n_obs, obs_3xn, *opts, field, p, tag_avg,
+                   &dg);
+        hip_check(hipStreamSynchronize(s->stream), "batch sync");
+        for (int f = 0; f < F; ++f) {
+            if (dg[f].lapack_info != 0 && !diag)
+                fail(HBEM_ESINGULAR, "LU found an exactly zero pivot");
+            if (diag) diag[f] = dg[f];
+        }
+    });
+}
+
+int32_t hbem_evaluate_field(hbem_session *s, int32_t F, const float *field_k, const float *p_re,
+                            const float *p_im, const float *q_re, const float *q_im, int32_t n_obs,
+                            const float *obs_3xn, float *out_re, float *out_im) {
+    return guarded([&] {
+        if (!s) fail(HBEM_EINVAL, "hbem_evaluate_field: null session");
+        if (F <= 0 || n_obs <= 0) fail(HBEM_EINVAL, "hbem_evaluate_field: empty batch");
+        if (!field_k || !p_re || !p_im || !q_re || !q_im || !obs_3xn || !out_re || !out_im)
+            fail(HBEM_EINVAL, "hbem_evaluate_field: null array");
+        std::lock_guard<std::mutex> lock(s->mu);
+        set_device(s);
+        hipStream_t st = s->stream;
+        const int n = s->plan.n_dofs, T = s->plan.n_triangles;
+        std::vector<float> fk(field_k, field_k + F);
+        upload(s->fk, fk, st);
+        std::vector<float> p = interleave(p_re, p_im, (size_t)F * n);
+        upload(s->x, p, st);
+        std::vector<float> q = interleave(q_re, q_im, (size_t)F * T);
+        upload(s->q, q, st);
+        std::vector<float> obs(obs_3xn, obs_3xn + 3 * (size_t)n_obs);
+        upload(s->obs, obs, st);
+        s->field.reserve(sizeof(float) * 2 * (size_t)F * n_obs);
+        hbem::launch_fill_zero(s->field.as<float>(), (size_t)2 * F * n_obs, st);
+        hbem::launch_field(s->dg, F, s->fk.as<float>(), s->x.as<float>(), n, s->q.as<float>(),
+                           n_obs, s->obs.as<float>(), s->field.as<float>(), st);
+        hip_check(hipGetLastError(), "field launch");
+        std::vector<float> fh((size_t)2 * F * n_obs);
+        hip_check(hipMemcpyAsync(fh.data(), s->field.ptr, sizeof(float) * fh.size(),
+                                 hipMemcpyDeviceToHost, st),
+                  "field D2H");
+        hip_check(hipStreamSynchronize(st), "field sync");
+        for (size_t i = 0; i < (size_t)F * n_obs; ++i) {
+            out_re[i] = fh[2 * i];
+            out_im[i] = fh[2 * i + 1];
+        }
+    });
+}
+
+int32_t hbem_dense_solve(int32_t n_sys, int32_t n, const float *A_re, const float *A_im,
+                         const float *b_re, const float *b_im, const hbem_solve_options *opts,
+                         float *x_re, float *x_im, hbem_case_diag *diag) {
+    return guarded([&] {
+        if (n_sys <= 0 || n <= 0) fail(HBEM_EINVAL, "hbem_dense_solve: empty problem");
+        if (!A_re || !A_im || !b_re || !b_im || !x_re || !x_im || !opts)
+            fail(HBEM_EINVAL, "hbem_dense_solve: null array");
+        const int dev = gfx950_device();
+        if (dev < 0) fail(HBEM_ENODEV, "no gfx950 (MI355X) device is visible");
+        // A transient session without geometry carries the workspaces.
+        hbem_session s;
+        s.device = dev;
+        set_device(&s);
+        hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+        const int N = (int)round_up(n, hbem::kLuPad);
+        const int64_t lda = N, a_stride = (int64_t)N * N;
+        const int B = n_sys;
+        s.A.reserve(sizeof(float) * 2 * (size_t)B * a_stride);
+        s.rowsplit.reserve(sizeof(float) * 2 * (size_t)B * n * n);
+        float *re_d = s.rowsplit.as<float>();
+        float *im_d = re_d + (size_t)B * n * n;
+        hip_check(hipMemcpyAsync(re_d, A_re, sizeof(float) * (size_t)B * n * n,
+                                 hipMemcpyHostToDevice, s.stream),
+                  "A_re H2D");
+        hip_check(hipMemcpyAsync(im_d, A_im, sizeof(float) * (size_t)B * n * n,
+                                 hipMemcpyHostToDevice, s.stream),
+                  "A_im H2D");
+        hbem::launch_fill_zero(s.A.as<float>(), (size_t)2 * B * a_stride, s.stream);
+        hbem::launch_rowmajor_split_to_colmajor(re_d, im_d, n, B, s.A.as<float>(), lda, a_stride,
+                                                s.stream);
+        std::vector<float> rh((size_t)2 * B * N, 0.0f);
+        for (int b = 0; b < B; ++b)
+            for (int i = 0; i < n; ++i) {
+                rh[((size_t)b * N + i) * 2] = b_re[(size_t)b * n + i];
+                rh[((size_t)b * N + i) * 2 + 1] = b_im[(size_t)b * n + i];
+            }
+        upload(s.rhs, rh, s.stream);
+        std::vector<hbem_case_diag> dg;
+        double solve_s = 0.0;
+        solve_resident(&s, B, n, N, *opts, &dg, &solve_s);
+        std::vector<float> xh((size_t)2 * B * N);
+        hip_check(hipMemcpyAsync(xh.data(), s.x.ptr, sizeof(float) * xh.size(),
+                                 hipMemcpyDeviceToHost, s.stream),
+                  "x D2H");
+        hip_check(hipStreamSynchronize(s.stream), "dense solve sync");
+        for (int b = 0; b < B; ++b) {
+            if (dg[b].lapack_info != 0 && !diag)
+                fail(HBEM_ESINGULAR, "LU found an exactly zero pivot");
+            for (int i = 0; i < n; ++i) {
+                x_re[(size_t)b * n + i] = xh[((size_t)b * N + i) * 2];
+                x_im[(size_t)b * n + i] = xh[((size_t)b * N + i) * 2 + 1];
+            }
+            if (diag) {
+                diag[b] = dg[b];
+                diag[b].assembly_s = 0.0;
+                diag[b].field_s = 0.0;
+                diag[b].dense_solve_s = solve_s / B;
+            }
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,597 @@
+// gfx950 kernels of the assembly and field stages of libhbem.
+//
+//   k_assemble_regular  -- every non-touching (test, trial, image) triangle
+//                          pair with the 6x6-point rule, fused double-layer
+//                          3x3 block + single-layer right-hand side
+//                          (reference: assemble_matrix_pair_atomic and
+//                          assemble_rhs_source_regular, main.swift:3698-3736
+//                          and 3993-4133).
+//   k_assemble_singular -- touching pairs with the Sauter-Schwab rules plus
+//                          the -M/2 mass term (reference: duffy_delta_blocks
+//                          main.swift:4246-4431 folded by
+//                          reduceDuffyDeltaBlocks main.swift:6498-6561).
+//   k_field             -- exterior potential (evaluate_field_regular,
+//                          main.swift:4135-4203).
+//   k_tag_average       -- area-weighted tag means
+//                          (averageSurfacePressureForTag, main.swift:807-841).
+//
+// Matrices are column-major interleaved complex (re, im) float with leading
+// dimension lda; vectors are interleaved complex.
+//
+// Kernel of every stage, for r = y - x (trial minus test / source minus
+// observer), complex k = k_re + i k_im:
+//   G      = e^{i k_re r} e^{-k_im r} / (4 pi r)
+//   dG/dn  = G (i k_re - k_im - 1/r) (r . n_y) / r
+// The phase is range-reduced in revolutions (t = k_re r / 2pi, t -= rint(t))
+// before the hardware sin/cos, which take revolutions.
+#include <hip/hip_runtime.h>
+
+#include "hbem_internal.h"
+
+namespace hbem {
+namespace {
+
+constexpr float kInv4Pi = 0.07957747154594767f;
+constexpr float kInv2Pi = 0.15915494309189535f;
+
+struct GD {
+    float gre, gim, dre, dim;
+};
+
+// G and dG/dn_y for d = y - x and normal n (FP32).  Returns zeros for
+// r^2 <= 1e-14 (the reference GPU guard, main.swift:3443-3480).
+template <bool kComplexK>
+__device__ __forceinline__ GD kernel_gd(float dx, float dy, float dz, float nx, float ny,
+                                        float nz, float k, float krev, float kim) {
+    const float r2raw = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const bool ok = r2raw > 1.0e-14f;
+    const float r2 = ok ? r2raw : 1.0e-14f;  // keeps every lane finite
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    float t = r * krev;
+    t = t - __builtin_rintf(t);
+    const float s = __builtin_amdgcn_sinf(t);
+    const float c = __builtin_amdgcn_cosf(t);
+    float scale = kInv4Pi * rinv;
+    if (kComplexK) scale *= __expf(-kim * r);
+    scale = ok ? scale : 0.0f;
+    const float gre = c * scale;
+    const float gim = s * scale;
+    const float proj = fmaf(dx, nx, fmaf(dy, ny, dz * nz)) * rinv;
+    const float fre = kComplexK ? (-rinv - kim) : -rinv;
+    GD o;
+    o.gre = gre;
+    o.gim = gim;
+    o.dre = (gre * fre - gim * k) * proj;
+    o.dim = (gre * k + gim * fre) * proj;
+    return o;
+}
+
+__device__ __forceinline__ float mask_sign(int mask, int bit) { return (mask & bit) ? -1.0f : 1.0f; }
+
+// ---------------------------------------------------------------------------
+// Regular pairs.
+// Block = 256 threads = 4 waves.  Lanes own trial triangles (one per lane,
+// trial record held in registers); each wave walks a run of test triangles
+// whose records are wave-uniform (scalar loads).  grid.z = frequency.
+// ---------------------------------------------------------------------------
+
+constexpr int kRegTestsPerWave = 8;
+
+template <int kMasks, bool kComplexK>
+__global__ __launch_bounds__(256) void k_assemble_regular(
+    DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
+    const float *__restrict__ q, float *__restrict__ A, int64_t lda, int64_t a_stride,
+    float *__restrict__ rhs, int64_t rhs_stride) {
+    const int T = g.n_triangles;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int trial = blockIdx.x * 64 + lane;
+    const int b = blockIdx.z;
+    const bool trial_ok = trial < T;
+    const int tr = trial_ok ? trial : 0;
+
+    const float k = k_real[b];
+    const float kim = kComplexK ? k_imag[b] : 0.0f;
+    const float krev = k * kInv2Pi;
+
+    // Trial record (SoA, coalesced).
+    float yp[18];
+#pragma unroll
+    for (int j = 0; j < 18; ++j) yp[j] = g.tri_f_soa[(size_t)j * T + tr];
+    const float nx = g.tri_f_soa[(size_t)18 * T + tr];
+    const float ny = g.tri_f_soa[(size_t)19 * T + tr];
+    const float nz = g.tri_f_soa[(size_t)20 * T + tr];
+    const float jac_s = g.tri_f_soa[(size_t)22 * T + tr];
+    int sdof[3], svid[3], skid[kMasks][3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        sdof[l] = g.tri_i_soa[(size_t)l * T + tr];
+        svid[l] = g.tri_i_soa[(size_t)(3 + l) * T + tr];
+    }
+#pragma unroll
+    for (int m = 0; m < kMasks; ++m)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) skid[m][l] = g.tri_i_soa[(size_t)(6 + 3 * m + l) * T + tr];
+    const float qre = trial_ok ? q[((size_t)b * T + tr) * 2] : 0.0f;
+    const float qim = trial_ok ? q[((size_t)b * T + tr) * 2 + 1] : 0.0f;
+    const bool wave_has_source = __any(qre != 0.0f || qim != 0.0f);
+
+    float *Ab = A + (size_t)b * a_stride * 2;
+    float *rb = rhs + (size_t)b * rhs_stride * 2;
+
+    const int test0 = (blockIdx.y * 4 + wave) * kRegTestsPerWave;
+    for (int ti = 0; ti < kRegTestsPerWave; ++ti) {
+        const int test = test0 + ti;
+        if (test >= T) break;
+        const float *tf = g.tri_f_aos + (size_t)test * kTriF;
+        const int32_t *tii = g.tri_i_aos + (size_t)test * kTriI;
+        const float jac = tf[22] * jac_s * g.row_weight;
+
+        float dlp[9][2];
+        float slp[3][2];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dlp[i][0] = dlp[i][1] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) slp[i][0] = slp[i][1] = 0.0f;
+
+#pragma unroll
+        for (int m = 0; m < kMasks; ++m) {
+            // Touching (test, trial image) combinations belong to the
+            // singular kernel.  Image 0 compares vertex indices, images by
+            // coordinate-key ids (the reference's pair criteria).
+            bool touch = false;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if (m == 0)
+                        touch |= (tii[3 + i] == svid[j]);
+                    else
+                        touch |= (tii[6 + i] == skid[m][j]);
+                }
+            if (touch || !trial_ok) continue;
+            const int mask = g.masks[m];
+            const float sx = mask_sign(mask, 1), sy = mask_sign(mask, 2), sz = mask_sign(mask, 4);
+            const float mnx = sx * nx, mny = sy * ny, mnz = sz * nz;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                const float xa = tf[3 * a], ya = tf[3 * a + 1], za = tf[3 * a + 2];
+                float e[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+                float gs[2] = {0.f, 0.f};
+#pragma unroll
+                for (int bq = 0; bq < 6; ++bq) {
+                    const GD v = kernel_gd<kComplexK>(sx * yp[3 * bq] - xa, sy * yp[3 * bq + 1] - ya,
+                                                      sz * yp[3 * bq + 2] - za, mnx, mny, mnz, k,
+                                                      krev, kim);
+                    const float wb = (float)kRule6W[bq];
+                    const float w1 = wb * (float)kRule6X[bq];
+                    const float w2 = wb * (float)kRule6Y[bq];
+                    const float w0 = wb - w1 - w2;
+                    e[0][0] = fmaf(v.dre, w0, e[0][0]);
+                    e[0][1] = fmaf(v.dim, w0, e[0][1]);
+                    e[1][0] = fmaf(v.dre, w1, e[1][0]);
+                    e[1][1] = fmaf(v.dim, w1, e[1][1]);
+                    e[2][0] = fmaf(v.dre, w2, e[2][0]);
+                    e[2][1] = fmaf(v.dim, w2, e[2][1]);
+                    gs[0] = fmaf(v.gre, wb, gs[0]);
+                    gs[1] = fmaf(v.gim, wb, gs[1]);
+                }
+                const float wa = (float)kRule6W[a];
+                const float ta[3] = {wa * (float)(1.0 - kRule6X[a] - kRule6Y[a]),
+                                     wa * (float)kRule6X[a], wa * (float)kRule6Y[a]};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        dlp[3 * i + j][0] = fmaf(ta[i], e[j][0], dlp[3 * i + j][0]);
+                        dlp[3 * i + j][1] = fmaf(ta[i], e[j][1], dlp[3 * i + j][1]);
+                    }
+                    slp[i][0] = fmaf(ta[i], gs[0], slp[i][0]);
+                    slp[i][1] = fmaf(ta[i], gs[1], slp[i][1]);
+                }
+            }
+        }
+
+        if (trial_ok) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int row = tii[i];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const size_t idx = ((size_t)sdof[j] * lda + row) * 2;
+                    const float vre = dlp[3 * i + j][0] * jac, vim = dlp[3 * i + j][1] * jac;
+                    if (vre != 0.0f || vim != 0.0f) {
+                        atomicAdd(Ab + idx, vre);
+                        atomicAdd(Ab + idx + 1, vim);
+                    }
+                }
+            }
+        }
+        if (wave_has_source) {
+            // rhs_i += 2^planes * slp_i * q_trial, reduced over the wave.
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                float cre = (slp[i][0] * qre - slp[i][1] * qim) * jac;
+                float cim = (slp[i][0] * qim + slp[i][1] * qre) * jac;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    cre += __shfl_xor(cre, off);
+                    cim += __shfl_xor(cim, off);
+                }
+                if (lane == 0) {
+                    atomicAdd(rb + (size_t)tii[i] * 2, cre);
+                    atomicAdd(rb + (size_t)tii[i] * 2 + 1, cim);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Singular pairs: one wave per (pair, frequency); lanes stride over the
+// Sauter-Schwab points of the pair's kind.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void remap_point(float x, float y, int kind, int l1, int l2,
+                                            float &ox, float &oy) {
+    // remapSingular (main.swift:2709-2735)
+    if (kind == 1) {
+        ox = x;
+        oy = y;
+    } else if (kind == 2) {
+        const int vc = 3 - l1 - l2;
+        const float ax = (l1 == 1), ay = (l1 == 2);
+        const float bx = (l2 == 1), by = (l2 == 2);
+        const float cx = (vc == 1), cy = (vc == 2);
+        ox = ax + x * (bx - ax) + y * (cx - ax);
+        oy = ay + x * (by - ay) + y * (cy - ay);
+    } else {
+        if (l1 == 0) {
+            ox = x;
+            oy = y;
+        } else if (l1 == 1) {
+            ox = 1.0f - x - y;
+            oy = y;
+        } else {
+            ox = x;
+            oy = 1.0f - x - y;
+        }
+    }
+}
+
+template <bool kComplexK>
+__global__ __launch_bounds__(256) void k_assemble_singular(
+    DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
+    const float *__restrict__ q, float *__restrict__ A, int64_t lda, int64_t a_stride,
+    float *__restrict__ rhs, int64_t rhs_stride) {
+    const int lane = threadIdx.x & 63;
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int b = blockIdx.y;
+    if (pair >= g.n_pairs) return;
+    const int T = g.n_triangles;
+    const int test = g.pair_tt[2 * pair];
+    const int trial = g.pair_tt[2 * pair + 1];
+    const int code = g.pair_code[pair];
+    const int tm = code & 7, sm = (code >> 3) & 7, kind = (code >> 6) & 3;
+    const int tl1 = (code >> 8) & 3, tl2 = (code >> 10) & 3;
+    const int sl1 = (code >> 12) & 3, sl2 = (code >> 14) & 3;
+    const bool t_odd = __popc(tm) & 1, s_odd = __popc(sm) & 1;
+
+    const float k = k_real[b];
+    const float kim = kComplexK ? k_imag[b] : 0.0f;
+    const float krev = k * kInv2Pi;
+
+    const float *tf = g.tri_f_aos + (size_t)test * kTriF;
+    const float *sf = g.tri_f_aos + (size_t)trial * kTriF;
+    const float tsx = mask_sign(tm, 1), tsy = mask_sign(tm, 2), tsz = mask_sign(tm, 4);
+    const float ssx = mask_sign(sm, 1), ssy = mask_sign(sm, 2), ssz = mask_sign(sm, 4);
+    const float nx = ssx * sf[18], ny = ssy * sf[19], nz = ssz * sf[20];
+    const float t0x = tf[23], t0y = tf[24], t0z = tf[25];
+    const float t1x = tf[26] - t0x, t1y = tf[27] - t0y, t1z = tf[28] - t0z;
+    const float t2x = tf[29] - t0x, t2y = tf[30] - t0y, t2z = tf[31] - t0z;
+    const float s0x = sf[23], s0y = sf[24], s0z = sf[25];
+    const float s1x = sf[26] - s0x, s1y = sf[27] - s0y, s1z = sf[28] - s0z;
+    const float s2x = sf[29] - s0x, s2y = sf[30] - s0y, s2z = sf[31] - s0z;
+
+    float dlp[9][2], slp[3][2];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dlp[i][0] = dlp[i][1] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) slp[i][0] = slp[i][1] = 0.0f;
+
+    const int off = g.rule_offset[kind - 1];
+    const int cnt = g.rule_count[kind - 1];
+    for (int p = lane; p < cnt; p += 64) {
+        const float *r = g.rule + (size_t)(off + p) * 5;
+        float a, bb, txi, teta, sxi, seta;
+        remap_point(r[0], r[1], kind, tl1, tl2, a, bb);
+        txi = t_odd ? bb : a;
+        teta = t_odd ? a : bb;
+        remap_point(r[2], r[3], kind, sl1, sl2, a, bb);
+        sxi = s_odd ? bb : a;
+        seta = s_odd ? a : bb;
+        const float w = r[4];
+        const float px = tsx * (t0x + txi * t1x + teta * t2x);
+        const float py = tsy * (t0y + txi * t1y + teta * t2y);
+        const float pz = tsz * (t0z + txi * t1z + teta * t2z);
+        const float qx = ssx * (s0x + sxi * s1x + seta * s2x);
+        const float qy = ssy * (s0y + sxi * s1y + seta * s2y);
+        const float qz = ssz * (s0z + sxi * s1z + seta * s2z);
+        const GD v = kernel_gd<kComplexK>(qx - px, qy - py, qz - pz, nx, ny, nz, k, krev, kim);
+        const float tb[3] = {(1.0f - txi - teta) * w, txi * w, teta * w};
+        const float sb[3] = {1.0f - sxi - seta, sxi, seta};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            slp[i][0] = fmaf(v.gre, tb[i], slp[i][0]);
+            slp[i][1] = fmaf(v.gim, tb[i], slp[i][1]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float c = tb[i] * sb[j];
+                dlp[3 * i + j][0] = fmaf(v.dre, c, dlp[3 * i + j][0]);
+                dlp[3 * i + j][1] = fmaf(v.dim, c, dlp[3 * i + j][1]);
+            }
+        }
+    }
+    // Wave reduction of the 24 accumulators.
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) dlp[i][c] += __shfl_xor(dlp[i][c], o);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) slp[i][c] += __shfl_xor(slp[i][c], o);
+
+    const float jac = tf[22] * sf[22];
+    const bool mass = (test == trial) && (tm == sm) && (kind == 1);
+    const float area = tf[21];
+    float *Ab = A + (size_t)b * a_stride * 2;
+    float *rb = rhs + (size_t)b * rhs_stride * 2;
+    if (lane < 9) {
+        const int i = lane / 3, j = lane % 3;
+        float vre = 0.f, vim = 0.f;
+        // select accumulator lane-wise without dynamic register indexing
+#pragma unroll
+        for (int s = 0; s < 9; ++s)
+            if (s == lane) {
+                vre = dlp[s][0];
+                vim = dlp[s][1];
+            }
+        vre *= jac;
+        vim *= jac;
+        if (mass) vre -= 0.5f * area * (i == j ? (1.0f / 6.0f) : (1.0f / 12.0f));
+        const int row = g.tri_i_aos[(size_t)test * kTriI + i];
+        const int col = g.tri_i_aos[(size_t)trial * kTriI + j];
+        const size_t idx = ((size_t)col * lda + row) * 2;
+        atomicAdd(Ab + idx, vre);
+        atomicAdd(Ab + idx + 1, vim);
+    } else if (lane < 12) {
+        const int i = lane - 9;
+        const float qre = q[((size_t)b * T + trial) * 2];
+        const float qim = q[((size_t)b * T + trial) * 2 + 1];
+        if (qre != 0.0f || qim != 0.0f) {
+            float sre = 0.f, sim = 0.f;
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+                if (s == i) {
+                    sre = slp[s][0];
+                    sim = slp[s][1];
+                }
+            sre *= jac;
+            sim *= jac;
+            const int row = g.tri_i_aos[(size_t)test * kTriI + i];
+            atomicAdd(rb + (size_t)row * 2, sre * qre - sim * qim);
+            atomicAdd(rb + (size_t)row * 2 + 1, sre * qim + sim * qre);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exterior field.  Lanes own observation points; a wave walks a chunk of
+// triangles (wave-uniform records).  grid = (obs blocks, triangle chunks, B).
+// ---------------------------------------------------------------------------
+
+constexpr int kFieldChunk = 64;
+
+template <int kMasks>
+__global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__restrict__ field_k,
+                                               const float *__restrict__ p, int64_t p_stride,
+                                               const float *__restrict__ q, int n_obs,
+                                               const float *__restrict__ obs,
+                                               float *__restrict__ out) {
+    const int T = g.n_triangles;
+    const int o = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.z;
+    const bool ok = o < n_obs;
+    const int oo = ok ? o : 0;
+    const float ox = obs[oo], oy = obs[n_obs + oo], oz = obs[2 * n_obs + oo];
+    const float k = field_k[b];
+    const float krev = k * kInv2Pi;
+    const float *pb = p + (size_t)b * p_stride * 2;
+    const float *qb = q + (size_t)b * T * 2;
+    float acc_re = 0.f, acc_im = 0.f;
+    const int t_begin = blockIdx.y * kFieldChunk;
+    const int t_end = min(T, t_begin + kFieldChunk);
+    for (int t = t_begin; t < t_end; ++t) {
+        const float *tf = g.tri_f_aos + (size_t)t * kTriF;
+        const int32_t *ti = g.tri_i_aos + (size_t)t * kTriI;
+        const float p0r = pb[2 * ti[0]], p0i = pb[2 * ti[0] + 1];
+        const float p1r = pb[2 * ti[1]], p1i = pb[2 * ti[1] + 1];
+        const float p2r = pb[2 * ti[2]], p2i = pb[2 * ti[2] + 1];
+        const float qre = qb[2 * t], qim = qb[2 * t + 1];
+        const float jac = tf[22];
+        float tre = 0.f, tim = 0.f;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            const float xi = (float)kRule6X[a], eta = (float)kRule6Y[a];
+            const float b0 = 1.0f - xi - eta;
+            const float pr = b0 * p0r + xi * p1r + eta * p2r;
+            const float pi = b0 * p0i + xi * p1i + eta * p2i;
+            const float w = (float)kRule6W[a];
+#pragma unroll
+            for (int m = 0; m < kMasks; ++m) {
+                const int mask = g.masks[m];
+                const float sx = mask_sign(mask, 1), sy = mask_sign(mask, 2), sz = mask_sign(mask, 4);
+                const GD v = kernel_gd<false>(sx * tf[3 * a] - ox, sy * tf[3 * a + 1] - oy,
+                                              sz * tf[3 * a + 2] - oz, sx * tf[18], sy * tf[19],
+                                              sz * tf[20], k, krev, 0.0f);
+                // (dG/dn p - G q) w
+                const float cre = (v.dre * pr - v.dim * pi) - (v.gre * qre - v.gim * qim);
+                const float cim = (v.dre * pi + v.dim * pr) - (v.gre * qim + v.gim * qre);
+                tre = fmaf(cre, w, tre);
+                tim = fmaf(cim, w, tim);
+            }
+        }
+        acc_re = fmaf(tre, jac, acc_re);
+        acc_im = fmaf(tim, jac, acc_im);
+    }
+    if (ok) {
+        atomicAdd(out + ((size_t)b * n_obs + o) * 2, acc_re);
+        atomicAdd(out + ((size_t)b * n_obs + o) * 2 + 1, acc_im);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tag averages: one block per (tag, frequency), FP64 accumulation.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_tag_average(DeviceGeometry g, const float *__restrict__ p,
+                                                     int64_t p_stride, int n_tags,
+                                                     const int32_t *__restrict__ tags,
+                                                     double *__restrict__ out) {
+    const int t_idx = blockIdx.x;
+    const int b = blockIdx.y;
+    const int tag = tags[t_idx];
+    const int T = g.n_triangles;
+    const float *pb = p + (size_t)b * p_stride * 2;
+    double sr = 0.0, si = 0.0, sa = 0.0;
+    for (int t = threadIdx.x; t < T; t += 256) {
+        const int32_t *ti = g.tri_i_aos + (size_t)t * kTriI;
+        if (ti[18] != tag) continue;
+        const double area = (double)g.tri_f_aos[(size_t)t * kTriF + 21];
+        if (!(area > 0.0)) continue;
+        const double ar = ((double)pb[2 * ti[0]] + (double)pb[2 * ti[1]] + (double)pb[2 * ti[2]]) / 3.0;
+        const double ai =
+            ((double)pb[2 * ti[0] + 1] + (double)pb[2 * ti[1] + 1] + (double)pb[2 * ti[2] + 1]) / 3.0;
+        sr += ar * area;
+        si += ai * area;
+        sa += area;
+    }
+    __shared__ double red[3][256];
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    red[2][threadIdx.x] = sa;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+            red[2][threadIdx.x] += red[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double ta = red[2][0];
+        double* o = out + ((size_t)b * n_tags + t_idx) * 2;
+        if (ta < 1e-30) {
+            o[0] = 0.0;
+            o[1] = 0.0;
+        } else {
+            o[0] = red[0][0] / ta;
+            o[1] = red[1][0] / ta;
+        }
+    }
+}
+
+__global__ void k_fill_zero(float *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+__global__ void k_fill_zero_f64(double *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+}  // namespace
+
+void launch_fill_zero(float *ptr, size_t n, hipStream_t stream) {
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, stream, ptr, n);
+}
+
+void launch_fill_zero_f64(double *ptr, size_t n, hipStream_t stream) {
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_zero_f64, dim3((unsigned)blocks), dim3(256), 0, stream, ptr, n);
+}
+
+void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *k_real,
+                             const float *k_imag, const float *q, float *A, int64_t lda,
+                             int64_t a_stride, float *rhs, int64_t rhs_stride,
+                             hipStream_t stream) {
+    const int T = g.n_triangles;
+    dim3 grid((T + 63) / 64, (T + 4 * kRegTestsPerWave - 1) / (4 * kRegTestsPerWave), n_batch);
+    const bool ck = k_imag != nullptr;
+#define HBEM_REG(M, CK)                                                                      \
+    hipLaunchKernelGGL((k_assemble_regular<M, CK>), grid, dim3(256), 0, stream, g, k_real,   \
+                       k_imag, q, A, lda, a_stride, rhs, rhs_stride)
+    switch (g.n_masks) {
+    case 1: if (ck) HBEM_REG(1, true); else HBEM_REG(1, false); break;
+    case 2: if (ck) HBEM_REG(2, true); else HBEM_REG(2, false); break;
+    default: if (ck) HBEM_REG(4, true); else HBEM_REG(4, false); break;
+    }
+#undef HBEM_REG
+}
+
+void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float *k_real,
+                              const float *k_imag, const float *q, float *A, int64_t lda,
+                              int64_t a_stride, float *rhs, int64_t rhs_stride,
+                              hipStream_t stream) {
+    if (g.n_pairs == 0) return;
+    dim3 grid((g.n_pairs + 3) / 4, n_batch);
+    if (k_imag != nullptr)
+        hipLaunchKernelGGL((k_assemble_singular<true>), grid, dim3(256), 0, stream, g, k_real,
+                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
+    else
+        hipLaunchKernelGGL((k_assemble_singular<false>), grid, dim3(256), 0, stream, g, k_real,
+                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
+}
+
+void launch_field(const DeviceGeometry &g, int n_batch, const float *field_k, const float *p,
+                  int64_t p_stride, const float *q, int n_obs, const float *obs_3xn, float *out,
+                  hipStream_t stream) {
+    if (n_obs == 0) return;
+    dim3 grid((n_obs + 255) / 256, (g.n_triangles + kFieldChunk - 1) / kFieldChunk, n_batch);
+    switch (g.n_masks) {
+    case 1:
+        hipLaunchKernelGGL((k_field<1>), grid, dim3(256), 0, stream, g, field_k, p, p_stride, q,
+                           n_obs, obs_3xn, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_field<2>), grid, dim3(256), 0, stream, g, field_k, p, p_stride, q,
+                           n_obs, obs_3xn, out);
+        break;
+    default:
+        hipLaunchKernelGGL((k_field<4>), grid, dim3(256), 0, stream, g, field_k, p, p_stride, q,
+                           n_obs, obs_3xn, out);
+        break;
+    }
+}
+
+void launch_tag_average(const DeviceGeometry &g, int n_batch, const float *p, int64_t p_stride,
+                        int n_tags, const int32_t *tags_dev, double *out, hipStream_t stream) {
+    if (n_tags == 0) return;
+    hipLaunchKernelGGL(k_tag_average, dim3(n_tags, n_batch), dim3(256), 0, stream, g, p, p_stride,
+                       n_tags, tags_dev, out);
+}
+
+}  // namespace hbem

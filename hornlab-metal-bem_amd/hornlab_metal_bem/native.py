@@ -1,0 +1,655 @@
+"""ctypes boundary to libhbem (include/hbem.h).
+
+``HipBemSession`` takes the place of the reference's
+``MetalNativeStandardSession`` (hornlab_metal_bem/metal/native.py:233-1879):
+same operation names and argument meaning, but an in-process C ABI instead of
+a helper subprocess exchanging JSON manifests and binary files.  Results are
+returned as arrays instead of file descriptors.
+
+There is no CPU fallback: if the library or a gfx950 device is missing every
+entry point raises ``AssemblyBackendUnavailable``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Any, Callable
+
+import numpy as np
+from numpy.typing import NDArray
+
+from .backends import AssemblyBackendUnavailable
+from .geometry import SYMMETRY_CODES, GeometryBuffers, validate_native_symmetry_plane
+
+HBEM_ABI_VERSION = 1
+_LIB_NAME = "libhbem.so"
+
+SOLVE_MODES = {"fp32": 0, "fp32_ir": 1, "mixed_fp16": 2, "mixed_bf16": 3}
+
+
+class _Geometry(C.Structure):
+    _fields_ = [
+        ("n_vertices", C.c_int32),
+        ("n_triangles", C.c_int32),
+        ("p1_dof_count", C.c_int32),
+        ("symmetry", C.c_int32),
+        ("vertices_3xn", C.POINTER(C.c_float)),
+        ("triangles_3xm", C.POINTER(C.c_int32)),
+        ("p1_local2global", C.POINTER(C.c_int32)),
+        ("areas", C.POINTER(C.c_float)),
+        ("normals_3xm", C.POINTER(C.c_float)),
+        ("physical_tags", C.POINTER(C.c_int32)),
+    ]
+
+
+class _SessionInfo(C.Structure):
+    _fields_ = [
+        ("duffy_coincident", C.c_int64),
+        ("duffy_edge", C.c_int64),
+        ("duffy_vertex", C.c_int64),
+        ("duffy_image_pairs", C.c_int64),
+        ("duffy_unique_slots", C.c_int64),
+        ("n_triangles", C.c_int32),
+        ("p1_dof_count", C.c_int32),
+        ("symmetry", C.c_int32),
+        ("device_cu_count", C.c_int32),
+        ("setup_seconds", C.c_double),
+        ("device_name", C.c_char * 64),
+    ]
+
+
+class _SolveOptions(C.Structure):
+    _fields_ = [
+        ("solve_mode", C.c_int32),
+        ("refine_max_iterations", C.c_int32),
+        ("refine_tolerance", C.c_double),
+        ("batch_frequencies", C.c_int32),
+        ("estimate_rcond", C.c_int32),
+        ("n_tags", C.c_int32),
+        ("tags", C.POINTER(C.c_int32)),
+        ("impedance_tag", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class _CaseDiag(C.Structure):
+    _fields_ = [
+        ("assembly_s", C.c_double),
+        ("dense_solve_s", C.c_double),
+        ("field_s", C.c_double),
+        ("lapack_info", C.c_int32),
+        ("refine_iterations", C.c_int32),
+        ("dense_solve_rcond", C.c_double),
+        ("dense_solve_condition_1norm", C.c_double),
+        ("refine_residual_rel", C.c_double),
+        ("regular_assembly_s", C.c_double),
+        ("singular_assembly_s", C.c_double),
+        ("batch_index", C.c_int32),
+        ("batch_size", C.c_int32),
+    ]
+
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTED_SYMBOLS = (
+    "hbem_abi_version",
+    "hbem_last_error",
+    "hbem_device_count",
+    "hbem_session_create",
+    "hbem_session_destroy",
+    "hbem_session_get_info",
+    "hbem_assemble",
+    "hbem_solve_batch",
+    "hbem_solve_batch_device",
+    "hbem_evaluate_field",
+    "hbem_dense_solve",
+)
+
+
+def diag_to_dict(d: "_CaseDiag") -> dict[str, Any]:
+    return {
+        "assembly_seconds": float(d.assembly_s),
+        "regular_assembly_seconds": float(d.regular_assembly_s),
+        "singular_assembly_seconds": float(d.singular_assembly_s),
+        "dense_solve_seconds": float(d.dense_solve_s),
+        "field_seconds": float(d.field_s),
+        "lapack_info": int(d.lapack_info),
+        "dense_solve_rcond": float(d.dense_solve_rcond),
+        "dense_solve_condition_1norm": float(d.dense_solve_condition_1norm),
+        "dense_solve_refine_iterations": int(d.refine_iterations),
+        "dense_solve_refine_residual_rel": float(d.refine_residual_rel),
+        "batch_index": int(d.batch_index),
+        "batch_size": int(d.batch_size),
+    }
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+
+def library_path() -> Path:
+    override = os.environ.get("HBEM_LIBRARY")
+    if override:
+        return Path(override)
+    return Path(__file__).resolve().parent / "_lib" / _LIB_NAME
+
+
+def _f32p(a: NDArray | None):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i32p(a: NDArray | None):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f64p(a: NDArray | None):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def load_library() -> C.CDLL:
+    """Load libhbem.so (once).  Raises AssemblyBackendUnavailable if absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        path = library_path()
+        if not path.exists():
+            raise AssemblyBackendUnavailable(
+                f"libhbem is not built ({path} missing); run __graft_entry__.build() or make"
+            )
+        lib = C.CDLL(str(path))
+        lib.hbem_abi_version.restype = C.c_int32
+        lib.hbem_last_error.restype = C.c_char_p
+        lib.hbem_device_count.restype = C.c_int32
+        lib.hbem_session_create.argtypes = [C.POINTER(_Geometry), C.POINTER(C.c_void_p)]
+        lib.hbem_session_create.restype = C.c_int32
+        lib.hbem_session_destroy.argtypes = [C.c_void_p]
+        lib.hbem_session_destroy.restype = C.c_int32
+        lib.hbem_session_get_info.argtypes = [C.c_void_p, C.POINTER(_SessionInfo)]
+        lib.hbem_session_get_info.restype = C.c_int32
+        fp = C.POINTER(C.c_float)
+        lib.hbem_assemble.argtypes = [C.c_void_p, C.c_float, C.c_float, fp, fp, fp, fp, fp, fp]
+        lib.hbem_assemble.restype = C.c_int32
+        lib.hbem_solve_batch.argtypes = [
+            C.c_void_p, C.c_int32, fp, fp, fp, fp, fp, C.c_int32, fp,
+            C.POINTER(_SolveOptions), fp, fp, fp, fp,
+            C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_CaseDiag),
+        ]
+        lib.hbem_solve_batch.restype = C.c_int32
+        vp = C.c_void_p
+        lib.hbem_solve_batch_device.argtypes = [
+            C.c_void_p, C.c_int32, vp, vp, vp, vp, C.c_int32, vp, C.POINTER(_SolveOptions),
+            vp, vp, vp, C.POINTER(_CaseDiag),
+        ]
+        lib.hbem_solve_batch_device.restype = C.c_int32
+        lib.hbem_evaluate_field.argtypes = [
+            C.c_void_p, C.c_int32, fp, fp, fp, fp, fp, C.c_int32, fp, fp, fp,
+        ]
+        lib.hbem_evaluate_field.restype = C.c_int32
+        lib.hbem_dense_solve.argtypes = [
+            C.c_int32, C.c_int32, fp, fp, fp, fp, C.POINTER(_SolveOptions), fp, fp,
+            C.POINTER(_CaseDiag),
+        ]
+        lib.hbem_dense_solve.restype = C.c_int32
+        if lib.hbem_abi_version() != HBEM_ABI_VERSION:
+            raise AssemblyBackendUnavailable(
+                f"libhbem ABI {lib.hbem_abi_version()} != expected {HBEM_ABI_VERSION}; rebuild"
+            )
+        _lib = lib
+        return lib
+
+
+def _check(rc: int, op: str) -> None:
+    if rc == 0:
+        return
+    lib = load_library()
+    msg = (lib.hbem_last_error() or b"").decode("utf-8", "replace")
+    if rc == -2:
+        raise AssemblyBackendUnavailable(f"HIP BEM backend failed during {op}: {msg}")
+    raise RuntimeError(f"HIP BEM backend failed during {op}: {msg}")
+
+
+@dataclass(frozen=True)
+class NativeRuntimeStatus:
+    library_path: Path | None
+    library_loaded: bool
+    device_count: int
+    unavailable_reasons: tuple[str, ...] = ()
+
+    @property
+    def available(self) -> bool:
+        return self.library_loaded and self.device_count > 0
+
+
+def discover_native_runtime(run_smoke_test: bool = False) -> NativeRuntimeStatus:
+    """Library + device probe (stands in for native.py:95-179)."""
+    path = library_path()
+    try:
+        lib = load_library()
+    except (AssemblyBackendUnavailable, OSError) as exc:
+        return NativeRuntimeStatus(path, False, 0, (str(exc),))
+    count = int(lib.hbem_device_count())
+    reasons = () if count > 0 else ("no gfx950 (MI355X) device is visible",)
+    return NativeRuntimeStatus(path, True, count, reasons)
+
+
+def _complex_out(re: NDArray[np.float32], im: NDArray[np.float32]) -> NDArray[np.complex64]:
+    return (re + 1j * im).astype(np.complex64)
+
+
+@dataclass
+class DenseSolveFieldResult:
+    """Per-frequency solve + field result (the fields of the reference's
+    DenseSolveFieldResult, hornlab_metal_bem/metal/session.py:353-376, as
+    in-memory arrays)."""
+
+    frequency_hz: float
+    pressure: NDArray[np.complex64] | None
+    field: NDArray[np.complex64]
+    assembly_s: float
+    dense_solve_s: float
+    field_s: float
+    lapack_info: int
+    impedance: complex | None
+    surface_pressure_avg: dict[int, complex] | None
+    diagnostics: dict[str, Any] = field(default_factory=dict)
+    field_row_index: int | None = None
+
+
+class HipBemSession:
+    """Resident GPU session: geometry, singular-pair plan and workspaces stay
+    on the device across calls."""
+
+    def __init__(self, handle: C.c_void_p, buffers: GeometryBuffers, symmetry_plane: str | None):
+        self._handle = handle
+        self.geometry_buffers = buffers
+        self.symmetry_plane = symmetry_plane
+        self._closed = False
+
+    @classmethod
+    def create_session(
+        cls,
+        geometry_buffers: GeometryBuffers,
+        *,
+        symmetry_plane: str | None = None,
+        check_open_edges: bool = True,
+        aperture_tag: int | None = None,
+        velocity_source_tags: Any | None = None,
+        **_ignored: Any,
+    ) -> "HipBemSession":
+        if aperture_tag is not None:
+            raise AssemblyBackendUnavailable(
+                "aperture_tag (coupled infinite baffle) is not supported by the HIP path"
+            )
+        plane = validate_native_symmetry_plane(
+            geometry_buffers, symmetry_plane, check_open_edges=check_open_edges
+        )
+        lib = load_library()
+        b = geometry_buffers
+        keep = (
+            np.ascontiguousarray(b.vertices_3xn_f32, dtype=np.float32),
+            np.ascontiguousarray(b.triangles_3xm_i32, dtype=np.int32),
+            np.ascontiguousarray(b.p1_local2global_i32, dtype=np.int32),
+            np.ascontiguousarray(b.triangle_areas_f32, dtype=np.float32),
+            np.ascontiguousarray(b.triangle_normals_3xm_f32, dtype=np.float32),
+            np.ascontiguousarray(b.physical_tags_i32, dtype=np.int32),
+        )
+        g = _Geometry(
+            b.n_vertices, b.n_triangles, b.p1_dof_count, SYMMETRY_CODES[plane],
+            _f32p(keep[0]), _i32p(keep[1]), _i32p(keep[2]), _f32p(keep[3]), _f32p(keep[4]),
+            _i32p(keep[5]),
+        )
+        handle = C.c_void_p()
+        _check(lib.hbem_session_create(C.byref(g), C.byref(handle)), "create_session")
+        return cls(handle, b, plane)
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self) -> None:
+        if not self._closed and self._handle:
+            load_library().hbem_session_destroy(self._handle)
+        self._closed = True
+
+    def __enter__(self) -> "HipBemSession":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ensure_open(self) -> None:
+        if self._closed:
+            raise RuntimeError("HIP BEM session is closed")
+
+    def info(self) -> dict[str, Any]:
+        self._ensure_open()
+        out = _SessionInfo()
+        _check(load_library().hbem_session_get_info(self._handle, C.byref(out)), "get_info")
+        return {
+            "planned_pairs": {
+                "coincident": int(out.duffy_coincident),
+                "edge": int(out.duffy_edge),
+                "vertex": int(out.duffy_vertex),
+                "total": int(out.duffy_coincident + out.duffy_edge + out.duffy_vertex),
+            },
+            "image_adjacent_pairs": int(out.duffy_image_pairs),
+            "unique_triplets": int(out.duffy_unique_slots),
+            "raw_triplets_if_expanded": 9
+            * int(out.duffy_coincident + out.duffy_edge + out.duffy_vertex),
+            "n_triangles": int(out.n_triangles),
+            "p1_dof_count": int(out.p1_dof_count),
+            "symmetry_code": int(out.symmetry),
+            "device_cu_count": int(out.device_cu_count),
+            "device_name": out.device_name.decode("utf-8", "replace"),
+            "resident_context_seconds": float(out.setup_seconds),
+        }
+
+    # -- operations ---------------------------------------------------------
+    def assemble_standard_neumann(
+        self,
+        frequency_hz: float,
+        k_real: float,
+        neumann_dp0: NDArray[Any],
+        *,
+        k_imag: float = 0.0,
+        operation_id: str | None = None,
+    ) -> tuple[NDArray[np.complex64], NDArray[np.complex64]]:
+        """Corrected system at one wavenumber: (A (n, n) row-major, rhs (n,))."""
+        self._ensure_open()
+        q = np.asarray(neumann_dp0)
+        T, n = self.geometry_buffers.n_triangles, self.geometry_buffers.p1_dof_count
+        if q.shape != (T,):
+            raise ValueError(f"neumann_dp0 must have shape {(T,)}, got {q.shape}")
+        if not np.all(np.isfinite(q)):
+            raise ValueError("neumann_dp0 must contain only finite values")
+        qre = np.ascontiguousarray(q.real, dtype=np.float32)
+        qim = np.ascontiguousarray(q.imag, dtype=np.float32)
+        are = np.empty((n, n), np.float32)
+        aim = np.empty((n, n), np.float32)
+        rre = np.empty(n, np.float32)
+        rim = np.empty(n, np.float32)
+        _check(
+            load_library().hbem_assemble(
+                self._handle, float(np.float32(k_real)), float(np.float32(k_imag)), _f32p(qre),
+                _f32p(qim), _f32p(are), _f32p(aim), _f32p(rre), _f32p(rim),
+            ),
+            "assemble_standard_neumann",
+        )
+        return _complex_out(are, aim), _complex_out(rre, rim)
+
+    def assemble_solve_evaluate_standard_neumann_batch(
+        self,
+        frequency_hz: NDArray[Any],
+        k_real: NDArray[Any],
+        neumann_dp0: NDArray[Any],
+        observation_points: NDArray[Any],
+        *,
+        k_imag_f32: NDArray[Any] | None = None,
+        field_k_real: NDArray[Any] | None = None,
+        source_tags: list[int] | tuple[int, ...] | None = None,
+        impedance_source_tag: int | None = None,
+        write_surface_pressure: bool = True,
+        solve_mode: str = "fp32",
+        refine_max_iterations: int = 8,
+        refine_tolerance: float = 1e-7,
+        batch_frequencies: int | None = None,
+        estimate_rcond: bool = True,
+        on_case_result: Callable[[int, DenseSolveFieldResult], bool | None] | None = None,
+        dense_solve_dtype: str | None = None,
+        **_ignored: Any,
+    ) -> list[DenseSolveFieldResult]:
+        """Assemble, solve and evaluate every frequency; returns one result per
+        frequency in order.  ``on_case_result(i, result)`` is called in order;
+        returning exactly False stops after that case."""
+        self._ensure_open()
+        freqs = np.asarray(frequency_hz, dtype=np.float64).reshape(-1)
+        k = np.ascontiguousarray(np.asarray(k_real, dtype=np.float32).reshape(-1))
+        F = int(freqs.size)
+        if F == 0:
+            raise ValueError("frequency_hz must be a non-empty 1D array")
+        if k.shape != freqs.shape:
+            raise ValueError("k_real must have the same shape as frequency_hz")
+        kim = (
+            np.zeros(F, np.float32)
+            if k_imag_f32 is None
+            else np.ascontiguousarray(np.asarray(k_imag_f32, dtype=np.float32).reshape(-1))
+        )
+        if kim.shape != freqs.shape:
+            raise ValueError("k_imag_f32 must have the same shape as frequency_hz")
+        if np.any(kim < 0):
+            raise ValueError("k_imag_f32 must be non-negative")
+        fk = k if field_k_real is None else np.ascontiguousarray(np.asarray(field_k_real, np.float32))
+        T, n = self.geometry_buffers.n_triangles, self.geometry_buffers.p1_dof_count
+        q = np.asarray(neumann_dp0)
+        if q.shape != (F, T):
+            raise ValueError(f"neumann_dp0 must have shape {(F, T)}, got {q.shape}")
+        if not np.iscomplexobj(q):
+            raise ValueError("neumann_dp0 must be complex")
+        if not np.all(np.isfinite(q)):
+            raise ValueError("neumann_dp0 must contain only finite values")
+        obs = np.asarray(observation_points, dtype=np.float32)
+        if obs.ndim != 2 or 3 not in obs.shape:
+            raise ValueError("observation_points must have shape (3, n_obs) or (n_obs, 3)")
+        if obs.shape[0] != 3:
+            obs = obs.T
+        obs = np.ascontiguousarray(obs, dtype=np.float32)
+        n_obs = int(obs.shape[1])
+        if dense_solve_dtype == "float64" and solve_mode == "fp32":
+            solve_mode = "fp32_ir"
+        if solve_mode not in SOLVE_MODES:
+            raise ValueError(f"solve_mode must be one of {sorted(SOLVE_MODES)}")
+        tags = np.ascontiguousarray(list(source_tags or []), dtype=np.int32)
+        imp_tag = int(impedance_source_tag) if impedance_source_tag is not None else (
+            int(min(tags)) if tags.size else 2
+        )
+        opts = _SolveOptions()
+        opts.solve_mode = SOLVE_MODES[solve_mode]
+        opts.refine_max_iterations = int(refine_max_iterations)
+        opts.refine_tolerance = float(refine_tolerance)
+        opts.batch_frequencies = int(batch_frequencies or 0)
+        opts.estimate_rcond = 1 if estimate_rcond else 0
+        opts.n_tags = int(tags.size)
+        opts.tags = _i32p(tags) if tags.size else None
+        opts.impedance_tag = imp_tag
+        qre = np.ascontiguousarray(q.real, dtype=np.float32)
+        qim = np.ascontiguousarray(q.imag, dtype=np.float32)
+        fre = np.zeros((F, n_obs), np.float32)
+        fim = np.zeros((F, n_obs), np.float32)
+        pre = np.zeros((F, n), np.float32)
+        pim = np.zeros((F, n), np.float32)
+        imp = np.zeros((F, 2), np.float64)
+        pav = np.zeros((F, max(1, tags.size), 2), np.float64)
+        diag = (_CaseDiag * F)()
+        _check(
+            load_library().hbem_solve_batch(
+                self._handle, F, _f32p(k), _f32p(kim) if np.any(kim != 0) else None, _f32p(fk),
+                _f32p(qre), _f32p(qim), n_obs, _f32p(obs), C.byref(opts), _f32p(fre), _f32p(fim),
+                _f32p(pre), _f32p(pim), _f64p(imp), _f64p(pav) if tags.size else None, diag,
+            ),
+            "assemble_solve_evaluate_standard_neumann_batch",
+        )
+        results: list[DenseSolveFieldResult] = []
+        for i in range(F):
+            d = diag[i]
+            diagnostics: dict[str, Any] = {
+                "implementation": "hip_gfx950_resident_assembly_lu_field_batch",
+                "assembly_mode": "corrected",
+                "lapack_info": int(d.lapack_info),
+                "dense_solve_condition_1norm": float(d.dense_solve_condition_1norm),
+                "dense_solve_dtype": "float32" if solve_mode == "fp32" else solve_mode,
+                "dense_solve_refine_iterations": int(d.refine_iterations),
+                "field_k_real_f32": float(fk[i]),
+                "assembly_k_imag_f32": float(kim[i]),
+                "complex_k": bool(kim[i] != 0),
+                "symmetry_plane": self.symmetry_plane,
+                "pressure_shape": [n],
+                "field_shape": [n_obs],
+            }
+            diagnostics["regular_assembly_seconds"] = float(d.regular_assembly_s)
+            diagnostics["singular_assembly_seconds"] = float(d.singular_assembly_s)
+            diagnostics["resident_batch"] = {"index": int(d.batch_index), "size": int(d.batch_size)}
+            if d.dense_solve_rcond >= 0:
+                diagnostics["dense_solve_rcond"] = float(d.dense_solve_rcond)
+            if d.refine_residual_rel >= 0:
+                diagnostics["dense_solve_refine_residual_rel"] = float(d.refine_residual_rel)
+            res = DenseSolveFieldResult(
+                frequency_hz=float(freqs[i]),
+                pressure=_complex_out(pre[i], pim[i]) if write_surface_pressure else None,
+                field=_complex_out(fre[i], fim[i]),
+                assembly_s=float(d.assembly_s),
+                dense_solve_s=float(d.dense_solve_s),
+                field_s=float(d.field_s),
+                lapack_info=int(d.lapack_info),
+                impedance=complex(imp[i, 0], imp[i, 1]),
+                surface_pressure_avg=(
+                    {int(t): complex(pav[i, j, 0], pav[i, j, 1]) for j, t in enumerate(tags)}
+                    if tags.size
+                    else None
+                ),
+                diagnostics=diagnostics,
+                field_row_index=i,
+            )
+            results.append(res)
+            if on_case_result is not None and on_case_result(i, res) is False:
+                break
+        return results
+
+    def solve_batch_device(
+        self,
+        n_freq: int,
+        k_real_ptr: int,
+        field_k_ptr: int,
+        q_ptr: int,
+        n_obs: int,
+        obs_ptr: int,
+        field_ptr: int,
+        tag_avg_ptr: int,
+        *,
+        k_imag_ptr: int | None = None,
+        p_ptr: int | None = None,
+        source_tags: list[int] | tuple[int, ...] = (),
+        impedance_source_tag: int = 2,
+        solve_mode: str = "fp32",
+        refine_max_iterations: int = 8,
+        refine_tolerance: float = 1e-7,
+        batch_frequencies: int | None = None,
+        estimate_rcond: bool = False,
+    ) -> list[dict[str, Any]]:
+        """Device-resident batch (hbem_solve_batch_device): every pointer is
+        a device address on the session's GPU (e.g. ``tensor.data_ptr()``).
+        Returns the per-case diagnostics."""
+        self._ensure_open()
+        tags = np.ascontiguousarray(list(source_tags), dtype=np.int32)
+        opts = _SolveOptions()
+        opts.solve_mode = SOLVE_MODES[solve_mode]
+        opts.refine_max_iterations = int(refine_max_iterations)
+        opts.refine_tolerance = float(refine_tolerance)
+        opts.batch_frequencies = int(batch_frequencies or 0)
+        opts.estimate_rcond = 1 if estimate_rcond else 0
+        opts.n_tags = int(tags.size)
+        opts.tags = _i32p(tags) if tags.size else None
+        opts.impedance_tag = int(impedance_source_tag)
+        diag = (_CaseDiag * int(n_freq))()
+        _check(
+            load_library().hbem_solve_batch_device(
+                self._handle, int(n_freq), k_real_ptr, k_imag_ptr, field_k_ptr, q_ptr, int(n_obs),
+                obs_ptr, C.byref(opts), field_ptr, p_ptr, tag_avg_ptr, diag,
+            ),
+            "solve_batch_device",
+        )
+        return [diag_to_dict(d) for d in diag]
+
+    def evaluate_standard_exterior_batch(
+        self,
+        frequency_hz: NDArray[Any],
+        k_real: NDArray[Any],
+        pressure_p1: NDArray[Any],
+        neumann_dp0: NDArray[Any],
+        observation_points: NDArray[Any],
+        **_ignored: Any,
+    ) -> NDArray[np.complex64]:
+        """Field (F, n_obs) from given surface pressure and Neumann data."""
+        self._ensure_open()
+        k = np.ascontiguousarray(np.asarray(k_real, dtype=np.float32).reshape(-1))
+        F = int(k.size)
+        T, n = self.geometry_buffers.n_triangles, self.geometry_buffers.p1_dof_count
+        p = np.asarray(pressure_p1).reshape(F, n)
+        q = np.asarray(neumann_dp0).reshape(F, T)
+        obs = np.asarray(observation_points, dtype=np.float32)
+        if obs.shape[0] != 3:
+            obs = obs.T
+        obs = np.ascontiguousarray(obs)
+        n_obs = int(obs.shape[1])
+        fre = np.zeros((F, n_obs), np.float32)
+        fim = np.zeros((F, n_obs), np.float32)
+        _check(
+            load_library().hbem_evaluate_field(
+                self._handle, F, _f32p(k),
+                _f32p(np.ascontiguousarray(p.real, np.float32)),
+                _f32p(np.ascontiguousarray(p.imag, np.float32)),
+                _f32p(np.ascontiguousarray(q.real, np.float32)),
+                _f32p(np.ascontiguousarray(q.imag, np.float32)),
+                n_obs, _f32p(obs), _f32p(fre), _f32p(fim),
+            ),
+            "evaluate_standard_exterior_batch",
+        )
+        return _complex_out(fre, fim)
+
+
+# The reference's class name for the same role.
+MetalNativeStandardSession = HipBemSession
+
+
+def dense_solve(
+    A: NDArray[Any],
+    b: NDArray[Any],
+    *,
+    solve_mode: str = "fp32",
+    refine_max_iterations: int = 8,
+    refine_tolerance: float = 1e-7,
+    estimate_rcond: bool = True,
+) -> tuple[NDArray[np.complex64], list[dict[str, Any]]]:
+    """GPU LU solve of stacked systems A (S, n, n) x = b (S, n)."""
+    A = np.asarray(A)
+    b = np.asarray(b)
+    if A.ndim == 2:
+        A, b = A[None], b[None]
+    S, n, n2 = A.shape
+    if n != n2 or b.shape != (S, n):
+        raise ValueError("A must be (S, n, n) and b (S, n)")
+    opts = _SolveOptions()
+    opts.solve_mode = SOLVE_MODES[solve_mode]
+    opts.refine_max_iterations = int(refine_max_iterations)
+    opts.refine_tolerance = float(refine_tolerance)
+    opts.estimate_rcond = 1 if estimate_rcond else 0
+    xr = np.zeros((S, n), np.float32)
+    xi = np.zeros((S, n), np.float32)
+    diag = (_CaseDiag * S)()
+    _check(
+        load_library().hbem_dense_solve(
+            S, n,
+            _f32p(np.ascontiguousarray(A.real, np.float32)),
+            _f32p(np.ascontiguousarray(A.imag, np.float32)),
+            _f32p(np.ascontiguousarray(b.real, np.float32)),
+            _f32p(np.ascontiguousarray(b.imag, np.float32)),
+            C.byref(opts), _f32p(xr), _f32p(xi), diag,
+        ),
+        "dense_solve",
+    )
+    info = [
+        {
+            "lapack_info": int(d.lapack_info),
+            "dense_solve_rcond": float(d.dense_solve_rcond),
+            "dense_solve_condition_1norm": float(d.dense_solve_condition_1norm),
+            "dense_solve_refine_iterations": int(d.refine_iterations),
+            "dense_solve_refine_residual_rel": float(d.refine_residual_rel),
+            "dense_solve_s": float(d.dense_solve_s),
+        }
+        for d in diag
+    ]
+    return _complex_out(xr, xi), info

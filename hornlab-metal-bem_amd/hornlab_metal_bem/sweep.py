@@ -1,0 +1,301 @@
+"""Frequency sweep on the HIP path (reference: hornlab_metal_bem/sweep.py).
+
+Builds the frequency grid, wavenumbers, observation points and DP0 Neumann
+rows, runs the resident GPU batch (assemble -> LU -> field -> reductions) and
+assembles the ``SolveResult`` with the reference's fields, directivity and
+diagnostic policies.
+"""
+from __future__ import annotations
+
+import logging
+import math
+import time
+from typing import Any
+
+import numpy as np
+from numpy.typing import NDArray
+
+from ._constants import REFERENCE_PRESSURE, SPEED_OF_SOUND
+from .backends import AssemblyBackendUnavailable
+from .bie import _build_driver_neumann_coeffs, _build_source_face_scale
+from .config import NATIVE_SYMMETRY_PLANES, BIEFormulation, SolveConfig
+from .geometry import build_geometry_buffers
+from .mesh import LoadedMesh, make_pure_function_spaces
+from .observation import ObservationFrame, build_observation_points
+from .result import SolveResult
+
+logger = logging.getLogger(__name__)
+
+
+def _build_frequency_grid(config: SolveConfig) -> NDArray[np.float64]:
+    if config.freq_spacing == "log":
+        return np.geomspace(config.freq_min_hz, config.freq_max_hz, config.freq_count)
+    return np.linspace(config.freq_min_hz, config.freq_max_hz, config.freq_count)
+
+
+def should_route_native_metal(config: SolveConfig) -> bool:
+    """Reject configurations the HIP path does not implement."""
+    if config.native_symmetry_plane is not None and config.native_symmetry_plane not in NATIVE_SYMMETRY_PLANES:
+        raise AssemblyBackendUnavailable(
+            "native_symmetry_plane must be None or one of "
+            + ", ".join(repr(p) for p in NATIVE_SYMMETRY_PLANES)
+        )
+    if config.chief_points is not None:
+        raise AssemblyBackendUnavailable("chief_points (CHIEF least squares) is not built on the HIP path")
+    if config.aperture_tag is not None:
+        raise AssemblyBackendUnavailable("aperture_tag (coupled infinite baffle) is not built on the HIP path")
+    if config.impedance_sources or config.impedance_source_callback is not None:
+        raise AssemblyBackendUnavailable("impedance_sources (Robin boundary) is not built on the HIP path")
+    return True
+
+
+should_route_native = should_route_native_metal
+
+
+def _k_values_for_native(frequencies, config: SolveConfig):
+    """k = 2 pi f / c in float32; complex_k adds k_im = shift * k
+    (reference: sweep.py:150-161)."""
+    k_real = (2.0 * np.pi * np.asarray(frequencies, np.float64) / SPEED_OF_SOUND).astype(np.float32)
+    if config.formulation == BIEFormulation.COMPLEX_K:
+        k_imag = (k_real.astype(np.float64) * config.complex_k_shift).astype(np.float32)
+    else:
+        k_imag = np.zeros_like(k_real)
+    return k_real, k_imag
+
+
+def _build_neumann_rows(dp0_space, physical_tags, frequencies, config, source_face_scale=None):
+    rows = [
+        _build_driver_neumann_coeffs(
+            dp0_space, physical_tags, 2.0 * np.pi * float(f), config, np.complex64,
+            impedance_tags=set(), source_face_scale=source_face_scale,
+        )
+        for f in frequencies
+    ]
+    return np.stack(rows, axis=0)
+
+
+def _field_points_3xn(obs_points: NDArray[np.float64]) -> NDArray[np.float32]:
+    P, N, _ = obs_points.shape
+    return np.ascontiguousarray(obs_points.reshape(P * N, 3).T, dtype=np.float32)
+
+
+def _mesh_max_edge_m(mesh: LoadedMesh) -> float:
+    V = np.asarray(mesh.grid.vertices, np.float64).T
+    E = np.asarray(mesh.grid.elements, np.int64).T
+    if E.size == 0:
+        return 0.0
+    p0, p1, p2 = V[E[:, 0]], V[E[:, 1]], V[E[:, 2]]
+    return float(max(np.linalg.norm(p1 - p0, axis=1).max(), np.linalg.norm(p2 - p1, axis=1).max(),
+                     np.linalg.norm(p0 - p2, axis=1).max()))
+
+
+def _directivity_from_pressure(pressure: NDArray[np.complex128], on_axis_idx: int) -> NDArray[np.float64]:
+    """20 log10(|p| / p_ref) with a -120 dB floor, minus the on-axis column
+    (reference: sweep.py:523-532)."""
+    floor = REFERENCE_PRESSURE * 10.0 ** (-120.0 / 20.0)
+    spl = 20.0 * np.log10(np.maximum(np.abs(pressure), floor) / REFERENCE_PRESSURE)
+    return spl - spl[:, on_axis_idx][:, None]
+
+
+def _apply_dense_solve_policy(diagnostics: dict, *, threshold: float) -> None:
+    """rcond below the threshold marks the frequency suspect (reference:
+    sweep.py:383-412)."""
+    diagnostics["dense_solve_rcond_warning_threshold"] = float(threshold)
+    diagnostics["dense_solve_suspect"] = False
+    if threshold <= 0.0:
+        return
+    raw = diagnostics.get("dense_solve_rcond")
+    if raw is None:
+        diagnostics["dense_solve_policy_available"] = False
+        return
+    diagnostics["dense_solve_policy_available"] = True
+    rcond = float(raw)
+    if math.isfinite(rcond) and rcond < threshold:
+        diagnostics["dense_solve_suspect"] = True
+        diagnostics["dense_solve_recommendation"] = (
+            "Treat this frequency as suspect; nudge or densify the frequency grid around the "
+            "resonance and compare against nearby points. This flag detects dense-solve "
+            "conditioning only."
+        )
+
+
+def _apply_mesh_resolution_policy(diagnostics: dict, *, frequency_hz: float, mesh_max_edge_m: float,
+                                  elements_per_wavelength_min: float) -> None:
+    """Elements-per-wavelength check (reference: sweep.py:415-440)."""
+    diagnostics["mesh_max_edge_m"] = float(mesh_max_edge_m)
+    diagnostics["mesh_elements_per_wavelength_min"] = float(elements_per_wavelength_min)
+    if mesh_max_edge_m <= 0.0:
+        diagnostics["mesh_elements_per_wavelength"] = math.inf
+        diagnostics["mesh_max_valid_frequency_hz"] = math.inf
+        diagnostics["mesh_resolution_suspect"] = False
+        return
+    epw = (SPEED_OF_SOUND / float(frequency_hz)) / float(mesh_max_edge_m)
+    diagnostics["mesh_elements_per_wavelength"] = float(epw)
+    diagnostics["mesh_max_valid_frequency_hz"] = float(
+        SPEED_OF_SOUND / (float(elements_per_wavelength_min) * float(mesh_max_edge_m))
+    )
+    diagnostics["mesh_resolution_suspect"] = bool(epw < elements_per_wavelength_min)
+
+
+def _prepare(mesh: LoadedMesh, frame: ObservationFrame, config: SolveConfig):
+    mesh_tags = {int(t) for t in np.unique(mesh.physical_tags)}
+    missing = sorted(set(config.velocity_sources) - mesh_tags)
+    if missing:
+        raise ValueError(
+            f"velocity_sources tags {missing} are not present in the mesh; "
+            f"available physical tags: {sorted(mesh_tags)}"
+        )
+    obs_points, angles = build_observation_points(frame, config.observation)
+    p1, dp0 = make_pure_function_spaces(mesh.grid)
+    buffers = build_geometry_buffers(mesh.grid, mesh.physical_tags, p1, dp0)
+    face_scale = _build_source_face_scale(
+        mesh.grid, mesh.physical_tags, config, frame.axis, frame.source_center
+    )
+    return obs_points, angles, p1, dp0, buffers, face_scale
+
+
+def run_sweep_native_metal(
+    mesh: LoadedMesh, frequencies: NDArray[np.float64], frame: ObservationFrame, config: SolveConfig
+) -> SolveResult:
+    """Standard-Neumann sweep on the GPU (reference: sweep.py:612-926)."""
+    from .native import HipBemSession
+
+    should_route_native_metal(config)
+    freqs = np.asarray(frequencies, dtype=np.float64).reshape(-1)
+    if freqs.size == 0:
+        raise ValueError("frequencies must contain at least one value")
+    t_total = time.time()
+    obs_points, angles, p1, dp0, buffers, face_scale = _prepare(mesh, frame, config)
+    source_tags = list(config.velocity_sources.keys())
+    impedance_tag = min(config.velocity_sources.keys(), default=2)
+    on_axis = int(np.argmin(np.abs(angles)))
+    P, N, _ = obs_points.shape
+    pts = _field_points_3xn(obs_points)
+    n_sphere = 0
+    if config.observation.sphere_points is not None:
+        sph = np.ascontiguousarray(np.asarray(config.observation.sphere_points, np.float64).T, np.float32)
+        pts = np.concatenate([pts, sph], axis=1)
+        n_sphere = int(sph.shape[1])
+    max_edge = _mesh_max_edge_m(mesh)
+    k_real, k_imag = _k_values_for_native(freqs, config)
+    q_rows = _build_neumann_rows(dp0, mesh.physical_tags, freqs, config, face_scale)
+
+    pressure_rows, spl_rows, imp_rows, surf_rows, diag_rows, log = [], [], [], [], [], []
+    pavg: dict[int, list[complex]] = {t: [] for t in source_tags}
+    done: list[float] = []
+    stream = config.on_frequency_result is not None
+    chunk = (config.hip_batch_frequencies or 8) if stream else freqs.size
+    stop = False
+    with HipBemSession.create_session(
+        buffers, symmetry_plane=config.native_symmetry_plane,
+        check_open_edges=config.native_check_open_edges,
+    ) as session:
+        for c0 in range(0, freqs.size, chunk):
+            sl = slice(c0, min(freqs.size, c0 + chunk))
+            results = session.assemble_solve_evaluate_standard_neumann_batch(
+                freqs[sl], k_real[sl], q_rows[sl], pts,
+                k_imag_f32=k_imag[sl], field_k_real=k_real[sl], source_tags=source_tags,
+                impedance_source_tag=impedance_tag,
+                write_surface_pressure=config.return_surface_pressure,
+                solve_mode=config.effective_hip_solve_mode,
+                refine_max_iterations=config.hip_refine_max_iterations,
+                refine_tolerance=config.hip_refine_tolerance,
+                batch_frequencies=config.hip_batch_frequencies,
+                estimate_rcond=config.hip_estimate_rcond,
+            )
+            for j, r in enumerate(results):
+                i = c0 + j
+                f = float(freqs[i])
+                flat = np.asarray(r.field, np.complex128)
+                pressure = flat[: P * N].reshape(P, N)
+                sphere = flat[P * N : P * N + n_sphere] if n_sphere else None
+                directivity = _directivity_from_pressure(pressure, on_axis)
+                diag = dict(r.diagnostics)
+                _apply_dense_solve_policy(diag, threshold=config.dense_solve_rcond_warning_threshold)
+                _apply_mesh_resolution_policy(
+                    diag, frequency_hz=f, mesh_max_edge_m=max_edge,
+                    elements_per_wavelength_min=config.mesh_elements_per_wavelength_min,
+                )
+                entry = {
+                    "frequency_hz": f,
+                    "iterations": None,
+                    "timing_s": r.assembly_s + r.dense_solve_s + r.field_s,
+                    "backend": "hip_gfx950_resident_assembly_lu_field_batch",
+                    "assembly_s": r.assembly_s,
+                    "dense_solve_s": r.dense_solve_s,
+                    "field_s": r.field_s,
+                    "lapack_info": r.lapack_info,
+                    "impedance": r.impedance,
+                    "native_diagnostics": diag,
+                    "observation_sphere_pressure_complex": sphere,
+                }
+                for t in source_tags:
+                    pavg[t].append((r.surface_pressure_avg or {}).get(int(t), 0j))
+                done.append(f)
+                pressure_rows.append(pressure)
+                spl_rows.append(directivity)
+                imp_rows.append(r.impedance)
+                if config.return_surface_pressure:
+                    surf_rows.append(np.asarray(r.pressure, np.complex128))
+                diag_rows.append(diag)
+                log.append(entry)
+                if config.progress_callback is not None:
+                    config.progress_callback(i, freqs.size, f)
+                if config.on_frequency_result is not None:
+                    cb = {
+                        **entry,
+                        "observation_pressure_complex": pressure,
+                        "observation_directivity_db": directivity,
+                        "observation_angles_deg": angles,
+                        "observation_planes": config.observation.planes,
+                    }
+                    if config.on_frequency_result(i, f, cb) is False:
+                        logger.info("Early stop requested after %.1f Hz", f)
+                        stop = True
+                        break
+            if stop:
+                break
+    timings = {
+        "solve_s": sum(e["timing_s"] for e in log),
+        "assembly_s": sum(e["assembly_s"] for e in log),
+        "dense_solve_s": sum(e["dense_solve_s"] for e in log),
+        "directivity_s": sum(e["field_s"] for e in log),
+        "total_s": time.time() - t_total,
+    }
+    sp_avg = {t: np.array(v, dtype=np.complex128) for t, v in pavg.items()}
+    return SolveResult(
+        frequencies_hz=np.array(done, dtype=np.float64),
+        pressure_complex=np.stack(pressure_rows, axis=0),
+        directivity_db=np.stack(spl_rows, axis=0),
+        impedance=np.array(imp_rows, dtype=np.complex128),
+        observation_angles_deg=angles,
+        observation_points=obs_points,
+        observation_planes=config.observation.planes,
+        config=config,
+        mesh_info=mesh.info,
+        timings=timings,
+        solver_log=log,
+        surface_pressure_avg=sp_avg if sp_avg else None,
+        surface_pressure_complex=np.stack(surf_rows, axis=0) if config.return_surface_pressure else None,
+        native_diagnostics=diag_rows,
+    )
+
+
+run_sweep = run_sweep_native_metal
+
+
+def run_sweep_native_metal_multi_source(
+    mesh: LoadedMesh, frequencies, frame: ObservationFrame, config: SolveConfig, sources: list[dict]
+) -> list[SolveResult]:
+    """One SolveResult per velocity-source dict (reference: sweep.py:957-1301).
+    Each source runs through the same resident path; ``surface_pressure_avg``
+    covers the union of every source's tags."""
+    from dataclasses import replace
+
+    all_tags = sorted({int(t) for src in sources for t in src})
+    out = []
+    for src in sources:
+        vs = {t: src.get(t, 0.0) for t in all_tags}
+        cfg = replace(config, velocity_sources=vs)
+        out.append(run_sweep_native_metal(mesh, frequencies, frame, cfg))
+    return out

@@ -1,0 +1,203 @@
+/*
+ * hbem.h -- C ABI of libhbem, the MI355X (gfx950) per-frequency BEM hot path
+ * behind hornlab_metal_bem's solve()/native_config() API.
+ *
+ * The reference crosses this boundary as a process + file IPC to a Swift/
+ * Metal helper (hornlab_metal_bem/metal/native.py:233-1879, contract in
+ * docs/native-ipc.md).  libhbem replaces that helper with an in-process
+ * library: every entry point below names the reference call it stands in
+ * for.  Conventions:
+ *   - plain pointers and sizes only; arrays are C-contiguous little-endian;
+ *   - the caller owns every host buffer it passes; the library never frees
+ *     caller memory and never keeps a caller pointer past the call;
+ *   - a session owns its device memory (geometry, singular-pair plan,
+ *     quadrature tables, per-batch workspaces) until hbem_session_destroy;
+ *   - every function returns 0 on success or a negative HBEM_E* code; the
+ *     message of the last failure on the calling thread is returned by
+ *     hbem_last_error();
+ *   - calls on distinct sessions may run concurrently; calls on one session
+ *     are serialised by the session.
+ */
+#ifndef HBEM_H
+#define HBEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HBEM_ABI_VERSION 1
+
+#define HBEM_OK 0
+#define HBEM_EINVAL -1      /* bad argument / contract violation */
+#define HBEM_ENODEV -2      /* no usable gfx950 device */
+#define HBEM_EHIP -3        /* HIP runtime error */
+#define HBEM_ENOMEM -4      /* device allocation failed */
+#define HBEM_ESINGULAR -5   /* LU hit an exactly zero pivot (lapack_info > 0) */
+#define HBEM_EUNSUPPORTED -6
+
+/* Symmetry codes: MS:697-717 (Geometry.symmetryPlaneCode). */
+#define HBEM_SYM_NONE 0
+#define HBEM_SYM_YZ 1
+#define HBEM_SYM_XZ 2
+#define HBEM_SYM_YZ_XZ 3
+#define HBEM_SYM_XY 4
+
+/* Geometry buffers in the layout of the reference's MetalGeometryBuffers
+ * (hornlab_metal_bem/metal/geometry.py:36-65), already validated and
+ * zero-based. */
+typedef struct hbem_geometry {
+    int32_t n_vertices;
+    int32_t n_triangles;
+    int32_t p1_dof_count;
+    int32_t symmetry;               /* HBEM_SYM_* */
+    const float *vertices_3xn;      /* [3][n_vertices] */
+    const int32_t *triangles_3xm;   /* [3][n_triangles] */
+    const int32_t *p1_local2global; /* [n_triangles][3] */
+    const float *areas;             /* [n_triangles] */
+    const float *normals_3xm;       /* [3][n_triangles] */
+    const int32_t *physical_tags;   /* [n_triangles] */
+} hbem_geometry;
+
+/* Singular-pair plan of a session; the fields the reference reports under
+ * duffy_corrections.planned_pairs (MS:610-640, MS:2195-2254). */
+typedef struct hbem_session_info {
+    int64_t duffy_coincident;
+    int64_t duffy_edge;
+    int64_t duffy_vertex;
+    int64_t duffy_image_pairs;      /* pairs with a non-zero mask */
+    int64_t duffy_unique_slots;     /* distinct (row, col) entries touched */
+    int32_t n_triangles;
+    int32_t p1_dof_count;
+    int32_t symmetry;
+    int32_t device_cu_count;
+    double setup_seconds;           /* resident_context_seconds analogue */
+    char device_name[64];
+} hbem_session_info;
+
+/* Dense-solve precision: dense_solve_dtype of config.py:236-246 plus the
+ * mixed-precision modes of this library. */
+#define HBEM_SOLVE_F32 0        /* FP32 complex LU (reference cgesv) */
+#define HBEM_SOLVE_F32_IR 1     /* FP32 LU + FP64-residual refinement */
+#define HBEM_SOLVE_MIXED_F16 2  /* FP16 MFMA trailing update + refinement */
+#define HBEM_SOLVE_MIXED_BF16 3 /* BF16 MFMA trailing update + refinement */
+
+typedef struct hbem_solve_options {
+    int32_t solve_mode;            /* HBEM_SOLVE_* */
+    int32_t refine_max_iterations; /* refinement sweeps (0 = none) */
+    double refine_tolerance;       /* stop at ||b-Ax||/||b|| <= tol */
+    int32_t batch_frequencies;     /* systems resident at once (0 = auto) */
+    int32_t estimate_rcond;        /* 1: cgecon-style rcond estimate */
+    int32_t n_tags;                /* tags reduced into surface_pressure_avg */
+    const int32_t *tags;           /* [n_tags] */
+    int32_t impedance_tag;         /* tag of the impedance reduction */
+    int32_t reserved[8];
+} hbem_solve_options;
+
+/* Per-frequency diagnostics: the case fields the reference returns
+ * (native.py:1205-1320, keys native.py:2082-2137). */
+typedef struct hbem_case_diag {
+    double assembly_s;                   /* whole assembly stage, per case */
+    double dense_solve_s;
+    double field_s;                      /* field + reductions */
+    int32_t lapack_info;
+    int32_t refine_iterations;
+    double dense_solve_rcond;            /* < 0 when not estimated */
+    double dense_solve_condition_1norm;  /* ||A||_1 */
+    double refine_residual_rel;          /* ||b-Ax||_inf/||b||_inf, FP64 */
+    double regular_assembly_s;           /* regular-pair kernel alone (HIP events) */
+    double singular_assembly_s;          /* singular-pair kernel alone (HIP events) */
+    int32_t batch_index;                 /* resident chunk the case ran in */
+    int32_t batch_size;                  /* cases in that chunk (times are chunk / size) */
+} hbem_case_diag;
+
+/* ABI version of the loaded library. */
+int32_t hbem_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *hbem_last_error(void);
+
+/* Device probe: number of visible gfx950 devices (0 if none).  Stands in for
+ * discover_native_runtime (native.py:95-179). */
+int32_t hbem_device_count(void);
+
+typedef struct hbem_session hbem_session;
+
+/* Upload geometry and build the resident plan (incidence, singular pairs,
+ * quadrature tables).  Replaces MetalNativeStandardSession.create_session
+ * (native.py:252-377) and ResidentMetalContext.init (MS:5370-5507). */
+int32_t hbem_session_create(const hbem_geometry *geometry, hbem_session **out);
+
+/* Release every device buffer of the session.  Replaces close()
+ * (native.py:1709-1716). */
+int32_t hbem_session_destroy(hbem_session *session);
+
+int32_t hbem_session_get_info(hbem_session *session, hbem_session_info *info);
+
+/* Assemble the corrected standard-Neumann system at one wavenumber:
+ * A = (K - M/2) 2^planes (row-major n x n, split re/im float32) and
+ * rhs = 2^planes V q.  Replaces assemble_standard_neumann
+ * (native.py:388-480; helper op MS:8128-8310 in "corrected" mode).
+ * q_re/q_im: DP0 Neumann data [n_triangles]. */
+int32_t hbem_assemble(hbem_session *session, float k_real, float k_imag, const float *q_re,
+                      const float *q_im, float *A_re, float *A_im, float *rhs_re, float *rhs_im);
+
+/* Assemble, solve and evaluate the exterior field for n_freq frequencies.
+ * Replaces assemble_solve_evaluate_standard_neumann_batch
+ * (native.py:737-1203; helper MS:8869-9878).
+ *   k_real/k_imag [F]: assembly wavenumber (k_imag >= 0 for complex_k);
+ *   field_k [F]: real wavenumber of the field kernel;
+ *   q_re/q_im [F][n_triangles]: DP0 Neumann rows;
+ *   obs_3xn [3][n_obs]: observation points;
+ * outputs (host):
+ *   field_re/im [F][n_obs];
+ *   p_re/im [F][p1_dof_count] or NULL (write_surface_pressure);
+ *   impedance [F][2] or NULL (area-mean pressure on opts->impedance_tag);
+ *   pavg [F][n_tags][2] or NULL;
+ *   diag [F] or NULL. */
+int32_t hbem_solve_batch(hbem_session *session, int32_t n_freq, const float *k_real,
+                         const float *k_imag, const float *field_k, const float *q_re,
+                         const float *q_im, int32_t n_obs, const float *obs_3xn,
+                         const hbem_solve_options *opts, float *field_re, float *field_im,
+                         float *p_re, float *p_im, double *impedance, double *pavg,
+                         hbem_case_diag *diag);
+
+/* hbem_solve_batch with every array in device memory of the session's GPU
+ * (for callers that keep data resident, e.g. a PyTorch wrapper).  No
+ * counterpart in the reference, whose helper only reads files.
+ *   k_real, field_k [F], k_imag [F] or NULL (real k);
+ *   q [F][n_triangles] interleaved complex float;
+ *   obs_3xn [3][n_obs];
+ *   field [F][n_obs] interleaved complex float (overwritten);
+ *   p [F][p1_dof_count] interleaved complex float or NULL;
+ *   tag_avg [F][1 + n_tags][2] double: impedance tag, then opts->tags;
+ *   diag [F] host array or NULL.
+ * Returns after the results are complete on the device. */
+int32_t hbem_solve_batch_device(hbem_session *session, int32_t n_freq, const float *k_real,
+                                const float *k_imag, const float *field_k, const float *q,
+                                int32_t n_obs, const float *obs_3xn,
+                                const hbem_solve_options *opts, float *field, float *p,
+                                double *tag_avg, hbem_case_diag *diag);
+
+/* Exterior field from given surface data.  Replaces
+ * evaluate_standard_exterior_batch (native.py:1552-1707; kernel
+ * MS:4135-4203).  p [F][p1_dof_count], q [F][n_triangles], out [F][n_obs]. */
+int32_t hbem_evaluate_field(hbem_session *session, int32_t n_freq, const float *field_k,
+                            const float *p_re, const float *p_im, const float *q_re,
+                            const float *q_im, int32_t n_obs, const float *obs_3xn,
+                            float *out_re, float *out_im);
+
+/* Dense complex solve of caller-supplied systems on the GPU LU (row-major
+ * split float32 A [F][n][n], b [F][n] -> x [F][n]).  Replaces the
+ * Accelerate cgesv call site solveDenseAccelerateCgesvMulti (MS:7834-7941).
+ * diag may be NULL. */
+int32_t hbem_dense_solve(int32_t n_sys, int32_t n, const float *A_re, const float *A_im,
+                         const float *b_re, const float *b_im, const hbem_solve_options *opts,
+                         float *x_re, float *x_im, hbem_case_diag *diag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HBEM_H */

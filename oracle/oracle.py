@@ -1,0 +1,244 @@
+"""Python front of the CPU oracle (hbem_oracle.c).  TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker / CPU baseline, never by the product path.  The dense solve uses
+scipy's LAPACK (zgesv/cgesv, OpenBLAS), standing in for the reference's
+Accelerate cgesv (main.swift:7834-7941) and zgesv (main.swift:7101-7185).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+BUILD = HERE / "_build"
+
+
+class _Geom(C.Structure):
+    _fields_ = [
+        ("vertices", C.c_void_p),
+        ("triangles", C.c_void_p),
+        ("l2g", C.c_void_p),
+        ("areas", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("n_vertices", C.c_int32),
+        ("n_triangles", C.c_int32),
+        ("n_dofs", C.c_int32),
+        ("symmetry", C.c_int32),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+_libs: dict[int, C.CDLL] = {}
+
+
+def lib(bits: int = 64) -> C.CDLL:
+    if bits not in _libs:
+        path = BUILD / f"libhbem_oracle{bits}.so"
+        if not path.exists():
+            build()
+        L = C.CDLL(str(path))
+        L.hbo_duffy_rule.restype = C.c_int
+        L.hbo_duffy_rule_size.restype = C.c_int
+        L.hbo_duffy_pairs.restype = C.c_int
+        L.hbo_near_pairs.restype = C.c_int
+        L.hbo_apply_duffy.restype = C.c_int64
+        L.hbo_apply_near.restype = C.c_int64
+        L.hbo_pair_block.restype = C.c_int
+        L.hbo_set_threads.restype = C.c_int
+        real = C.c_double if bits == 64 else C.c_float
+        L.hbo_kernel.argtypes = [real] * 8 + [C.c_void_p]
+        L.hbo_assemble_regular.argtypes = [C.c_void_p, real, real] + [C.c_void_p] * 8
+        L.hbo_apply_duffy.argtypes = [C.c_void_p, C.c_int, real, real] + [C.c_void_p] * 9
+        L.hbo_apply_near.argtypes = [C.c_void_p, C.c_int, C.c_double, real, real] + [C.c_void_p] * 8
+        L.hbo_field.argtypes = [C.c_void_p, real] + [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 3
+        L.hbo_pair_block.argtypes = [C.c_void_p] + [C.c_int] * 6 + [real, real, C.c_void_p, C.c_void_p]
+        L.hbo_near_pairs.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int]
+        L.hbo_duffy_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.hbo_surface_average.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hbo_duffy_rule.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 5
+        L.hbo_rule6.argtypes = [C.c_void_p] * 3
+        _libs[bits] = L
+    return _libs[bits]
+
+
+SYMMETRY = {None: 0, "yz": 1, "xz": 2, "yz+xz": 3, "xy": 4}
+
+
+@dataclass
+class Geometry:
+    """Keeps the float32/int32 buffers alive while the C side reads them."""
+
+    vertices: np.ndarray
+    triangles: np.ndarray
+    l2g: np.ndarray
+    areas: np.ndarray
+    normals: np.ndarray
+    n_dofs: int
+    symmetry: int
+    tags: np.ndarray
+
+    @classmethod
+    def from_buffers(cls, b, symmetry_plane: str | None = None) -> "Geometry":
+        return cls(
+            np.ascontiguousarray(b.vertices_3xn_f32, np.float32),
+            np.ascontiguousarray(b.triangles_3xm_i32, np.int32),
+            np.ascontiguousarray(b.p1_local2global_i32, np.int32),
+            np.ascontiguousarray(b.triangle_areas_f32, np.float32),
+            np.ascontiguousarray(b.triangle_normals_3xm_f32, np.float32),
+            int(b.p1_dof_count),
+            SYMMETRY[symmetry_plane],
+            np.ascontiguousarray(b.physical_tags_i32, np.int32),
+        )
+
+    @property
+    def n_triangles(self) -> int:
+        return int(self.triangles.shape[1])
+
+    def c(self) -> _Geom:
+        return _Geom(
+            self.vertices.ctypes.data, self.triangles.ctypes.data, self.l2g.ctypes.data,
+            self.areas.ctypes.data, self.normals.ctypes.data, int(self.vertices.shape[1]),
+            self.n_triangles, self.n_dofs, self.symmetry,
+        )
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _dt(bits):
+    return np.float64 if bits == 64 else np.float32
+
+
+def set_threads(n: int) -> int:
+    r = lib(64).hbo_set_threads(int(n))
+    lib(32).hbo_set_threads(int(n))
+    return r
+
+
+def rule6(bits=64):
+    d = _dt(bits)
+    x, y, w = np.zeros(6, d), np.zeros(6, d), np.zeros(6, d)
+    lib(bits).hbo_rule6(_p(x), _p(y), _p(w))
+    return x, y, w
+
+
+def duffy_rule(kind: int, order: int = 4, bits=64):
+    d = _dt(bits)
+    n = lib(bits).hbo_duffy_rule_size(kind, order)
+    arrs = [np.zeros(n, d) for _ in range(5)]
+    got = lib(bits).hbo_duffy_rule(kind, order, *[_p(a) for a in arrs])
+    assert got == n
+    return arrs  # test x, test y, trial x, trial y, weight
+
+
+def kernel(d, n, k, kim=0.0, bits=64):
+    out = np.zeros(4, _dt(bits))
+    lib(bits).hbo_kernel(*[float(v) for v in d], *[float(v) for v in n], float(k), float(kim), _p(out))
+    return complex(out[0], out[1]), complex(out[2], out[3])
+
+
+def duffy_pairs(geom: Geometry):
+    counts = np.zeros(3, np.int64)
+    g = geom.c()
+    n = lib(64).hbo_duffy_pairs(C.byref(g), None, 0, _p(counts))
+    pairs = np.zeros((max(n, 1), 9), np.int32)
+    lib(64).hbo_duffy_pairs(C.byref(g), _p(pairs), n, _p(counts))
+    return pairs[:n], {"coincident": int(counts[0]), "edge": int(counts[1]), "vertex": int(counts[2])}
+
+
+def near_pairs(geom: Geometry, threshold: float):
+    g = geom.c()
+    n = lib(64).hbo_near_pairs(C.byref(g), float(threshold), None, 0)
+    pairs = np.zeros((max(n, 1), 9), np.int32)
+    lib(64).hbo_near_pairs(C.byref(g), float(threshold), _p(pairs), n)
+    return pairs[:n]
+
+
+def pair_block(geom: Geometry, test, trial, tm=0, sm=0, mode=0, order=4, k=1.0, kim=0.0, bits=64):
+    d = _dt(bits)
+    slp, dlp = np.zeros(6, d), np.zeros(18, d)
+    g = geom.c()
+    rc = lib(bits).hbo_pair_block(C.byref(g), test, trial, tm, sm, mode, order, float(k), float(kim), _p(slp), _p(dlp))
+    if rc != 0:
+        raise ValueError("pair_block: invalid pair/mode")
+    return slp[0::2] + 1j * slp[1::2], (dlp[0::2] + 1j * dlp[1::2]).reshape(3, 3)
+
+
+def _split(q, n, d):
+    if q is None:
+        return None, None
+    q = np.asarray(q, np.complex128).reshape(n)
+    return np.ascontiguousarray(q.real, d), np.ascontiguousarray(q.imag, d)
+
+
+def assemble(geom: Geometry, k: float, q=None, *, kim: float = 0.0, beta=None,
+             duffy: bool = True, duffy_order: int = 4, near_level: int = 0,
+             near_threshold: float = 1.5, bits: int = 64):
+    """Corrected assembly: regular 6x6 on all pairs + Sauter-Schwab deltas on
+    touching pairs (+ optional subdivided near-field deltas).  Returns
+    (A (n,n) complex row-major, rhs (n,), stats)."""
+    d = _dt(bits)
+    n, T = geom.n_dofs, geom.n_triangles
+    Are, Aim = np.zeros((n, n), d), np.zeros((n, n), d)
+    rre, rim = np.zeros(n, d), np.zeros(n, d)
+    qre, qim = _split(q if q is not None else np.zeros(T), T, d)
+    bre, bim = _split(beta, T, d)
+    g = geom.c()
+    L = lib(bits)
+    L.hbo_assemble_regular(C.byref(g), k, kim, _p(qre), _p(qim), _p(bre), _p(bim), _p(Are), _p(Aim), _p(rre), _p(rim))
+    stats = {}
+    if duffy:
+        counts = np.zeros(3, np.int64)
+        u = L.hbo_apply_duffy(C.byref(g), duffy_order, k, kim, _p(qre), _p(qim), _p(bre), _p(bim),
+                              _p(Are), _p(Aim), _p(rre), _p(rim), _p(counts))
+        stats["planned_pairs"] = {"coincident": int(counts[0]), "edge": int(counts[1]), "vertex": int(counts[2])}
+        stats["unique_triplets"] = int(u)
+    if near_level > 0:
+        stats["near_pairs"] = int(L.hbo_apply_near(C.byref(g), near_level, near_threshold, k, kim, _p(qre), _p(qim),
+                                                   _p(bre), _p(bim), _p(Are), _p(Aim), _p(rre), _p(rim)))
+    return Are + 1j * Aim, rre + 1j * rim, stats
+
+
+def field(geom: Geometry, k: float, p, q, obs_3xn, bits: int = 64):
+    d = _dt(bits)
+    n, T = geom.n_dofs, geom.n_triangles
+    pre, pim = _split(p, n, d)
+    qre, qim = _split(q, T, d)
+    obs = np.ascontiguousarray(obs_3xn, np.float32)
+    m = int(obs.shape[1])
+    ore, oim = np.zeros(m, d), np.zeros(m, d)
+    g = geom.c()
+    lib(bits).hbo_field(C.byref(g), k, _p(pre), _p(pim), _p(qre), _p(qim), m, _p(obs), _p(ore), _p(oim))
+    return ore + 1j * oim
+
+
+def surface_average(geom: Geometry, p, tag: int, bits: int = 64):
+    d = _dt(bits)
+    pre, pim = _split(p, geom.n_dofs, d)
+    out = np.zeros(3, np.float64)
+    g = geom.c()
+    lib(bits).hbo_surface_average(C.byref(g), _p(geom.tags), int(tag), _p(pre), _p(pim), _p(out))
+    return complex(out[0], out[1]), float(out[2])
+
+
+def solve(A, b, bits: int = 64):
+    """LAPACK solve: zgesv (FP64) or cgesv (FP32)."""
+    from scipy.linalg import lapack
+
+    if bits == 64:
+        lu, piv, x, info = lapack.zgesv(np.asarray(A, np.complex128), np.asarray(b, np.complex128))
+    else:
+        lu, piv, x, info = lapack.cgesv(np.asarray(A, np.complex64), np.asarray(b, np.complex64))
+    if info != 0:
+        raise np.linalg.LinAlgError(f"gesv info={info}")
+    return x
