@@ -133,7 +133,8 @@ struct hbem_session {
     hipStream_t stream = nullptr;
     hbem::Plan plan;
     hbem::DeviceGeometry dg{};
-    DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, pair_tt, pair_code, rule;
+    DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, pair_tt, pair_code, rule, tile_start,
+        tile_count, tile_ndofs, tile_dofs;
     // per-call workspaces
     DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, q, kre, kim, fk, obs, field, tags,
         tagout, norms, anorm, active, rowsplit, rcx;
@@ -145,7 +146,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &pair_tt, &pair_code,
-                          &rule, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
+                          &rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag})
@@ -413,6 +414,10 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         upload(s->pair_tt, P.pair_tt, s->stream);
         upload(s->pair_code, P.pair_code, s->stream);
         upload(s->rule, P.rule, s->stream);
+        upload(s->tile_start, P.tile_start, s->stream);
+        upload(s->tile_count, P.tile_count, s->stream);
+        upload(s->tile_ndofs, P.tile_ndofs, s->stream);
+        upload(s->tile_dofs, P.tile_dofs, s->stream);
         hbem::DeviceGeometry &g = s->dg;
         g.tri_f_aos = s->tri_f_aos.as<float>();
         g.tri_i_aos = s->tri_i_aos.as<int32_t>();
@@ -432,6 +437,12 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         g.n_masks = P.n_masks;
         for (int i = 0; i < hbem::kMaxMasks; ++i) g.masks[i] = P.masks[i];
         g.row_weight = P.row_weight;
+        g.tile_start = s->tile_start.as<int32_t>();
+        g.tile_count = s->tile_count.as<int32_t>();
+        g.tile_ndofs = s->tile_ndofs.as<int32_t>();
+        g.tile_dofs = s->tile_dofs.as<int32_t>();
+        g.n_tiles = (int)P.tile_start.size();
+        g.max_tile_dofs = P.max_tile_dofs;
         hip_check(hipStreamSynchronize(s->stream), "session upload");
         s->setup_seconds = now_s() - t0;
         *out = s.release();

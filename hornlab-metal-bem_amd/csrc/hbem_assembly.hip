@@ -71,31 +71,143 @@ __device__ __forceinline__ float mask_sign(int mask, int bit) { return (mask & b
 
 // ---------------------------------------------------------------------------
 // Regular pairs.
-// Block = 256 threads = 4 waves.  Lanes own trial triangles (one per lane,
-// trial record held in registers); each wave walks a run of test triangles
-// whose records are wave-uniform (scalar loads).  grid.z = frequency.
+// One block = one (test tile, trial tile) pair of up to 64 x 64 triangles
+// for one frequency (grid.z).  Lanes own the trial triangles of the tile
+// (records in registers); each wave walks 16 test triangles whose records
+// are wave-uniform (scalar loads).  The 3x3 blocks are summed in an LDS
+// image of the tile's (rows x cols) sub-matrix with LDS float atomics, and
+// the block ends with one global atomic add per touched matrix entry.
+// Mirror signs are compile-time constants of the symmetry code.
 // ---------------------------------------------------------------------------
 
-constexpr int kRegTestsPerWave = 8;
+constexpr int kTestsPerWave = kTileTris / 4;
 
-template <int kMasks, bool kComplexK>
+constexpr int masks_of(int sym) { return sym == 0 ? 1 : (sym == 3 ? 4 : 2); }
+constexpr int mask_at(int sym, int m) { return m == 0 ? 0 : (sym == 3 ? m : sym); }
+constexpr float sgn(int mask, int bit) { return (mask & bit) ? -1.0f : 1.0f; }
+
+// Accumulates the DLP (and, for kSource, SLP) of one (test, trial image)
+// combination over the 6 x 6 rule.  dlp[i][j] holds -4 pi * DLP / jac and
+// slp[i] holds 4 pi * SLP / jac (constants folded into the weights).
+template <int kMask, bool kComplexK, bool kSource>
+__device__ __forceinline__ void regular_image(const float *__restrict__ tf, const float (&yp)[18],
+                                              float nx, float ny, float nz, float k, float krev,
+                                              float kim, float (&dlp)[9][2], float (&slp)[3][2]) {
+    constexpr float sx = sgn(kMask, 1), sy = sgn(kMask, 2), sz = sgn(kMask, 4);
+    const float mnx = sx * nx, mny = sy * ny, mnz = sz * nz;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        const float xa = tf[3 * a], ya = tf[3 * a + 1], za = tf[3 * a + 2];
+        float e[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+        float gs[2] = {0.f, 0.f};
+#pragma unroll
+        for (int bq = 0; bq < 6; ++bq) {
+            const float dx = sx * yp[3 * bq] - xa;
+            const float dy = sy * yp[3 * bq + 1] - ya;
+            const float dz = sz * yp[3 * bq + 2] - za;
+            const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), 1.0e-14f);
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            float t = r * krev;
+            t = t - __builtin_rintf(t);
+            const float s = __builtin_amdgcn_sinf(t);
+            const float c = __builtin_amdgcn_cosf(t);
+            float damp = 1.0f;
+            if (kComplexK) damp = __expf(-kim * r);
+            const float fr = kComplexK ? rinv + kim : rinv;
+            // u = (r . n) / r^2 (times the damping); dG/dn = -u G0 (c fr + s k, s fr - c k)
+            const float u = fmaf(dx, mnx, fmaf(dy, mny, dz * mnz)) * (rinv * rinv) * damp;
+            const float cfr = c * fr;
+            const float t1 = fmaf(s, k, cfr) * u;
+            const float t2 = fmaf(-c, k, s * fr) * u;
+            const float wb = (float)kRule6W[bq];
+            const float w1 = wb * (float)kRule6X[bq];
+            const float w2 = wb * (float)kRule6Y[bq];
+            const float w0 = wb - w1 - w2;
+            e[0][0] = fmaf(t1, w0, e[0][0]);
+            e[0][1] = fmaf(t2, w0, e[0][1]);
+            e[1][0] = fmaf(t1, w1, e[1][0]);
+            e[1][1] = fmaf(t2, w1, e[1][1]);
+            e[2][0] = fmaf(t1, w2, e[2][0]);
+            e[2][1] = fmaf(t2, w2, e[2][1]);
+            if (kSource) {
+                const float g = rinv * damp * wb;
+                gs[0] = fmaf(c, g, gs[0]);
+                gs[1] = fmaf(s, g, gs[1]);
+            }
+        }
+        const float wa = (float)kRule6W[a];
+        const float ta[3] = {wa * (float)(1.0 - kRule6X[a] - kRule6Y[a]), wa * (float)kRule6X[a],
+                             wa * (float)kRule6Y[a]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                dlp[3 * i + j][0] = fmaf(ta[i], e[j][0], dlp[3 * i + j][0]);
+                dlp[3 * i + j][1] = fmaf(ta[i], e[j][1], dlp[3 * i + j][1]);
+            }
+            if (kSource) {
+                slp[i][0] = fmaf(ta[i], gs[0], slp[i][0]);
+                slp[i][1] = fmaf(ta[i], gs[1], slp[i][1]);
+            }
+        }
+    }
+}
+
+// True when test and the trial's image share a vertex: image 0 by vertex
+// index, images by coordinate-key id (the singular-pair criteria).
+template <int kM>
+__device__ __forceinline__ bool touching(const int32_t *__restrict__ tii, const int (&svid)[3],
+                                         const int (&skid)[3]) {
+    bool touch = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) touch |= (kM == 0) ? (tii[3 + i] == svid[j]) : (tii[6 + i] == skid[j]);
+    return touch;
+}
+
+template <int SYM, bool kComplexK, bool kSource>
+__device__ __forceinline__ void regular_pair(const float *__restrict__ tf,
+                                             const int32_t *__restrict__ tii, const float (&yp)[18],
+                                             float nx, float ny, float nz, const int (&svid)[3],
+                                             const int (&skid)[kMaxMasks][3], float k, float krev,
+                                             float kim, float (&dlp)[9][2], float (&slp)[3][2]) {
+    if (!touching<0>(tii, svid, skid[0]))
+        regular_image<0, kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
+    if constexpr (masks_of(SYM) > 1) {
+        if (!touching<1>(tii, svid, skid[1]))
+            regular_image<mask_at(SYM, 1), kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
+    }
+    if constexpr (masks_of(SYM) > 2) {
+        if (!touching<2>(tii, svid, skid[2]))
+            regular_image<mask_at(SYM, 2), kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
+        if (!touching<3>(tii, svid, skid[3]))
+            regular_image<mask_at(SYM, 3), kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
+    }
+}
+
+template <int SYM, bool kComplexK>
 __global__ __launch_bounds__(256) void k_assemble_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, float *__restrict__ A, int64_t lda, int64_t a_stride,
     float *__restrict__ rhs, int64_t rhs_stride) {
+    constexpr int M = masks_of(SYM);
+    extern __shared__ float acc[];  // [trial dofs][test dofs][2]
     const int T = g.n_triangles;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int trial = blockIdx.x * 64 + lane;
-    const int b = blockIdx.z;
-    const bool trial_ok = trial < T;
-    const int tr = trial_ok ? trial : 0;
+    const int s_tile = blockIdx.x, t_tile = blockIdx.y, b = blockIdx.z;
+    const int s0 = g.tile_start[s_tile], scount = g.tile_count[s_tile], snd = g.tile_ndofs[s_tile];
+    const int t0 = g.tile_start[t_tile], tcount = g.tile_count[t_tile], tnd = g.tile_ndofs[t_tile];
+    for (int e = threadIdx.x; e < 2 * snd * tnd; e += 256) acc[e] = 0.0f;
 
     const float k = k_real[b];
     const float kim = kComplexK ? k_imag[b] : 0.0f;
     const float krev = k * kInv2Pi;
 
-    // Trial record (SoA, coalesced).
+    const bool trial_ok = lane < scount;
+    const int tr = s0 + (trial_ok ? lane : 0);
     float yp[18];
 #pragma unroll
     for (int j = 0; j < 18; ++j) yp[j] = g.tri_f_soa[(size_t)j * T + tr];
@@ -103,117 +215,64 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     const float ny = g.tri_f_soa[(size_t)19 * T + tr];
     const float nz = g.tri_f_soa[(size_t)20 * T + tr];
     const float jac_s = g.tri_f_soa[(size_t)22 * T + tr];
-    int sdof[3], svid[3], skid[kMasks][3];
+    int svid[3], skid[kMaxMasks][3], slc[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
-        sdof[l] = g.tri_i_soa[(size_t)l * T + tr];
         svid[l] = g.tri_i_soa[(size_t)(3 + l) * T + tr];
+        slc[l] = g.tri_i_soa[(size_t)(20 + l) * T + tr];
     }
 #pragma unroll
-    for (int m = 0; m < kMasks; ++m)
+    for (int m = 0; m < kMaxMasks; ++m)
 #pragma unroll
-        for (int l = 0; l < 3; ++l) skid[m][l] = g.tri_i_soa[(size_t)(6 + 3 * m + l) * T + tr];
-    const float qre = trial_ok ? q[((size_t)b * T + tr) * 2] : 0.0f;
-    const float qim = trial_ok ? q[((size_t)b * T + tr) * 2 + 1] : 0.0f;
+        for (int l = 0; l < 3; ++l)
+            skid[m][l] = (m < M) ? g.tri_i_soa[(size_t)(6 + 3 * m + l) * T + tr] : -1;
+    const int orig = g.tri_i_soa[(size_t)19 * T + tr];
+    const float qre = trial_ok ? q[((size_t)b * T + orig) * 2] : 0.0f;
+    const float qim = trial_ok ? q[((size_t)b * T + orig) * 2 + 1] : 0.0f;
     const bool wave_has_source = __any(qre != 0.0f || qim != 0.0f);
-
-    float *Ab = A + (size_t)b * a_stride * 2;
     float *rb = rhs + (size_t)b * rhs_stride * 2;
+    // -4 pi and 4 pi folded out of the kernels, jac = (2 A_test)(2 A_trial)
+    const float dscale = -kInv4Pi * jac_s * g.row_weight;
+    __syncthreads();
 
-    const int test0 = (blockIdx.y * 4 + wave) * kRegTestsPerWave;
-    for (int ti = 0; ti < kRegTestsPerWave; ++ti) {
-        const int test = test0 + ti;
-        if (test >= T) break;
+    for (int ti = 0; ti < kTestsPerWave; ++ti) {
+        const int lt = wave * kTestsPerWave + ti;
+        if (lt >= tcount) break;
+        const int test = t0 + lt;
         const float *tf = g.tri_f_aos + (size_t)test * kTriF;
         const int32_t *tii = g.tri_i_aos + (size_t)test * kTriI;
-        const float jac = tf[22] * jac_s * g.row_weight;
-
-        float dlp[9][2];
-        float slp[3][2];
+        const float jt = tf[22];
+        float dlp[9][2], slp[3][2];
 #pragma unroll
         for (int i = 0; i < 9; ++i) dlp[i][0] = dlp[i][1] = 0.0f;
 #pragma unroll
         for (int i = 0; i < 3; ++i) slp[i][0] = slp[i][1] = 0.0f;
-
-#pragma unroll
-        for (int m = 0; m < kMasks; ++m) {
-            // Touching (test, trial image) combinations belong to the
-            // singular kernel.  Image 0 compares vertex indices, images by
-            // coordinate-key ids (the reference's pair criteria).
-            bool touch = false;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    if (m == 0)
-                        touch |= (tii[3 + i] == svid[j]);
-                    else
-                        touch |= (tii[6 + i] == skid[m][j]);
-                }
-            if (touch || !trial_ok) continue;
-            const int mask = g.masks[m];
-            const float sx = mask_sign(mask, 1), sy = mask_sign(mask, 2), sz = mask_sign(mask, 4);
-            const float mnx = sx * nx, mny = sy * ny, mnz = sz * nz;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) {
-                const float xa = tf[3 * a], ya = tf[3 * a + 1], za = tf[3 * a + 2];
-                float e[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
-                float gs[2] = {0.f, 0.f};
-#pragma unroll
-                for (int bq = 0; bq < 6; ++bq) {
-                    const GD v = kernel_gd<kComplexK>(sx * yp[3 * bq] - xa, sy * yp[3 * bq + 1] - ya,
-                                                      sz * yp[3 * bq + 2] - za, mnx, mny, mnz, k,
-                                                      krev, kim);
-                    const float wb = (float)kRule6W[bq];
-                    const float w1 = wb * (float)kRule6X[bq];
-                    const float w2 = wb * (float)kRule6Y[bq];
-                    const float w0 = wb - w1 - w2;
-                    e[0][0] = fmaf(v.dre, w0, e[0][0]);
-                    e[0][1] = fmaf(v.dim, w0, e[0][1]);
-                    e[1][0] = fmaf(v.dre, w1, e[1][0]);
-                    e[1][1] = fmaf(v.dim, w1, e[1][1]);
-                    e[2][0] = fmaf(v.dre, w2, e[2][0]);
-                    e[2][1] = fmaf(v.dim, w2, e[2][1]);
-                    gs[0] = fmaf(v.gre, wb, gs[0]);
-                    gs[1] = fmaf(v.gim, wb, gs[1]);
-                }
-                const float wa = (float)kRule6W[a];
-                const float ta[3] = {wa * (float)(1.0 - kRule6X[a] - kRule6Y[a]),
-                                     wa * (float)kRule6X[a], wa * (float)kRule6Y[a]};
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        dlp[3 * i + j][0] = fmaf(ta[i], e[j][0], dlp[3 * i + j][0]);
-                        dlp[3 * i + j][1] = fmaf(ta[i], e[j][1], dlp[3 * i + j][1]);
-                    }
-                    slp[i][0] = fmaf(ta[i], gs[0], slp[i][0]);
-                    slp[i][1] = fmaf(ta[i], gs[1], slp[i][1]);
-                }
-            }
-        }
-
         if (trial_ok) {
+            if (wave_has_source)
+                regular_pair<SYM, kComplexK, true>(tf, tii, yp, nx, ny, nz, svid, skid, k, krev, kim,
+                                                   dlp, slp);
+            else
+                regular_pair<SYM, kComplexK, false>(tf, tii, yp, nx, ny, nz, svid, skid, k, krev,
+                                                    kim, dlp, slp);
+            const float sc = dscale * jt;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const int row = tii[i];
+                const int lr = tii[20 + i];
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    const size_t idx = ((size_t)sdof[j] * lda + row) * 2;
-                    const float vre = dlp[3 * i + j][0] * jac, vim = dlp[3 * i + j][1] * jac;
-                    if (vre != 0.0f || vim != 0.0f) {
-                        atomicAdd(Ab + idx, vre);
-                        atomicAdd(Ab + idx + 1, vim);
-                    }
+                    float *cell = acc + 2 * (slc[j] * tnd + lr);
+                    atomicAdd(cell, dlp[3 * i + j][0] * sc);
+                    atomicAdd(cell + 1, dlp[3 * i + j][1] * sc);
                 }
             }
         }
         if (wave_has_source) {
-            // rhs_i += 2^planes * slp_i * q_trial, reduced over the wave.
+            // rhs_i += 2^planes * SLP_i * q_trial, reduced over the wave
+            const float gsc = kInv4Pi * jac_s * jt * g.row_weight;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                float cre = (slp[i][0] * qre - slp[i][1] * qim) * jac;
-                float cim = (slp[i][0] * qim + slp[i][1] * qre) * jac;
+                float cre = (slp[i][0] * qre - slp[i][1] * qim) * gsc;
+                float cim = (slp[i][0] * qim + slp[i][1] * qre) * gsc;
 #pragma unroll
                 for (int off = 32; off >= 1; off >>= 1) {
                     cre += __shfl_xor(cre, off);
@@ -225,6 +284,18 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
                 }
             }
         }
+    }
+    __syncthreads();
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const int32_t *rows = g.tile_dofs + (size_t)t_tile * kTileMaxDofs;
+    const int32_t *cols = g.tile_dofs + (size_t)s_tile * kTileMaxDofs;
+    for (int e = threadIdx.x; e < snd * tnd; e += 256) {
+        const float vre = acc[2 * e], vim = acc[2 * e + 1];
+        if (vre == 0.0f && vim == 0.0f) continue;
+        const int lc = e / tnd, lr = e - lc * tnd;
+        const size_t idx = ((size_t)cols[lc] * lda + rows[lr]) * 2;
+        atomicAdd(Ab + idx, vre);
+        atomicAdd(Ab + idx + 1, vim);
     }
 }
 
@@ -372,8 +443,9 @@ __global__ __launch_bounds__(256) void k_assemble_singular(
         atomicAdd(Ab + idx + 1, vim);
     } else if (lane < 12) {
         const int i = lane - 9;
-        const float qre = q[((size_t)b * T + trial) * 2];
-        const float qim = q[((size_t)b * T + trial) * 2 + 1];
+        const int orig = g.tri_i_aos[(size_t)trial * kTriI + 19];
+        const float qre = q[((size_t)b * T + orig) * 2];
+        const float qim = q[((size_t)b * T + orig) * 2 + 1];
         if (qre != 0.0f || qim != 0.0f) {
             float sre = 0.f, sim = 0.f;
 #pragma unroll
@@ -423,7 +495,7 @@ __global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__
         const float p0r = pb[2 * ti[0]], p0i = pb[2 * ti[0] + 1];
         const float p1r = pb[2 * ti[1]], p1i = pb[2 * ti[1] + 1];
         const float p2r = pb[2 * ti[2]], p2i = pb[2 * ti[2] + 1];
-        const float qre = qb[2 * t], qim = qb[2 * t + 1];
+        const float qre = qb[2 * ti[19]], qim = qb[2 * ti[19] + 1];
         const float jac = tf[22];
         float tre = 0.f, tim = 0.f;
 #pragma unroll
@@ -538,15 +610,17 @@ void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *
                              const float *k_imag, const float *q, float *A, int64_t lda,
                              int64_t a_stride, float *rhs, int64_t rhs_stride,
                              hipStream_t stream) {
-    const int T = g.n_triangles;
-    dim3 grid((T + 63) / 64, (T + 4 * kRegTestsPerWave - 1) / (4 * kRegTestsPerWave), n_batch);
+    dim3 grid(g.n_tiles, g.n_tiles, n_batch);
+    const size_t shm = sizeof(float) * 2 * (size_t)g.max_tile_dofs * g.max_tile_dofs;
     const bool ck = k_imag != nullptr;
-#define HBEM_REG(M, CK)                                                                      \
-    hipLaunchKernelGGL((k_assemble_regular<M, CK>), grid, dim3(256), 0, stream, g, k_real,   \
+#define HBEM_REG(S, CK)                                                                       \
+    hipLaunchKernelGGL((k_assemble_regular<S, CK>), grid, dim3(256), shm, stream, g, k_real, \
                        k_imag, q, A, lda, a_stride, rhs, rhs_stride)
-    switch (g.n_masks) {
+    switch (g.symmetry) {
+    case 0: if (ck) HBEM_REG(0, true); else HBEM_REG(0, false); break;
     case 1: if (ck) HBEM_REG(1, true); else HBEM_REG(1, false); break;
     case 2: if (ck) HBEM_REG(2, true); else HBEM_REG(2, false); break;
+    case 3: if (ck) HBEM_REG(3, true); else HBEM_REG(3, false); break;
     default: if (ck) HBEM_REG(4, true); else HBEM_REG(4, false); break;
     }
 #undef HBEM_REG

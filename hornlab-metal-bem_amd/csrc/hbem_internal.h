@@ -43,8 +43,17 @@ constexpr int kTriF = 32;
 //   [3..5]   vertex indices
 //   [6+3m .. 8+3m] coordinate-key ids of the vertices under mask index m
 //   [18]     physical tag
-//   [19]     pad
-constexpr int kTriI = 20;
+//   [19]     original (caller) triangle index
+//   [20..22] position of each dof in the triangle's tile dof list
+//   [23]     pad
+constexpr int kTriI = 24;
+
+// Triangles are stored in Morton order of their centroids and grouped into
+// tiles of up to kTileTris consecutive triangles with at most kTileMaxDofs
+// distinct dofs; the regular-assembly kernel accumulates one tile x tile
+// block of the matrix in LDS.
+constexpr int kTileTris = 64;
+constexpr int kTileMaxDofs = 90;  // 90^2 complex floats = 64800 B of LDS
 
 struct Plan {
     int n_vertices = 0;
@@ -78,6 +87,13 @@ struct Plan {
     std::vector<float> areas;        // [T]
     std::vector<int32_t> tags;       // [T]
     std::vector<int32_t> l2g;        // [T][3]
+
+    // Tiles of device-order triangles.
+    std::vector<int32_t> tile_start;   // [n_tiles]
+    std::vector<int32_t> tile_count;   // [n_tiles]
+    std::vector<int32_t> tile_ndofs;   // [n_tiles]
+    std::vector<int32_t> tile_dofs;    // [n_tiles][kTileMaxDofs], -1 padded
+    int max_tile_dofs = 0;
 };
 
 // Builds the plan.  Returns an empty string on success, else a message.
@@ -111,6 +127,12 @@ struct DeviceGeometry {
     int n_masks;
     int masks[kMaxMasks];
     float row_weight;
+    const int32_t *tile_start;
+    const int32_t *tile_count;
+    const int32_t *tile_ndofs;
+    const int32_t *tile_dofs;
+    int n_tiles;
+    int max_tile_dofs;
 };
 
 // Zero n floats.

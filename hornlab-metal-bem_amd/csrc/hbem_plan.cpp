@@ -175,12 +175,100 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
                                       uniq.begin());
     auto key_id = [&](int m, int v) { return kid[static_cast<size_t>(m) * nv + v]; };
 
-    // Triangle records.
+    // Device triangle order: Morton order of the centroids, so that runs of
+    // consecutive triangles (tiles) cover compact patches with few distinct
+    // dofs.  perm[p] = original index of device triangle p.
+    std::vector<int32_t> perm(T), inv(T);
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        std::vector<double> cen(3 * static_cast<size_t>(T));
+        for (int t = 0; t < T; ++t)
+            for (int c = 0; c < 3; ++c) {
+                const double v = (static_cast<double>(vtx[c * nv + tri[t]]) + vtx[c * nv + tri[T + t]] +
+                                  vtx[c * nv + tri[2 * T + t]]) / 3.0;
+                cen[3 * t + c] = v;
+                lo[c] = std::min(lo[c], v);
+                hi[c] = std::max(hi[c], v);
+            }
+        const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-30});
+        auto spread = [](uint64_t x) {  // 21 bits -> every third bit
+            x &= 0x1fffff;
+            x = (x | x << 32) & 0x1f00000000ffffULL;
+            x = (x | x << 16) & 0x1f0000ff0000ffULL;
+            x = (x | x << 8) & 0x100f00f00f00f00fULL;
+            x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+            x = (x | x << 2) & 0x1249249249249249ULL;
+            return x;
+        };
+        std::vector<std::pair<uint64_t, int32_t>> code(T);
+        for (int t = 0; t < T; ++t) {
+            uint64_t c = 0;
+            for (int k = 0; k < 3; ++k) {
+                const double u = (cen[3 * t + k] - lo[k]) / span;
+                const uint64_t q = static_cast<uint64_t>(std::min(u, 1.0) * 2097151.0);
+                c |= spread(q) << k;
+            }
+            code[t] = {c, t};
+        }
+        std::sort(code.begin(), code.end());
+        for (int p = 0; p < T; ++p) {
+            perm[p] = code[p].second;
+            inv[code[p].second] = p;
+        }
+    }
+    // Tiles: runs of up to kTileTris consecutive device triangles, cut early
+    // when the run's distinct dofs would exceed kTileMaxDofs.
+    P->tile_start.clear();
+    P->tile_count.clear();
+    P->tile_ndofs.clear();
+    P->tile_dofs.clear();
+    std::vector<int32_t> ldof(3 * static_cast<size_t>(T), 0);
+    {
+        int p = 0;
+        std::vector<int32_t> dofs;
+        while (p < T) {
+            dofs.clear();
+            const int start = p;
+            while (p < T && p - start < kTileTris) {
+                const int t = perm[p];
+                int add = 0;
+                for (int l = 0; l < 3; ++l)
+                    if (std::find(dofs.begin(), dofs.end(), l2g[t * 3 + l]) == dofs.end()) ++add;
+                // the same dof may appear twice in one triangle only for
+                // degenerate input, which build_plan rejects via areas
+                if ((int)dofs.size() + add > kTileMaxDofs) break;
+                for (int l = 0; l < 3; ++l) {
+                    const int d = l2g[t * 3 + l];
+                    auto it = std::find(dofs.begin(), dofs.end(), d);
+                    int pos;
+                    if (it == dofs.end()) {
+                        pos = (int)dofs.size();
+                        dofs.push_back(d);
+                    } else {
+                        pos = (int)(it - dofs.begin());
+                    }
+                    ldof[3 * static_cast<size_t>(p) + l] = pos;
+                }
+                ++p;
+            }
+            P->tile_start.push_back(start);
+            P->tile_count.push_back(p - start);
+            P->tile_ndofs.push_back((int)dofs.size());
+            dofs.resize(kTileMaxDofs, -1);
+            P->tile_dofs.insert(P->tile_dofs.end(), dofs.begin(), dofs.end());
+        }
+        P->max_tile_dofs = *std::max_element(P->tile_ndofs.begin(), P->tile_ndofs.end());
+    }
+
+    // Triangle records, in device order.
     P->tri_f_aos.assign(static_cast<size_t>(T) * kTriF, 0.0f);
     P->tri_i_aos.assign(static_cast<size_t>(T) * kTriI, -1);
-    for (int t = 0; t < T; ++t) {
-        float *f = &P->tri_f_aos[static_cast<size_t>(t) * kTriF];
-        int32_t *ii = &P->tri_i_aos[static_cast<size_t>(t) * kTriI];
+    for (int p = 0; p < T; ++p) {
+        const int t = perm[p];
+        float *f = &P->tri_f_aos[static_cast<size_t>(p) * kTriF];
+        int32_t *ii = &P->tri_i_aos[static_cast<size_t>(p) * kTriI];
+        ii[19] = t;
+        for (int l = 0; l < 3; ++l) ii[20 + l] = ldof[3 * static_cast<size_t>(p) + l];
         const int v0 = tri[t], v1 = tri[T + t], v2 = tri[2 * T + t];
         for (int a = 0; a < 6; ++a) {
             const double xi = kRule6X[a], eta = kRule6Y[a], b0 = 1.0 - xi - eta;
@@ -318,8 +406,8 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
     std::unordered_set<int64_t> slots;
     slots.reserve(pairs.size() * 4);
     for (size_t p = 0; p < pairs.size(); ++p) {
-        P->pair_tt[2 * p] = pairs[p].test;
-        P->pair_tt[2 * p + 1] = pairs[p].trial;
+        P->pair_tt[2 * p] = inv[pairs[p].test];
+        P->pair_tt[2 * p + 1] = inv[pairs[p].trial];
         P->pair_code[p] = pairs[p].code;
         P->kind_count[pairs[p].kind - 1]++;
         if ((pairs[p].code & 63) != 0) P->image_pairs++;
