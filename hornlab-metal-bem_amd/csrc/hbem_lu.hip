@@ -166,54 +166,174 @@ __global__ __launch_bounds__(1024) void k_lu_panel(float *A, int64_t lda, int64_
     }
 }
 
-// Row swaps of panel [k, k+nb) on every column outside the panel, then the
-// unit-lower triangular solve U12 = L11^{-1} A12 on columns >= k + nb.
-// One thread per column.
+// Register-resident panel for N <= 1024: thread t owns row k + t of the
+// nb = 32 panel columns; two block barriers per column (reduction, row
+// exchange through LDS).  Same pivot rule as k_lu_panel.
+__global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, int64_t a_stride,
+                                                       int N, int k, int32_t *ipiv,
+                                                       int32_t *info) {
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const int row = k + tid;
+    const bool own = row < N;
+    c32 v[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+        v[c] = own ? ld(Ab, (size_t)(k + c) * lda + row) : c32{0.0f, 0.0f};
+    __shared__ float red_v[16];
+    __shared__ int red_i[16];
+    __shared__ c32 prow[2][kNB];
+    __shared__ c32 orow[2][kNB];
+    for (int j = 0; j < kNB; ++j) {
+        const int col = k + j;
+        // v[j] by predicated selection keeps every register index static
+        c32 vj = v[0];
+#pragma unroll
+        for (int c = 1; c < kNB; ++c)
+            if (c == j) vj = v[c];
+        float best = -1.0f;
+        int bi = N;
+        if (own && row >= col) {
+            best = fabsf(vj.re) + fabsf(vj.im);
+            bi = row;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const float ov = __shfl_xor(best, o);
+            const int oi = __shfl_xor(bi, o);
+            if (ov > best || (ov == best && oi < bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            red_v[wave] = best;
+            red_i[wave] = bi;
+        }
+        __syncthreads();
+        float bv = red_v[0];
+        int p = red_i[0];
+#pragma unroll
+        for (int w = 1; w < 16; ++w)
+            if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < p)) {
+                bv = red_v[w];
+                p = red_i[w];
+            }
+        if (p >= N) p = col;
+        const int buf = j & 1;
+        if (row == p) {
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) prow[buf][c] = v[c];
+        }
+        if (row == col && p != col) {
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) orow[buf][c] = v[c];
+        }
+        if (tid == 0) {
+            ipiv[(size_t)b * N + col] = p;
+            if (!(bv > 0.0f) && info[b] == 0) info[b] = col + 1;
+        }
+        __syncthreads();
+        if (row == col) {
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) v[c] = prow[buf][c];
+        } else if (row == p) {
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) v[c] = orow[buf][c];
+        }
+        const c32 piv = prow[buf][j];
+        const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
+        if (own && row > col) {
+            c32 l = v[0];
+#pragma unroll
+            for (int c = 1; c < kNB; ++c)
+                if (c == j) l = v[c];
+            if (nz) l = cmul(l, cinv(piv));
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) {
+                if (c == j) v[c] = l;
+                if (c > j) {
+                    const c32 u = prow[buf][c];
+                    v[c].re -= l.re * u.re - l.im * u.im;
+                    v[c].im -= l.re * u.im + l.im * u.re;
+                }
+            }
+        }
+    }
+    if (own) {
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) st(Ab, (size_t)(k + c) * lda + row, v[c]);
+    }
+}
+
+// Row swaps of panel [k, k + nb) on every column outside the panel, then
+// the unit-lower solve U12 = L11^{-1} A12 on columns >= k + nb.  The nb
+// sequential swaps are composed once per block into a gather list (<= 2 nb
+// rows), so each column issues all its loads before any store.
 __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int64_t a_stride,
                                                       int N, int k, int nb,
                                                       const int32_t *ipiv) {
     const int b = blockIdx.y;
     float *Ab = A + (size_t)b * a_stride * 2;
     __shared__ c32 L11[kNB][kNB + 1];
-    __shared__ int piv[kNB];
+    __shared__ int trow[2 * kNB];  // destination rows (first nb: k..k+nb-1)
+    __shared__ int tsrc[2 * kNB];  // source rows
+    __shared__ int tcount;
     for (int e = threadIdx.x; e < nb * nb; e += 256) {
-        const int i = e % nb, j = e / nb;  // row i, col j of the diagonal block
+        const int i = e % nb, j = e / nb;
         L11[i][j] = ld(Ab, (size_t)(k + j) * lda + k + i);
     }
-    if (threadIdx.x < nb) piv[threadIdx.x] = ipiv[(size_t)b * N + k + threadIdx.x];
+    if (threadIdx.x == 0) {
+        int cnt = nb;
+        for (int j = 0; j < nb; ++j) {
+            trow[j] = k + j;
+            tsrc[j] = k + j;
+        }
+        for (int j = 0; j < nb; ++j) {
+            const int p = ipiv[(size_t)b * N + k + j];
+            if (p == k + j) continue;
+            int ip = -1;
+            for (int m = 0; m < cnt; ++m)
+                if (trow[m] == p) ip = m;
+            if (ip < 0) {
+                ip = cnt++;
+                trow[ip] = p;
+                tsrc[ip] = p;
+            }
+            const int t = tsrc[j];  // swap the sources of positions k+j and p
+            tsrc[j] = tsrc[ip];
+            tsrc[ip] = t;
+        }
+        tcount = cnt;
+    }
     __syncthreads();
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= N || (c >= k && c < k + nb)) return;
     const size_t cc = (size_t)c * lda;
-    for (int j = 0; j < nb; ++j) {
-        const int p = piv[j];
-        if (p != k + j) {
-            const c32 x = ld(Ab, cc + k + j), y = ld(Ab, cc + p);
-            st(Ab, cc + k + j, y);
-            st(Ab, cc + p, x);
+    const int cnt = tcount;
+    c32 vals[2 * kNB];
+#pragma unroll
+    for (int m = 0; m < 2 * kNB; ++m)
+        if (m < cnt) vals[m] = ld(Ab, cc + tsrc[m]);
+    if (c >= k + nb) {
+        // forward substitution with unit L11 on the first nb (= kNB) rows
+#pragma unroll
+        for (int i = 1; i < kNB; ++i) {
+            c32 acc = vals[i];
+#pragma unroll
+            for (int j = 0; j < i; ++j) {
+                const c32 l = L11[i][j];
+                acc.re -= l.re * vals[j].re - l.im * vals[j].im;
+                acc.im -= l.re * vals[j].im + l.im * vals[j].re;
+            }
+            vals[i] = acc;
         }
     }
-    if (c < k + nb) return;
-    c32 x[kNB];
 #pragma unroll
-    for (int i = 0; i < kNB; ++i)
-        if (i < nb) x[i] = ld(Ab, cc + k + i);
-#pragma unroll
-    for (int i = 0; i < kNB; ++i) {
-        if (i >= nb) break;
-        c32 acc = x[i];
-#pragma unroll
-        for (int j = 0; j < kNB; ++j) {
-            if (j >= i) break;
-            const c32 l = L11[i][j];
-            acc.re -= l.re * x[j].re - l.im * x[j].im;
-            acc.im -= l.re * x[j].im + l.im * x[j].re;
-        }
-        x[i] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < kNB; ++i)
-        if (i < nb) st(Ab, cc + k + i, x[i]);
+    for (int m = 0; m < 2 * kNB; ++m)
+        if (m < cnt) st(Ab, cc + trow[m], vals[m]);
 }
 
 // Trailing update A22 -= L21 * U12 over rows/cols [k + nb, N), 64 x 64
@@ -275,6 +395,95 @@ __global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t 
     }
 }
 
+// MFMA trailing update A22 -= L21 * U12 (FP32 matrix cores,
+// v_mfma_f32_16x16x4_f32).  64 x 64 complex output tile per block; each of
+// the 4 waves owns a 32 x 32 sub-tile as 2 x 2 MFMA tiles with separate real
+// and imaginary accumulators (re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).
+// The MFMA computes the transposed tile C^T = U^T L^T so that the 16 lanes
+// of an output row group cover 16 consecutive matrix rows (coalesced
+// column-major stores).  Operands are staged in LDS as separate re/im planes
+// with an 80-float row stride (conflict-free reads of the 4 k-rows a wave
+// touches per instruction).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kGS = 80;
+
+__global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int64_t a_stride,
+                                                      int N, int k) {
+    const int b = blockIdx.z;
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const int s = k + kNB;
+    const int r0 = s + blockIdx.x * 64;
+    const int c0 = s + blockIdx.y * 64;
+    __shared__ float Lr[kNB][kGS], Li[kNB][kGS], Ur[kNB][kGS], Ui[kNB][kGS];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < (kNB * 64) / 256; ++it) {
+        const int e = tid + 256 * it;
+        const int kk = e / 64, rr = e % 64;
+        const c32 v = (r0 + rr < N) ? ld(Ab, (size_t)(k + kk) * lda + r0 + rr) : c32{0.f, 0.f};
+        Lr[kk][rr] = v.re;
+        Li[kk][rr] = v.im;
+    }
+#pragma unroll
+    for (int it = 0; it < (kNB * 64) / 256; ++it) {
+        const int e = tid + 256 * it;
+        const int cc = e / kNB, kk = e % kNB;
+        const c32 v = (c0 + cc < N) ? ld(Ab, (size_t)(c0 + cc) * lda + k + kk) : c32{0.f, 0.f};
+        Ur[kk][cc] = v.re;
+        Ui[kk][cc] = v.im;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    f32x4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            accr[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acci[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+    for (int ks = 0; ks < kNB; ks += 4) {
+        const int kk = ks + lk;
+        float ur[2], ui[2], lr[2], lim[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            ur[t] = Ur[kk][wc + 16 * t + li];
+            ui[t] = Ui[kk][wc + 16 * t + li];
+            lr[t] = Lr[kk][wr + 16 * t + li];
+            lim[t] = Li[kk][wr + 16 * t + li];
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                // D^T[col][row]: A operand = U^T (col, k), B operand = L^T (k, row)
+                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
+                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[ti], lim[tj], accr[ti][tj], 0, 0, 0);
+                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
+                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = c0 + wc + 16 * ti + 4 * lk + v;
+                const int row = r0 + wr + 16 * tj + li;
+                if (row < N && col < N) {
+                    const size_t idx = (size_t)col * lda + row;
+                    c32 x = ld(Ab, idx);
+                    x.re -= accr[ti][tj][v];
+                    x.im -= acci[ti][tj][v];
+                    st(Ab, idx, x);
+                }
+            }
+}
+
 // ---------------------------------------------------------------------------
 // Triangular solves, one 1024-thread block per system, vector in LDS.
 // ---------------------------------------------------------------------------
@@ -295,6 +504,91 @@ __device__ __forceinline__ c32 block_sum(c32 v, c32 *red) {
         s.im += red[w].im;
     }
     return s;
+}
+
+// x <- A^{-1} x with the LU factors: pivots, then L and U sweeps in blocks
+// of 32 rows.  Wave 0 solves each 32 x 32 diagonal block with shuffles; all
+// threads update the remaining rows.  One block per system, x in LDS.
+__global__ __launch_bounds__(1024) void k_lu_solve_blocked(const float *A, int64_t lda,
+                                                           int64_t a_stride, int N,
+                                                           const int32_t *ipiv, float *X,
+                                                           int64_t ldx) {
+    extern __shared__ c32 xs[];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const float *Ab = A + (size_t)b * a_stride * 2;
+    const int32_t *pv = ipiv + (size_t)b * N;
+    float *xb = X + (size_t)b * ldx * 2;
+    for (int r = tid; r < N; r += 1024) xs[r] = ld(xb, r);
+    __syncthreads();
+    if (tid == 0)
+        for (int j = 0; j < N; ++j) {
+            const int p = pv[j];
+            if (p != j) {
+                const c32 t = xs[j];
+                xs[j] = xs[p];
+                xs[p] = t;
+            }
+        }
+    __syncthreads();
+    constexpr int BS = 32;
+    for (int kb = 0; kb < N; kb += BS) {
+        if (tid < 64) {
+            c32 x = lane < BS ? xs[kb + lane] : c32{0.0f, 0.0f};
+            for (int j = 0; j < BS - 1; ++j) {
+                const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                if (lane > j && lane < BS) {
+                    const c32 l = ld(Ab, (size_t)(kb + j) * lda + kb + lane);
+                    x.re -= l.re * xr - l.im * xi;
+                    x.im -= l.re * xi + l.im * xr;
+                }
+            }
+            if (lane < BS) xs[kb + lane] = x;
+        }
+        __syncthreads();
+        for (int r = kb + BS + tid; r < N; r += 1024) {
+            c32 acc = xs[r];
+#pragma unroll 8
+            for (int j = 0; j < BS; ++j) {
+                const c32 l = ld(Ab, (size_t)(kb + j) * lda + r);
+                const c32 xj = xs[kb + j];
+                acc.re -= l.re * xj.re - l.im * xj.im;
+                acc.im -= l.re * xj.im + l.im * xj.re;
+            }
+            xs[r] = acc;
+        }
+        __syncthreads();
+    }
+    for (int kb = N - BS; kb >= 0; kb -= BS) {
+        if (tid < 64) {
+            c32 x = lane < BS ? xs[kb + lane] : c32{0.0f, 0.0f};
+            for (int j = BS - 1; j >= 0; --j) {
+                if (lane == j) x = cmul(x, cinv(ld(Ab, (size_t)(kb + j) * lda + kb + j)));
+                const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                if (lane < j) {
+                    const c32 u = ld(Ab, (size_t)(kb + j) * lda + kb + lane);
+                    x.re -= u.re * xr - u.im * xi;
+                    x.im -= u.re * xi + u.im * xr;
+                }
+            }
+            if (lane < BS) xs[kb + lane] = x;
+        }
+        __syncthreads();
+        for (int r = tid; r < kb; r += 1024) {
+            c32 acc = xs[r];
+#pragma unroll 8
+            for (int j = 0; j < BS; ++j) {
+                const c32 u = ld(Ab, (size_t)(kb + j) * lda + r);
+                const c32 xj = xs[kb + j];
+                acc.re -= u.re * xj.re - u.im * xj.im;
+                acc.im -= u.re * xj.im + u.im * xj.re;
+            }
+            xs[r] = acc;
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < N; r += 1024) st(xb, r, xs[r]);
 }
 
 // trans = 0: x <- A^{-1} x ;  trans = 2: x <- A^{-H} x.
@@ -505,25 +799,39 @@ void launch_norm1(const float *A, int64_t lda, int64_t a_stride, int n, int B, d
 }
 
 void lu_factor(const LuWorkspace &w, hipStream_t stream) {
+    static_assert(kLuPad % kNB == 0, "padded order must hold whole panels");
     // int32 zero has the bit pattern of float zero
     launch_fill_zero(reinterpret_cast<float *>(w.info), (size_t)w.B, stream);
     for (int k = 0; k < w.N; k += kNB) {
         const int nb = std::min(kNB, w.N - k);
-        hipLaunchKernelGGL(k_lu_panel, dim3(w.B), dim3(1024), 0, stream, w.A, w.lda, w.a_stride,
-                           w.N, k, nb, w.ipiv, w.info);
+        if (w.N <= 1024 && nb == kNB)
+            hipLaunchKernelGGL(k_lu_panel_reg, dim3(w.B), dim3(1024), 0, stream, w.A, w.lda,
+                               w.a_stride, w.N, k, w.ipiv, w.info);
+        else
+            hipLaunchKernelGGL(k_lu_panel, dim3(w.B), dim3(1024), 0, stream, w.A, w.lda,
+                               w.a_stride, w.N, k, nb, w.ipiv, w.info);
         hipLaunchKernelGGL(k_lu_swap_trsm, dim3((w.N + 255) / 256, w.B), dim3(256), 0, stream, w.A,
                            w.lda, w.a_stride, w.N, k, nb, w.ipiv);
         const int rest = w.N - k - nb;
-        if (rest > 0)
-            hipLaunchKernelGGL(k_lu_gemm, dim3((rest + 63) / 64, (rest + 63) / 64, w.B), dim3(256),
-                               0, stream, w.A, w.lda, w.a_stride, w.N, k, nb);
+        if (rest > 0) {
+            if (nb == kNB)
+                hipLaunchKernelGGL(k_lu_gemm_mfma, dim3((rest + 63) / 64, (rest + 63) / 64, w.B),
+                                   dim3(256), 0, stream, w.A, w.lda, w.a_stride, w.N, k);
+            else
+                hipLaunchKernelGGL(k_lu_gemm, dim3((rest + 63) / 64, (rest + 63) / 64, w.B),
+                                   dim3(256), 0, stream, w.A, w.lda, w.a_stride, w.N, k, nb);
+        }
     }
 }
 
 void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream) {
     const size_t shm = sizeof(float) * 2 * (size_t)w.N;
-    hipLaunchKernelGGL(k_lu_solve, dim3(w.B), dim3(1024), shm, stream, w.A, w.lda, w.a_stride, w.N,
-                       w.ipiv, x, ldx, trans);
+    if (trans == 0 && w.N % 32 == 0)
+        hipLaunchKernelGGL(k_lu_solve_blocked, dim3(w.B), dim3(1024), shm, stream, w.A, w.lda,
+                           w.a_stride, w.N, w.ipiv, x, ldx);
+    else
+        hipLaunchKernelGGL(k_lu_solve, dim3(w.B), dim3(1024), shm, stream, w.A, w.lda,
+                           w.a_stride, w.N, w.ipiv, x, ldx, trans);
 }
 
 void launch_residual_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,
