@@ -133,7 +133,7 @@ struct hbem_session {
     hipStream_t stream = nullptr;
     hbem::Plan plan;
     hbem::DeviceGeometry dg{};
-    DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, pair_tt, pair_code, rule, tile_start,
+    DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
         tile_count, tile_ndofs, tile_dofs;
     // per-call workspaces
     DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, q, kre, kim, fk, obs, field, tags,
@@ -145,8 +145,8 @@ struct hbem_session {
     int cu_count = 0;
 
     ~hbem_session() {
-        for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &pair_tt, &pair_code,
-                          &rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
+        for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag})
@@ -411,9 +411,9 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         upload(s->tri_i_aos, P.tri_i_aos, s->stream);
         upload(s->tri_f_soa, P.tri_f_soa, s->stream);
         upload(s->tri_i_soa, P.tri_i_soa, s->stream);
-        upload(s->pair_tt, P.pair_tt, s->stream);
-        upload(s->pair_code, P.pair_code, s->stream);
-        upload(s->rule, P.rule, s->stream);
+        upload(s->sg_f, P.sg_f, s->stream);
+        upload(s->sg_i, P.sg_i, s->stream);
+        upload(s->sg_rule, P.sg_rule, s->stream);
         upload(s->tile_start, P.tile_start, s->stream);
         upload(s->tile_count, P.tile_count, s->stream);
         upload(s->tile_ndofs, P.tile_ndofs, s->stream);
@@ -423,14 +423,15 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         g.tri_i_aos = s->tri_i_aos.as<int32_t>();
         g.tri_f_soa = s->tri_f_soa.as<float>();
         g.tri_i_soa = s->tri_i_soa.as<int32_t>();
-        g.pair_tt = s->pair_tt.as<int32_t>();
-        g.pair_code = s->pair_code.as<int32_t>();
-        g.rule = s->rule.as<float>();
+        g.sg_f = s->sg_f.as<float>();
+        g.sg_i = s->sg_i.as<int32_t>();
+        g.sg_rule = s->sg_rule.as<float>();
+        g.sg_np = P.sg_np;
         for (int i = 0; i < 3; ++i) {
-            g.rule_offset[i] = P.rule_offset[i];
-            g.rule_count[i] = P.rule_count[i];
+            g.sg_waves[i] = P.sg_waves[i];
+            g.sg_rule_offset[i] = P.sg_rule_offset[i];
+            g.sg_rule_count[i] = P.sg_rule_count[i];
         }
-        g.n_pairs = (int)P.pair_code.size();
         g.n_triangles = P.n_triangles;
         g.n_dofs = P.n_dofs;
         g.symmetry = P.symmetry;

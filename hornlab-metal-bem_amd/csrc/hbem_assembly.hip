@@ -6,9 +6,9 @@
 //                          (reference: assemble_matrix_pair_atomic and
 //                          assemble_rhs_source_regular, main.swift:3698-3736
 //                          and 3993-4133).
-//   k_assemble_singular -- touching pairs with the Sauter-Schwab rules plus
-//                          the -M/2 mass term (reference: duffy_delta_blocks
-//                          main.swift:4246-4431 folded by
+//   k_assemble_singular -- touching pair classes with the Sauter-Schwab rules
+//                          plus the -M/2 mass term (reference:
+//                          duffy_delta_blocks main.swift:4246-4431 folded by
 //                          reduceDuffyDeltaBlocks main.swift:6498-6561).
 //   k_field             -- exterior potential (evaluate_field_regular,
 //                          main.swift:4135-4203).
@@ -300,165 +300,192 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
 }
 
 // ---------------------------------------------------------------------------
-// Singular pairs: one wave per (pair, frequency); lanes stride over the
-// Sauter-Schwab points of the pair's kind.
+// Singular pair classes.  One lane per class, kFB frequencies per lane; a
+// wave holds classes of one kind, so the Sauter-Schwab points are
+// wave-uniform and stream through scalar loads.  Per point and lane:
+// the distance vector from the class's affine maps (cancellation-free on
+// shared vertices and edges), one rsq, then per frequency sin/cos and the
+// fused 3x3 double-layer block plus 3 single-layer moments.  Sums run in
+// FP32 over runs of kSgRun points and are carried in FP64 across runs
+// (blocked compensated summation).  Block deltas go to A / rhs with one
+// atomic add per entry.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void remap_point(float x, float y, int kind, int l1, int l2,
-                                            float &ox, float &oy) {
-    // remapSingular (main.swift:2709-2735)
-    if (kind == 1) {
-        ox = x;
-        oy = y;
-    } else if (kind == 2) {
-        const int vc = 3 - l1 - l2;
-        const float ax = (l1 == 1), ay = (l1 == 2);
-        const float bx = (l2 == 1), by = (l2 == 2);
-        const float cx = (vc == 1), cy = (vc == 2);
-        ox = ax + x * (bx - ax) + y * (cx - ax);
-        oy = ay + x * (by - ay) + y * (cy - ay);
-    } else {
-        if (l1 == 0) {
-            ox = x;
-            oy = y;
-        } else if (l1 == 1) {
-            ox = 1.0f - x - y;
-            oy = y;
-        } else {
-            ox = x;
-            oy = 1.0f - x - y;
-        }
-    }
-}
+constexpr int kSgRun = 32;
 
-template <bool kComplexK>
+template <bool kComplexK, int kFB>
 __global__ __launch_bounds__(256) void k_assemble_singular(
-    DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
+    DeviceGeometry g, const float *__restrict__ rule, int n_batch,
+    const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, float *__restrict__ A, int64_t lda, int64_t a_stride,
     float *__restrict__ rhs, int64_t rhs_stride) {
     const int lane = threadIdx.x & 63;
-    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int b = blockIdx.y;
-    if (pair >= g.n_pairs) return;
-    const int T = g.n_triangles;
-    const int test = g.pair_tt[2 * pair];
-    const int trial = g.pair_tt[2 * pair + 1];
-    const int code = g.pair_code[pair];
-    const int tm = code & 7, sm = (code >> 3) & 7, kind = (code >> 6) & 3;
-    const int tl1 = (code >> 8) & 3, tl2 = (code >> 10) & 3;
-    const int sl1 = (code >> 12) & 3, sl2 = (code >> 14) & 3;
-    const bool t_odd = __popc(tm) & 1, s_odd = __popc(sm) & 1;
-
-    const float k = k_real[b];
-    const float kim = kComplexK ? k_imag[b] : 0.0f;
-    const float krev = k * kInv2Pi;
-
-    const float *tf = g.tri_f_aos + (size_t)test * kTriF;
-    const float *sf = g.tri_f_aos + (size_t)trial * kTriF;
-    const float tsx = mask_sign(tm, 1), tsy = mask_sign(tm, 2), tsz = mask_sign(tm, 4);
-    const float ssx = mask_sign(sm, 1), ssy = mask_sign(sm, 2), ssz = mask_sign(sm, 4);
-    const float nx = ssx * sf[18], ny = ssy * sf[19], nz = ssz * sf[20];
-    const float t0x = tf[23], t0y = tf[24], t0z = tf[25];
-    const float t1x = tf[26] - t0x, t1y = tf[27] - t0y, t1z = tf[28] - t0z;
-    const float t2x = tf[29] - t0x, t2y = tf[30] - t0y, t2z = tf[31] - t0z;
-    const float s0x = sf[23], s0y = sf[24], s0z = sf[25];
-    const float s1x = sf[26] - s0x, s1y = sf[27] - s0y, s1z = sf[28] - s0z;
-    const float s2x = sf[29] - s0x, s2y = sf[30] - s0y, s2z = sf[31] - s0z;
-
-    float dlp[9][2], slp[3][2];
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int w1 = g.sg_waves[0], w2 = w1 + g.sg_waves[1], w3 = w2 + g.sg_waves[2];
+    if (w >= w3) return;
+    const int kind = w < w1 ? 0 : (w < w2 ? 1 : 2);
+    const int np = g.sg_np;
+    const int c = w * 64 + lane;
+    const float *__restrict__ F = g.sg_f;
+    float gf[18];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) dlp[i][0] = dlp[i][1] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) slp[i][0] = slp[i][1] = 0.0f;
+    for (int j = 0; j < 18; ++j) gf[j] = F[(size_t)j * np + c];
+    const float r0x = gf[0], r0y = gf[1], r0z = gf[2];
+    const float e1x = gf[3], e1y = gf[4], e1z = gf[5];
+    const float e2x = gf[6], e2y = gf[7], e2z = gf[8];
+    const float g1x = gf[9], g1y = gf[10], g1z = gf[11];
+    const float g2x = gf[12], g2y = gf[13], g2z = gf[14];
+    const float nx = gf[15], ny = gf[16], nz = gf[17];
 
-    const int off = g.rule_offset[kind - 1];
-    const int cnt = g.rule_count[kind - 1];
-    for (int p = lane; p < cnt; p += 64) {
-        const float *r = g.rule + (size_t)(off + p) * 5;
-        float a, bb, txi, teta, sxi, seta;
-        remap_point(r[0], r[1], kind, tl1, tl2, a, bb);
-        txi = t_odd ? bb : a;
-        teta = t_odd ? a : bb;
-        remap_point(r[2], r[3], kind, sl1, sl2, a, bb);
-        sxi = s_odd ? bb : a;
-        seta = s_odd ? a : bb;
-        const float w = r[4];
-        const float px = tsx * (t0x + txi * t1x + teta * t2x);
-        const float py = tsy * (t0y + txi * t1y + teta * t2y);
-        const float pz = tsz * (t0z + txi * t1z + teta * t2z);
-        const float qx = ssx * (s0x + sxi * s1x + seta * s2x);
-        const float qy = ssy * (s0y + sxi * s1y + seta * s2y);
-        const float qz = ssz * (s0z + sxi * s1z + seta * s2z);
-        const GD v = kernel_gd<kComplexK>(qx - px, qy - py, qz - pz, nx, ny, nz, k, krev, kim);
-        const float tb[3] = {(1.0f - txi - teta) * w, txi * w, teta * w};
-        const float sb[3] = {1.0f - sxi - seta, sxi, seta};
+    const int b0 = blockIdx.y * kFB;
+    float kk[kFB], krev[kFB], kim[kFB];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            slp[i][0] = fmaf(v.gre, tb[i], slp[i][0]);
-            slp[i][1] = fmaf(v.gim, tb[i], slp[i][1]);
+    for (int f = 0; f < kFB; ++f) {
+        const int b = min(b0 + f, n_batch - 1);
+        kk[f] = k_real[b];
+        krev[f] = kk[f] * kInv2Pi;
+        kim[f] = kComplexK ? k_imag[b] : 0.0f;
+    }
+
+    const float *__restrict__ rp = rule + (size_t)g.sg_rule_offset[kind] * kSgRule;
+    const int cnt = g.sg_rule_count[kind];
+    double dd[kFB][9][2], ds[kFB][3][2];
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const float c = tb[i] * sb[j];
-                dlp[3 * i + j][0] = fmaf(v.dre, c, dlp[3 * i + j][0]);
-                dlp[3 * i + j][1] = fmaf(v.dim, c, dlp[3 * i + j][1]);
+    for (int f = 0; f < kFB; ++f) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dd[f][i][0] = dd[f][i][1] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ds[f][i][0] = ds[f][i][1] = 0.0;
+    }
+    for (int p0 = 0; p0 < cnt; p0 += kSgRun) {
+        float ad[kFB][9][2], as[kFB][3][2];
+#pragma unroll
+        for (int f = 0; f < kFB; ++f) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) ad[f][i][0] = ad[f][i][1] = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) as[f][i][0] = as[f][i][1] = 0.0f;
+        }
+        // Two-stage software pipeline over the points: the scalar loads of
+        // the next point are in flight while the current one is evaluated.
+        // distance vector of a point (its first use of the point's loads)
+        auto dist = [&](const float (&pt)[kSgRule], float &rx, float &ry, float &rz) {
+            const float dx = pt[0], dy = pt[1], xs = pt[2], ys = pt[3];
+            rx = fmaf(ys, g2x, fmaf(xs, g1x, fmaf(dy, e2x, fmaf(dx, e1x, r0x))));
+            ry = fmaf(ys, g2y, fmaf(xs, g1y, fmaf(dy, e2y, fmaf(dx, e1y, r0y))));
+            rz = fmaf(ys, g2z, fmaf(xs, g1z, fmaf(dy, e2z, fmaf(dx, e1z, r0z))));
+        };
+        auto eval_point = [&](const float (&pt)[kSgRule], float rx, float ry, float rz) {
+            const float r2raw = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+            const bool ok = r2raw > 1.0e-14f;
+            const float r2 = ok ? r2raw : 1.0e-14f;
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            const float gs = ok ? kInv4Pi * rinv : 0.0f;
+            const float proj = fmaf(rx, nx, fmaf(ry, ny, rz * nz)) * rinv;
+#pragma unroll
+            for (int f = 0; f < kFB; ++f) {
+                float t = r * krev[f];
+                t = t - __builtin_rintf(t);
+                const float sn = __builtin_amdgcn_sinf(t);
+                const float cs = __builtin_amdgcn_cosf(t);
+                float sc = gs;
+                if (kComplexK) sc *= __expf(-kim[f] * r);
+                const float gre = cs * sc, gim = sn * sc;
+                const float fre = kComplexK ? (-rinv - kim[f]) : -rinv;
+                const float pre = gre * proj, pim = gim * proj;
+                const float dre = fmaf(pre, fre, -pim * kk[f]);
+                const float dim = fmaf(pre, kk[f], pim * fre);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    ad[f][i][0] = fmaf(dre, pt[4 + i], ad[f][i][0]);
+                    ad[f][i][1] = fmaf(dim, pt[4 + i], ad[f][i][1]);
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    as[f][i][0] = fmaf(gre, pt[13 + i], as[f][i][0]);
+                    as[f][i][1] = fmaf(gim, pt[13 + i], as[f][i][1]);
+                }
+            }
+        };
+        float pa[kSgRule], pb[kSgRule];
+#pragma unroll
+        for (int j = 0; j < kSgRule; ++j) pa[j] = rp[(size_t)p0 * kSgRule + j];
+        // Scalar loads return out of order, so a wait on one point's loads
+        // is a wait on all: each point's loads are consumed (dist) before
+        // the next point's loads are issued, and the scheduling barriers keep
+        // that order.
+        for (int p = p0; p < p0 + kSgRun; p += 2) {
+            float rx, ry, rz;
+            dist(pa, rx, ry, rz);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < kSgRule; ++j) pb[j] = rp[(size_t)(p + 1) * kSgRule + j];
+            __builtin_amdgcn_sched_barrier(0);
+            eval_point(pa, rx, ry, rz);
+            __builtin_amdgcn_sched_barrier(0);
+            dist(pb, rx, ry, rz);
+            __builtin_amdgcn_sched_barrier(0);
+            const int pn = (p + 2 < cnt) ? p + 2 : p;
+#pragma unroll
+            for (int j = 0; j < kSgRule; ++j) pa[j] = rp[(size_t)pn * kSgRule + j];
+            __builtin_amdgcn_sched_barrier(0);
+            eval_point(pb, rx, ry, rz);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int f = 0; f < kFB; ++f) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                dd[f][i][0] += (double)ad[f][i][0];
+                dd[f][i][1] += (double)ad[f][i][1];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                ds[f][i][0] += (double)as[f][i][0];
+                ds[f][i][1] += (double)as[f][i][1];
             }
         }
     }
-    // Wave reduction of the 24 accumulators.
-#pragma unroll
-    for (int i = 0; i < 9; ++i)
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) dlp[i][c] += __shfl_xor(dlp[i][c], o);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) slp[i][c] += __shfl_xor(slp[i][c], o);
 
-    const float jac = tf[22] * sf[22];
-    const bool mass = (test == trial) && (tm == sm) && (kind == 1);
-    const float area = tf[21];
-    float *Ab = A + (size_t)b * a_stride * 2;
-    float *rb = rhs + (size_t)b * rhs_stride * 2;
-    if (lane < 9) {
-        const int i = lane / 3, j = lane % 3;
-        float vre = 0.f, vim = 0.f;
-        // select accumulator lane-wise without dynamic register indexing
+    const int orig = g.sg_i[(size_t)6 * np + c];
+    if (orig < 0) return;
+    const int T = g.n_triangles;
+    const double scale = (double)F[(size_t)18 * np + c];
+    const float mc = F[(size_t)19 * np + c];
+    int rows[3], cols[3];
 #pragma unroll
-        for (int s = 0; s < 9; ++s)
-            if (s == lane) {
-                vre = dlp[s][0];
-                vim = dlp[s][1];
+    for (int a = 0; a < 3; ++a) {
+        rows[a] = g.sg_i[(size_t)a * np + c];
+        cols[a] = g.sg_i[(size_t)(3 + a) * np + c];
+    }
+#pragma unroll
+    for (int f = 0; f < kFB; ++f) {
+        const int b = b0 + f;
+        if (b >= n_batch) break;
+        float *Ab = A + (size_t)b * a_stride * 2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb) {
+                float vre = (float)(dd[f][3 * a + bb][0] * scale);
+                const float vim = (float)(dd[f][3 * a + bb][1] * scale);
+                if (mc != 0.0f) vre += mc * (rows[a] == cols[bb] ? (1.0f / 6.0f) : (1.0f / 12.0f));
+                const size_t idx = ((size_t)cols[bb] * lda + rows[a]) * 2;
+                atomicAdd(Ab + idx, vre);
+                atomicAdd(Ab + idx + 1, vim);
             }
-        vre *= jac;
-        vim *= jac;
-        if (mass) vre -= 0.5f * area * (i == j ? (1.0f / 6.0f) : (1.0f / 12.0f));
-        const int row = g.tri_i_aos[(size_t)test * kTriI + i];
-        const int col = g.tri_i_aos[(size_t)trial * kTriI + j];
-        const size_t idx = ((size_t)col * lda + row) * 2;
-        atomicAdd(Ab + idx, vre);
-        atomicAdd(Ab + idx + 1, vim);
-    } else if (lane < 12) {
-        const int i = lane - 9;
-        const int orig = g.tri_i_aos[(size_t)trial * kTriI + 19];
         const float qre = q[((size_t)b * T + orig) * 2];
         const float qim = q[((size_t)b * T + orig) * 2 + 1];
         if (qre != 0.0f || qim != 0.0f) {
-            float sre = 0.f, sim = 0.f;
+            float *rb = rhs + (size_t)b * rhs_stride * 2;
 #pragma unroll
-            for (int s = 0; s < 3; ++s)
-                if (s == i) {
-                    sre = slp[s][0];
-                    sim = slp[s][1];
-                }
-            sre *= jac;
-            sim *= jac;
-            const int row = g.tri_i_aos[(size_t)test * kTriI + i];
-            atomicAdd(rb + (size_t)row * 2, sre * qre - sim * qim);
-            atomicAdd(rb + (size_t)row * 2 + 1, sre * qim + sim * qre);
+            for (int a = 0; a < 3; ++a) {
+                const float sre = (float)(ds[f][a][0] * scale);
+                const float sim = (float)(ds[f][a][1] * scale);
+                atomicAdd(rb + (size_t)rows[a] * 2, sre * qre - sim * qim);
+                atomicAdd(rb + (size_t)rows[a] * 2 + 1, sre * qim + sim * qre);
+            }
         }
     }
 }
@@ -630,14 +657,18 @@ void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float 
                               const float *k_imag, const float *q, float *A, int64_t lda,
                               int64_t a_stride, float *rhs, int64_t rhs_stride,
                               hipStream_t stream) {
-    if (g.n_pairs == 0) return;
-    dim3 grid((g.n_pairs + 3) / 4, n_batch);
+    const int waves = g.sg_waves[0] + g.sg_waves[1] + g.sg_waves[2];
+    if (waves == 0 || n_batch <= 0) return;
+    constexpr int kFB = 2;
+    dim3 grid((waves + 3) / 4, (n_batch + kFB - 1) / kFB);
     if (k_imag != nullptr)
-        hipLaunchKernelGGL((k_assemble_singular<true>), grid, dim3(256), 0, stream, g, k_real,
-                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
+        hipLaunchKernelGGL((k_assemble_singular<true, kFB>), grid, dim3(256), 0, stream, g,
+                           g.sg_rule, n_batch, k_real, k_imag, q, A, lda, a_stride, rhs,
+                           rhs_stride);
     else
-        hipLaunchKernelGGL((k_assemble_singular<false>), grid, dim3(256), 0, stream, g, k_real,
-                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
+        hipLaunchKernelGGL((k_assemble_singular<false, kFB>), grid, dim3(256), 0, stream, g,
+                           g.sg_rule, n_batch, k_real, k_imag, q, A, lda, a_stride, rhs,
+                           rhs_stride);
 }
 
 void launch_field(const DeviceGeometry &g, int n_batch, const float *field_k, const float *p,

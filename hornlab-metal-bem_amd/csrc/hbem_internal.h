@@ -69,20 +69,46 @@ struct Plan {
     std::vector<float> tri_f_soa;    // [kTriF][T]
     std::vector<int32_t> tri_i_soa;  // [kTriI][T]
 
-    // Singular pairs sorted by kind (1, 2, 3).  Each pair is two ints:
-    //   pair_tt[2p] = test, pair_tt[2p+1] = trial
-    //   pair_code[p] = tm | sm<<3 | kind<<6 | tl1<<8 | tl2<<10 | sl1<<12 | sl2<<14
-    std::vector<int32_t> pair_tt;
-    std::vector<int32_t> pair_code;
+    // Counts of the full singular pair list (every touching (test image,
+    // trial image) combination), as the reference reports them.
     int64_t kind_count[3] = {0, 0, 0};
     int64_t image_pairs = 0;
     int64_t unique_slots = 0;
 
-    // Sauter-Schwab rule tables for kinds 1..3, concatenated: per point
-    // (test x, test y, trial x, trial y, weight).
-    std::vector<float> rule;         // [n_points][5]
-    int rule_offset[3] = {0, 0, 0};
-    int rule_count[3] = {0, 0, 0};
+    // Singular pair classes.  Two touching combinations (tm, sm) and
+    // (tm', sm') of one (test, trial) with tm^sm == tm'^sm', the same mask
+    // parity and the same shared locals differ by one isometry applied to
+    // both triangles, so their blocks are equal; each class is evaluated
+    // once with its size as weight.  Classes are sorted by kind and every
+    // kind is padded to a multiple of 64 (one lane per class, kind-uniform
+    // waves).  For class c the quadrature point of rule coordinates
+    // (x, y | x', y') has
+    //   r = Q - P = R0 + (x'-x) E1 + (y'-y) E2 + x' G1 + y' G2
+    // with P(x,y) = P0 + x E1 + y E2 the test image map (shared-vertex remap
+    // and odd-mask swap folded in), Q = Q0 + x' F1 + y' F2 the trial image
+    // map, R0 = Q0 - P0, G1 = F1 - E1, G2 = F2 - E2, all formed in double;
+    // R0, G1, G2 vanish on shared vertices and edges.  Rule-local basis a of
+    // the test (b of the trial) is the P1 function of the original local
+    // vertex that the map sends rule vertex a to.
+    // Float fields, [kSgF][sg_np]:
+    //   [0..2] R0  [3..5] E1  [6..8] E2  [9..11] G1  [12..14] G2
+    //   [15..17] trial image normal  [18] (2 A_t)(2 A_s) * weight
+    //   [19] mass coefficient -A_t/2 * weight on coincident self pairs, else 0
+    // Int fields, [kSgI][sg_np]:
+    //   [0..2] row dof of test basis a  [3..5] column dof of trial basis b
+    //   [6] trial original index (-1 on padding)
+    std::vector<float> sg_f;
+    std::vector<int32_t> sg_i;
+    int sg_np = 0;                 // padded class count
+    int sg_waves[3] = {0, 0, 0};   // 64-lane groups per kind
+    int64_t sg_classes = 0;        // unpadded class count
+
+    // Sauter-Schwab rules of kinds 1..3, concatenated; per point kSgRule
+    // floats: (x'-x, y'-y, x', y', w rho_a rho'_b (a, b = 0..2, 9 values),
+    // w rho_a (3 values)) with rho = (1-x-y, x, y), rho' = (1-x'-y', x', y').
+    std::vector<float> sg_rule;
+    int sg_rule_offset[3] = {0, 0, 0};  // in points
+    int sg_rule_count[3] = {0, 0, 0};
 
     std::vector<float> areas;        // [T]
     std::vector<int32_t> tags;       // [T]
@@ -102,9 +128,13 @@ std::string build_plan(int n_vertices, int n_triangles, int n_dofs, int symmetry
                        const int32_t *l2g, const float *areas, const float *normals_3xm,
                        const int32_t *tags, Plan *plan);
 
-// Sauter-Schwab tables of one kind (1D Gauss order 4).  Returns the number of
-// points written (rows of 5 floats).
-int duffy_rule_f32(int kind, std::vector<float> *out);
+constexpr int kSgF = 20;
+constexpr int kSgI = 7;
+constexpr int kSgRule = 16;
+
+// Sauter-Schwab points of one kind (1D Gauss order 4) in double: rows of
+// (x, y, x', y', w).  Returns the number of points written.
+int duffy_rule(int kind, std::vector<double> *out);
 
 // ---------------------------------------------------------------------------
 // Device-side launch wrappers (each enqueues on `stream`, no sync).
@@ -115,12 +145,13 @@ struct DeviceGeometry {
     const int32_t *tri_i_aos;
     const float *tri_f_soa;
     const int32_t *tri_i_soa;
-    const int32_t *pair_tt;
-    const int32_t *pair_code;
-    const float *rule;
-    int rule_offset[3];
-    int rule_count[3];
-    int n_pairs;
+    const float *sg_f;
+    const int32_t *sg_i;
+    const float *sg_rule;
+    int sg_np;
+    int sg_waves[3];
+    int sg_rule_offset[3];
+    int sg_rule_count[3];
     int n_triangles;
     int n_dofs;
     int symmetry;

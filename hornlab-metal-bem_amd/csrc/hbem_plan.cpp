@@ -54,6 +54,12 @@ int image_local(int local, int mask) {
     return local == 1 ? 2 : (local == 2 ? 1 : local);
 }
 
+// One touching (test image, trial image) combination.  tl*/sl* are the
+// image-local indices of the shared vertices (kinds 2 and 3).
+struct PairRec {
+    int test, trial, tm, sm, kind, tl1, tl2, sl1, sl2;
+};
+
 struct Shared {
     int count = 0, tl1 = 0, tl2 = 0, sl1 = 0, sl2 = 0;
     void add(int t, int s) {
@@ -70,7 +76,7 @@ struct Shared {
 
 }  // namespace
 
-int duffy_rule_f32(int kind, std::vector<float> *out) {
+int duffy_rule(int kind, std::vector<double> *out) {
     // Gauss-Legendre order 4 on [0, 1] (gaussRule1D4, main.swift:2626-2635).
     const double s65 = std::sqrt(6.0 / 5.0);
     const double x1 = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * s65) / 2.0;
@@ -82,11 +88,11 @@ int duffy_rule_f32(int kind, std::vector<float> *out) {
     int n = 0;
     auto put = [&](double a1, double a2, double b1, double b2, double w) {
         // (x - y, y) form of the region maps (appendDuffy, main.swift:2637-2650)
-        out->push_back(static_cast<float>(a1 - a2));
-        out->push_back(static_cast<float>(a2));
-        out->push_back(static_cast<float>(b1 - b2));
-        out->push_back(static_cast<float>(b2));
-        out->push_back(static_cast<float>(w));
+        out->push_back(a1 - a2);
+        out->push_back(a2);
+        out->push_back(b1 - b2);
+        out->push_back(b2);
+        out->push_back(w);
         ++n;
     };
     // Sauter-Schwab region maps (duffyRule, main.swift:2652-2699).
@@ -120,6 +126,173 @@ int duffy_rule_f32(int kind, std::vector<float> *out) {
                 }
     return n;
 }
+
+namespace {
+
+// Shared-vertex remap of a rule point onto the image-local reference
+// triangle (remapSingular, main.swift:2709-2735): kind 2 sends the rule's
+// vertices (0, 1, 2) to image-local (l1, l2, the third vertex); kind 3
+// moves the shared vertex l1 to the rule origin.
+void remap_point(double x, double y, int kind, int l1, int l2, double &ox, double &oy) {
+    if (kind == 2) {
+        const int vc = 3 - l1 - l2;
+        const double ax = (l1 == 1), ay = (l1 == 2);
+        const double bx = (l2 == 1), by = (l2 == 2);
+        const double cx = (vc == 1), cy = (vc == 2);
+        ox = ax + x * (bx - ax) + y * (cx - ax);
+        oy = ay + x * (by - ay) + y * (cy - ay);
+    } else if (kind == 3 && l1 == 1) {
+        ox = 1.0 - x - y;
+        oy = y;
+    } else if (kind == 3 && l1 == 2) {
+        ox = x;
+        oy = 1.0 - x - y;
+    } else {
+        ox = x;
+        oy = y;
+    }
+}
+
+// Affine map of rule coordinates onto one side's mirrored triangle:
+// point(x, y) = p0 + x e1 + y e2, and loc[a] = original local vertex at rule
+// vertex a.  Odd masks swap the image parameters (image local order
+// (0, 2, 1), main.swift:2118-2143).
+struct SideMap {
+    double p0[3], e1[3], e2[3];
+    int loc[3];
+};
+
+SideMap side_map(int nv, int T, const float *vtx, const int32_t *tri, int t, int mask, int kind,
+                 int l1, int l2) {
+    double v[3][3];
+    for (int l = 0; l < 3; ++l)
+        for (int c = 0; c < 3; ++c) {
+            const double x = vtx[c * nv + tri[l * T + t]];
+            v[l][c] = (mask & (1 << c)) ? -x : x;
+        }
+    const bool odd = __builtin_popcount(static_cast<unsigned>(mask)) & 1;
+    SideMap m;
+    double at[3][3];
+    const double rx[3] = {0.0, 1.0, 0.0}, ry[3] = {0.0, 0.0, 1.0};
+    for (int a = 0; a < 3; ++a) {
+        double u, w;
+        remap_point(rx[a], ry[a], kind, l1, l2, u, w);
+        if (odd) std::swap(u, w);
+        m.loc[a] = (u > 0.5) ? 1 : (w > 0.5 ? 2 : 0);
+        for (int c = 0; c < 3; ++c) at[a][c] = v[m.loc[a]][c];
+    }
+    for (int c = 0; c < 3; ++c) {
+        m.p0[c] = at[0][c];
+        m.e1[c] = at[1][c] - at[0][c];
+        m.e2[c] = at[2][c] - at[0][c];
+    }
+    return m;
+}
+
+// Groups the full singular pair list into classes (see Plan::sg_f) and
+// writes the per-class records and the per-point rule tables.
+void build_singular_classes(int nv, int T, const float *vtx, const int32_t *tri,
+                            const int32_t *l2g, const float *areas, const float *normals,
+                            const std::vector<PairRec> &pairs, Plan *P) {
+    struct Cls {
+        int rep;     // index into pairs
+        int weight;  // members
+    };
+    // class key: test, trial, tm^sm, parity(tm), kind, locals
+    std::unordered_map<uint64_t, int> index;
+    index.reserve(pairs.size());
+    std::vector<Cls> cls[3];
+    for (size_t i = 0; i < pairs.size(); ++i) {
+        const PairRec &p = pairs[i];
+        const uint64_t par = __builtin_popcount(static_cast<unsigned>(p.tm)) & 1;
+        const uint64_t key = (static_cast<uint64_t>(p.test) << 36) ^
+                             (static_cast<uint64_t>(p.trial) << 12) ^
+                             (static_cast<uint64_t>(p.tm ^ p.sm) << 9) ^ (par << 8) ^
+                             (static_cast<uint64_t>(p.tl1) << 6) ^ (static_cast<uint64_t>(p.tl2) << 4) ^
+                             (static_cast<uint64_t>(p.sl1) << 2) ^ static_cast<uint64_t>(p.sl2);
+        // the kind is a function of the locals and the pair, but keep classes
+        // per kind so that waves stay kind-uniform
+        const uint64_t full = key * 4 + static_cast<uint64_t>(p.kind);
+        auto it = index.find(full);
+        if (it == index.end()) {
+            index.emplace(full, static_cast<int>(cls[p.kind - 1].size()));
+            cls[p.kind - 1].push_back(Cls{static_cast<int>(i), 1});
+        } else {
+            cls[p.kind - 1][it->second].weight++;
+        }
+    }
+    int np = 0;
+    int64_t nclasses = 0;
+    for (int k = 0; k < 3; ++k) {
+        P->sg_waves[k] = static_cast<int>((cls[k].size() + 63) / 64);
+        np += 64 * P->sg_waves[k];
+        nclasses += static_cast<int64_t>(cls[k].size());
+    }
+    P->sg_np = np;
+    P->sg_classes = nclasses;
+    P->sg_f.assign(static_cast<size_t>(kSgF) * np, 0.0f);
+    P->sg_i.assign(static_cast<size_t>(kSgI) * np, 0);
+    for (int p = 0; p < np; ++p) P->sg_i[static_cast<size_t>(6) * np + p] = -1;
+    int base = 0;
+    for (int k = 0; k < 3; ++k) {
+        for (size_t c = 0; c < cls[k].size(); ++c) {
+            const PairRec &pr = pairs[cls[k][c].rep];
+            const int slot = base + static_cast<int>(c);
+            const SideMap tmap = side_map(nv, T, vtx, tri, pr.test, pr.tm, pr.kind, pr.tl1, pr.tl2);
+            const SideMap smap = side_map(nv, T, vtx, tri, pr.trial, pr.sm, pr.kind, pr.sl1, pr.sl2);
+            float *f = P->sg_f.data();
+            auto put = [&](int field, double v) { f[static_cast<size_t>(field) * np + slot] = static_cast<float>(v); };
+            for (int d = 0; d < 3; ++d) {
+                put(0 + d, smap.p0[d] - tmap.p0[d]);
+                put(3 + d, tmap.e1[d]);
+                put(6 + d, tmap.e2[d]);
+                put(9 + d, smap.e1[d] - tmap.e1[d]);
+                put(12 + d, smap.e2[d] - tmap.e2[d]);
+                const double nrm = normals[d * T + pr.trial];
+                put(15 + d, (pr.sm & (1 << d)) ? -nrm : nrm);
+            }
+            const double w = cls[k][c].weight;
+            put(18, 4.0 * static_cast<double>(areas[pr.test]) * static_cast<double>(areas[pr.trial]) * w);
+            const bool mass = pr.kind == 1 && pr.test == pr.trial && pr.tm == pr.sm;
+            put(19, mass ? -0.5 * static_cast<double>(areas[pr.test]) * w : 0.0);
+            int32_t *ii = P->sg_i.data();
+            for (int a = 0; a < 3; ++a) {
+                ii[static_cast<size_t>(a) * np + slot] = l2g[pr.test * 3 + tmap.loc[a]];
+                ii[static_cast<size_t>(3 + a) * np + slot] = l2g[pr.trial * 3 + smap.loc[a]];
+            }
+            ii[static_cast<size_t>(6) * np + slot] = pr.trial;
+        }
+        base += 64 * P->sg_waves[k];
+    }
+
+    P->sg_rule.clear();
+    int off = 0;
+    std::vector<double> pts;
+    for (int kind = 1; kind <= 3; ++kind) {
+        pts.clear();
+        const int cnt = duffy_rule(kind, &pts);
+        P->sg_rule_offset[kind - 1] = off;
+        P->sg_rule_count[kind - 1] = cnt;
+        off += cnt;
+        for (int i = 0; i < cnt; ++i) {
+            const double x = pts[5 * i], y = pts[5 * i + 1], xs = pts[5 * i + 2], ys = pts[5 * i + 3];
+            const double w = pts[5 * i + 4];
+            const double rho[3] = {1.0 - x - y, x, y}, rhos[3] = {1.0 - xs - ys, xs, ys};
+            float row[kSgRule];
+            row[0] = static_cast<float>(xs - x);
+            row[1] = static_cast<float>(ys - y);
+            row[2] = static_cast<float>(xs);
+            row[3] = static_cast<float>(ys);
+            for (int a = 0; a < 3; ++a) {
+                for (int b = 0; b < 3; ++b) row[4 + 3 * a + b] = static_cast<float>(w * rho[a] * rhos[b]);
+                row[13 + a] = static_cast<float>(w * rho[a]);
+            }
+            P->sg_rule.insert(P->sg_rule.end(), row, row + kSgRule);
+        }
+    }
+}
+
+}  // namespace
 
 std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, const int32_t *tri,
                        const int32_t *l2g, const float *areas, const float *normals,
@@ -302,9 +475,6 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
     }
 
     // Singular pair list.
-    struct PairRec {
-        int test, trial, code, kind;
-    };
     std::vector<PairRec> pairs;
     pairs.reserve(static_cast<size_t>(T) * 16 * M);
     auto append = [&](int test, int trial, int tm, int sm, Shared s) {
@@ -319,9 +489,7 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
         } else {
             return;
         }
-        const int code = tm | (sm << 3) | (kind << 6) | (s.tl1 << 8) | (s.tl2 << 10) |
-                         (s.sl1 << 12) | (s.sl2 << 14);
-        pairs.push_back(PairRec{test, trial, code, kind});
+        pairs.push_back(PairRec{test, trial, tm, sm, kind, s.tl1, s.tl2, s.sl1, s.sl2});
     };
     auto shared_by_index = [&](int test, int trial) {
         Shared s;
@@ -397,35 +565,19 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
                 }
             }
     }
-    std::stable_sort(pairs.begin(), pairs.end(),
-                     [](const PairRec &a, const PairRec &b) { return a.kind < b.kind; });
-    P->pair_tt.resize(pairs.size() * 2);
-    P->pair_code.resize(pairs.size());
     P->kind_count[0] = P->kind_count[1] = P->kind_count[2] = 0;
     P->image_pairs = 0;
     std::unordered_set<int64_t> slots;
     slots.reserve(pairs.size() * 4);
-    for (size_t p = 0; p < pairs.size(); ++p) {
-        P->pair_tt[2 * p] = inv[pairs[p].test];
-        P->pair_tt[2 * p + 1] = inv[pairs[p].trial];
-        P->pair_code[p] = pairs[p].code;
-        P->kind_count[pairs[p].kind - 1]++;
-        if ((pairs[p].code & 63) != 0) P->image_pairs++;
+    for (const PairRec &pr : pairs) {
+        P->kind_count[pr.kind - 1]++;
+        if (pr.tm != 0 || pr.sm != 0) P->image_pairs++;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
-                slots.insert(static_cast<int64_t>(l2g[pairs[p].test * 3 + i]) * n +
-                             l2g[pairs[p].trial * 3 + j]);
+                slots.insert(static_cast<int64_t>(l2g[pr.test * 3 + i]) * n + l2g[pr.trial * 3 + j]);
     }
     P->unique_slots = static_cast<int64_t>(slots.size());
-
-    P->rule.clear();
-    int off = 0;
-    for (int kind = 1; kind <= 3; ++kind) {
-        const int c = duffy_rule_f32(kind, &P->rule);
-        P->rule_offset[kind - 1] = off;
-        P->rule_count[kind - 1] = c;
-        off += c;
-    }
+    build_singular_classes(nv, T, vtx, tri, l2g, areas, normals, pairs, P);
     P->areas.assign(areas, areas + T);
     P->tags.assign(T, 0);
     if (tags) P->tags.assign(tags, tags + T);
