@@ -134,7 +134,7 @@ struct hbem_session {
     hbem::Plan plan;
     hbem::DeviceGeometry dg{};
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
-        tile_count, tile_ndofs, tile_dofs;
+        tile_count, tile_ndofs, tile_dofs, tile_pairs;
     // per-call workspaces
     DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, q, kre, kim, fk, obs, field, tags,
         tagout, norms, anorm, active, rowsplit, rcx;
@@ -146,7 +146,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag})
@@ -418,6 +418,7 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         upload(s->tile_count, P.tile_count, s->stream);
         upload(s->tile_ndofs, P.tile_ndofs, s->stream);
         upload(s->tile_dofs, P.tile_dofs, s->stream);
+        upload(s->tile_pairs, P.tile_pairs, s->stream);
         hbem::DeviceGeometry &g = s->dg;
         g.tri_f_aos = s->tri_f_aos.as<float>();
         g.tri_i_aos = s->tri_i_aos.as<int32_t>();
@@ -442,6 +443,8 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         g.tile_count = s->tile_count.as<int32_t>();
         g.tile_ndofs = s->tile_ndofs.as<int32_t>();
         g.tile_dofs = s->tile_dofs.as<int32_t>();
+        g.tile_pairs = s->tile_pairs.as<int32_t>();
+        g.n_tile_pairs = (int)(P.tile_pairs.size() / 2);
         g.n_tiles = (int)P.tile_start.size();
         g.max_tile_dofs = P.max_tile_dofs;
         hip_check(hipStreamSynchronize(s->stream), "session upload");

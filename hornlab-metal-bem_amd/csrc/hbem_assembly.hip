@@ -71,40 +71,47 @@ __device__ __forceinline__ float mask_sign(int mask, int bit) { return (mask & b
 
 // ---------------------------------------------------------------------------
 // Regular pairs.
-// One block = one (test tile, trial tile) pair of up to 64 x 64 triangles
-// for one frequency (grid.z).  Lanes own the trial triangles of the tile
-// (records in registers); each wave walks 16 test triangles whose records
-// are wave-uniform (scalar loads).  The 3x3 blocks are summed in an LDS
-// image of the tile's (rows x cols) sub-matrix with LDS float atomics, and
-// the block ends with one global atomic add per touched matrix entry.
-// Mirror signs are compile-time constants of the symmetry code.
+// One block = one unordered pair {T, S} of tiles of up to 64 triangles, for
+// one frequency (grid.y).  Lanes own the triangles s of S (records in
+// registers); each wave walks 16 triangles t of T, whose records are
+// wave-uniform (scalar loads).  The quadrature kernel is reciprocal: for an
+// image mask m the point sets of (test t, trial s^m) and (test s, trial t^m)
+// are isometric, |S_m y_b - x_a| = |y_b - S_m x_a|, so one evaluation of
+// r, 1/r, sin and cos at (a, b) serves both blocks:
+//   (t, s^m):  d = y_b - S_m x_a,  proj =  d . n_s
+//   (s, t^m):                      proj = -d . (S_m n_t)
+// (S_m is its own inverse and orthogonal).  The two 3x3 blocks are summed in
+// two LDS images of the tiles' (rows x cols) sub-matrices with LDS float
+// atomics and leave with one global atomic add per touched entry.  On a
+// diagonal block (T == S) lane s does the pair {t, s} only for s >= t, the
+// second direction only for s > t.  Images run in a runtime loop (mirror
+// signs applied to the uniform test points).
 // ---------------------------------------------------------------------------
 
 constexpr int kTestsPerWave = kTileTris / 4;
 
-constexpr int masks_of(int sym) { return sym == 0 ? 1 : (sym == 3 ? 4 : 2); }
-constexpr int mask_at(int sym, int m) { return m == 0 ? 0 : (sym == 3 ? m : sym); }
-constexpr float sgn(int mask, int bit) { return (mask & bit) ? -1.0f : 1.0f; }
-
-// Accumulates the DLP (and, for kSource, SLP) of one (test, trial image)
-// combination over the 6 x 6 rule.  dlp[i][j] holds -4 pi * DLP / jac and
-// slp[i] holds 4 pi * SLP / jac (constants folded into the weights).
-template <int kMask, bool kComplexK, bool kSource>
-__device__ __forceinline__ void regular_image(const float *__restrict__ tf, const float (&yp)[18],
-                                              float nx, float ny, float nz, float k, float krev,
-                                              float kim, float (&dlp)[9][2], float (&slp)[3][2]) {
-    constexpr float sx = sgn(kMask, 1), sy = sgn(kMask, 2), sz = sgn(kMask, 4);
-    const float mnx = sx * nx, mny = sy * ny, mnz = sz * nz;
+// Both directions of one (t, s, image) combination over the 6 x 6 rule.
+// dF[i][j] (test basis i on t, trial basis j on s) and dR[i][j] (test basis
+// i on s, trial basis j on t) hold -4 pi DLP / jac; sF[i] (basis on t) and
+// sR[i] (basis on s) hold 4 pi SLP / jac with the source's unit density.
+template <bool kComplexK, bool kSource>
+__device__ __forceinline__ void regular_both(const float (&xa_m)[18], float ntx, float nty,
+                                             float ntz, const float (&yp)[18], float nsx,
+                                             float nsy, float nsz, float k, float krev,
+                                             float kim, float (&dF)[9][2], float (&dR)[9][2],
+                                             float (&sF)[3][2], float (&sR)[3][2]) {
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
-        const float xa = tf[3 * a], ya = tf[3 * a + 1], za = tf[3 * a + 2];
-        float e[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
-        float gs[2] = {0.f, 0.f};
+        const float xa = xa_m[3 * a], ya = xa_m[3 * a + 1], za = xa_m[3 * a + 2];
+        float eF[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+        float eR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+        float gF[2] = {0.f, 0.f};
+        float gR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
 #pragma unroll
         for (int bq = 0; bq < 6; ++bq) {
-            const float dx = sx * yp[3 * bq] - xa;
-            const float dy = sy * yp[3 * bq + 1] - ya;
-            const float dz = sz * yp[3 * bq + 2] - za;
+            const float dx = yp[3 * bq] - xa;
+            const float dy = yp[3 * bq + 1] - ya;
+            const float dz = yp[3 * bq + 2] - za;
             const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), 1.0e-14f);
             const float rinv = __builtin_amdgcn_rsqf(r2);
             const float r = r2 * rinv;
@@ -112,28 +119,47 @@ __device__ __forceinline__ void regular_image(const float *__restrict__ tf, cons
             t = t - __builtin_rintf(t);
             const float s = __builtin_amdgcn_sinf(t);
             const float c = __builtin_amdgcn_cosf(t);
+            float q2 = rinv * rinv;  // 1/r^2 (times the damping)
             float damp = 1.0f;
-            if (kComplexK) damp = __expf(-kim * r);
+            if (kComplexK) {
+                damp = __expf(-kim * r);
+                q2 *= damp;
+            }
             const float fr = kComplexK ? rinv + kim : rinv;
-            // u = (r . n) / r^2 (times the damping); dG/dn = -u G0 (c fr + s k, s fr - c k)
-            const float u = fmaf(dx, mnx, fmaf(dy, mny, dz * mnz)) * (rinv * rinv) * damp;
-            const float cfr = c * fr;
-            const float t1 = fmaf(s, k, cfr) * u;
-            const float t2 = fmaf(-c, k, s * fr) * u;
+            // -4 pi dG/dn = proj/r^2 * (c fr + s k, s fr - c k) e^{-kim r}
+            const float p1 = fmaf(s, k, c * fr);
+            const float p2 = fmaf(-c, k, s * fr);
+            const float uF = fmaf(dx, nsx, fmaf(dy, nsy, dz * nsz)) * q2;
+            const float uR = -fmaf(dx, ntx, fmaf(dy, nty, dz * ntz)) * q2;
             const float wb = (float)kRule6W[bq];
             const float w1 = wb * (float)kRule6X[bq];
             const float w2 = wb * (float)kRule6Y[bq];
             const float w0 = wb - w1 - w2;
-            e[0][0] = fmaf(t1, w0, e[0][0]);
-            e[0][1] = fmaf(t2, w0, e[0][1]);
-            e[1][0] = fmaf(t1, w1, e[1][0]);
-            e[1][1] = fmaf(t2, w1, e[1][1]);
-            e[2][0] = fmaf(t1, w2, e[2][0]);
-            e[2][1] = fmaf(t2, w2, e[2][1]);
+            const float f1 = p1 * uF, f2 = p2 * uF;
+            const float r1 = p1 * uR, r2v = p2 * uR;
+            eF[0][0] = fmaf(f1, w0, eF[0][0]);
+            eF[0][1] = fmaf(f2, w0, eF[0][1]);
+            eF[1][0] = fmaf(f1, w1, eF[1][0]);
+            eF[1][1] = fmaf(f2, w1, eF[1][1]);
+            eF[2][0] = fmaf(f1, w2, eF[2][0]);
+            eF[2][1] = fmaf(f2, w2, eF[2][1]);
+            eR[0][0] = fmaf(r1, w0, eR[0][0]);
+            eR[0][1] = fmaf(r2v, w0, eR[0][1]);
+            eR[1][0] = fmaf(r1, w1, eR[1][0]);
+            eR[1][1] = fmaf(r2v, w1, eR[1][1]);
+            eR[2][0] = fmaf(r1, w2, eR[2][0]);
+            eR[2][1] = fmaf(r2v, w2, eR[2][1]);
             if (kSource) {
-                const float g = rinv * damp * wb;
-                gs[0] = fmaf(c, g, gs[0]);
-                gs[1] = fmaf(s, g, gs[1]);
+                const float g = rinv * damp;
+                const float gc = c * g, gs = s * g;
+                gF[0] = fmaf(gc, wb, gF[0]);
+                gF[1] = fmaf(gs, wb, gF[1]);
+                gR[0][0] = fmaf(gc, w0, gR[0][0]);
+                gR[0][1] = fmaf(gs, w0, gR[0][1]);
+                gR[1][0] = fmaf(gc, w1, gR[1][0]);
+                gR[1][1] = fmaf(gs, w1, gR[1][1]);
+                gR[2][0] = fmaf(gc, w2, gR[2][0]);
+                gR[2][1] = fmaf(gs, w2, gR[2][1]);
             }
         }
         const float wa = (float)kRule6W[a];
@@ -143,68 +169,69 @@ __device__ __forceinline__ void regular_image(const float *__restrict__ tf, cons
         for (int i = 0; i < 3; ++i) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                dlp[3 * i + j][0] = fmaf(ta[i], e[j][0], dlp[3 * i + j][0]);
-                dlp[3 * i + j][1] = fmaf(ta[i], e[j][1], dlp[3 * i + j][1]);
+                dF[3 * i + j][0] = fmaf(ta[i], eF[j][0], dF[3 * i + j][0]);
+                dF[3 * i + j][1] = fmaf(ta[i], eF[j][1], dF[3 * i + j][1]);
+                dR[3 * i + j][0] = fmaf(ta[j], eR[i][0], dR[3 * i + j][0]);
+                dR[3 * i + j][1] = fmaf(ta[j], eR[i][1], dR[3 * i + j][1]);
             }
             if (kSource) {
-                slp[i][0] = fmaf(ta[i], gs[0], slp[i][0]);
-                slp[i][1] = fmaf(ta[i], gs[1], slp[i][1]);
+                sF[i][0] = fmaf(ta[i], gF[0], sF[i][0]);
+                sF[i][1] = fmaf(ta[i], gF[1], sF[i][1]);
+                sR[i][0] = fmaf(wa, gR[i][0], sR[i][0]);
+                sR[i][1] = fmaf(wa, gR[i][1], sR[i][1]);
             }
         }
     }
 }
 
-// True when test and the trial's image share a vertex: image 0 by vertex
-// index, images by coordinate-key id (the singular-pair criteria).
-template <int kM>
-__device__ __forceinline__ bool touching(const int32_t *__restrict__ tii, const int (&svid)[3],
-                                         const int (&skid)[3]) {
-    bool touch = false;
+template <bool kComplexK, bool kSource>
+__device__ __forceinline__ void regular_images(const float *__restrict__ tf, int n_masks,
+                                               int packed_masks, unsigned skip,
+                                               const float (&yp)[18], float nsx, float nsy,
+                                               float nsz, float k, float krev, float kim,
+                                               float (&dF)[9][2], float (&dR)[9][2],
+                                               float (&sF)[3][2], float (&sR)[3][2]) {
+    for (int m = 0; m < n_masks; ++m) {
+        if ((skip >> m) & 1u) continue;
+        const int mask = (packed_masks >> (3 * m)) & 7;
+        const float sx = (mask & 1) ? -1.0f : 1.0f;
+        const float sy = (mask & 2) ? -1.0f : 1.0f;
+        const float sz = (mask & 4) ? -1.0f : 1.0f;
+        float xm[18];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) touch |= (kM == 0) ? (tii[3 + i] == svid[j]) : (tii[6 + i] == skid[j]);
-    return touch;
-}
-
-template <int SYM, bool kComplexK, bool kSource>
-__device__ __forceinline__ void regular_pair(const float *__restrict__ tf,
-                                             const int32_t *__restrict__ tii, const float (&yp)[18],
-                                             float nx, float ny, float nz, const int (&svid)[3],
-                                             const int (&skid)[kMaxMasks][3], float k, float krev,
-                                             float kim, float (&dlp)[9][2], float (&slp)[3][2]) {
-    if (!touching<0>(tii, svid, skid[0]))
-        regular_image<0, kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
-    if constexpr (masks_of(SYM) > 1) {
-        if (!touching<1>(tii, svid, skid[1]))
-            regular_image<mask_at(SYM, 1), kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
-    }
-    if constexpr (masks_of(SYM) > 2) {
-        if (!touching<2>(tii, svid, skid[2]))
-            regular_image<mask_at(SYM, 2), kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
-        if (!touching<3>(tii, svid, skid[3]))
-            regular_image<mask_at(SYM, 3), kComplexK, kSource>(tf, yp, nx, ny, nz, k, krev, kim, dlp, slp);
+        for (int a = 0; a < 6; ++a) {
+            xm[3 * a] = sx * tf[3 * a];
+            xm[3 * a + 1] = sy * tf[3 * a + 1];
+            xm[3 * a + 2] = sz * tf[3 * a + 2];
+        }
+        regular_both<kComplexK, kSource>(xm, sx * tf[18], sy * tf[19], sz * tf[20], yp, nsx, nsy,
+                                         nsz, k, krev, kim, dF, dR, sF, sR);
     }
 }
 
-template <int SYM, bool kComplexK>
+template <bool kComplexK>
 __global__ __launch_bounds__(256) void k_assemble_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, float *__restrict__ A, int64_t lda, int64_t a_stride,
     float *__restrict__ rhs, int64_t rhs_stride) {
-    constexpr int M = masks_of(SYM);
-    extern __shared__ float acc[];  // [trial dofs][test dofs][2]
+    extern __shared__ float acc[];  // F: [S dofs][T dofs][2], then R: [T dofs][S dofs][2]
     const int T = g.n_triangles;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int s_tile = blockIdx.x, t_tile = blockIdx.y, b = blockIdx.z;
+    const int s_tile = g.tile_pairs[2 * blockIdx.x], t_tile = g.tile_pairs[2 * blockIdx.x + 1];
+    const int b = blockIdx.y;
+    const bool diag = s_tile == t_tile;
     const int s0 = g.tile_start[s_tile], scount = g.tile_count[s_tile], snd = g.tile_ndofs[s_tile];
     const int t0 = g.tile_start[t_tile], tcount = g.tile_count[t_tile], tnd = g.tile_ndofs[t_tile];
-    for (int e = threadIdx.x; e < 2 * snd * tnd; e += 256) acc[e] = 0.0f;
+    float *accF = acc;
+    float *accR = acc + 2 * snd * tnd;
+    for (int e = threadIdx.x; e < 4 * snd * tnd; e += 256) acc[e] = 0.0f;
 
     const float k = k_real[b];
     const float kim = kComplexK ? k_imag[b] : 0.0f;
     const float krev = k * kInv2Pi;
+    const int M = g.n_masks;
+    const int packed_masks = g.masks[0] | (g.masks[1] << 3) | (g.masks[2] << 6) | (g.masks[3] << 9);
 
     const bool trial_ok = lane < scount;
     const int tr = s0 + (trial_ok ? lane : 0);
@@ -215,11 +242,12 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     const float ny = g.tri_f_soa[(size_t)19 * T + tr];
     const float nz = g.tri_f_soa[(size_t)20 * T + tr];
     const float jac_s = g.tri_f_soa[(size_t)22 * T + tr];
-    int svid[3], skid[kMaxMasks][3], slc[3];
+    int svid[3], skid[kMaxMasks][3], slc[3], sdof[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
         svid[l] = g.tri_i_soa[(size_t)(3 + l) * T + tr];
         slc[l] = g.tri_i_soa[(size_t)(20 + l) * T + tr];
+        sdof[l] = g.tri_i_soa[(size_t)l * T + tr];
     }
 #pragma unroll
     for (int m = 0; m < kMaxMasks; ++m)
@@ -233,6 +261,7 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     float *rb = rhs + (size_t)b * rhs_stride * 2;
     // -4 pi and 4 pi folded out of the kernels, jac = (2 A_test)(2 A_trial)
     const float dscale = -kInv4Pi * jac_s * g.row_weight;
+    const float gscale = kInv4Pi * jac_s * g.row_weight;
     __syncthreads();
 
     for (int ti = 0; ti < kTestsPerWave; ++ti) {
@@ -242,58 +271,110 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         const float *tf = g.tri_f_aos + (size_t)test * kTriF;
         const int32_t *tii = g.tri_i_aos + (size_t)test * kTriI;
         const float jt = tf[22];
-        float dlp[9][2], slp[3][2];
+        const int torig = tii[19];
+        const float tqre = q[((size_t)b * T + torig) * 2];
+        const float tqim = q[((size_t)b * T + torig) * 2 + 1];
+        const bool test_source = (tqre != 0.0f || tqim != 0.0f);
+        // lanes and directions of this (t, s) pair
+        const bool fwd = trial_ok && (!diag || lane >= lt);
+        const bool rev = trial_ok && (!diag || lane > lt);
+        // skip the singular combinations: t touches s^m (image 0 by vertex
+        // index, images by coordinate key of t against the keys of s under
+        // mask m); s touches t^m exactly when t touches s^m
+        unsigned skip = fwd ? 0u : 0xfu;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) dlp[i][0] = dlp[i][1] = 0.0f;
+        for (int m = 0; m < kMaxMasks; ++m) {
+            bool touch = false;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) slp[i][0] = slp[i][1] = 0.0f;
-        if (trial_ok) {
-            if (wave_has_source)
-                regular_pair<SYM, kComplexK, true>(tf, tii, yp, nx, ny, nz, svid, skid, k, krev, kim,
-                                                   dlp, slp);
-            else
-                regular_pair<SYM, kComplexK, false>(tf, tii, yp, nx, ny, nz, svid, skid, k, krev,
-                                                    kim, dlp, slp);
-            const float sc = dscale * jt;
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    touch |= (m == 0) ? (tii[3 + i] == svid[j]) : (tii[6 + i] == skid[m][j]);
+            if (m < M && touch) skip |= 1u << m;
+        }
+        float dF[9][2], dR[9][2], sF[3][2], sR[3][2];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dF[i][0] = dF[i][1] = dR[i][0] = dR[i][1] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sF[i][0] = sF[i][1] = sR[i][0] = sR[i][1] = 0.0f;
+        const bool src = wave_has_source || test_source;
+        if (src)
+            regular_images<kComplexK, true>(tf, M, packed_masks, skip, yp, nx, ny, nz, k, krev, kim, dF,
+                                            dR, sF, sR);
+        else
+            regular_images<kComplexK, false>(tf, M, packed_masks, skip, yp, nx, ny, nz, k, krev, kim, dF,
+                                             dR, sF, sR);
+        const float sc = dscale * jt;
+        if (fwd) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const int lr = tii[20 + i];
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    float *cell = acc + 2 * (slc[j] * tnd + lr);
-                    atomicAdd(cell, dlp[3 * i + j][0] * sc);
-                    atomicAdd(cell + 1, dlp[3 * i + j][1] * sc);
+                    float *cell = accF + 2 * (slc[j] * tnd + lr);
+                    atomicAdd(cell, dF[3 * i + j][0] * sc);
+                    atomicAdd(cell + 1, dF[3 * i + j][1] * sc);
                 }
             }
         }
-        if (wave_has_source) {
-            // rhs_i += 2^planes * SLP_i * q_trial, reduced over the wave
-            const float gsc = kInv4Pi * jac_s * jt * g.row_weight;
+        if (rev) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                float cre = (slp[i][0] * qre - slp[i][1] * qim) * gsc;
-                float cim = (slp[i][0] * qim + slp[i][1] * qre) * gsc;
+            for (int j = 0; j < 3; ++j) {
+                const int lc = tii[20 + j];
 #pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) {
-                    cre += __shfl_xor(cre, off);
-                    cim += __shfl_xor(cim, off);
+                for (int i = 0; i < 3; ++i) {
+                    float *cell = accR + 2 * (lc * snd + slc[i]);
+                    atomicAdd(cell, dR[3 * i + j][0] * sc);
+                    atomicAdd(cell + 1, dR[3 * i + j][1] * sc);
                 }
-                if (lane == 0) {
-                    atomicAdd(rb + (size_t)tii[i] * 2, cre);
-                    atomicAdd(rb + (size_t)tii[i] * 2 + 1, cim);
+            }
+        }
+        if (src) {
+            const float gsc = gscale * jt;
+            if (wave_has_source) {
+                // rhs_t += 2^planes SLP(t, s) q_s, reduced over the wave
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    float cre = fwd ? (sF[i][0] * qre - sF[i][1] * qim) * gsc : 0.0f;
+                    float cim = fwd ? (sF[i][0] * qim + sF[i][1] * qre) * gsc : 0.0f;
+#pragma unroll
+                    for (int off = 32; off >= 1; off >>= 1) {
+                        cre += __shfl_xor(cre, off);
+                        cim += __shfl_xor(cim, off);
+                    }
+                    if (lane == 0) {
+                        atomicAdd(rb + (size_t)tii[i] * 2, cre);
+                        atomicAdd(rb + (size_t)tii[i] * 2 + 1, cim);
+                    }
+                }
+            }
+            if (test_source && rev) {
+                // rhs_s += 2^planes SLP(s, t) q_t
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    atomicAdd(rb + (size_t)sdof[i] * 2, (sR[i][0] * tqre - sR[i][1] * tqim) * gsc);
+                    atomicAdd(rb + (size_t)sdof[i] * 2 + 1, (sR[i][0] * tqim + sR[i][1] * tqre) * gsc);
                 }
             }
         }
     }
     __syncthreads();
     float *Ab = A + (size_t)b * a_stride * 2;
-    const int32_t *rows = g.tile_dofs + (size_t)t_tile * kTileMaxDofs;
-    const int32_t *cols = g.tile_dofs + (size_t)s_tile * kTileMaxDofs;
+    const int32_t *tdofs = g.tile_dofs + (size_t)t_tile * kTileMaxDofs;
+    const int32_t *sdofs = g.tile_dofs + (size_t)s_tile * kTileMaxDofs;
     for (int e = threadIdx.x; e < snd * tnd; e += 256) {
-        const float vre = acc[2 * e], vim = acc[2 * e + 1];
+        const float vre = accF[2 * e], vim = accF[2 * e + 1];
         if (vre == 0.0f && vim == 0.0f) continue;
         const int lc = e / tnd, lr = e - lc * tnd;
-        const size_t idx = ((size_t)cols[lc] * lda + rows[lr]) * 2;
+        const size_t idx = ((size_t)sdofs[lc] * lda + tdofs[lr]) * 2;
+        atomicAdd(Ab + idx, vre);
+        atomicAdd(Ab + idx + 1, vim);
+    }
+    for (int e = threadIdx.x; e < snd * tnd; e += 256) {
+        const float vre = accR[2 * e], vim = accR[2 * e + 1];
+        if (vre == 0.0f && vim == 0.0f) continue;
+        const int lc = e / snd, lr = e - lc * snd;
+        const size_t idx = ((size_t)tdofs[lc] * lda + sdofs[lr]) * 2;
         atomicAdd(Ab + idx, vre);
         atomicAdd(Ab + idx + 1, vim);
     }
@@ -637,20 +718,16 @@ void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *
                              const float *k_imag, const float *q, float *A, int64_t lda,
                              int64_t a_stride, float *rhs, int64_t rhs_stride,
                              hipStream_t stream) {
-    dim3 grid(g.n_tiles, g.n_tiles, n_batch);
-    const size_t shm = sizeof(float) * 2 * (size_t)g.max_tile_dofs * g.max_tile_dofs;
-    const bool ck = k_imag != nullptr;
-#define HBEM_REG(S, CK)                                                                       \
-    hipLaunchKernelGGL((k_assemble_regular<S, CK>), grid, dim3(256), shm, stream, g, k_real, \
-                       k_imag, q, A, lda, a_stride, rhs, rhs_stride)
-    switch (g.symmetry) {
-    case 0: if (ck) HBEM_REG(0, true); else HBEM_REG(0, false); break;
-    case 1: if (ck) HBEM_REG(1, true); else HBEM_REG(1, false); break;
-    case 2: if (ck) HBEM_REG(2, true); else HBEM_REG(2, false); break;
-    case 3: if (ck) HBEM_REG(3, true); else HBEM_REG(3, false); break;
-    default: if (ck) HBEM_REG(4, true); else HBEM_REG(4, false); break;
-    }
-#undef HBEM_REG
+    if (g.n_tile_pairs == 0 || n_batch <= 0) return;
+    dim3 grid(g.n_tile_pairs, n_batch);
+    // two tables of (max tile dofs)^2 complex floats
+    const size_t shm = sizeof(float) * 4 * (size_t)g.max_tile_dofs * g.max_tile_dofs;
+    if (k_imag != nullptr)
+        hipLaunchKernelGGL((k_assemble_regular<true>), grid, dim3(256), shm, stream, g, k_real,
+                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
+    else
+        hipLaunchKernelGGL((k_assemble_regular<false>), grid, dim3(256), shm, stream, g, k_real,
+                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
 }
 
 void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float *k_real,

@@ -50,10 +50,11 @@ constexpr int kTriI = 24;
 
 // Triangles are stored in Morton order of their centroids and grouped into
 // tiles of up to kTileTris consecutive triangles with at most kTileMaxDofs
-// distinct dofs; the regular-assembly kernel accumulates one tile x tile
-// block of the matrix in LDS.
+// distinct dofs; the regular-assembly kernel accumulates the two tile x
+// tile blocks of a tile pair in LDS (2 * 90^2 complex floats = 129600 B at
+// most).
 constexpr int kTileTris = 64;
-constexpr int kTileMaxDofs = 90;  // 90^2 complex floats = 64800 B of LDS
+constexpr int kTileMaxDofs = 90;
 
 struct Plan {
     int n_vertices = 0;
@@ -119,6 +120,7 @@ struct Plan {
     std::vector<int32_t> tile_count;   // [n_tiles]
     std::vector<int32_t> tile_ndofs;   // [n_tiles]
     std::vector<int32_t> tile_dofs;    // [n_tiles][kTileMaxDofs], -1 padded
+    std::vector<int32_t> tile_pairs;   // [n_tile_pairs][2] = (S, T), S >= T
     int max_tile_dofs = 0;
 };
 
@@ -162,7 +164,9 @@ struct DeviceGeometry {
     const int32_t *tile_count;
     const int32_t *tile_ndofs;
     const int32_t *tile_dofs;
+    const int32_t *tile_pairs;
     int n_tiles;
+    int n_tile_pairs;
     int max_tile_dofs;
 };
 

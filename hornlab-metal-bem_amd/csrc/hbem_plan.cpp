@@ -198,21 +198,25 @@ void build_singular_classes(int nv, int T, const float *vtx, const int32_t *tri,
         int rep;     // index into pairs
         int weight;  // members
     };
-    // class key: test, trial, tm^sm, parity(tm), kind, locals
+    // Class key: (test, trial, tm^sm, kind) and, for coincident and edge
+    // pairs, the mask parity and the shared locals.  The vertex rule gives
+    // the same block under the odd-mask swap of both triangles (its point
+    // set maps onto itself); the coincident and edge rules do not, so their
+    // classes keep one representative per parity.  Triangle indices fit 24
+    // bits (build_plan rejects larger meshes).
     std::unordered_map<uint64_t, int> index;
     index.reserve(pairs.size());
     std::vector<Cls> cls[3];
     for (size_t i = 0; i < pairs.size(); ++i) {
         const PairRec &p = pairs[i];
-        const uint64_t par = __builtin_popcount(static_cast<unsigned>(p.tm)) & 1;
-        const uint64_t key = (static_cast<uint64_t>(p.test) << 36) ^
-                             (static_cast<uint64_t>(p.trial) << 12) ^
-                             (static_cast<uint64_t>(p.tm ^ p.sm) << 9) ^ (par << 8) ^
-                             (static_cast<uint64_t>(p.tl1) << 6) ^ (static_cast<uint64_t>(p.tl2) << 4) ^
-                             (static_cast<uint64_t>(p.sl1) << 2) ^ static_cast<uint64_t>(p.sl2);
-        // the kind is a function of the locals and the pair, but keep classes
-        // per kind so that waves stay kind-uniform
-        const uint64_t full = key * 4 + static_cast<uint64_t>(p.kind);
+        uint64_t key = (static_cast<uint64_t>(p.test) << 38) | (static_cast<uint64_t>(p.trial) << 14) |
+                       (static_cast<uint64_t>(p.tm ^ p.sm) << 11) | (static_cast<uint64_t>(p.kind) << 9);
+        if (p.kind != 3) {
+            const uint64_t par = __builtin_popcount(static_cast<unsigned>(p.tm)) & 1;
+            key |= (par << 8) | (static_cast<uint64_t>(p.tl1) << 6) | (static_cast<uint64_t>(p.tl2) << 4) |
+                   (static_cast<uint64_t>(p.sl1) << 2) | static_cast<uint64_t>(p.sl2);
+        }
+        const uint64_t full = key;
         auto it = index.find(full);
         if (it == index.end()) {
             index.emplace(full, static_cast<int>(cls[p.kind - 1].size()));
@@ -298,6 +302,7 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
                        const int32_t *l2g, const float *areas, const float *normals,
                        const int32_t *tags, Plan *P) {
     if (nv <= 0 || T <= 0 || n <= 0) return "geometry must have vertices, triangles and dofs";
+    if (T >= (1 << 24)) return "at most 16777215 triangles are supported";
     if (!(symmetry == 0 || symmetry == 1 || symmetry == 2 || symmetry == 3 || symmetry == 4))
         return "symmetry code must be 0, 1, 2, 3 or 4";
     P->n_vertices = nv;
@@ -431,6 +436,13 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
             P->tile_dofs.insert(P->tile_dofs.end(), dofs.begin(), dofs.end());
         }
         P->max_tile_dofs = *std::max_element(P->tile_ndofs.begin(), P->tile_ndofs.end());
+        const int nt = static_cast<int>(P->tile_start.size());
+        P->tile_pairs.clear();
+        for (int t = 0; t < nt; ++t)
+            for (int s = t; s < nt; ++s) {
+                P->tile_pairs.push_back(s);
+                P->tile_pairs.push_back(t);
+            }
     }
 
     // Triangle records, in device order.

@@ -308,3 +308,40 @@ def test_near_field_subdivision_moves_toward_fine_reference():
     s3, d3 = O.pair_block(g, 0, 1, mode=2, order=3, k=2.0)
     assert rel_l2(s1, s3) < rel_l2(s0, s3)
     assert rel_l2(d1, d3) < rel_l2(d0, d3)
+
+
+def test_singular_mirror_classes_are_invariant():
+    """The GPU evaluates each class of touching combinations once: members
+    (tm, sm) with equal tm^sm, equal locals and, for coincident and edge
+    pairs, equal mask parity give the same block (both triangles moved by one
+    isometry); vertex pairs are parity-free.  Coincident and edge pairs of
+    opposite parity differ at quadrature level, which shows the check can
+    fail.  FP64 oracle blocks, quarter horn."""
+    buf = horn_buffers(edge_length=0.03)
+    g = O.Geometry.from_buffers(buf, "yz+xz")
+    pairs, _ = O.duffy_pairs(g)
+    k = 150.0
+    same = {1: 0.0, 2: 0.0, 3: 0.0}
+    cross = {1: 0.0, 2: 0.0}
+    rng = np.random.default_rng(3)
+    for idx in rng.choice(len(pairs), 1500, replace=False):
+        t, s, tm, sm, kind = (int(v) for v in pairs[idx, :5])
+        if tm == 0:
+            continue
+        odd = bin(tm).count("1") % 2
+        # same-parity partner: tm' = tm ^ 3 keeps the parity, tm' = 0 flips odd ones
+        partners = [(tm ^ 3, sm ^ 3)] if odd else [(0, tm ^ sm)]
+        if not odd or kind == 3:
+            partners.append((0, tm ^ sm))
+        a_slp, a_dlp = O.pair_block(g, t, s, tm, sm, mode=1, k=k)
+        scale = np.abs(a_slp).max() + np.abs(a_dlp).max()
+        for ptm, psm in partners:
+            b_slp, b_dlp = O.pair_block(g, t, s, ptm, psm, mode=1, k=k)
+            e = max(np.abs(a_slp - b_slp).max(), np.abs(a_dlp - b_dlp).max()) / scale
+            same[kind] = max(same[kind], e)
+        if odd and kind != 3:
+            b_slp, b_dlp = O.pair_block(g, t, s, 0, tm ^ sm, mode=1, k=k)
+            e = max(np.abs(a_slp - b_slp).max(), np.abs(a_dlp - b_dlp).max()) / scale
+            cross[kind] = max(cross[kind], e)
+    assert max(same.values()) < 1e-10, same
+    assert cross[1] > 1e-8 and cross[2] > 1e-8, cross
