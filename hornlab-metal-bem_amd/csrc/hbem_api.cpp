@@ -136,7 +136,7 @@ struct hbem_session {
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
         tile_count, tile_ndofs, tile_dofs, tile_pairs;
     // per-call workspaces
-    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, q, kre, kim, fk, obs, field, tags,
+    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, q, kre, kim, fk, obs, field, tags,
         tagout, norms, anorm, active, rowsplit, rcx;
     // staging of host-input batches
     DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag;
@@ -146,7 +146,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag})
@@ -546,8 +546,9 @@ void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_optio
     }
     s->ipiv.reserve(sizeof(int32_t) * (size_t)B * N);
     s->info.reserve(sizeof(int32_t) * B);
+    s->gl.reserve(sizeof(int32_t) * (size_t)B * hbem::lu_gather_ints(N));
     hbem::LuWorkspace w{s->A.as<float>(), lda, a_stride, N, n, B, s->ipiv.as<int32_t>(),
-                        s->info.as<int32_t>()};
+                        s->info.as<int32_t>(), s->gl.as<int32_t>()};
     hbem::lu_factor(w, st);
     s->x.reserve(sizeof(float) * 2 * (size_t)B * N);
     hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)B * N,

@@ -211,9 +211,14 @@ struct LuWorkspace {
     int B;
     int32_t *ipiv;     // [B][N]
     int32_t *info;     // [B]
+    int32_t *gl;       // [B][N / 32][kLuGather] composed row exchanges per step
 };
 
 constexpr int kLuPad = 64;
+constexpr int kLuGather = 132;  // ints per step of LuWorkspace::gl
+
+// ints of LuWorkspace::gl per system
+inline size_t lu_gather_ints(int N) { return (size_t)(N / 32) * kLuGather; }
 
 // Pads rows/cols n..N-1 with the identity (zeros elsewhere).
 void launch_pad_identity(float *A, int64_t lda, int64_t a_stride, int n, int N, int B,

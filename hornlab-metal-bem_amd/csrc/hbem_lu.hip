@@ -12,8 +12,11 @@
 // pivot search of a non-singular matrix.
 //
 // Pivoting follows LAPACK cgetf2: the pivot of a column is the first row of
-// maximal |re| + |im| (icamax), ipiv[j] is the row swapped with row j, and
-// swaps are applied to the whole row (left factor columns included).
+// maximal |re| + |im| (icamax) and ipiv[j] is the row exchanged with row j.
+// Unlike cgetrf, a step's exchanges are not applied to the factor columns
+// of earlier steps: L is stored block column by block column in the row
+// order of its own step, and the solves interleave the exchanges with the
+// block sweeps (the same operator, fewer strided row swaps).
 #include <hip/hip_runtime.h>
 
 #include "hbem_internal.h"
@@ -22,6 +25,7 @@ namespace hbem {
 namespace {
 
 constexpr int kNB = 32;  // panel width
+static_assert(kLuGather == 4 * kNB + 4, "gather record layout");
 
 struct c32 {
     float re, im;
@@ -80,61 +84,99 @@ __global__ __launch_bounds__(256) void k_norm1(const float *A, int64_t lda, int6
 }
 
 // ---------------------------------------------------------------------------
-// Panel factorisation: one 1024-thread block per system factors columns
-// [k, k + nb) over rows [k, N).
+// Panel factorisation.  Rows are exchanged inside the panel only; columns
+// left of the panel keep the row order of their own step (the solves apply
+// each step's exchanges between the block sweeps), and columns right of the
+// panel receive the step's exchanges in k_lu_swap_trsm through the composed
+// gather list written here.
 // ---------------------------------------------------------------------------
 
+// Composes the kNB row exchanges (k + j <-> piv[j], j = 0..kNB-1, in order)
+// of one panel into a gather list: position trow[m] receives the old row
+// tsrc[m] for m < count <= 2 kNB.  Record layout (kGl ints): [0] count,
+// [1 .. 2kNB] trow, [2kNB+1 .. 4kNB] tsrc.  Run by the 64 lanes of one wave.
+__device__ void build_gather_list(const int32_t *piv, int k, int32_t *out) {
+    const int lane = threadIdx.x & 63;
+    int trow = k + lane, tsrc = k + lane;
+    int cnt = kNB;
+    for (int j = 0; j < kNB; ++j) {
+        const int p = piv[j];
+        if (p == k + j) continue;
+        const unsigned long long hit = __ballot(lane < cnt && trow == p);
+        int ip;
+        if (hit != 0ull) {
+            ip = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+        } else {
+            ip = cnt++;
+            if (lane == ip) {
+                trow = p;
+                tsrc = p;
+            }
+        }
+        const int sj = __shfl(tsrc, j), sip = __shfl(tsrc, ip);
+        if (lane == j) tsrc = sip;
+        if (lane == ip) tsrc = sj;
+    }
+    if (lane < cnt) {
+        out[1 + lane] = trow;
+        out[1 + 2 * kNB + lane] = tsrc;
+    }
+    if (lane == 0) out[0] = cnt;
+}
+
+// Order key of a pivot candidate: larger |re| + |im| first, then the smaller
+// row (the first maximum, as icamax).  Magnitudes are >= 0, so their float
+// bits order like the values.
+__device__ __forceinline__ unsigned long long pivot_key(c32 v, int row) {
+    const float mag = fabsf(v.re) + fabsf(v.im);
+    return (static_cast<unsigned long long>(__float_as_uint(mag)) << 32) |
+           static_cast<unsigned long long>(0xffffffffu - static_cast<unsigned>(row));
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long key) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned lo = __shfl_xor(static_cast<unsigned>(key), o);
+        const unsigned hi = __shfl_xor(static_cast<unsigned>(key >> 32), o);
+        const unsigned long long other = (static_cast<unsigned long long>(hi) << 32) | lo;
+        key = other > key ? other : key;
+    }
+    return key;
+}
+
+// General panel (any N): one 1024-thread block per system factors columns
+// [k, k + kNB) over rows [k, N) in global memory.
 __global__ __launch_bounds__(1024) void k_lu_panel(float *A, int64_t lda, int64_t a_stride, int N,
-                                                   int k, int nb, int32_t *ipiv, int32_t *info) {
+                                                   int k, int32_t *ipiv, int32_t *info,
+                                                   int32_t *gl, int64_t gl_stride) {
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     float *Ab = A + (size_t)b * a_stride * 2;
-    __shared__ float red_v[16];
-    __shared__ int red_i[16];
-    __shared__ int piv_row;
+    __shared__ unsigned long long red[16];
+    __shared__ int piv_s[kNB];
     __shared__ c32 urow[kNB];
-    for (int j = 0; j < nb; ++j) {
+    for (int j = 0; j < kNB; ++j) {
         const int col = k + j;
-        float best = -1.0f;
-        int bi = N;
+        unsigned long long key = 0ull;
         for (int r = col + tid; r < N; r += 1024) {
-            const c32 v = ld(Ab, (size_t)col * lda + r);
-            const float a = fabsf(v.re) + fabsf(v.im);
-            if (a > best) {
-                best = a;
-                bi = r;
-            }
+            const unsigned long long kr = pivot_key(ld(Ab, (size_t)col * lda + r), r);
+            key = kr > key ? kr : key;
         }
-        for (int o = 32; o >= 1; o >>= 1) {
-            const float ov = __shfl_xor(best, o);
-            const int oi = __shfl_xor(bi, o);
-            if (ov > best || (ov == best && oi < bi)) {
-                best = ov;
-                bi = oi;
-            }
-        }
-        if (lane == 0) {
-            red_v[wave] = best;
-            red_i[wave] = bi;
-        }
+        key = wave_max_u64(key);
+        if (lane == 0) red[wave] = key;
         __syncthreads();
+        unsigned long long best = red[0];
+#pragma unroll
+        for (int w = 1; w < 16; ++w) best = red[w] > best ? red[w] : best;
+        const int p = static_cast<int>(0xffffffffu - static_cast<unsigned>(best));
         if (tid == 0) {
-            float bv = red_v[0];
-            int bidx = red_i[0];
-            for (int w = 1; w < 16; ++w)
-                if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bidx)) {
-                    bv = red_v[w];
-                    bidx = red_i[w];
-                }
-            if (bidx >= N) bidx = col;
-            piv_row = bidx;
-            ipiv[(size_t)b * N + col] = bidx;
-            if (!(bv > 0.0f) && info[b] == 0) info[b] = col + 1;
+            piv_s[j] = p;
+            ipiv[(size_t)b * N + col] = p;
+            if (!(__uint_as_float(static_cast<unsigned>(best >> 32)) > 0.0f) && info[b] == 0)
+                info[b] = col + 1;
         }
-        __syncthreads();
-        const int p = piv_row;
-        if (p != col && tid < nb) {
+        if (p != col && tid < kNB) {
             const size_t cc = (size_t)(k + tid) * lda;
             const c32 x = ld(Ab, cc + col), y = ld(Ab, cc + p);
             st(Ab, cc + col, y);
@@ -142,11 +184,11 @@ __global__ __launch_bounds__(1024) void k_lu_panel(float *A, int64_t lda, int64_
         }
         __syncthreads();
         const c32 piv = ld(Ab, (size_t)col * lda + col);
-        if (tid < nb - j - 1) urow[tid] = ld(Ab, (size_t)(col + 1 + tid) * lda + col);
+        if (tid < kNB - j - 1) urow[tid] = ld(Ab, (size_t)(col + 1 + tid) * lda + col);
         __syncthreads();
         const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
         const c32 inv = nz ? cinv(piv) : c32{0.0f, 0.0f};
-        const int rest = nb - j - 1;
+        const int rest = kNB - j - 1;
         for (int r = col + 1 + tid; r < N; r += 1024) {
             c32 l = ld(Ab, (size_t)col * lda + r);
             if (nz) {
@@ -164,256 +206,172 @@ __global__ __launch_bounds__(1024) void k_lu_panel(float *A, int64_t lda, int64_
         }
         __syncthreads();
     }
+    if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
 }
 
-// Register-resident panel for N <= 1024: thread t owns row k + t of the
-// nb = 32 panel columns; two block barriers per column (reduction, row
-// exchange through LDS).  Same pivot rule as k_lu_panel.
+// Register panel for N - k <= 1024: thread t holds the kNB panel values of
+// one row, the column loop is unrolled so that every register index is
+// static.  A row exchange swaps only the logical row numbers of the two
+// threads involved (no data moves); the pivot thread publishes its row
+// through LDS and every thread writes its values back to its final logical
+// row.  The pivot is reduced per wave with shuffles and across waves with
+// one LDS atomic max per wave into a double-buffered slot.  Two block
+// barriers per column; same pivot rule as k_lu_panel.
 __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, int64_t a_stride,
                                                        int N, int k, int32_t *ipiv,
-                                                       int32_t *info) {
+                                                       int32_t *info, int32_t *gl,
+                                                       int64_t gl_stride) {
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     float *Ab = A + (size_t)b * a_stride * 2;
-    const int row = k + tid;
-    const bool own = row < N;
+    const bool own = k + tid < N;
+    int myrow = k + tid;  // logical row held by this thread
     c32 v[kNB];
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
-        v[c] = own ? ld(Ab, (size_t)(k + c) * lda + row) : c32{0.0f, 0.0f};
-    __shared__ float red_v[16];
-    __shared__ int red_i[16];
-    __shared__ c32 prow[2][kNB];
-    __shared__ c32 orow[2][kNB];
+        v[c] = own ? ld(Ab, (size_t)(k + c) * lda + myrow) : c32{0.0f, 0.0f};
+    __shared__ unsigned long long best_s[2];
+    __shared__ c32 prow[kNB];
+    __shared__ int piv_s[kNB];
+    __shared__ int bad_s;
+    if (tid == 0) {
+        best_s[0] = 0ull;
+        best_s[1] = 0ull;
+        bad_s = 0;
+    }
+    __syncthreads();
+#pragma unroll
     for (int j = 0; j < kNB; ++j) {
         const int col = k + j;
-        // v[j] by predicated selection keeps every register index static
-        c32 vj = v[0];
-#pragma unroll
-        for (int c = 1; c < kNB; ++c)
-            if (c == j) vj = v[c];
-        float best = -1.0f;
-        int bi = N;
-        if (own && row >= col) {
-            best = fabsf(vj.re) + fabsf(vj.im);
-            bi = row;
-        }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const float ov = __shfl_xor(best, o);
-            const int oi = __shfl_xor(bi, o);
-            if (ov > best || (ov == best && oi < bi)) {
-                best = ov;
-                bi = oi;
-            }
-        }
-        if (lane == 0) {
-            red_v[wave] = best;
-            red_i[wave] = bi;
-        }
+        unsigned long long key = (own && myrow >= col) ? pivot_key(v[j], myrow) : 0ull;
+        key = wave_max_u64(key);
+        if (lane == 0) atomicMax(&best_s[j & 1], key);
         __syncthreads();
-        float bv = red_v[0];
-        int p = red_i[0];
+        const unsigned long long best = best_s[j & 1];
+        const int p = static_cast<int>(0xffffffffu - static_cast<unsigned>(best));
+        const bool holder = own && myrow == p;
+        if (holder) {
 #pragma unroll
-        for (int w = 1; w < 16; ++w)
-            if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < p)) {
-                bv = red_v[w];
-                p = red_i[w];
-            }
-        if (p >= N) p = col;
-        const int buf = j & 1;
-        if (row == p) {
-#pragma unroll
-            for (int c = 0; c < kNB; ++c) prow[buf][c] = v[c];
-        }
-        if (row == col && p != col) {
-#pragma unroll
-            for (int c = 0; c < kNB; ++c) orow[buf][c] = v[c];
+            for (int c = j; c < kNB; ++c) prow[c] = v[c];
         }
         if (tid == 0) {
+            best_s[(j + 1) & 1] = 0ull;  // its last readers passed the previous barrier
+            piv_s[j] = p;
             ipiv[(size_t)b * N + col] = p;
-            if (!(bv > 0.0f) && info[b] == 0) info[b] = col + 1;
+            if (!(__uint_as_float(static_cast<unsigned>(best >> 32)) > 0.0f) && bad_s == 0)
+                bad_s = col + 1;
         }
+        // exchange logical rows col <-> p
+        if (holder)
+            myrow = col;
+        else if (own && myrow == col)
+            myrow = p;
         __syncthreads();
-        if (row == col) {
-#pragma unroll
-            for (int c = 0; c < kNB; ++c) v[c] = prow[buf][c];
-        } else if (row == p) {
-#pragma unroll
-            for (int c = 0; c < kNB; ++c) v[c] = orow[buf][c];
-        }
-        const c32 piv = prow[buf][j];
-        const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
-        if (own && row > col) {
-            c32 l = v[0];
-#pragma unroll
-            for (int c = 1; c < kNB; ++c)
-                if (c == j) l = v[c];
+        // rows below the pivot: l = a / pivot, a_c -= l u_c
+        if (own && myrow > col) {
+            const c32 piv = prow[j];
+            const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
+            c32 l = v[j];
             if (nz) l = cmul(l, cinv(piv));
+            v[j] = l;
 #pragma unroll
-            for (int c = 0; c < kNB; ++c) {
-                if (c == j) v[c] = l;
-                if (c > j) {
-                    const c32 u = prow[buf][c];
-                    v[c].re -= l.re * u.re - l.im * u.im;
-                    v[c].im -= l.re * u.im + l.im * u.re;
-                }
+            for (int c = j + 1; c < kNB; ++c) {
+                const c32 u = prow[c];
+                v[c].re = fmaf(-l.re, u.re, fmaf(l.im, u.im, v[c].re));
+                v[c].im = fmaf(-l.re, u.im, fmaf(-l.im, u.re, v[c].im));
             }
         }
     }
     if (own) {
 #pragma unroll
-        for (int c = 0; c < kNB; ++c) st(Ab, (size_t)(k + c) * lda + row, v[c]);
+        for (int c = 0; c < kNB; ++c) st(Ab, (size_t)(k + c) * lda + myrow, v[c]);
     }
+    if (tid == 0 && bad_s != 0 && info[b] == 0) info[b] = bad_s;
+    if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
 }
 
-// Row swaps of panel [k, k + nb) on every column outside the panel, then
-// the unit-lower solve U12 = L11^{-1} A12 on columns >= k + nb.  The nb
-// sequential swaps are composed once per block into a gather list (<= 2 nb
-// rows), so each column issues all its loads before any store.
+// Row exchanges of the step at column k (its gather list) and the
+// unit-lower solve U12 = L11^{-1} A12, on columns [c_begin, c_end), all right
+// of the panel; 64 columns per block.  The gathered rows are staged in LDS
+// (loads coalesced along each column), lane = column for the solve.
 __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int64_t a_stride,
-                                                      int N, int k, int nb,
-                                                      const int32_t *ipiv) {
+                                                      int k, int c_begin, int c_end,
+                                                      const int32_t *gl, int64_t gl_stride) {
     const int b = blockIdx.y;
     float *Ab = A + (size_t)b * a_stride * 2;
+    const int cb = c_begin + blockIdx.x * 64;
+    const int ncols = min(64, c_end - cb);
     __shared__ c32 L11[kNB][kNB + 1];
-    __shared__ int trow[2 * kNB];  // destination rows (first nb: k..k+nb-1)
-    __shared__ int tsrc[2 * kNB];  // source rows
-    __shared__ int tcount;
-    for (int e = threadIdx.x; e < nb * nb; e += 256) {
-        const int i = e % nb, j = e / nb;
+    __shared__ c32 tile[2 * kNB][64 + 1];
+    __shared__ int trow[2 * kNB], tsrc[2 * kNB];
+    __shared__ int cnt_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t *g = gl + (size_t)b * gl_stride;
+    if (tid == 0) cnt_s = g[0];
+    __syncthreads();
+    const int cnt = cnt_s;
+    if (tid < cnt) {
+        trow[tid] = g[1 + tid];
+        tsrc[tid] = g[1 + 2 * kNB + tid];
+    }
+    for (int e = tid; e < kNB * kNB; e += 256) {
+        const int i = e % kNB, j = e / kNB;
         L11[i][j] = ld(Ab, (size_t)(k + j) * lda + k + i);
     }
-    if (threadIdx.x == 0) {
-        int cnt = nb;
-        for (int j = 0; j < nb; ++j) {
-            trow[j] = k + j;
-            tsrc[j] = k + j;
-        }
-        for (int j = 0; j < nb; ++j) {
-            const int p = ipiv[(size_t)b * N + k + j];
-            if (p == k + j) continue;
-            int ip = -1;
-            for (int m = 0; m < cnt; ++m)
-                if (trow[m] == p) ip = m;
-            if (ip < 0) {
-                ip = cnt++;
-                trow[ip] = p;
-                tsrc[ip] = p;
-            }
-            const int t = tsrc[j];  // swap the sources of positions k+j and p
-            tsrc[j] = tsrc[ip];
-            tsrc[ip] = t;
-        }
-        tcount = cnt;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int cc = wave + 4 * it;
+        if (cc < ncols && lane < cnt) tile[lane][cc] = ld(Ab, (size_t)(cb + cc) * lda + tsrc[lane]);
     }
     __syncthreads();
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= N || (c >= k && c < k + nb)) return;
-    const size_t cc = (size_t)c * lda;
-    const int cnt = tcount;
-    c32 vals[2 * kNB];
+    if (tid < ncols) {
+        c32 x[kNB];
 #pragma unroll
-    for (int m = 0; m < 2 * kNB; ++m)
-        if (m < cnt) vals[m] = ld(Ab, cc + tsrc[m]);
-    if (c >= k + nb) {
-        // forward substitution with unit L11 on the first nb (= kNB) rows
+        for (int i = 0; i < kNB; ++i) x[i] = tile[i][tid];
 #pragma unroll
         for (int i = 1; i < kNB; ++i) {
-            c32 acc = vals[i];
+            c32 acc = x[i];
 #pragma unroll
             for (int j = 0; j < i; ++j) {
                 const c32 l = L11[i][j];
-                acc.re -= l.re * vals[j].re - l.im * vals[j].im;
-                acc.im -= l.re * vals[j].im + l.im * vals[j].re;
+                acc.re = fmaf(-l.re, x[j].re, fmaf(l.im, x[j].im, acc.re));
+                acc.im = fmaf(-l.re, x[j].im, fmaf(-l.im, x[j].re, acc.im));
             }
-            vals[i] = acc;
+            x[i] = acc;
         }
-    }
 #pragma unroll
-    for (int m = 0; m < 2 * kNB; ++m)
-        if (m < cnt) st(Ab, cc + trow[m], vals[m]);
-}
-
-// Trailing update A22 -= L21 * U12 over rows/cols [k + nb, N), 64 x 64
-// output tiles, 256 threads with a 4 x 4 complex micro-tile each.
-__global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t a_stride, int N,
-                                                 int k, int nb) {
-    const int b = blockIdx.z;
-    float *Ab = A + (size_t)b * a_stride * 2;
-    const int s = k + nb;
-    const int r0 = s + blockIdx.x * 64;
-    const int c0 = s + blockIdx.y * 64;
-    __shared__ c32 Ls[kNB][64];
-    __shared__ c32 Us[kNB][64 + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < nb * 64; e += 256) {
-        const int rr = e % 64, l = e / 64;
-        Ls[l][rr] = (r0 + rr < N) ? ld(Ab, (size_t)(k + l) * lda + r0 + rr) : c32{0.0f, 0.0f};
-    }
-    for (int e = tid; e < nb * 64; e += 256) {
-        const int l = e % nb, cc = e / nb;
-        Us[l][cc] = (c0 + cc < N) ? ld(Ab, (size_t)(c0 + cc) * lda + k + l) : c32{0.0f, 0.0f};
+        for (int i = 0; i < kNB; ++i) tile[i][tid] = x[i];
     }
     __syncthreads();
-    const int tr = tid % 16, tc = tid / 16;
-    c32 acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = c32{0.0f, 0.0f};
-    for (int l = 0; l < nb; ++l) {
-        c32 a[4], u[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = Ls[l][tr + 16 * i];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) u[j] = Us[l][tc + 16 * j];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[i][j].re = fmaf(a[i].re, u[j].re, acc[i][j].re);
-                acc[i][j].re = fmaf(-a[i].im, u[j].im, acc[i][j].re);
-                acc[i][j].im = fmaf(a[i].re, u[j].im, acc[i][j].im);
-                acc[i][j].im = fmaf(a[i].im, u[j].re, acc[i][j].im);
-            }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (c0 + tc + 16 * j >= N) continue;
-        const size_t cc = (size_t)(c0 + tc + 16 * j) * lda;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (r0 + tr + 16 * i >= N) continue;
-            const size_t idx = cc + r0 + tr + 16 * i;
-            c32 v = ld(Ab, idx);
-            v.re -= acc[i][j].re;
-            v.im -= acc[i][j].im;
-            st(Ab, idx, v);
-        }
+    for (int it = 0; it < 16; ++it) {
+        const int cc = wave + 4 * it;
+        if (cc < ncols && lane < cnt) st(Ab, (size_t)(cb + cc) * lda + trow[lane], tile[lane][cc]);
     }
 }
 
 // MFMA trailing update A22 -= L21 * U12 (FP32 matrix cores,
-// v_mfma_f32_16x16x4_f32).  64 x 64 complex output tile per block; each of
-// the 4 waves owns a 32 x 32 sub-tile as 2 x 2 MFMA tiles with separate real
-// and imaginary accumulators (re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).
-// The MFMA computes the transposed tile C^T = U^T L^T so that the 16 lanes
-// of an output row group cover 16 consecutive matrix rows (coalesced
-// column-major stores).  Operands are staged in LDS as separate re/im planes
-// with an 80-float row stride (conflict-free reads of the 4 k-rows a wave
-// touches per instruction).
+// v_mfma_f32_16x16x4_f32) over rows [k + kNB, N) and columns
+// [c_begin, c_end).  64 x 64 complex output tile per block; each of the 4
+// waves owns a 32 x 32 sub-tile as 2 x 2 MFMA tiles with separate real and
+// imaginary accumulators (re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).  The
+// MFMA computes the transposed tile C^T = U^T L^T so that the 16 lanes of an
+// output row group cover 16 consecutive matrix rows (coalesced column-major
+// stores).  Operands are staged in LDS as separate re/im planes with an
+// 80-float row stride (conflict-free reads of the 4 k-rows a wave touches
+// per instruction).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kGS = 80;
 
 __global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int64_t a_stride,
-                                                      int N, int k) {
+                                                      int N, int k, int c_begin, int c_end) {
     const int b = blockIdx.z;
     float *Ab = A + (size_t)b * a_stride * 2;
     const int s = k + kNB;
     const int r0 = s + blockIdx.x * 64;
-    const int c0 = s + blockIdx.y * 64;
+    const int c0 = c_begin + blockIdx.y * 64;
     __shared__ float Lr[kNB][kGS], Li[kNB][kGS], Ur[kNB][kGS], Ui[kNB][kGS];
     const int tid = threadIdx.x;
 #pragma unroll
@@ -428,7 +386,8 @@ __global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int
     for (int it = 0; it < (kNB * 64) / 256; ++it) {
         const int e = tid + 256 * it;
         const int cc = e / kNB, kk = e % kNB;
-        const c32 v = (c0 + cc < N) ? ld(Ab, (size_t)(c0 + cc) * lda + k + kk) : c32{0.f, 0.f};
+        const c32 v =
+            (c0 + cc < c_end) ? ld(Ab, (size_t)(c0 + cc) * lda + k + kk) : c32{0.f, 0.f};
         Ur[kk][cc] = v.re;
         Ui[kk][cc] = v.im;
     }
@@ -474,7 +433,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int
             for (int v = 0; v < 4; ++v) {
                 const int col = c0 + wc + 16 * ti + 4 * lk + v;
                 const int row = r0 + wr + 16 * tj + li;
-                if (row < N && col < N) {
+                if (row < N && col < c_end) {
                     const size_t idx = (size_t)col * lda + row;
                     c32 x = ld(Ab, idx);
                     x.re -= accr[ti][tj][v];
@@ -485,35 +444,55 @@ __global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves, one 1024-thread block per system, vector in LDS.
+// Triangular solves with the factors: one 1024-thread block per system,
+// x in LDS, sweeps in blocks of kNB rows.  The diagonal block of each sweep
+// step is staged in LDS and solved by one wave with shuffles; the coupling
+// to the other rows is a block GEMV over all threads.
+//   trans 0: x <- A^{-1} x = U^{-1} L~_{S-1}^{-1} P_{S-1}^T ... L~_0^{-1} P_0^T x
+//   trans 2: x <- A^{-H} x = P_0 L~_0^{-H} ... P_{S-1} L~_{S-1}^{-H} U^{-H} x
+// where P_s^T applies the exchanges of step s in order and L~_s is the unit
+// lower block column of step s in that step's row order.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ c32 block_sum(c32 v, c32 *red) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int o = 32; o >= 1; o >>= 1) {
-        v.re += __shfl_xor(v.re, o);
-        v.im += __shfl_xor(v.im, o);
-    }
-    __syncthreads();
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    c32 s{0.0f, 0.0f};
-    const int nw = blockDim.x >> 6;
-    for (int w = 0; w < nw; ++w) {
-        s.re += red[w].re;
-        s.im += red[w].im;
-    }
-    return s;
+__device__ __forceinline__ void stage_diag(const float *Ab, int64_t lda, int kb, c32 (*D)[kNB + 1]) {
+    const int i = threadIdx.x % kNB, j = threadIdx.x / kNB;  // 1024 threads = 32 x 32
+    D[i][j] = ld(Ab, (size_t)(kb + j) * lda + kb + i);
 }
 
-// x <- A^{-1} x with the LU factors: pivots, then L and U sweeps in blocks
-// of 32 rows.  Wave 0 solves each 32 x 32 diagonal block with shuffles; all
-// threads update the remaining rows.  One block per system, x in LDS.
-__global__ __launch_bounds__(1024) void k_lu_solve_blocked(const float *A, int64_t lda,
-                                                           int64_t a_stride, int N,
-                                                           const int32_t *ipiv, float *X,
-                                                           int64_t ldx) {
+// xs[kb + c] -= sum_r conj(A[r][kb + c]) xs[r] over rows r in [r0, r1)
+// (32 outputs, all threads; partial sums in part).
+__device__ __forceinline__ void block_gemv_h(const float *Ab, int64_t lda, int kb, int r0, int r1,
+                                             c32 *xs, c32 (*part)[kNB + 1]) {
+    const int c = threadIdx.x / kNB, chunk = threadIdx.x % kNB;
+    c32 acc{0.0f, 0.0f};
+    const size_t cc = (size_t)(kb + c) * lda;
+    for (int r = r0 + chunk; r < r1; r += kNB) {
+        const c32 a = ld(Ab, cc + r);
+        const c32 x = xs[r];
+        acc.re += a.re * x.re + a.im * x.im;
+        acc.im += a.re * x.im - a.im * x.re;
+    }
+    part[chunk][c] = acc;
+    __syncthreads();
+    if (threadIdx.x < kNB) {
+        c32 s{0.0f, 0.0f};
+#pragma unroll 8
+        for (int q = 0; q < kNB; ++q) {
+            s.re += part[q][threadIdx.x].re;
+            s.im += part[q][threadIdx.x].im;
+        }
+        xs[kb + threadIdx.x].re -= s.re;
+        xs[kb + threadIdx.x].im -= s.im;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, int64_t a_stride,
+                                                   int N, const int32_t *ipiv, float *X,
+                                                   int64_t ldx, int trans) {
     extern __shared__ c32 xs[];
+    __shared__ c32 D[kNB][kNB + 1];
+    __shared__ c32 part[kNB][kNB + 1];
+    __shared__ int pv_s[kNB];
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -521,168 +500,129 @@ __global__ __launch_bounds__(1024) void k_lu_solve_blocked(const float *A, int64
     const int32_t *pv = ipiv + (size_t)b * N;
     float *xb = X + (size_t)b * ldx * 2;
     for (int r = tid; r < N; r += 1024) xs[r] = ld(xb, r);
-    __syncthreads();
-    if (tid == 0)
-        for (int j = 0; j < N; ++j) {
-            const int p = pv[j];
-            if (p != j) {
-                const c32 t = xs[j];
-                xs[j] = xs[p];
-                xs[p] = t;
-            }
-        }
-    __syncthreads();
-    constexpr int BS = 32;
-    for (int kb = 0; kb < N; kb += BS) {
-        if (tid < 64) {
-            c32 x = lane < BS ? xs[kb + lane] : c32{0.0f, 0.0f};
-            for (int j = 0; j < BS - 1; ++j) {
-                const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
-                if (lane > j && lane < BS) {
-                    const c32 l = ld(Ab, (size_t)(kb + j) * lda + kb + lane);
-                    x.re -= l.re * xr - l.im * xi;
-                    x.im -= l.re * xi + l.im * xr;
-                }
-            }
-            if (lane < BS) xs[kb + lane] = x;
-        }
-        __syncthreads();
-        for (int r = kb + BS + tid; r < N; r += 1024) {
-            c32 acc = xs[r];
-#pragma unroll 8
-            for (int j = 0; j < BS; ++j) {
-                const c32 l = ld(Ab, (size_t)(kb + j) * lda + r);
-                const c32 xj = xs[kb + j];
-                acc.re -= l.re * xj.re - l.im * xj.im;
-                acc.im -= l.re * xj.im + l.im * xj.re;
-            }
-            xs[r] = acc;
-        }
-        __syncthreads();
-    }
-    for (int kb = N - BS; kb >= 0; kb -= BS) {
-        if (tid < 64) {
-            c32 x = lane < BS ? xs[kb + lane] : c32{0.0f, 0.0f};
-            for (int j = BS - 1; j >= 0; --j) {
-                if (lane == j) x = cmul(x, cinv(ld(Ab, (size_t)(kb + j) * lda + kb + j)));
-                const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
-                if (lane < j) {
-                    const c32 u = ld(Ab, (size_t)(kb + j) * lda + kb + lane);
-                    x.re -= u.re * xr - u.im * xi;
-                    x.im -= u.re * xi + u.im * xr;
-                }
-            }
-            if (lane < BS) xs[kb + lane] = x;
-        }
-        __syncthreads();
-        for (int r = tid; r < kb; r += 1024) {
-            c32 acc = xs[r];
-#pragma unroll 8
-            for (int j = 0; j < BS; ++j) {
-                const c32 u = ld(Ab, (size_t)(kb + j) * lda + r);
-                const c32 xj = xs[kb + j];
-                acc.re -= u.re * xj.re - u.im * xj.im;
-                acc.im -= u.re * xj.im + u.im * xj.re;
-            }
-            xs[r] = acc;
-        }
-        __syncthreads();
-    }
-    for (int r = tid; r < N; r += 1024) st(xb, r, xs[r]);
-}
-
-// trans = 0: x <- A^{-1} x ;  trans = 2: x <- A^{-H} x.
-__global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, int64_t a_stride,
-                                                   int N, const int32_t *ipiv, float *X,
-                                                   int64_t ldx, int trans) {
-    extern __shared__ c32 xs[];
-    __shared__ c32 red[16];
-    const int b = blockIdx.x;
-    const int tid = threadIdx.x;
-    const float *Ab = A + (size_t)b * a_stride * 2;
-    const int32_t *pv = ipiv + (size_t)b * N;
-    float *xb = X + (size_t)b * ldx * 2;
-    for (int r = tid; r < N; r += 1024) xs[r] = ld(xb, r);
-    __syncthreads();
     if (trans == 0) {
-        if (tid == 0)
-            for (int j = 0; j < N; ++j) {
-                const int p = pv[j];
-                if (p != j) {
-                    const c32 t = xs[j];
-                    xs[j] = xs[p];
-                    xs[p] = t;
+        for (int kb = 0; kb < N; kb += kNB) {
+            if (tid < kNB) pv_s[tid] = pv[kb + tid];
+            stage_diag(Ab, lda, kb, D);
+            __syncthreads();
+            if (tid == 0)
+                for (int j = 0; j < kNB; ++j) {
+                    const int p = pv_s[j];
+                    if (p != kb + j) {
+                        const c32 t = xs[kb + j];
+                        xs[kb + j] = xs[p];
+                        xs[p] = t;
+                    }
                 }
+            __syncthreads();
+            if (tid < 64) {
+                c32 x = lane < kNB ? xs[kb + lane] : c32{0.0f, 0.0f};
+                for (int j = 0; j < kNB - 1; ++j) {
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane > j && lane < kNB) {
+                        const c32 l = D[lane][j];
+                        x.re -= l.re * xr - l.im * xi;
+                        x.im -= l.re * xi + l.im * xr;
+                    }
+                }
+                if (lane < kNB) xs[kb + lane] = x;
             }
-        __syncthreads();
-        // L y = P b, unit lower, column sweep.
-        for (int j = 0; j < N - 1; ++j) {
-            const c32 xj = xs[j];
-            for (int r = j + 1 + tid; r < N; r += 1024) {
-                const c32 l = ld(Ab, (size_t)j * lda + r);
-                xs[r].re -= l.re * xj.re - l.im * xj.im;
-                xs[r].im -= l.re * xj.im + l.im * xj.re;
+            __syncthreads();
+            for (int r = kb + kNB + tid; r < N; r += 1024) {
+                c32 acc = xs[r];
+#pragma unroll 8
+                for (int j = 0; j < kNB; ++j) {
+                    const c32 l = ld(Ab, (size_t)(kb + j) * lda + r);
+                    const c32 xj = xs[kb + j];
+                    acc.re -= l.re * xj.re - l.im * xj.im;
+                    acc.im -= l.re * xj.im + l.im * xj.re;
+                }
+                xs[r] = acc;
             }
             __syncthreads();
         }
-        // U x = y, column sweep from the bottom.
-        for (int j = N - 1; j >= 0; --j) {
-            const c32 u = ld(Ab, (size_t)j * lda + j);
-            const c32 xj = cmul(xs[j], cinv(u));
+        for (int kb = N - kNB; kb >= 0; kb -= kNB) {
+            stage_diag(Ab, lda, kb, D);
             __syncthreads();
-            if (tid == 0) xs[j] = xj;
-            for (int r = tid; r < j; r += 1024) {
-                const c32 a = ld(Ab, (size_t)j * lda + r);
-                xs[r].re -= a.re * xj.re - a.im * xj.im;
-                xs[r].im -= a.re * xj.im + a.im * xj.re;
+            if (tid < 64) {
+                c32 x = lane < kNB ? xs[kb + lane] : c32{0.0f, 0.0f};
+                for (int j = kNB - 1; j >= 0; --j) {
+                    if (lane == j) x = cmul(x, cinv(D[j][j]));
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane < j) {
+                        const c32 u = D[lane][j];
+                        x.re -= u.re * xr - u.im * xi;
+                        x.im -= u.re * xi + u.im * xr;
+                    }
+                }
+                if (lane < kNB) xs[kb + lane] = x;
+            }
+            __syncthreads();
+            for (int r = tid; r < kb; r += 1024) {
+                c32 acc = xs[r];
+#pragma unroll 8
+                for (int j = 0; j < kNB; ++j) {
+                    const c32 u = ld(Ab, (size_t)(kb + j) * lda + r);
+                    const c32 xj = xs[kb + j];
+                    acc.re -= u.re * xj.re - u.im * xj.im;
+                    acc.im -= u.re * xj.im + u.im * xj.re;
+                }
+                xs[r] = acc;
             }
             __syncthreads();
         }
     } else {
-        // U^H y = b: y_j = (b_j - sum_{i<j} conj(U_ij) y_i) / conj(U_jj).
-        for (int j = 0; j < N; ++j) {
-            c32 part{0.0f, 0.0f};
-            for (int i = tid; i < j; i += 1024) {
-                const c32 u = cconj(ld(Ab, (size_t)j * lda + i));
-                const c32 t = cmul(u, xs[i]);
-                part.re += t.re;
-                part.im += t.im;
-            }
-            const c32 s = block_sum(part, red);
-            if (tid == 0) {
-                const c32 d = cconj(ld(Ab, (size_t)j * lda + j));
-                c32 v = xs[j];
-                v.re -= s.re;
-                v.im -= s.im;
-                xs[j] = cmul(v, cinv(d));
-            }
+        // U^H y = x, forward over the blocks: couple to the solved rows above,
+        // then the lower triangular (U11)^H.
+        for (int kb = 0; kb < N; kb += kNB) {
             __syncthreads();
-        }
-        // L^H z = y (unit): z_j = y_j - sum_{i>j} conj(L_ij) z_i.
-        for (int j = N - 2; j >= 0; --j) {
-            c32 part{0.0f, 0.0f};
-            for (int i = j + 1 + tid; i < N; i += 1024) {
-                const c32 l = cconj(ld(Ab, (size_t)j * lda + i));
-                const c32 t = cmul(l, xs[i]);
-                part.re += t.re;
-                part.im += t.im;
-            }
-            const c32 s = block_sum(part, red);
-            if (tid == 0) {
-                xs[j].re -= s.re;
-                xs[j].im -= s.im;
-            }
+            if (kb > 0) block_gemv_h(Ab, lda, kb, 0, kb, xs, part);
+            stage_diag(Ab, lda, kb, D);
             __syncthreads();
-        }
-        if (tid == 0)
-            for (int j = N - 1; j >= 0; --j) {
-                const int p = pv[j];
-                if (p != j) {
-                    const c32 t = xs[j];
-                    xs[j] = xs[p];
-                    xs[p] = t;
+            if (tid < 64) {
+                c32 x = lane < kNB ? xs[kb + lane] : c32{0.0f, 0.0f};
+                for (int j = 0; j < kNB; ++j) {
+                    if (lane == j) x = cmul(x, cinv(cconj(D[j][j])));
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane > j && lane < kNB) {
+                        const c32 u = cconj(D[j][lane]);
+                        x.re -= u.re * xr - u.im * xi;
+                        x.im -= u.re * xi + u.im * xr;
+                    }
                 }
+                if (lane < kNB) xs[kb + lane] = x;
             }
+        }
+        // L~_s^{-H} then P_s, s = S-1 .. 0.
+        for (int kb = N - kNB; kb >= 0; kb -= kNB) {
+            __syncthreads();
+            if (kb + kNB < N) block_gemv_h(Ab, lda, kb, kb + kNB, N, xs, part);
+            if (tid < kNB) pv_s[tid] = pv[kb + tid];
+            stage_diag(Ab, lda, kb, D);
+            __syncthreads();
+            if (tid < 64) {
+                c32 x = lane < kNB ? xs[kb + lane] : c32{0.0f, 0.0f};
+                for (int j = kNB - 1; j >= 0; --j) {
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane < j) {
+                        const c32 l = cconj(D[j][lane]);
+                        x.re -= l.re * xr - l.im * xi;
+                        x.im -= l.re * xi + l.im * xr;
+                    }
+                }
+                if (lane < kNB) xs[kb + lane] = x;
+            }
+            __syncthreads();
+            if (tid == 0)
+                for (int j = kNB - 1; j >= 0; --j) {
+                    const int p = pv_s[j];
+                    if (p != kb + j) {
+                        const c32 t = xs[kb + j];
+                        xs[kb + j] = xs[p];
+                        xs[p] = t;
+                    }
+                }
+        }
         __syncthreads();
     }
     for (int r = tid; r < N; r += 1024) st(xb, r, xs[r]);
@@ -802,36 +742,29 @@ void lu_factor(const LuWorkspace &w, hipStream_t stream) {
     static_assert(kLuPad % kNB == 0, "padded order must hold whole panels");
     // int32 zero has the bit pattern of float zero
     launch_fill_zero(reinterpret_cast<float *>(w.info), (size_t)w.B, stream);
+    const int64_t gls = (int64_t)(w.N / kNB) * kLuGather;
     for (int k = 0; k < w.N; k += kNB) {
-        const int nb = std::min(kNB, w.N - k);
-        if (w.N <= 1024 && nb == kNB)
+        int32_t *glk = w.gl + (size_t)(k / kNB) * kLuGather;
+        if (w.N - k <= 1024)
             hipLaunchKernelGGL(k_lu_panel_reg, dim3(w.B), dim3(1024), 0, stream, w.A, w.lda,
-                               w.a_stride, w.N, k, w.ipiv, w.info);
+                               w.a_stride, w.N, k, w.ipiv, w.info, glk, gls);
         else
             hipLaunchKernelGGL(k_lu_panel, dim3(w.B), dim3(1024), 0, stream, w.A, w.lda,
-                               w.a_stride, w.N, k, nb, w.ipiv, w.info);
-        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((w.N + 255) / 256, w.B), dim3(256), 0, stream, w.A,
-                           w.lda, w.a_stride, w.N, k, nb, w.ipiv);
-        const int rest = w.N - k - nb;
+                               w.a_stride, w.N, k, w.ipiv, w.info, glk, gls);
+        const int s = k + kNB, rest = w.N - s;
         if (rest > 0) {
-            if (nb == kNB)
-                hipLaunchKernelGGL(k_lu_gemm_mfma, dim3((rest + 63) / 64, (rest + 63) / 64, w.B),
-                                   dim3(256), 0, stream, w.A, w.lda, w.a_stride, w.N, k);
-            else
-                hipLaunchKernelGGL(k_lu_gemm, dim3((rest + 63) / 64, (rest + 63) / 64, w.B),
-                                   dim3(256), 0, stream, w.A, w.lda, w.a_stride, w.N, k, nb);
+            hipLaunchKernelGGL(k_lu_swap_trsm, dim3((rest + 63) / 64, w.B), dim3(256), 0, stream,
+                               w.A, w.lda, w.a_stride, k, s, w.N, glk, gls);
+            hipLaunchKernelGGL(k_lu_gemm_mfma, dim3((rest + 63) / 64, (rest + 63) / 64, w.B),
+                               dim3(256), 0, stream, w.A, w.lda, w.a_stride, w.N, k, s, w.N);
         }
     }
 }
 
 void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream) {
     const size_t shm = sizeof(float) * 2 * (size_t)w.N;
-    if (trans == 0 && w.N % 32 == 0)
-        hipLaunchKernelGGL(k_lu_solve_blocked, dim3(w.B), dim3(1024), shm, stream, w.A, w.lda,
-                           w.a_stride, w.N, w.ipiv, x, ldx);
-    else
-        hipLaunchKernelGGL(k_lu_solve, dim3(w.B), dim3(1024), shm, stream, w.A, w.lda,
-                           w.a_stride, w.N, w.ipiv, x, ldx, trans);
+    hipLaunchKernelGGL(k_lu_solve, dim3(w.B), dim3(1024), shm, stream, w.A, w.lda, w.a_stride,
+                       w.N, w.ipiv, x, ldx, trans);
 }
 
 void launch_residual_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,
