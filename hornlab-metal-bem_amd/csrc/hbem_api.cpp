@@ -136,7 +136,8 @@ struct hbem_session {
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
         tile_count, tile_ndofs, tile_dofs, tile_pairs;
     // per-call workspaces
-    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, q, kre, kim, fk, obs, field, tags,
+    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, amax, lu_scale, q, kre, kim, fk,
+        obs, field, tags,
         tagout, norms, anorm, active, rowsplit, rcx;
     // staging of host-input batches
     DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag;
@@ -146,7 +147,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &amax, &lu_scale, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag})
@@ -531,13 +532,18 @@ void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_optio
     const bool refine = opt.solve_mode == HBEM_SOLVE_F32_IR ||
                         opt.solve_mode == HBEM_SOLVE_MIXED_F16 ||
                         opt.solve_mode == HBEM_SOLVE_MIXED_BF16;
-    if (opt.solve_mode == HBEM_SOLVE_MIXED_F16 || opt.solve_mode == HBEM_SOLVE_MIXED_BF16)
-        fail(HBEM_EUNSUPPORTED, "mixed-precision LU modes are not built in this library version");
+    const int precision = opt.solve_mode == HBEM_SOLVE_MIXED_F16    ? 1
+                          : opt.solve_mode == HBEM_SOLVE_MIXED_BF16 ? 2
+                                                                    : 0;
     Timer t;
     t.start(st);
     hbem::launch_pad_identity(s->A.as<float>(), lda, a_stride, n, N, B, st);
     s->anorm.reserve(sizeof(double) * B);
-    hbem::launch_norm1(s->A.as<float>(), lda, a_stride, n, B, s->anorm.as<double>(), st);
+    s->amax.reserve(sizeof(double) * B);
+    s->lu_scale.reserve(sizeof(float) * B);
+    hbem::launch_norm1(s->A.as<float>(), lda, a_stride, n, B, s->anorm.as<double>(),
+                       s->amax.as<double>(), st);
+    hbem::launch_pow2_scale(s->amax.as<double>(), s->lu_scale.as<float>(), B, st);
     if (refine) {
         s->A_orig.reserve(sizeof(float) * 2 * (size_t)B * a_stride);
         hip_check(hipMemcpyAsync(s->A_orig.ptr, s->A.ptr, sizeof(float) * 2 * (size_t)B * a_stride,
@@ -549,7 +555,7 @@ void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_optio
     s->gl.reserve(sizeof(int32_t) * (size_t)B * hbem::lu_gather_ints(N));
     hbem::LuWorkspace w{s->A.as<float>(), lda, a_stride, N, n, B, s->ipiv.as<int32_t>(),
                         s->info.as<int32_t>(), s->gl.as<int32_t>()};
-    hbem::lu_factor(w, st);
+    hbem::lu_factor(w, precision, s->lu_scale.as<float>(), st);
     s->x.reserve(sizeof(float) * 2 * (size_t)B * N);
     hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)B * N,
                              hipMemcpyDeviceToDevice, st),

@@ -94,12 +94,17 @@ constexpr int kTestsPerWave = kTileTris / 4;
 // dF[i][j] (test basis i on t, trial basis j on s) and dR[i][j] (test basis
 // i on s, trial basis j on t) hold -4 pi DLP / jac; sF[i] (basis on t) and
 // sR[i] (basis on s) hold 4 pi SLP / jac with the source's unit density.
+// Both triangles are flat, so the normal projections of d = y_b - S_m x_a
+// depend on one point only:
+//   d . n_s         = (v0_s - S_m x_a) . n_s          = hF[a]
+//   d . (S_m n_t)   = (y_b - S_m v0_t) . (S_m n_t)    = hR[b]
+// (differences of nearby points first, so the rounding scales with |d|).
 template <bool kComplexK, bool kSource>
-__device__ __forceinline__ void regular_both(const float (&xa_m)[18], float ntx, float nty,
-                                             float ntz, const float (&yp)[18], float nsx,
-                                             float nsy, float nsz, float k, float krev,
-                                             float kim, float (&dF)[9][2], float (&dR)[9][2],
-                                             float (&sF)[3][2], float (&sR)[3][2]) {
+__device__ __forceinline__ void regular_both(const float (&xa_m)[18], const float (&hF)[6],
+                                             const float (&hR)[6], const float (&yp)[18],
+                                             float k, float krev, float kim, float (&dF)[9][2],
+                                             float (&dR)[9][2], float (&sF)[3][2],
+                                             float (&sR)[3][2]) {
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
         const float xa = xa_m[3 * a], ya = xa_m[3 * a + 1], za = xa_m[3 * a + 2];
@@ -115,8 +120,8 @@ __device__ __forceinline__ void regular_both(const float (&xa_m)[18], float ntx,
             const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), 1.0e-14f);
             const float rinv = __builtin_amdgcn_rsqf(r2);
             const float r = r2 * rinv;
-            float t = r * krev;
-            t = t - __builtin_rintf(t);
+            // phase in revolutions, reduced to [0, 1) (exact)
+            const float t = __builtin_amdgcn_fractf(r * krev);
             const float s = __builtin_amdgcn_sinf(t);
             const float c = __builtin_amdgcn_cosf(t);
             float q2 = rinv * rinv;  // 1/r^2 (times the damping)
@@ -129,8 +134,8 @@ __device__ __forceinline__ void regular_both(const float (&xa_m)[18], float ntx,
             // -4 pi dG/dn = proj/r^2 * (c fr + s k, s fr - c k) e^{-kim r}
             const float p1 = fmaf(s, k, c * fr);
             const float p2 = fmaf(-c, k, s * fr);
-            const float uF = fmaf(dx, nsx, fmaf(dy, nsy, dz * nsz)) * q2;
-            const float uR = -fmaf(dx, ntx, fmaf(dy, nty, dz * ntz)) * q2;
+            const float uF = hF[a] * q2;
+            const float uR = -hR[bq] * q2;
             const float wb = (float)kRule6W[bq];
             const float w1 = wb * (float)kRule6X[bq];
             const float w2 = wb * (float)kRule6Y[bq];
@@ -184,28 +189,45 @@ __device__ __forceinline__ void regular_both(const float (&xa_m)[18], float ntx,
     }
 }
 
+// All images of one (t, s): tf is t's float record (uniform), yp / v0s /
+// ns the lane's points, vertex 0 and normal.
 template <bool kComplexK, bool kSource>
 __device__ __forceinline__ void regular_images(const float *__restrict__ tf, int n_masks,
                                                int packed_masks, unsigned skip,
-                                               const float (&yp)[18], float nsx, float nsy,
-                                               float nsz, float k, float krev, float kim,
-                                               float (&dF)[9][2], float (&dR)[9][2],
+                                               const float (&yp)[18], const float (&v0s)[3],
+                                               const float (&ns)[3], float k, float krev,
+                                               float kim, float (&dF)[9][2], float (&dR)[9][2],
                                                float (&sF)[3][2], float (&sR)[3][2]) {
     for (int m = 0; m < n_masks; ++m) {
         if ((skip >> m) & 1u) continue;
         const int mask = (packed_masks >> (3 * m)) & 7;
-        const float sx = (mask & 1) ? -1.0f : 1.0f;
-        const float sy = (mask & 2) ? -1.0f : 1.0f;
-        const float sz = (mask & 4) ? -1.0f : 1.0f;
-        float xm[18];
+        const float sg[3] = {(mask & 1) ? -1.0f : 1.0f, (mask & 2) ? -1.0f : 1.0f,
+                             (mask & 4) ? -1.0f : 1.0f};
+        float xm[18], hF[6], hR[6];
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
-            xm[3 * a] = sx * tf[3 * a];
-            xm[3 * a + 1] = sy * tf[3 * a + 1];
-            xm[3 * a + 2] = sz * tf[3 * a + 2];
+            float h = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                xm[3 * a + c] = sg[c] * tf[3 * a + c];
+                h = fmaf(v0s[c] - xm[3 * a + c], ns[c], h);
+            }
+            hF[a] = h;
         }
-        regular_both<kComplexK, kSource>(xm, sx * tf[18], sy * tf[19], sz * tf[20], yp, nsx, nsy,
-                                         nsz, k, krev, kim, dF, dR, sF, sR);
+        float nt[3], v0t[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            nt[c] = sg[c] * tf[18 + c];
+            v0t[c] = sg[c] * tf[23 + c];
+        }
+#pragma unroll
+        for (int bq = 0; bq < 6; ++bq) {
+            float h = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) h = fmaf(yp[3 * bq + c] - v0t[c], nt[c], h);
+            hR[bq] = h;
+        }
+        regular_both<kComplexK, kSource>(xm, hF, hR, yp, k, krev, kim, dF, dR, sF, sR);
     }
 }
 
@@ -238,9 +260,12 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     float yp[18];
 #pragma unroll
     for (int j = 0; j < 18; ++j) yp[j] = g.tri_f_soa[(size_t)j * T + tr];
-    const float nx = g.tri_f_soa[(size_t)18 * T + tr];
-    const float ny = g.tri_f_soa[(size_t)19 * T + tr];
-    const float nz = g.tri_f_soa[(size_t)20 * T + tr];
+    float ns[3], v0s[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ns[c] = g.tri_f_soa[(size_t)(18 + c) * T + tr];
+        v0s[c] = g.tri_f_soa[(size_t)(23 + c) * T + tr];
+    }
     const float jac_s = g.tri_f_soa[(size_t)22 * T + tr];
     int svid[3], skid[kMaxMasks][3], slc[3], sdof[3];
 #pragma unroll
@@ -299,11 +324,11 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         for (int i = 0; i < 3; ++i) sF[i][0] = sF[i][1] = sR[i][0] = sR[i][1] = 0.0f;
         const bool src = wave_has_source || test_source;
         if (src)
-            regular_images<kComplexK, true>(tf, M, packed_masks, skip, yp, nx, ny, nz, k, krev, kim, dF,
-                                            dR, sF, sR);
+            regular_images<kComplexK, true>(tf, M, packed_masks, skip, yp, v0s, ns, k, krev, kim,
+                                            dF, dR, sF, sR);
         else
-            regular_images<kComplexK, false>(tf, M, packed_masks, skip, yp, nx, ny, nz, k, krev, kim, dF,
-                                             dR, sF, sR);
+            regular_images<kComplexK, false>(tf, M, packed_masks, skip, yp, v0s, ns, k, krev,
+                                             kim, dF, dR, sF, sR);
         const float sc = dscale * jt;
         if (fwd) {
 #pragma unroll

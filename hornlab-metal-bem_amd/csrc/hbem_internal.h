@@ -215,6 +215,7 @@ struct LuWorkspace {
 };
 
 constexpr int kLuPad = 64;
+constexpr int kLuOuter = 128;   // outer block of the two-level LU (exchange granularity)
 constexpr int kLuGather = 132;  // ints per step of LuWorkspace::gl
 
 // ints of LuWorkspace::gl per system
@@ -224,11 +225,17 @@ inline size_t lu_gather_ints(int N) { return (size_t)(N / 32) * kLuGather; }
 void launch_pad_identity(float *A, int64_t lda, int64_t a_stride, int n, int N, int B,
                          hipStream_t stream);
 
-// 1-norm of each matrix (max column sum of |a_ij|) -> norms[B] (double).
+// 1-norm of each matrix (max column sum of |a_ij|) -> norms[B] and the
+// largest modulus -> amax[B] (double).
 void launch_norm1(const float *A, int64_t lda, int64_t a_stride, int n, int B, double *norms,
-                  hipStream_t stream);
+                  double *amax, hipStream_t stream);
 
-void lu_factor(const LuWorkspace &w, hipStream_t stream);
+// scale[b] = 2^-e with 2^e ~ amax[b] (operand scale of the FP16 update).
+void launch_pow2_scale(const double *amax, float *scale, int B, hipStream_t stream);
+
+// precision of the rank-kLuOuter trailing updates: 0 FP32, 1 FP16 (with
+// scale[B]), 2 BF16; everything else runs in FP32.
+void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStream_t stream);
 
 // Solves A x = b (trans = 0) or A^H x = b (trans = 2) in place on x
 // [B][ldx] interleaved complex, using the LU factors and pivots.

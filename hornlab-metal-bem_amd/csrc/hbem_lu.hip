@@ -19,6 +19,9 @@
 // block sweeps (the same operator, fewer strided row swaps).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "hbem_internal.h"
 
 namespace hbem {
@@ -63,24 +66,34 @@ __global__ __launch_bounds__(256) void k_pad_identity(float *A, int64_t lda, int
     }
 }
 
-// max_c sum_r |a_rc| over the true n x n block (clange '1').  One wave per
-// column; norms[] must be zero on entry.
+// norms[b] = max_c sum_r |a_rc| over the true n x n block (clange '1', FP32
+// moduli summed in FP64) and amax[b] = max |a_rc|.  One wave per column;
+// both must be zero on entry.
 __global__ __launch_bounds__(256) void k_norm1(const float *A, int64_t lda, int64_t a_stride, int n,
-                                               double *norms) {
+                                               double *norms, double *amax) {
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int b = blockIdx.y;
     if (c >= n) return;
     const float *Ab = A + (size_t)b * a_stride * 2;
     double s = 0.0;
+    float m = 0.0f;
     for (int r = lane; r < n; r += 64) {
         const c32 v = ld(Ab, (size_t)c * lda + r);
-        s += sqrt((double)v.re * v.re + (double)v.im * v.im);
+        const float a = __builtin_sqrtf(fmaf(v.re, v.re, v.im * v.im));
+        s += (double)a;
+        m = fmaxf(m, a);
     }
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0)
+    for (int o = 32; o >= 1; o >>= 1) {
+        s += __shfl_xor(s, o);
+        m = fmaxf(m, __shfl_xor(m, o));
+    }
+    if (lane == 0) {
         atomicMax(reinterpret_cast<unsigned long long *>(norms + b),
                   static_cast<unsigned long long>(__double_as_longlong(s)));
+        atomicMax(reinterpret_cast<unsigned long long *>(amax + b),
+                  static_cast<unsigned long long>(__double_as_longlong((double)m)));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -291,13 +304,15 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
     if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
 }
 
-// Row exchanges of the step at column k (its gather list) and the
-// unit-lower solve U12 = L11^{-1} A12, on columns [c_begin, c_end), all right
-// of the panel; 64 columns per block.  The gathered rows are staged in LDS
+// Row exchanges of the step at column k (its gather list) and/or the
+// unit-lower solve U12 = L11^{-1} A12 with that step's diagonal block, on
+// columns [c_begin, c_end); 64 columns per block.  mode 0: exchanges only,
+// 1: exchanges then solve, 2: solve only.  The rows are staged in LDS
 // (loads coalesced along each column), lane = column for the solve.
 __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int64_t a_stride,
                                                       int k, int c_begin, int c_end,
-                                                      const int32_t *gl, int64_t gl_stride) {
+                                                      const int32_t *gl, int64_t gl_stride,
+                                                      int mode) {
     const int b = blockIdx.y;
     float *Ab = A + (size_t)b * a_stride * 2;
     const int cb = c_begin + blockIdx.x * 64;
@@ -308,17 +323,18 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int
     __shared__ int cnt_s;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t *g = gl + (size_t)b * gl_stride;
-    if (tid == 0) cnt_s = g[0];
+    if (tid == 0) cnt_s = (mode == 2) ? kNB : g[0];
     __syncthreads();
     const int cnt = cnt_s;
     if (tid < cnt) {
-        trow[tid] = g[1 + tid];
-        tsrc[tid] = g[1 + 2 * kNB + tid];
+        trow[tid] = (mode == 2) ? k + tid : g[1 + tid];
+        tsrc[tid] = (mode == 2) ? k + tid : g[1 + 2 * kNB + tid];
     }
-    for (int e = tid; e < kNB * kNB; e += 256) {
-        const int i = e % kNB, j = e / kNB;
-        L11[i][j] = ld(Ab, (size_t)(k + j) * lda + k + i);
-    }
+    if (mode != 0)
+        for (int e = tid; e < kNB * kNB; e += 256) {
+            const int i = e % kNB, j = e / kNB;
+            L11[i][j] = ld(Ab, (size_t)(k + j) * lda + k + i);
+        }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
@@ -326,7 +342,7 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int
         if (cc < ncols && lane < cnt) tile[lane][cc] = ld(Ab, (size_t)(cb + cc) * lda + tsrc[lane]);
     }
     __syncthreads();
-    if (tid < ncols) {
+    if (mode != 0 && tid < ncols) {
         c32 x[kNB];
 #pragma unroll
         for (int i = 0; i < kNB; ++i) x[i] = tile[i][tid];
@@ -352,49 +368,68 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int
     }
 }
 
-// MFMA trailing update A22 -= L21 * U12 (FP32 matrix cores,
-// v_mfma_f32_16x16x4_f32) over rows [k + kNB, N) and columns
-// [c_begin, c_end).  64 x 64 complex output tile per block; each of the 4
-// waves owns a 32 x 32 sub-tile as 2 x 2 MFMA tiles with separate real and
-// imaginary accumulators (re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).  The
-// MFMA computes the transposed tile C^T = U^T L^T so that the 16 lanes of an
-// output row group cover 16 consecutive matrix rows (coalesced column-major
-// stores).  Operands are staged in LDS as separate re/im planes with an
-// 80-float row stride (conflict-free reads of the 4 k-rows a wave touches
-// per instruction).
+// Trailing update C -= L * U on MFMA (FP32 accumulation) over rows
+// [r_begin, r_end), columns [c_begin, c_end) and inner range
+// [k_begin, k_end) (a multiple of kNB): L = A[rows][k range],
+// U = A[k range][cols].  64 x 64 complex output tile per block; each of the
+// 4 waves owns a 32 x 32 sub-tile as 2 x 2 MFMA tiles with separate real
+// and imaginary accumulators (re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).
+// The MFMA computes the transposed tile C^T = U^T L^T so that the 16 lanes
+// of an output row group cover 16 consecutive matrix rows (coalesced
+// column-major stores).  The inner range is staged through LDS kNB at a
+// time.
+//   kPrec = 0: FP32 operands, v_mfma_f32_16x16x4_f32, planes [k][row|col]
+//              with an 80-float stride (conflict-free reads of the 4 k-rows
+//              a wave touches per instruction);
+//   kPrec = 1: FP16 operands, v_mfma_f32_16x16x16_f16; U is multiplied by
+//              the power of two scale[b] before rounding (FP16 range) and the
+//              product by 1/scale[b];
+//   kPrec = 2: BF16 operands (round to nearest even),
+//              v_mfma_f32_16x16x16_bf16.
+//   FP16/BF16 planes are [row|col][k] so that the 4 consecutive k of a lane
+//   are one 8-byte LDS read.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int kGS = 80;
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+constexpr int kGS = 80;   // FP32 plane stride (floats)
+constexpr int kHS = 40;   // 16-bit plane stride (elements)
 
-__global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int64_t a_stride,
-                                                      int N, int k, int c_begin, int c_end) {
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float x) {
+    const unsigned u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<unsigned short>(u >> 16);  // inf/nan
+    const unsigned r = u + 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(r >> 16);
+}
+
+template <int kPrec>
+struct GemmLds;
+template <>
+struct GemmLds<0> {
+    float Lr[kNB][kGS], Li[kNB][kGS], Ur[kNB][kGS], Ui[kNB][kGS];
+};
+template <>
+struct GemmLds<1> {
+    _Float16 Lr[64][kHS], Li[64][kHS], Ur[64][kHS], Ui[64][kHS];
+};
+template <>
+struct GemmLds<2> {
+    unsigned short Lr[64][kHS], Li[64][kHS], Ur[64][kHS], Ui[64][kHS];
+};
+
+template <int kPrec>
+__global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t a_stride,
+                                                 int r_begin, int r_end, int c_begin, int c_end,
+                                                 int k_begin, int k_end, const float *scale) {
     const int b = blockIdx.z;
     float *Ab = A + (size_t)b * a_stride * 2;
-    const int s = k + kNB;
-    const int r0 = s + blockIdx.x * 64;
+    const int r0 = r_begin + blockIdx.x * 64;
     const int c0 = c_begin + blockIdx.y * 64;
-    __shared__ float Lr[kNB][kGS], Li[kNB][kGS], Ur[kNB][kGS], Ui[kNB][kGS];
+    __shared__ GemmLds<kPrec> sm;
     const int tid = threadIdx.x;
-#pragma unroll
-    for (int it = 0; it < (kNB * 64) / 256; ++it) {
-        const int e = tid + 256 * it;
-        const int kk = e / 64, rr = e % 64;
-        const c32 v = (r0 + rr < N) ? ld(Ab, (size_t)(k + kk) * lda + r0 + rr) : c32{0.f, 0.f};
-        Lr[kk][rr] = v.re;
-        Li[kk][rr] = v.im;
-    }
-#pragma unroll
-    for (int it = 0; it < (kNB * 64) / 256; ++it) {
-        const int e = tid + 256 * it;
-        const int cc = e / kNB, kk = e % kNB;
-        const c32 v =
-            (c0 + cc < c_end) ? ld(Ab, (size_t)(c0 + cc) * lda + k + kk) : c32{0.f, 0.f};
-        Ur[kk][cc] = v.re;
-        Ui[kk][cc] = v.im;
-    }
-    __syncthreads();
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
+    const float us = (kPrec == 1) ? scale[b] : 1.0f;
     f32x4 accr[2][2], acci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -403,28 +438,104 @@ __global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int
             accr[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
             acci[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
+    for (int k = k_begin; k < k_end; k += kNB) {
+        __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < kNB; ks += 4) {
-        const int kk = ks + lk;
-        float ur[2], ui[2], lr[2], lim[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            ur[t] = Ur[kk][wc + 16 * t + li];
-            ui[t] = Ui[kk][wc + 16 * t + li];
-            lr[t] = Lr[kk][wr + 16 * t + li];
-            lim[t] = Li[kk][wr + 16 * t + li];
+        for (int it = 0; it < (kNB * 64) / 256; ++it) {
+            const int e = tid + 256 * it;
+            const int kk = e / 64, rr = e % 64;
+            const c32 v =
+                (r0 + rr < r_end) ? ld(Ab, (size_t)(k + kk) * lda + r0 + rr) : c32{0.f, 0.f};
+            if constexpr (kPrec == 0) {
+                sm.Lr[kk][rr] = v.re;
+                sm.Li[kk][rr] = v.im;
+            } else if constexpr (kPrec == 1) {
+                sm.Lr[rr][kk] = static_cast<_Float16>(v.re);
+                sm.Li[rr][kk] = static_cast<_Float16>(v.im);
+            } else {
+                sm.Lr[rr][kk] = f32_to_bf16_rne(v.re);
+                sm.Li[rr][kk] = f32_to_bf16_rne(v.im);
+            }
         }
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-            for (int tj = 0; tj < 2; ++tj) {
-                // D^T[col][row]: A operand = U^T (col, k), B operand = L^T (k, row)
-                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
-                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[ti], lim[tj], accr[ti][tj], 0, 0, 0);
-                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
-                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
+        for (int it = 0; it < (kNB * 64) / 256; ++it) {
+            const int e = tid + 256 * it;
+            const int cc = e / kNB, kk = e % kNB;
+            const c32 v =
+                (c0 + cc < c_end) ? ld(Ab, (size_t)(c0 + cc) * lda + k + kk) : c32{0.f, 0.f};
+            if constexpr (kPrec == 0) {
+                sm.Ur[kk][cc] = v.re;
+                sm.Ui[kk][cc] = v.im;
+            } else if constexpr (kPrec == 1) {
+                sm.Ur[cc][kk] = static_cast<_Float16>(v.re * us);
+                sm.Ui[cc][kk] = static_cast<_Float16>(v.im * us);
+            } else {
+                sm.Ur[cc][kk] = f32_to_bf16_rne(v.re);
+                sm.Ui[cc][kk] = f32_to_bf16_rne(v.im);
             }
+        }
+        __syncthreads();
+        if constexpr (kPrec == 0) {
+#pragma unroll
+            for (int ks = 0; ks < kNB; ks += 4) {
+                const int kk = ks + lk;
+                float ur[2], ui[2], lr[2], lim[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    ur[t] = sm.Ur[kk][wc + 16 * t + li];
+                    ui[t] = sm.Ui[kk][wc + 16 * t + li];
+                    lr[t] = sm.Lr[kk][wr + 16 * t + li];
+                    lim[t] = sm.Li[kk][wr + 16 * t + li];
+                }
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj) {
+                        // D^T[col][row]: A operand = U^T (col, k), B operand = L^T (k, row)
+                        accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
+                        accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[ti], lim[tj], accr[ti][tj], 0, 0, 0);
+                        acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
+                        acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
+                    }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < kNB; ks += 16) {
+                const int kk = ks + 4 * lk;
+                using V = typename std::conditional<kPrec == 1, f16x4, i16x4>::type;
+                V ur[2], ui[2], uin[2], lr[2], lim[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    ur[t] = *reinterpret_cast<const V *>(&sm.Ur[wc + 16 * t + li][kk]);
+                    ui[t] = *reinterpret_cast<const V *>(&sm.Ui[wc + 16 * t + li][kk]);
+                    lr[t] = *reinterpret_cast<const V *>(&sm.Lr[wr + 16 * t + li][kk]);
+                    lim[t] = *reinterpret_cast<const V *>(&sm.Li[wr + 16 * t + li][kk]);
+                    // -Ui: flip the sign bits of the four 16-bit values
+                    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+                    u32x2 bits = *reinterpret_cast<const u32x2 *>(&ui[t]);
+                    bits ^= u32x2{0x80008000u, 0x80008000u};
+                    uin[t] = *reinterpret_cast<const V *>(&bits);
+                }
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj) {
+                        if constexpr (kPrec == 1) {
+                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
+                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(uin[ti], lim[tj], accr[ti][tj], 0, 0, 0);
+                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
+                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
+                        } else {
+                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
+                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(uin[ti], lim[tj], accr[ti][tj], 0, 0, 0);
+                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
+                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
+                        }
+                    }
+            }
+        }
     }
+    const float inv_us = 1.0f / us;  // exact: us is a power of two
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -433,14 +544,26 @@ __global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int
             for (int v = 0; v < 4; ++v) {
                 const int col = c0 + wc + 16 * ti + 4 * lk + v;
                 const int row = r0 + wr + 16 * tj + li;
-                if (row < N && col < c_end) {
+                if (row < r_end && col < c_end) {
                     const size_t idx = (size_t)col * lda + row;
                     c32 x = ld(Ab, idx);
-                    x.re -= accr[ti][tj][v];
-                    x.im -= acci[ti][tj][v];
+                    x.re -= accr[ti][tj][v] * inv_us;
+                    x.im -= acci[ti][tj][v] * inv_us;
                     st(Ab, idx, x);
                 }
             }
+}
+
+// Power-of-two scale 2^-e per system with 2^e ~ max |a_ij| (FP16 operand
+// range of the trailing update): scale[b] from the max-abs bits in amax[b].
+__global__ void k_pow2_scale(const double *amax, float *scale, int B) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= B) return;
+    const double m = amax[b];
+    int e = 0;
+    if (m > 0.0 && m < 1e300) (void)frexp(m, &e);
+    e = max(-100, min(100, e));
+    scale[b] = ldexpf(1.0f, -e);
 }
 
 // ---------------------------------------------------------------------------
@@ -450,8 +573,9 @@ __global__ __launch_bounds__(256) void k_lu_gemm_mfma(float *A, int64_t lda, int
 // to the other rows is a block GEMV over all threads.
 //   trans 0: x <- A^{-1} x = U^{-1} L~_{S-1}^{-1} P_{S-1}^T ... L~_0^{-1} P_0^T x
 //   trans 2: x <- A^{-H} x = P_0 L~_0^{-H} ... P_{S-1} L~_{S-1}^{-H} U^{-H} x
-// where P_s^T applies the exchanges of step s in order and L~_s is the unit
-// lower block column of step s in that step's row order.
+// where s runs over the outer blocks of kLuOuter columns, P_s^T applies the
+// exchanges of block s in order and L~_s is the unit lower block column of
+// block s in the row order at the end of that block.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void stage_diag(const float *Ab, int64_t lda, int kb, c32 (*D)[kNB + 1]) {
@@ -492,7 +616,7 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
     extern __shared__ c32 xs[];
     __shared__ c32 D[kNB][kNB + 1];
     __shared__ c32 part[kNB][kNB + 1];
-    __shared__ int pv_s[kNB];
+    __shared__ int pv_s[kLuOuter];
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -502,11 +626,13 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
     for (int r = tid; r < N; r += 1024) xs[r] = ld(xb, r);
     if (trans == 0) {
         for (int kb = 0; kb < N; kb += kNB) {
-            if (tid < kNB) pv_s[tid] = pv[kb + tid];
+            const int xw = min(kLuOuter, N - kb);  // exchanges of an outer block
+            const bool xstart = (kb % kLuOuter) == 0;
+            if (xstart && tid < xw) pv_s[tid] = pv[kb + tid];
             stage_diag(Ab, lda, kb, D);
             __syncthreads();
-            if (tid == 0)
-                for (int j = 0; j < kNB; ++j) {
+            if (xstart && tid == 0)
+                for (int j = 0; j < xw; ++j) {
                     const int p = pv_s[j];
                     if (p != kb + j) {
                         const c32 t = xs[kb + j];
@@ -593,11 +719,14 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
                 if (lane < kNB) xs[kb + lane] = x;
             }
         }
-        // L~_s^{-H} then P_s, s = S-1 .. 0.
+        // L~_s^{-H} then P_s, s = S-1 .. 0 (outer blocks, their kNB-row
+        // sub-blocks from the last).
         for (int kb = N - kNB; kb >= 0; kb -= kNB) {
             __syncthreads();
             if (kb + kNB < N) block_gemv_h(Ab, lda, kb, kb + kNB, N, xs, part);
-            if (tid < kNB) pv_s[tid] = pv[kb + tid];
+            const int xw = min(kLuOuter, N - kb);
+            const bool xstart = (kb % kLuOuter) == 0;
+            if (xstart && tid < xw) pv_s[tid] = pv[kb + tid];
             stage_diag(Ab, lda, kb, D);
             __syncthreads();
             if (tid < 64) {
@@ -613,8 +742,8 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
                 if (lane < kNB) xs[kb + lane] = x;
             }
             __syncthreads();
-            if (tid == 0)
-                for (int j = kNB - 1; j >= 0; --j) {
+            if (xstart && tid == 0)
+                for (int j = xw - 1; j >= 0; --j) {
                     const int p = pv_s[j];
                     if (p != kb + j) {
                         const c32 t = xs[kb + j];
@@ -732,31 +861,68 @@ void launch_pad_identity(float *A, int64_t lda, int64_t a_stride, int n, int N, 
 }
 
 void launch_norm1(const float *A, int64_t lda, int64_t a_stride, int n, int B, double *norms,
-                  hipStream_t stream) {
+                  double *amax, hipStream_t stream) {
     launch_fill_zero_f64(norms, (size_t)B, stream);
+    launch_fill_zero_f64(amax, (size_t)B, stream);
     hipLaunchKernelGGL(k_norm1, dim3((n + 3) / 4, B), dim3(256), 0, stream, A, lda, a_stride, n,
-                       norms);
+                       norms, amax);
 }
 
-void lu_factor(const LuWorkspace &w, hipStream_t stream) {
-    static_assert(kLuPad % kNB == 0, "padded order must hold whole panels");
+void launch_pow2_scale(const double *amax, float *scale, int B, hipStream_t stream) {
+    hipLaunchKernelGGL(k_pow2_scale, dim3((B + 255) / 256), dim3(256), 0, stream, amax, scale, B);
+}
+
+// Two-level blocked LU: outer blocks of kLuOuter columns are factored with
+// kNB-wide panels, their exchanges, unit-lower solves and rank-kNB updates
+// (FP32) inside the block; the rows of the block right of it then get all
+// of the block's exchanges, the block-triangular solve (kNB rows at a time
+// with FP32 rank-kNB updates of the block's remaining rows), and one
+// rank-kLuOuter update of the trailing matrix in the requested precision.
+void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStream_t stream) {
+    static_assert(kLuPad % kNB == 0 && kLuOuter % kNB == 0, "whole panels");
     // int32 zero has the bit pattern of float zero
     launch_fill_zero(reinterpret_cast<float *>(w.info), (size_t)w.B, stream);
-    const int64_t gls = (int64_t)(w.N / kNB) * kLuGather;
-    for (int k = 0; k < w.N; k += kNB) {
-        int32_t *glk = w.gl + (size_t)(k / kNB) * kLuGather;
-        if (w.N - k <= 1024)
-            hipLaunchKernelGGL(k_lu_panel_reg, dim3(w.B), dim3(1024), 0, stream, w.A, w.lda,
-                               w.a_stride, w.N, k, w.ipiv, w.info, glk, gls);
+    const int N = w.N, B = w.B;
+    const int64_t gls = (int64_t)(N / kNB) * kLuGather;
+    auto gl_of = [&](int k) { return w.gl + (size_t)(k / kNB) * kLuGather; };
+    auto swap_trsm = [&](int k, int c0, int c1, int mode) {
+        if (c1 <= c0) return;
+        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((c1 - c0 + 63) / 64, B), dim3(256), 0, stream,
+                           w.A, w.lda, w.a_stride, k, c0, c1, gl_of(k), gls, mode);
+    };
+    auto gemm = [&](int prec, int r0, int r1, int c0, int c1, int k0, int k1) {
+        if (r1 <= r0 || c1 <= c0) return;
+        const dim3 grid((r1 - r0 + 63) / 64, (c1 - c0 + 63) / 64, B);
+        if (prec == 1)
+            hipLaunchKernelGGL(k_lu_gemm<1>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride,
+                               r0, r1, c0, c1, k0, k1, scale);
+        else if (prec == 2)
+            hipLaunchKernelGGL(k_lu_gemm<2>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride,
+                               r0, r1, c0, c1, k0, k1, scale);
         else
-            hipLaunchKernelGGL(k_lu_panel, dim3(w.B), dim3(1024), 0, stream, w.A, w.lda,
-                               w.a_stride, w.N, k, w.ipiv, w.info, glk, gls);
-        const int s = k + kNB, rest = w.N - s;
-        if (rest > 0) {
-            hipLaunchKernelGGL(k_lu_swap_trsm, dim3((rest + 63) / 64, w.B), dim3(256), 0, stream,
-                               w.A, w.lda, w.a_stride, k, s, w.N, glk, gls);
-            hipLaunchKernelGGL(k_lu_gemm_mfma, dim3((rest + 63) / 64, (rest + 63) / 64, w.B),
-                               dim3(256), 0, stream, w.A, w.lda, w.a_stride, w.N, k, s, w.N);
+            hipLaunchKernelGGL(k_lu_gemm<0>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride,
+                               r0, r1, c0, c1, k0, k1, scale);
+    };
+    for (int K0 = 0; K0 < N; K0 += kLuOuter) {
+        const int K1 = std::min(N, K0 + kLuOuter);
+        for (int k = K0; k < K1; k += kNB) {
+            if (N - k <= 1024)
+                hipLaunchKernelGGL(k_lu_panel_reg, dim3(B), dim3(1024), 0, stream, w.A, w.lda,
+                                   w.a_stride, N, k, w.ipiv, w.info, gl_of(k), gls);
+            else
+                hipLaunchKernelGGL(k_lu_panel, dim3(B), dim3(1024), 0, stream, w.A, w.lda,
+                                   w.a_stride, N, k, w.ipiv, w.info, gl_of(k), gls);
+            swap_trsm(k, K0, k, 0);
+            swap_trsm(k, k + kNB, K1, 1);
+            gemm(0, k + kNB, N, k + kNB, K1, k, k + kNB);
+        }
+        if (K1 < N) {
+            for (int k = K0; k < K1; k += kNB) swap_trsm(k, K1, N, 0);
+            for (int k = K0; k < K1; k += kNB) {
+                swap_trsm(k, K1, N, 2);
+                gemm(0, k + kNB, K1, K1, N, k, k + kNB);
+            }
+            gemm(precision, K1, N, K1, N, K0, K1);
         }
     }
 }

@@ -191,3 +191,50 @@ def test_refinement_reaches_fp64_residual(require_gpu):
         ref = np.linalg.solve(A[s].astype(np.complex128), b[s].astype(np.complex128))
         assert rel_l2(x[s], ref) < 2e-6, rel_l2(x[s], ref)
         assert info[s]["dense_solve_refine_iterations"] >= 1
+
+
+def test_dense_solve_two_level_and_global_panel(require_gpu):
+    """Orders with several outer blocks and one with N > 1024 (first panels
+    factored in global memory) against numpy."""
+    from hornlab_metal_bem.native import dense_solve
+
+    rng = np.random.default_rng(11)
+    for n, count in ((700, 2), (1100, 1)):
+        A = (rng.standard_normal((count, n, n)) + 1j * rng.standard_normal((count, n, n))).astype(np.complex64)
+        b = (rng.standard_normal((count, n)) + 1j * rng.standard_normal((count, n))).astype(np.complex64)
+        x, info = dense_solve(A, b, estimate_rcond=True)
+        for s in range(count):
+            A64 = A[s].astype(np.complex128)
+            ref = np.linalg.solve(A64, b[s].astype(np.complex128))
+            # FP32 LU: error ~ cond * eps32
+            cond = np.linalg.cond(A64)
+            assert rel_l2(x[s], ref) < max(1e-4, 50 * cond * 6e-8), (n, s, rel_l2(x[s], ref), cond)
+            assert info[s]["lapack_info"] == 0
+            rc_ref = 1.0 / (np.linalg.norm(A64, 1) * np.linalg.norm(np.linalg.inv(A64), 1))
+            # lower bound of ||A^-1||_1 up to the FP32 solves inside the
+            # estimator (cond ~ 1e5 here), within ~3x
+            est = info[s]["dense_solve_rcond"]
+            assert rc_ref <= est * 1.01 and est < 3.0 * rc_ref, (rc_ref, est)
+
+
+@pytest.mark.parametrize("mode,raw_floor", [("mixed_fp16", 1e-6), ("mixed_bf16", 1e-5)])
+def test_mixed_precision_lu_refines_to_fp64_accuracy(require_gpu, mode, raw_floor):
+    """FP16/BF16 trailing updates: the unrefined solve is measurably less
+    accurate than FP32 (the low precision is in use), FP64-residual
+    refinement recovers the FP64 solution."""
+    from hornlab_metal_bem.native import dense_solve
+
+    rng = np.random.default_rng(5)
+    n = 600
+    A = ((rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))) / np.sqrt(n)).astype(np.complex64)
+    A += (2.0 * np.eye(n, dtype=np.complex64))[None]
+    b = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(np.complex64)
+    x0, _ = dense_solve(A, b, solve_mode=mode, refine_max_iterations=0)
+    x32, _ = dense_solve(A, b, solve_mode="fp32")
+    x, info = dense_solve(A, b, solve_mode=mode, refine_max_iterations=20, refine_tolerance=1e-12)
+    for s in range(2):
+        ref = np.linalg.solve(A[s].astype(np.complex128), b[s].astype(np.complex128))
+        e0, e32, e = rel_l2(x0[s], ref), rel_l2(x32[s], ref), rel_l2(x[s], ref)
+        assert e0 > max(raw_floor, 3 * e32), (mode, e0, e32)
+        assert e < 1e-6, (mode, e0, e)
+        assert info[s]["dense_solve_refine_iterations"] >= 1
