@@ -9,6 +9,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -136,8 +137,8 @@ struct hbem_session {
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
         tile_count, tile_ndofs, tile_dofs, tile_pairs;
     // per-call workspaces
-    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, amax, lu_scale, q, kre, kim, fk,
-        obs, field, tags,
+    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, amax, lu_scale, kv, kh, kn, kf,
+        q, kre, kim, fk, obs, field, tags,
         tagout, norms, anorm, active, rowsplit, rcx;
     // staging of host-input batches
     DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag;
@@ -147,7 +148,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &amax, &lu_scale, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag})
@@ -522,6 +523,170 @@ int32_t hbem_assemble(hbem_session *s, float k_real, float k_imag, const float *
 
 namespace {
 
+// GMRES-based iterative refinement (Carson-Higham GMRES-IR): residuals
+// r = b - A x in FP64 with the FP32 operator, correction equations
+// M^{-1} A d = M^{-1} r solved by GMRES with M the (low-precision) LU applied
+// in FP32, Arnoldi vectors and classical Gram-Schmidt with one
+// re-orthogonalisation pass in FP64 on the device; Givens rotations and the
+// small least-squares solve on the host.  x64 / r64 hold the iterate and its
+// residual on entry and exit; best[b] the relative residual ||r||/||b||
+// (inf-norms).  Returns the Arnoldi steps taken per system.
+void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int n, int N,
+                  const hbem_solve_options &opt, const std::function<void()> &residual,
+                  std::vector<double> &nrm, std::vector<double> &best,
+                  std::vector<int32_t> &steps) {
+    using cd = std::complex<double>;
+    hipStream_t st = s->stream;
+    const int64_t lda = N, a_stride = (int64_t)N * N;
+    const int m = std::max(1, std::min(opt.refine_max_iterations, 20));
+    const int max_restarts = 4;
+    const int64_t vstride = (int64_t)(m + 1) * N;
+    s->kv.reserve(sizeof(double) * 2 * (size_t)B * vstride);
+    s->kh.reserve(sizeof(double) * 2 * (size_t)B * (m + 1) * 2);
+    s->kn.reserve(sizeof(double) * B);
+    s->kf.reserve(sizeof(double) * B);
+    s->active.reserve(sizeof(int32_t) * B);
+    double *V = s->kv.as<double>();
+    double *h1 = s->kh.as<double>(), *h2 = h1 + (size_t)B * (m + 1) * 2;
+    auto vec = [&](int i) { return V + (size_t)i * N * 2; };  // vector i of system 0
+    std::vector<int32_t> active(B);
+    std::vector<double> hn(B), fac(B), hbuf((size_t)B * (m + 1) * 2 * 2);
+    // M^{-1} src (FP64, stride N) -> basis vector i (FP64)
+    auto precond = [&](const double *src, int64_t ld_src, int i) {
+        hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * B * N, st);
+        hbem::launch_copy_f64_to_f32(src, s->x.as<float>(), ld_src, N, n, B, st);
+        hbem::lu_solve(w, s->x.as<float>(), N, 0, st);
+        hbem::launch_copy_f32_to_f64(s->x.as<float>(), vec(i), N, vstride, n, B, st);
+    };
+    auto norms_of = [&](int i) {
+        hbem::launch_norm2sq_f64(vec(i), vstride, n, B, s->kn.as<double>(), st);
+        hip_check(hipMemcpyAsync(hn.data(), s->kn.ptr, sizeof(double) * B, hipMemcpyDeviceToHost,
+                                 st),
+                  "norm D2H");
+        hip_check(hipStreamSynchronize(st), "gmres sync");
+    };
+    auto scale_vec = [&](int i) {
+        hip_check(hipMemcpyAsync(s->kf.ptr, fac.data(), sizeof(double) * B, hipMemcpyHostToDevice,
+                                 st),
+                  "scale H2D");
+        hbem::launch_scale_f64(vec(i), vstride, vec(i), vstride, n, B, s->kf.as<double>(), st);
+    };
+    for (int restart = 0; restart < max_restarts; ++restart) {
+        bool any = false;
+        for (int b = 0; b < B; ++b) {
+            active[b] = best[b] > opt.refine_tolerance ? 1 : 0;
+            any |= active[b] != 0;
+        }
+        if (!any) break;
+        hbem::launch_fill_zero_f64(V, (size_t)2 * B * vstride, st);
+        precond(s->r64.as<double>(), N, 0);
+        norms_of(0);
+        std::vector<double> beta(B);
+        for (int b = 0; b < B; ++b) {
+            beta[b] = std::sqrt(hn[b]);
+            fac[b] = beta[b] > 0.0 ? 1.0 / beta[b] : 0.0;
+        }
+        scale_vec(0);
+        std::vector<std::vector<cd>> H(B, std::vector<cd>((size_t)(m + 1) * m, cd(0.0)));
+        std::vector<std::vector<cd>> g(B, std::vector<cd>(m + 1, cd(0.0)));
+        std::vector<std::vector<double>> cs(B, std::vector<double>(m, 0.0));
+        std::vector<std::vector<cd>> sn(B, std::vector<cd>(m, cd(0.0)));
+        std::vector<int> kk(B, 0);
+        std::vector<char> done(B, 0);
+        for (int b = 0; b < B; ++b) {
+            g[b][0] = cd(beta[b], 0.0);
+            if (!active[b] || beta[b] == 0.0) done[b] = 1;
+        }
+        int j = 0;
+        for (; j < m; ++j) {
+            // z = M^{-1} A v_j into v_{j+1}, then CGS2 against v_0..v_j
+            hbem::launch_gemv_f64(s->A_orig.as<float>(), lda, a_stride, n, B, vec(j), vstride,
+                                  s->r64.as<double>() + 0, N, st);
+            precond(s->r64.as<double>(), N, j + 1);
+            hbem::launch_dots_f64(V, N, vstride, j + 1, vec(j + 1), vstride, n, B, h1, m + 1, st);
+            hbem::launch_combine_f64(V, N, vstride, j + 1, h1, m + 1, -1.0, vec(j + 1), vstride, n,
+                                     B, nullptr, st);
+            hbem::launch_dots_f64(V, N, vstride, j + 1, vec(j + 1), vstride, n, B, h2, m + 1, st);
+            hbem::launch_combine_f64(V, N, vstride, j + 1, h2, m + 1, -1.0, vec(j + 1), vstride, n,
+                                     B, nullptr, st);
+            hip_check(hipMemcpyAsync(hbuf.data(), h1, sizeof(double) * hbuf.size(),
+                                     hipMemcpyDeviceToHost, st),
+                      "h D2H");
+            norms_of(j + 1);
+            bool all_done = true;
+            for (int b = 0; b < B; ++b) {
+                const double hnext = std::sqrt(hn[b]);
+                fac[b] = 0.0;
+                if (done[b]) continue;
+                std::vector<cd> col(j + 2);
+                for (int i = 0; i <= j; ++i) {
+                    const size_t o1 = ((size_t)b * (m + 1) + i) * 2;
+                    const size_t o2 = o1 + (size_t)B * (m + 1) * 2;
+                    col[i] = cd(hbuf[o1] + hbuf[o2], hbuf[o1 + 1] + hbuf[o2 + 1]);
+                }
+                col[j + 1] = cd(hnext, 0.0);
+                // previous rotations, then a new one zeroing col[j+1]
+                for (int i = 0; i < j; ++i) {
+                    const cd t = cs[b][i] * col[i] + sn[b][i] * col[i + 1];
+                    col[i + 1] = -std::conj(sn[b][i]) * col[i] + cs[b][i] * col[i + 1];
+                    col[i] = t;
+                }
+                const double a = std::abs(col[j]), hb = std::abs(col[j + 1]);
+                const double den = std::hypot(a, hb);
+                if (den == 0.0) {
+                    done[b] = 1;
+                    continue;
+                }
+                cs[b][j] = a / den;
+                sn[b][j] = (a == 0.0) ? cd(1.0, 0.0) : (col[j] / a) * std::conj(col[j + 1]) / den;
+                col[j] = cs[b][j] * col[j] + sn[b][j] * col[j + 1];
+                col[j + 1] = cd(0.0);
+                g[b][j + 1] = -std::conj(sn[b][j]) * g[b][j];
+                g[b][j] = cs[b][j] * g[b][j];
+                for (int i = 0; i <= j; ++i) H[b][(size_t)i * m + j] = col[i];
+                kk[b] = j + 1;
+                steps[b]++;
+                if (std::abs(g[b][j + 1]) <= 1e-13 * beta[b] || hnext <= 1e-300) {
+                    done[b] = 1;
+                } else {
+                    fac[b] = 1.0 / hnext;
+                }
+                all_done &= done[b] != 0;
+            }
+            if (all_done) break;
+            scale_vec(j + 1);
+        }
+        // y = R^{-1} g per system, x += V y
+        std::vector<double> ybuf((size_t)B * (m + 1) * 2, 0.0);
+        for (int b = 0; b < B; ++b) {
+            const int k = kk[b];
+            std::vector<cd> y(k);
+            for (int i = k - 1; i >= 0; --i) {
+                cd acc = g[b][i];
+                for (int l = i + 1; l < k; ++l) acc -= H[b][(size_t)i * m + l] * y[l];
+                y[i] = acc / H[b][(size_t)i * m + i];
+            }
+            for (int i = 0; i < k; ++i) {
+                ybuf[((size_t)b * (m + 1) + i) * 2] = y[i].real();
+                ybuf[((size_t)b * (m + 1) + i) * 2 + 1] = y[i].imag();
+            }
+        }
+        hip_check(hipMemcpyAsync(h1, ybuf.data(), sizeof(double) * ybuf.size(),
+                                 hipMemcpyHostToDevice, st),
+                  "y H2D");
+        hip_check(hipMemcpyAsync(s->active.ptr, active.data(), sizeof(int32_t) * B,
+                                 hipMemcpyHostToDevice, st),
+                  "active H2D");
+        hbem::launch_combine_f64(V, N, vstride, m + 1, h1, m + 1, 1.0, s->x64.as<double>(), N, n,
+                                 B, s->active.as<int32_t>(), st);
+        residual();
+        for (int b = 0; b < B; ++b) {
+            const double rel = nrm[2 * b + 1] > 0.0 ? nrm[2 * b] / nrm[2 * b + 1] : nrm[2 * b];
+            if (active[b]) best[b] = rel;
+        }
+    }
+}
+
 // Factor + solve + optional refinement of the B systems resident in s->A
 // (column-major, order N, true order n) with right-hand sides s->rhs.
 // Solution (FP32, interleaved, stride N) ends in s->x.  Fills diagnostics.
@@ -596,7 +761,13 @@ void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_optio
             best[b] = nrm[2 * b + 1] > 0.0 ? nrm[2 * b] / nrm[2 * b + 1] : nrm[2 * b];
             if (best[b] <= opt.refine_tolerance) active[b] = 0;
         }
-        for (int it = 0; it < opt.refine_max_iterations; ++it) {
+        // mixed precision: GMRES-IR; FP32 LU: classical refinement as the
+        // reference's FP64 dense mode (main.swift:6964-7070)
+        if (precision != 0) {
+            gmres_refine(s, w, B, n, N, opt, residual, nrm, best, iters);
+            std::fill(active.begin(), active.end(), 0);
+        }
+        for (int it = 0; it < opt.refine_max_iterations && precision == 0; ++it) {
             bool any = false;
             for (int b = 0; b < B; ++b) any |= active[b] != 0;
             if (!any) break;

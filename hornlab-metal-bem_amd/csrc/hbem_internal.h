@@ -256,6 +256,22 @@ void launch_copy_f64_to_f32(const double *src, float *dst, int64_t ld_src, int64
 void launch_axpy_f32_to_f64(const float *d, double *x, int64_t ld_d, int64_t ld_x, int n, int B,
                             const int32_t *active, hipStream_t stream);
 
+// Batched FP64 Krylov helpers (vectors interleaved complex double, stride
+// ld per system; a basis V holds vector i of system b at (b * vstride +
+// i * ldv) complex elements).
+void launch_gemv_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B, const double *x,
+                     int64_t ldx, double *y, int64_t ldy, hipStream_t stream);  // y = A x
+void launch_dots_f64(const double *V, int64_t ldv, int64_t vstride, int m, const double *w,
+                     int64_t ldw, int n, int B, double *h, int64_t ldh,
+                     hipStream_t stream);  // h_i = V_i^H w, i < m
+void launch_combine_f64(const double *V, int64_t ldv, int64_t vstride, int m, const double *c,
+                        int64_t ldc, double sign, double *w, int64_t ldw, int n, int B,
+                        const int32_t *active, hipStream_t stream);  // w += sign V c
+void launch_norm2sq_f64(const double *w, int64_t ldw, int n, int B, double *out,
+                        hipStream_t stream);
+void launch_scale_f64(const double *src, int64_t lds, double *dst, int64_t ldd, int n, int B,
+                      const double *f, hipStream_t stream);  // dst = f[b] src
+
 // Copy a row-major split (re, im) host-layout matrix block on device into
 // the column-major interleaved LU layout and back.
 void launch_rowmajor_split_to_colmajor(const float *re, const float *im, int n, int B, float *A,

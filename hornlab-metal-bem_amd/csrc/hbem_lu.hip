@@ -304,67 +304,80 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
     if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
 }
 
-// Row exchanges of the step at column k (its gather list) and/or the
-// unit-lower solve U12 = L11^{-1} A12 with that step's diagonal block, on
-// columns [c_begin, c_end); 64 columns per block.  mode 0: exchanges only,
-// 1: exchanges then solve, 2: solve only.  The rows are staged in LDS
-// (loads coalesced along each column), lane = column for the solve.
+// Row exchanges and/or the unit-lower solve U12 = L11^{-1} A12 of panel
+// steps on two column ranges, 64 columns per block: blocks
+// [0, nblk(a)) take range [a0, a1) with mode_a, the rest [b0, b1) with
+// mode_b.  mode 0: the exchanges of the n_steps consecutive steps from
+// column k (their gather lists, applied in order); 1: the exchanges of step
+// k, then the solve with its diagonal block; 2: the solve only.  Rows are
+// staged in LDS (loads coalesced along each column), lane = column for the
+// solve.
 __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int64_t a_stride,
-                                                      int k, int c_begin, int c_end,
-                                                      const int32_t *gl, int64_t gl_stride,
-                                                      int mode) {
+                                                      int k, int n_steps, int a0, int a1,
+                                                      int mode_a, int b0, int b1, int mode_b,
+                                                      const int32_t *gl, int64_t gl_stride) {
     const int b = blockIdx.y;
     float *Ab = A + (size_t)b * a_stride * 2;
-    const int cb = c_begin + blockIdx.x * 64;
-    const int ncols = min(64, c_end - cb);
+    const int nblk_a = (a1 - a0 + 63) / 64;
+    const bool in_a = (int)blockIdx.x < nblk_a;
+    const int mode = in_a ? mode_a : mode_b;
+    const int cb = in_a ? a0 + blockIdx.x * 64 : b0 + (blockIdx.x - nblk_a) * 64;
+    const int ncols = min(64, (in_a ? a1 : b1) - cb);
     __shared__ c32 L11[kNB][kNB + 1];
     __shared__ c32 tile[2 * kNB][64 + 1];
     __shared__ int trow[2 * kNB], tsrc[2 * kNB];
     __shared__ int cnt_s;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int32_t *g = gl + (size_t)b * gl_stride;
-    if (tid == 0) cnt_s = (mode == 2) ? kNB : g[0];
-    __syncthreads();
-    const int cnt = cnt_s;
-    if (tid < cnt) {
-        trow[tid] = (mode == 2) ? k + tid : g[1 + tid];
-        tsrc[tid] = (mode == 2) ? k + tid : g[1 + 2 * kNB + tid];
-    }
+    const int steps = (mode == 0) ? n_steps : 1;
     if (mode != 0)
         for (int e = tid; e < kNB * kNB; e += 256) {
             const int i = e % kNB, j = e / kNB;
             L11[i][j] = ld(Ab, (size_t)(k + j) * lda + k + i);
         }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < 16; ++it) {
-        const int cc = wave + 4 * it;
-        if (cc < ncols && lane < cnt) tile[lane][cc] = ld(Ab, (size_t)(cb + cc) * lda + tsrc[lane]);
-    }
-    __syncthreads();
-    if (mode != 0 && tid < ncols) {
-        c32 x[kNB];
-#pragma unroll
-        for (int i = 0; i < kNB; ++i) x[i] = tile[i][tid];
-#pragma unroll
-        for (int i = 1; i < kNB; ++i) {
-            c32 acc = x[i];
-#pragma unroll
-            for (int j = 0; j < i; ++j) {
-                const c32 l = L11[i][j];
-                acc.re = fmaf(-l.re, x[j].re, fmaf(l.im, x[j].im, acc.re));
-                acc.im = fmaf(-l.re, x[j].im, fmaf(-l.im, x[j].re, acc.im));
-            }
-            x[i] = acc;
+    for (int st_i = 0; st_i < steps; ++st_i) {
+        const int ks = k + kNB * st_i;
+        const int32_t *g = gl + (size_t)b * gl_stride + (size_t)st_i * kLuGather;
+        if (tid == 0) cnt_s = (mode == 2) ? kNB : g[0];
+        __syncthreads();
+        const int cnt = cnt_s;
+        if (tid < cnt) {
+            trow[tid] = (mode == 2) ? ks + tid : g[1 + tid];
+            tsrc[tid] = (mode == 2) ? ks + tid : g[1 + 2 * kNB + tid];
         }
+        __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kNB; ++i) tile[i][tid] = x[i];
-    }
-    __syncthreads();
+        for (int it = 0; it < 16; ++it) {
+            const int cc = wave + 4 * it;
+            if (cc < ncols && lane < cnt)
+                tile[lane][cc] = ld(Ab, (size_t)(cb + cc) * lda + tsrc[lane]);
+        }
+        __syncthreads();
+        if (mode != 0 && tid < ncols) {
+            c32 x[kNB];
 #pragma unroll
-    for (int it = 0; it < 16; ++it) {
-        const int cc = wave + 4 * it;
-        if (cc < ncols && lane < cnt) st(Ab, (size_t)(cb + cc) * lda + trow[lane], tile[lane][cc]);
+            for (int i = 0; i < kNB; ++i) x[i] = tile[i][tid];
+#pragma unroll
+            for (int i = 1; i < kNB; ++i) {
+                c32 acc = x[i];
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    const c32 l = L11[i][j];
+                    acc.re = fmaf(-l.re, x[j].re, fmaf(l.im, x[j].im, acc.re));
+                    acc.im = fmaf(-l.re, x[j].im, fmaf(-l.im, x[j].re, acc.im));
+                }
+                x[i] = acc;
+            }
+#pragma unroll
+            for (int i = 0; i < kNB; ++i) tile[i][tid] = x[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            const int cc = wave + 4 * it;
+            if (cc < ncols && lane < cnt)
+                st(Ab, (size_t)(cb + cc) * lda + trow[lane], tile[lane][cc]);
+        }
+        __syncthreads();
     }
 }
 
@@ -811,6 +824,120 @@ __global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t ld
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 Krylov helpers of the GMRES refinement (batched over systems,
+// vectors interleaved complex double with stride ld per system; the basis
+// V holds vectors [b][i][ld]).
+// ---------------------------------------------------------------------------
+
+// y = A x, A the FP32 operator (true n x n block), x and y FP64.
+__global__ __launch_bounds__(256) void k_gemv_f64(const float *A, int64_t lda, int64_t a_stride,
+                                                  int n, const double *x, int64_t ldx, double *y,
+                                                  int64_t ldy) {
+    const int b = blockIdx.y;
+    const int row = blockIdx.x * 256 + threadIdx.x;
+    const float *Ab = A + (size_t)b * a_stride * 2;
+    const double *xb = x + (size_t)b * ldx * 2;
+    __shared__ double xsh[2][256];
+    double ar = 0.0, ai = 0.0;
+    for (int c0 = 0; c0 < n; c0 += 256) {
+        __syncthreads();
+        const int c = c0 + threadIdx.x;
+        xsh[0][threadIdx.x] = c < n ? xb[2 * c] : 0.0;
+        xsh[1][threadIdx.x] = c < n ? xb[2 * c + 1] : 0.0;
+        __syncthreads();
+        if (row < n) {
+            const int cend = min(256, n - c0);
+            for (int cc = 0; cc < cend; ++cc) {
+                const c32 a = ld(Ab, (size_t)(c0 + cc) * lda + row);
+                const double xr = xsh[0][cc], xi = xsh[1][cc];
+                ar = fma((double)a.re, xr, ar);
+                ar = fma(-(double)a.im, xi, ar);
+                ai = fma((double)a.re, xi, ai);
+                ai = fma((double)a.im, xr, ai);
+            }
+        }
+    }
+    if (row < n) {
+        y[((size_t)b * ldy + row) * 2] = ar;
+        y[((size_t)b * ldy + row) * 2 + 1] = ai;
+    }
+}
+
+// h[b][i] = V_i^H w for i < m (grid.x = i, grid.y = b); h must be zero.
+__global__ __launch_bounds__(256) void k_dots_f64(const double *V, int64_t ldv, int64_t vstride,
+                                                  const double *w, int64_t ldw, int n, double *h,
+                                                  int64_t ldh) {
+    const int i = blockIdx.x, b = blockIdx.y;
+    const double *v = V + ((size_t)b * vstride + (size_t)i * ldv) * 2;
+    const double *wb = w + (size_t)b * ldw * 2;
+    double re = 0.0, im = 0.0;
+    for (int r = threadIdx.x; r < n; r += 256) {
+        const double vr = v[2 * r], vi = v[2 * r + 1], wr = wb[2 * r], wi = wb[2 * r + 1];
+        re = fma(vr, wr, fma(vi, wi, re));
+        im = fma(vr, wi, fma(-vi, wr, im));
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        re += __shfl_xor(re, o);
+        im += __shfl_xor(im, o);
+    }
+    __shared__ double red[2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wave] = re;
+        red[1][wave] = im;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        h[((size_t)b * ldh + i) * 2] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        h[((size_t)b * ldh + i) * 2 + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+// w += sign * sum_{i < m} c[b][i] V_i  (sign = -1: Gram-Schmidt, +1: x += V y);
+// systems with active[b] == 0 are left unchanged (active may be null).
+__global__ __launch_bounds__(256) void k_combine_f64(const double *V, int64_t ldv, int64_t vstride,
+                                                     int m, const double *c, int64_t ldc,
+                                                     double sign, double *w, int64_t ldw, int n,
+                                                     const int32_t *active) {
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double *v = V + ((size_t)b * vstride + (size_t)i * ldv) * 2;
+        const double cr = c[((size_t)b * ldc + i) * 2], ci = c[((size_t)b * ldc + i) * 2 + 1];
+        const double vr = v[2 * r], vi = v[2 * r + 1];
+        re = fma(cr, vr, fma(-ci, vi, re));
+        im = fma(cr, vi, fma(ci, vr, im));
+    }
+    double *wb = w + (size_t)b * ldw * 2;
+    wb[2 * r] += sign * re;
+    wb[2 * r + 1] += sign * im;
+}
+
+// out[b] = ||w_b||_2^2 (out must be zero).
+__global__ __launch_bounds__(256) void k_norm2sq_f64(const double *w, int64_t ldw, int n,
+                                                     double *out) {
+    const int b = blockIdx.y;
+    const double *wb = w + (size_t)b * ldw * 2;
+    double s = 0.0;
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256)
+        s = fma(wb[2 * r], wb[2 * r], fma(wb[2 * r + 1], wb[2 * r + 1], s));
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out + b, s);
+}
+
+// dst_b = src_b * f[b] (real factor per system); dst may alias src.
+__global__ void k_scale_f64(const double *src, int64_t lds, double *dst, int64_t ldd, int n,
+                            const double *f) {
+    const int b = blockIdx.y;
+    const double s = f[b];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < 2 * n; i += gridDim.x * 256)
+        dst[(size_t)b * ldd * 2 + i] = src[(size_t)b * lds * 2 + i] * s;
+}
+
 __global__ void k_copy_f32_to_f64(const float *src, double *dst, int64_t ls, int64_t ld, int n) {
     const int b = blockIdx.y;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < 2 * n; i += gridDim.x * 256)
@@ -885,10 +1012,16 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
     const int N = w.N, B = w.B;
     const int64_t gls = (int64_t)(N / kNB) * kLuGather;
     auto gl_of = [&](int k) { return w.gl + (size_t)(k / kNB) * kLuGather; };
-    auto swap_trsm = [&](int k, int c0, int c1, int mode) {
-        if (c1 <= c0) return;
-        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((c1 - c0 + 63) / 64, B), dim3(256), 0, stream,
-                           w.A, w.lda, w.a_stride, k, c0, c1, gl_of(k), gls, mode);
+    // exchanges / solves of step k (n_steps exchanges for mode 0) on up to
+    // two column ranges in one launch
+    auto swap_trsm = [&](int k, int n_steps, int a0, int a1, int mode_a, int b0, int b1,
+                         int mode_b) {
+        a1 = std::max(a0, a1);
+        b1 = std::max(b0, b1);
+        const int blocks = (a1 - a0 + 63) / 64 + (b1 - b0 + 63) / 64;
+        if (blocks == 0) return;
+        hipLaunchKernelGGL(k_lu_swap_trsm, dim3(blocks, B), dim3(256), 0, stream, w.A, w.lda,
+                           w.a_stride, k, n_steps, a0, a1, mode_a, b0, b1, mode_b, gl_of(k), gls);
     };
     auto gemm = [&](int prec, int r0, int r1, int c0, int c1, int k0, int k1) {
         if (r1 <= r0 || c1 <= c0) return;
@@ -912,14 +1045,13 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
             else
                 hipLaunchKernelGGL(k_lu_panel, dim3(B), dim3(1024), 0, stream, w.A, w.lda,
                                    w.a_stride, N, k, w.ipiv, w.info, gl_of(k), gls);
-            swap_trsm(k, K0, k, 0);
-            swap_trsm(k, k + kNB, K1, 1);
+            swap_trsm(k, 1, K0, k, 0, k + kNB, K1, 1);
             gemm(0, k + kNB, N, k + kNB, K1, k, k + kNB);
         }
         if (K1 < N) {
-            for (int k = K0; k < K1; k += kNB) swap_trsm(k, K1, N, 0);
+            swap_trsm(K0, (K1 - K0) / kNB, K1, N, 0, 0, 0, 0);
             for (int k = K0; k < K1; k += kNB) {
-                swap_trsm(k, K1, N, 2);
+                swap_trsm(k, 1, K1, N, 2, 0, 0, 0);
                 gemm(0, k + kNB, K1, K1, N, k, k + kNB);
             }
             gemm(precision, K1, N, K1, N, K0, K1);
@@ -939,6 +1071,38 @@ void launch_residual_f64(const float *A, int64_t lda, int64_t a_stride, int n, i
     launch_fill_zero_f64(norms, (size_t)2 * B, stream);
     hipLaunchKernelGGL(k_residual_f64, dim3((n + 255) / 256, B), dim3(256), 0, stream, A, lda,
                        a_stride, n, b, ldb, x, ldx, r, norms);
+}
+
+void launch_gemv_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B, const double *x,
+                     int64_t ldx, double *y, int64_t ldy, hipStream_t stream) {
+    hipLaunchKernelGGL(k_gemv_f64, dim3((n + 255) / 256, B), dim3(256), 0, stream, A, lda, a_stride,
+                       n, x, ldx, y, ldy);
+}
+
+void launch_dots_f64(const double *V, int64_t ldv, int64_t vstride, int m, const double *w,
+                     int64_t ldw, int n, int B, double *h, int64_t ldh, hipStream_t stream) {
+    hipLaunchKernelGGL(k_dots_f64, dim3(m, B), dim3(256), 0, stream, V, ldv, vstride, w, ldw, n, h,
+                       ldh);
+}
+
+void launch_combine_f64(const double *V, int64_t ldv, int64_t vstride, int m, const double *c,
+                        int64_t ldc, double sign, double *w, int64_t ldw, int n, int B,
+                        const int32_t *active, hipStream_t stream) {
+    hipLaunchKernelGGL(k_combine_f64, dim3((n + 255) / 256, B), dim3(256), 0, stream, V, ldv,
+                       vstride, m, c, ldc, sign, w, ldw, n, active);
+}
+
+void launch_norm2sq_f64(const double *w, int64_t ldw, int n, int B, double *out,
+                        hipStream_t stream) {
+    launch_fill_zero_f64(out, (size_t)B, stream);
+    hipLaunchKernelGGL(k_norm2sq_f64, dim3(std::min(64, (n + 255) / 256), B), dim3(256), 0, stream,
+                       w, ldw, n, out);
+}
+
+void launch_scale_f64(const double *src, int64_t lds, double *dst, int64_t ldd, int n, int B,
+                      const double *f, hipStream_t stream) {
+    hipLaunchKernelGGL(k_scale_f64, dim3((2 * n + 255) / 256, B), dim3(256), 0, stream, src, lds,
+                       dst, ldd, n, f);
 }
 
 void launch_copy_f32_to_f64(const float *src, double *dst, int64_t ld_src, int64_t ld_dst, int n,

@@ -238,3 +238,23 @@ def test_mixed_precision_lu_refines_to_fp64_accuracy(require_gpu, mode, raw_floo
         assert e0 > max(raw_floor, 3 * e32), (mode, e0, e32)
         assert e < 1e-6, (mode, e0, e)
         assert info[s]["dense_solve_refine_iterations"] >= 1
+
+
+@pytest.mark.parametrize("mode", ["mixed_fp16", "mixed_bf16"])
+def test_gmres_refinement_converges_beyond_classical_range(require_gpu, mode):
+    """cond(A) = 1e3: kappa * u_bf16 ~ 4, where classical refinement with the
+    BF16-updated LU stalls; GMRES-IR with the FP64 residual reaches FP64-level
+    accuracy."""
+    from hornlab_metal_bem.native import dense_solve
+
+    rng = np.random.default_rng(21)
+    n = 400
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    W, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    sig = np.geomspace(1.0, 1e-3, n)
+    A64 = (Q * sig) @ W.conj().T
+    A = A64.astype(np.complex64)[None]
+    b = (rng.standard_normal((1, n)) + 1j * rng.standard_normal((1, n))).astype(np.complex64)
+    x, info = dense_solve(A, b, solve_mode=mode, refine_max_iterations=20, refine_tolerance=1e-13)
+    ref = np.linalg.solve(A[0].astype(np.complex128), b[0].astype(np.complex128))
+    assert rel_l2(x[0], ref) < 1e-7, (mode, rel_l2(x[0], ref), info[0])
