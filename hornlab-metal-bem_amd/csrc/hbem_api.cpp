@@ -593,9 +593,13 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int n, int
         std::vector<std::vector<cd>> sn(B, std::vector<cd>(m, cd(0.0)));
         std::vector<int> kk(B, 0);
         std::vector<char> done(B, 0);
+        // inner target: the correction only has to carry the outer residual
+        // below the tolerance (with a factor 10 of margin)
+        std::vector<double> inner(B);
         for (int b = 0; b < B; ++b) {
             g[b][0] = cd(beta[b], 0.0);
             if (!active[b] || beta[b] == 0.0) done[b] = 1;
+            inner[b] = std::max(1e-14, 0.1 * opt.refine_tolerance / std::max(best[b], 1e-300));
         }
         int j = 0;
         for (; j < m; ++j) {
@@ -646,7 +650,7 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int n, int
                 for (int i = 0; i <= j; ++i) H[b][(size_t)i * m + j] = col[i];
                 kk[b] = j + 1;
                 steps[b]++;
-                if (std::abs(g[b][j + 1]) <= 1e-13 * beta[b] || hnext <= 1e-300) {
+                if (std::abs(g[b][j + 1]) <= inner[b] * beta[b] || hnext <= 1e-300) {
                     done[b] = 1;
                 } else {
                     fac[b] = 1.0 / hnext;
