@@ -34,20 +34,33 @@ TRANS_PER_EVAL = 4  # sqrt/rcp, sin, cos, (exp) per evaluation (fixed count)
 REFERENCE_FREQ_PER_S = 20.0  # reference README: 40 freqs in ~2 s on Apple M-series
 
 
-def workload(target_dofs: int = 1000, seed: int = 0):
+WORKLOADS = {
+    # quarter horn, yz+xz, ~1000 dofs, 40 log frequencies 500 Hz - 20 kHz, 3 x 37 points
+    "c2": dict(quarter=True, symmetry="yz+xz", dofs=1000, fmin=500.0, fmax=20000.0, nf=40,
+               angles=37),
+    # full horn, no symmetry, ~8000 dofs, 64 log frequencies 200 Hz - 16 kHz, 3 x 181 points
+    "c3": dict(quarter=False, symmetry=None, dofs=8000, fmin=200.0, fmax=16000.0, nf=64,
+               angles=181),
+}
+
+
+def workload(target_dofs: int | None = None, seed: int = 0, name: str = "c2"):
     from hornlab_metal_bem import ObservationConfig, native_config
     from hornlab_metal_bem.mesh import loaded_mesh_from_arrays
     from hornlab_metal_bem.procedural import HornSpec, edge_length_for_dofs, horn_body
 
+    w = WORKLOADS[name]
+    dofs = target_dofs or w["dofs"]
     spec = HornSpec(0.0127, 0.150, 0.150, 0.020, "exponential")
-    h = edge_length_for_dofs(spec, target_dofs, quarter=True, seed=seed)
-    V, F, T = horn_body(spec, edge_length=h, quarter=True, jitter=0.05, seed=seed)
-    mesh = loaded_mesh_from_arrays(V, F, T, native_symmetry_plane="yz+xz")
+    h = edge_length_for_dofs(spec, dofs, quarter=w["quarter"], seed=seed)
+    V, F, T = horn_body(spec, edge_length=h, quarter=w["quarter"], jitter=0.05, seed=seed)
+    sym = w["symmetry"]
+    mesh = loaded_mesh_from_arrays(V, F, T, native_symmetry_plane=sym)
     config = native_config(
-        freq_min_hz=500.0, freq_max_hz=20000.0, freq_count=40, freq_spacing="log",
-        native_symmetry_plane="yz+xz", source_motion="axial", velocity_sources={2: 1.0},
+        freq_min_hz=w["fmin"], freq_max_hz=w["fmax"], freq_count=w["nf"], freq_spacing="log",
+        native_symmetry_plane=sym, source_motion="axial", velocity_sources={2: 1.0},
         observation=ObservationConfig(planes=["horizontal", "vertical", "diagonal"], distance_m=2.0,
-                                      angle_count=37),
+                                      angle_count=w["angles"]),
     )
     return spec, h, mesh, config
 
@@ -71,13 +84,13 @@ def prepare_inputs(mesh, config):
     return freqs, k, q, pts, buffers
 
 
-def cpu_baseline(buffers, freqs, k, q, pts, sample_idx, threads):
+def cpu_baseline(buffers, sym, freqs, k, q, pts, sample_idx, threads):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
     O.set_threads(threads)
     os.environ.setdefault("OPENBLAS_NUM_THREADS", str(threads))
-    g = O.Geometry.from_buffers(buffers, "yz+xz")
+    g = O.Geometry.from_buffers(buffers, sym)
     t0 = time.perf_counter()
     for i in sample_idx:
         A, rhs, _ = O.assemble(g, float(k[i]), q[i], near_level=1, near_threshold=1.5, bits=32)
@@ -87,12 +100,12 @@ def cpu_baseline(buffers, freqs, k, q, pts, sample_idx, threads):
     return len(sample_idx) / dt, dt
 
 
-def fp64_reference(buffers, k, q, pts, idx, threads):
+def fp64_reference(buffers, sym, k, q, pts, idx, threads):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
     O.set_threads(threads)
-    g = O.Geometry.from_buffers(buffers, "yz+xz")
+    g = O.Geometry.from_buffers(buffers, sym)
     A, rhs, _ = O.assemble(g, float(k[idx]), q[idx], bits=64)
     p = O.solve(A, rhs, bits=64)
     return p, O.field(g, float(k[idx]), p, q[idx], pts, bits=64)
@@ -103,8 +116,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--dofs", type=int, default=1000)
-    ap.add_argument("--solve-mode", default="fp32", choices=["fp32", "fp32_ir", "mixed_fp16", "mixed_bf16"])
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--dofs", type=int, default=None)
+    ap.add_argument("--solve-mode", default=None, choices=["fp32", "fp32_ir", "mixed_fp16", "mixed_bf16"],
+                    help="default: fp32 for c2, mixed_fp16 (GMRES-IR to 1e-6) for c3")
+    ap.add_argument("--refine-tol", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -123,7 +139,11 @@ def main() -> int:
     if not rt.available:
         print("HIP backend unavailable: " + "; ".join(rt.unavailable_reasons), file=sys.stderr)
         return 1
-    spec, h, mesh, config = workload(args.dofs)
+    wl = WORKLOADS[args.workload]
+    sym = wl["symmetry"]
+    solve_mode = args.solve_mode or ("fp32" if args.workload == "c2" else "mixed_fp16")
+    refine_tol = args.refine_tol or (1e-7 if args.workload == "c2" else 1e-6)
+    spec, h, mesh, config = workload(args.dofs, name=args.workload)
     freqs, k, q, pts, buffers = prepare_inputs(mesh, config)
     F, T, n, n_obs = freqs.size, buffers.n_triangles, buffers.p1_dof_count, pts.shape[1]
     dev = torch.device("cuda:0")
@@ -135,14 +155,15 @@ def main() -> int:
     tag_d = torch.zeros(F * 2 * 2, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
-    with HipBemSession.create_session(buffers, symmetry_plane="yz+xz") as s:
+    with HipBemSession.create_session(buffers, symmetry_plane=sym) as s:
         info = s.info()
 
         def step():
             return s.solve_batch_device(
                 F, k_d.data_ptr(), k_d.data_ptr(), q_d.data_ptr(), n_obs, obs_d.data_ptr(),
                 field_d.data_ptr(), tag_d.data_ptr(), p_ptr=p_d.data_ptr(), source_tags=[2],
-                impedance_source_tag=2, solve_mode=args.solve_mode, estimate_rcond=False,
+                impedance_source_tag=2, solve_mode=solve_mode, estimate_rcond=False,
+                refine_tolerance=refine_tol,
             )
 
         for _ in range(args.warmup):
@@ -163,7 +184,7 @@ def main() -> int:
     reg_ms = per_step("regular_assembly_seconds")
     launches_per_step = len({d["batch_index"] for d in diags[0]})
     reg_launch_s = reg_ms * 1e-3 / launches_per_step
-    n_img = 4  # identity + 3 images for yz+xz
+    n_img = 4 if sym == "yz+xz" else (1 if sym is None else 2)
     evals_per_step = float(T) ** 2 * 36 * n_img * F
     achieved = evals_per_step / launches_per_step * FLOPS_PER_EVAL / reg_launch_s / 1e12
     p_gpu = (p_d.cpu().numpy().view(np.complex64)).reshape(F, n)
@@ -173,8 +194,8 @@ def main() -> int:
     cpu = None
     acc = {}
     if not args.no_cpu_baseline:
-        sample = [0, F // 2, F - 1]
-        cps, cdt = cpu_baseline(buffers, freqs, k, q, pts, sample, threads)
+        sample = [0, F // 2, F - 1] if args.workload == "c2" else [F // 2]
+        cps, cdt = cpu_baseline(buffers, sym, freqs, k, q, pts, sample, threads)
         cpu = {
             "value": round(cps, 4),
             "unit": "frequencies/s",
@@ -184,8 +205,8 @@ def main() -> int:
             "FP32 regular + Sauter-Schwab + level-1 near-field assembly, scipy cgesv, field; "
             f"{cdt:.1f} s",
         }
-        for i in (0, F - 1):
-            p64, f64 = fp64_reference(buffers, k, q, pts, i, threads)
+        for i in ((0, F - 1) if args.workload == "c2" else (F - 1,)):
+            p64, f64 = fp64_reference(buffers, sym, k, q, pts, i, threads)
             acc[f"{freqs[i]:.0f}Hz"] = {
                 "surface_p_rel_l2": float(np.linalg.norm(p_gpu[i] - p64) / np.linalg.norm(p64)),
                 "field_rel_l2": float(np.linalg.norm(f_gpu[i] - f64) / np.linalg.norm(f64)),
@@ -204,13 +225,15 @@ def main() -> int:
         "dtype": "c64",
         "data": "synthetic",
         "config": {
-            "workload": f"quarter exponential horn yz+xz, n={n} P1 dofs, T={T} triangles, "
-            f"{F} log frequencies 500-20000 Hz, 3x37 field points at 2 m",
+            "workload": f"{'quarter' if sym else 'full'} exponential horn {sym or 'no symmetry'}, "
+            f"n={n} P1 dofs, T={T} triangles, {F} log frequencies "
+            f"{wl['fmin']:.0f}-{wl['fmax']:.0f} Hz, 3x{wl['angles']} field points at 2 m",
             "n_dofs": n,
             "n_triangles": T,
             "frequencies": F,
             "edge_length_m": round(h, 5),
-            "solve_mode": args.solve_mode,
+            "solve_mode": solve_mode,
+            "refine_tolerance": refine_tol if solve_mode != "fp32" else None,
             "duffy_pairs": info["planned_pairs"],
         },
         "stages_ms_per_step": {

@@ -137,21 +137,22 @@ struct hbem_session {
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
         tile_count, tile_ndofs, tile_dofs, tile_pairs;
     // per-call workspaces
-    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, amax, lu_scale, kv, kh, kn, kf,
+    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, lu_scr_i, lu_scr_f, amax,
+        lu_scale, kv, kh, kn, kf,
         q, kre, kim, fk, obs, field, tags,
         tagout, norms, anorm, active, rowsplit, rcx;
     // staging of host-input batches
-    DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag;
+    DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag, io_beta, qt, l2g;
     double setup_seconds = 0.0;
     std::string device_name;
     int cu_count = 0;
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
-                          &io_p, &io_tag})
+                          &io_p, &io_tag, &io_beta, &qt, &l2g})
             b->release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -175,29 +176,39 @@ std::vector<float> interleave(const float *re, const float *im, size_t n) {
 // and s->rhs from device inputs: k_real/k_imag [B] (k_imag may be null for
 // real k) and interleaved complex Neumann rows q [B][T].  Optional timers
 // bracket the regular and the singular kernel on the session stream.
-void assemble_device(hbem_session *s, int B, const float *kre_d, const float *kim_d,
-                     const float *q_d, int64_t lda, int64_t a_stride, int64_t rhs_stride,
-                     Timer *t_reg, Timer *t_sing) {
+bool any_nonzero(const float *v, size_t n) {
+    if (!v) return false;
+    for (size_t i = 0; i < n; ++i)
+        if (v[i] != 0.0f) return true;
+    return false;
+}
+
+// B matrices (one per frequency) and B * S right-hand sides (q [B][S][T],
+// rhs vector b * S + src), beta [B][T] Robin admittance or null.
+void assemble_device(hbem_session *s, int B, int S, const float *kre_d, const float *kim_d,
+                     const float *q_d, const float *beta_d, int64_t lda, int64_t a_stride,
+                     int64_t rhs_stride, Timer *t_reg, Timer *t_sing) {
     const size_t a_floats = (size_t)B * a_stride * 2;
     s->A.reserve(a_floats * sizeof(float));
-    s->rhs.reserve((size_t)B * rhs_stride * 2 * sizeof(float));
+    s->rhs.reserve((size_t)B * S * rhs_stride * 2 * sizeof(float));
     hbem::launch_fill_zero(s->A.as<float>(), a_floats, s->stream);
-    hbem::launch_fill_zero(s->rhs.as<float>(), (size_t)B * rhs_stride * 2, s->stream);
+    hbem::launch_fill_zero(s->rhs.as<float>(), (size_t)B * S * rhs_stride * 2, s->stream);
     if (t_reg) t_reg->start(s->stream);
-    hbem::launch_assemble_regular(s->dg, B, kre_d, kim_d, q_d, s->A.as<float>(), lda, a_stride,
-                                  s->rhs.as<float>(), rhs_stride, s->stream);
+    hbem::launch_assemble_regular(s->dg, B, kre_d, kim_d, q_d, S, beta_d, s->A.as<float>(), lda,
+                                  a_stride, s->rhs.as<float>(), rhs_stride, s->stream);
     if (t_reg) t_reg->stop(s->stream);
     if (t_sing) t_sing->start(s->stream);
-    hbem::launch_assemble_singular(s->dg, B, kre_d, kim_d, q_d, s->A.as<float>(), lda, a_stride,
-                                   s->rhs.as<float>(), rhs_stride, s->stream);
+    hbem::launch_assemble_singular(s->dg, B, kre_d, kim_d, q_d, S, beta_d, s->A.as<float>(), lda,
+                                   a_stride, s->rhs.as<float>(), rhs_stride, s->stream);
     if (t_sing) t_sing->stop(s->stream);
     hip_check(hipGetLastError(), "assembly launch");
 }
 
 // Host-input twin of assemble_device (uploads k and split q rows).
 void assemble_batch(hbem_session *s, int B, const float *k_real_h, const float *k_imag_h,
-                    const float *q_re_h, const float *q_im_h, int64_t lda, int64_t a_stride,
-                    int64_t rhs_stride, bool complex_k) {
+                    const float *q_re_h, const float *q_im_h, const float *beta_re_h,
+                    const float *beta_im_h, int64_t lda, int64_t a_stride, int64_t rhs_stride,
+                    bool complex_k) {
     const int T = s->plan.n_triangles;
     std::vector<float> kr(k_real_h, k_real_h + B);
     upload(s->kre, kr, s->stream);
@@ -207,16 +218,18 @@ void assemble_batch(hbem_session *s, int B, const float *k_real_h, const float *
     }
     std::vector<float> q = interleave(q_re_h, q_im_h, (size_t)B * T);
     upload(s->q, q, s->stream);
-    assemble_device(s, B, s->kre.as<float>(), complex_k ? s->kim.as<float>() : nullptr,
-                    s->q.as<float>(), lda, a_stride, rhs_stride, nullptr, nullptr);
+    const bool robin = any_nonzero(beta_re_h, (size_t)B * T) || any_nonzero(beta_im_h, (size_t)B * T);
+    if (robin) {
+        std::vector<float> bz((size_t)B * T, 0.0f);
+        upload(s->io_beta, interleave(beta_re_h ? beta_re_h : bz.data(),
+                                      beta_im_h ? beta_im_h : bz.data(), (size_t)B * T),
+               s->stream);
+    }
+    assemble_device(s, B, 1, s->kre.as<float>(), complex_k ? s->kim.as<float>() : nullptr,
+                    s->q.as<float>(), robin ? s->io_beta.as<float>() : nullptr, lda, a_stride,
+                    rhs_stride, nullptr, nullptr);
 }
 
-bool any_nonzero(const float *v, size_t n) {
-    if (!v) return false;
-    for (size_t i = 0; i < n; ++i)
-        if (v[i] != 0.0f) return true;
-    return false;
-}
 
 // LAPACK clacn2-style 1-norm estimate of ||A^{-1}||_1 for each system of a
 // factored batch (Higham's algorithm, as in cgecon).  Returns rcond per
@@ -421,6 +434,7 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         upload(s->tile_ndofs, P.tile_ndofs, s->stream);
         upload(s->tile_dofs, P.tile_dofs, s->stream);
         upload(s->tile_pairs, P.tile_pairs, s->stream);
+        upload(s->l2g, P.l2g, s->stream);
         hbem::DeviceGeometry &g = s->dg;
         g.tri_f_aos = s->tri_f_aos.as<float>();
         g.tri_i_aos = s->tri_i_aos.as<int32_t>();
@@ -446,6 +460,7 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         g.tile_ndofs = s->tile_ndofs.as<int32_t>();
         g.tile_dofs = s->tile_dofs.as<int32_t>();
         g.tile_pairs = s->tile_pairs.as<int32_t>();
+        g.l2g = s->l2g.as<int32_t>();
         g.n_tile_pairs = (int)(P.tile_pairs.size() / 2);
         g.n_tiles = (int)P.tile_start.size();
         g.max_tile_dofs = P.max_tile_dofs;
@@ -483,7 +498,8 @@ int32_t hbem_session_get_info(hbem_session *s, hbem_session_info *info) {
 }
 
 int32_t hbem_assemble(hbem_session *s, float k_real, float k_imag, const float *q_re,
-                      const float *q_im, float *A_re, float *A_im, float *rhs_re, float *rhs_im) {
+                      const float *q_im, const float *beta_re, const float *beta_im, float *A_re,
+                      float *A_im, float *rhs_re, float *rhs_im) {
     return guarded([&] {
         if (!s) fail(HBEM_EINVAL, "hbem_assemble: null session");
         if (!A_re || !A_im || !rhs_re || !rhs_im)
@@ -494,8 +510,8 @@ int32_t hbem_assemble(hbem_session *s, float k_real, float k_imag, const float *
         set_device(s);
         const int n = s->plan.n_dofs;
         const int64_t lda = n, a_stride = (int64_t)n * n, rhs_stride = n;
-        assemble_batch(s, 1, &k_real, &k_imag, q_re, q_im, lda, a_stride, rhs_stride,
-                       k_imag != 0.0f);
+        assemble_batch(s, 1, &k_real, &k_imag, q_re, q_im, beta_re, beta_im, lda, a_stride,
+                       rhs_stride, k_imag != 0.0f);
         s->rowsplit.reserve((size_t)2 * n * n * sizeof(float));
         float *re_d = s->rowsplit.as<float>();
         float *im_d = re_d + (size_t)n * n;
@@ -531,7 +547,7 @@ namespace {
 // small least-squares solve on the host.  x64 / r64 hold the iterate and its
 // residual on entry and exit; best[b] the relative residual ||r||/||b||
 // (inf-norms).  Returns the Arnoldi steps taken per system.
-void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int n, int N,
+void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int vpm, int n, int N,
                   const hbem_solve_options &opt, const std::function<void()> &residual,
                   std::vector<double> &nrm, std::vector<double> &best,
                   std::vector<int32_t> &steps) {
@@ -555,7 +571,7 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int n, int
     auto precond = [&](const double *src, int64_t ld_src, int i) {
         hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * B * N, st);
         hbem::launch_copy_f64_to_f32(src, s->x.as<float>(), ld_src, N, n, B, st);
-        hbem::lu_solve(w, s->x.as<float>(), N, 0, st);
+        hbem::lu_solve(w, s->x.as<float>(), N, 0, st, vpm);
         hbem::launch_copy_f32_to_f64(s->x.as<float>(), vec(i), N, vstride, n, B, st);
     };
     auto norms_of = [&](int i) {
@@ -605,7 +621,7 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int n, int
         for (; j < m; ++j) {
             // z = M^{-1} A v_j into v_{j+1}, then CGS2 against v_0..v_j
             hbem::launch_gemv_f64(s->A_orig.as<float>(), lda, a_stride, n, B, vec(j), vstride,
-                                  s->r64.as<double>() + 0, N, st);
+                                  s->r64.as<double>() + 0, N, st, vpm);
             precond(s->r64.as<double>(), N, j + 1);
             hbem::launch_dots_f64(V, N, vstride, j + 1, vec(j + 1), vstride, n, B, h1, m + 1, st);
             hbem::launch_combine_f64(V, N, vstride, j + 1, h1, m + 1, -1.0, vec(j + 1), vstride, n,
@@ -692,12 +708,15 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int n, int
 }
 
 // Factor + solve + optional refinement of the B systems resident in s->A
-// (column-major, order N, true order n) with right-hand sides s->rhs.
-// Solution (FP32, interleaved, stride N) ends in s->x.  Fills diagnostics.
-void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_options &opt,
+// (column-major, order N, true order n) with the B * S right-hand sides in
+// s->rhs (vector v against matrix v / S).  Solutions (FP32, interleaved,
+// stride N) end in s->x.  Fills per-matrix diagnostics (refinement figures
+// are the worst over the matrix's right-hand sides).
+void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solve_options &opt,
                     std::vector<hbem_case_diag> *diag, double *solve_seconds) {
     hipStream_t st = s->stream;
     const int64_t lda = N, a_stride = (int64_t)N * N;
+    const int V = B * S;
     const bool refine = opt.solve_mode == HBEM_SOLVE_F32_IR ||
                         opt.solve_mode == HBEM_SOLVE_MIXED_F16 ||
                         opt.solve_mode == HBEM_SOLVE_MIXED_BF16;
@@ -722,14 +741,17 @@ void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_optio
     s->ipiv.reserve(sizeof(int32_t) * (size_t)B * N);
     s->info.reserve(sizeof(int32_t) * B);
     s->gl.reserve(sizeof(int32_t) * (size_t)B * hbem::lu_gather_ints(N));
+    s->lu_scr_i.reserve(sizeof(int32_t) * (size_t)B * hbem::kLuScratchI);
+    s->lu_scr_f.reserve(sizeof(float) * (size_t)B * hbem::kLuScratchF);
     hbem::LuWorkspace w{s->A.as<float>(), lda, a_stride, N, n, B, s->ipiv.as<int32_t>(),
-                        s->info.as<int32_t>(), s->gl.as<int32_t>()};
+                        s->info.as<int32_t>(), s->gl.as<int32_t>(), s->lu_scr_i.as<int32_t>(),
+                        s->lu_scr_f.as<float>()};
     hbem::lu_factor(w, precision, s->lu_scale.as<float>(), st);
-    s->x.reserve(sizeof(float) * 2 * (size_t)B * N);
-    hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)B * N,
+    s->x.reserve(sizeof(float) * 2 * (size_t)V * N);
+    hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)V * N,
                              hipMemcpyDeviceToDevice, st),
               "rhs copy");
-    hbem::lu_solve(w, s->x.as<float>(), N, 0, st);
+    hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
     std::vector<int32_t> info(B);
     hip_check(hipMemcpyAsync(info.data(), s->info.ptr, sizeof(int32_t) * B, hipMemcpyDeviceToHost,
                              st),
@@ -739,76 +761,76 @@ void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_optio
                              hipMemcpyDeviceToHost, st),
               "anorm D2H");
     hip_check(hipStreamSynchronize(st), "solve sync");
-    std::vector<int32_t> iters(B, 0);
-    std::vector<double> resid(B, -1.0);
+    std::vector<int32_t> iters(V, 0);
+    std::vector<double> resid(V, -1.0);
     if (refine && opt.refine_max_iterations > 0) {
-        s->x64.reserve(sizeof(double) * 2 * (size_t)B * N);
-        s->r64.reserve(sizeof(double) * 2 * (size_t)B * N);
-        s->xprev.reserve(sizeof(double) * 2 * (size_t)B * N);
-        s->norms.reserve(sizeof(double) * 2 * B);
-        s->active.reserve(sizeof(int32_t) * B);
-        hbem::launch_fill_zero_f64(s->x64.as<double>(), (size_t)2 * B * N, st);
-        hbem::launch_copy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, B, st);
-        std::vector<double> best(B, 0.0), nrm(2 * B);
-        std::vector<int32_t> active(B, 1);
+        s->x64.reserve(sizeof(double) * 2 * (size_t)V * N);
+        s->r64.reserve(sizeof(double) * 2 * (size_t)V * N);
+        s->xprev.reserve(sizeof(double) * 2 * (size_t)V * N);
+        s->norms.reserve(sizeof(double) * 2 * V);
+        s->active.reserve(sizeof(int32_t) * V);
+        hbem::launch_fill_zero_f64(s->x64.as<double>(), (size_t)2 * V * N, st);
+        hbem::launch_copy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, V, st);
+        std::vector<double> best(V, 0.0), nrm(2 * V);
+        std::vector<int32_t> active(V, 1);
         auto residual = [&]() {
-            hbem::launch_residual_f64(s->A_orig.as<float>(), lda, a_stride, n, B,
+            hbem::launch_residual_f64(s->A_orig.as<float>(), lda, a_stride, n, V,
                                       s->rhs.as<float>(), N, s->x64.as<double>(), N,
-                                      s->r64.as<double>(), s->norms.as<double>(), st);
-            hip_check(hipMemcpyAsync(nrm.data(), s->norms.ptr, sizeof(double) * 2 * B,
+                                      s->r64.as<double>(), s->norms.as<double>(), st, S);
+            hip_check(hipMemcpyAsync(nrm.data(), s->norms.ptr, sizeof(double) * 2 * V,
                                      hipMemcpyDeviceToHost, st),
                       "norms D2H");
             hip_check(hipStreamSynchronize(st), "residual sync");
         };
         residual();
-        for (int b = 0; b < B; ++b) {
-            best[b] = nrm[2 * b + 1] > 0.0 ? nrm[2 * b] / nrm[2 * b + 1] : nrm[2 * b];
-            if (best[b] <= opt.refine_tolerance) active[b] = 0;
+        for (int v = 0; v < V; ++v) {
+            best[v] = nrm[2 * v + 1] > 0.0 ? nrm[2 * v] / nrm[2 * v + 1] : nrm[2 * v];
+            if (best[v] <= opt.refine_tolerance) active[v] = 0;
         }
         // mixed precision: GMRES-IR; FP32 LU: classical refinement as the
         // reference's FP64 dense mode (main.swift:6964-7070)
         if (precision != 0) {
-            gmres_refine(s, w, B, n, N, opt, residual, nrm, best, iters);
+            gmres_refine(s, w, V, S, n, N, opt, residual, nrm, best, iters);
             std::fill(active.begin(), active.end(), 0);
         }
         for (int it = 0; it < opt.refine_max_iterations && precision == 0; ++it) {
             bool any = false;
-            for (int b = 0; b < B; ++b) any |= active[b] != 0;
+            for (int v = 0; v < V; ++v) any |= active[v] != 0;
             if (!any) break;
-            hip_check(hipMemcpyAsync(s->active.ptr, active.data(), sizeof(int32_t) * B,
+            hip_check(hipMemcpyAsync(s->active.ptr, active.data(), sizeof(int32_t) * V,
                                      hipMemcpyHostToDevice, st),
                       "active H2D");
-            hip_check(hipMemcpyAsync(s->xprev.ptr, s->x64.ptr, sizeof(double) * 2 * (size_t)B * N,
+            hip_check(hipMemcpyAsync(s->xprev.ptr, s->x64.ptr, sizeof(double) * 2 * (size_t)V * N,
                                      hipMemcpyDeviceToDevice, st),
                       "xprev copy");
-            hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * B * N, st);
-            hbem::launch_copy_f64_to_f32(s->r64.as<double>(), s->x.as<float>(), N, N, n, B, st);
-            hbem::lu_solve(w, s->x.as<float>(), N, 0, st);
-            hbem::launch_axpy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, B,
+            hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * V * N, st);
+            hbem::launch_copy_f64_to_f32(s->r64.as<double>(), s->x.as<float>(), N, N, n, V, st);
+            hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
+            hbem::launch_axpy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, V,
                                          s->active.as<int32_t>(), st);
             residual();
-            for (int b = 0; b < B; ++b) {
-                if (!active[b]) continue;
+            for (int v = 0; v < V; ++v) {
+                if (!active[v]) continue;
                 const double rel =
-                    nrm[2 * b + 1] > 0.0 ? nrm[2 * b] / nrm[2 * b + 1] : nrm[2 * b];
-                if (rel >= best[b]) {
+                    nrm[2 * v + 1] > 0.0 ? nrm[2 * v] / nrm[2 * v + 1] : nrm[2 * v];
+                if (rel >= best[v]) {
                     // stalled: keep the previous iterate (main.swift:7051-7058)
-                    hip_check(hipMemcpyAsync(s->x64.as<double>() + (size_t)b * N * 2,
-                                             s->xprev.as<double>() + (size_t)b * N * 2,
+                    hip_check(hipMemcpyAsync(s->x64.as<double>() + (size_t)v * N * 2,
+                                             s->xprev.as<double>() + (size_t)v * N * 2,
                                              sizeof(double) * 2 * (size_t)N,
                                              hipMemcpyDeviceToDevice, st),
                               "revert");
-                    active[b] = 0;
+                    active[v] = 0;
                     continue;
                 }
-                best[b] = rel;
-                iters[b]++;
-                if (rel <= opt.refine_tolerance) active[b] = 0;
+                best[v] = rel;
+                iters[v]++;
+                if (rel <= opt.refine_tolerance) active[v] = 0;
             }
         }
-        hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * B * N, st);
-        hbem::launch_copy_f64_to_f32(s->x64.as<double>(), s->x.as<float>(), N, N, n, B, st);
-        for (int b = 0; b < B; ++b) resid[b] = best[b];
+        hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * V * N, st);
+        hbem::launch_copy_f64_to_f32(s->x64.as<double>(), s->x.as<float>(), N, N, n, V, st);
+        for (int v = 0; v < V; ++v) resid[v] = best[v];
     }
     std::vector<double> rcond(B, -1.0);
     if (opt.estimate_rcond) estimate_rcond(s, w, anorm, &rcond);
@@ -818,8 +840,12 @@ void solve_resident(hbem_session *s, int B, int n, int N, const hbem_solve_optio
     for (int b = 0; b < B; ++b) {
         hbem_case_diag &d = (*diag)[b];
         d.lapack_info = info[b];
-        d.refine_iterations = iters[b];
-        d.refine_residual_rel = resid[b];
+        d.refine_iterations = 0;
+        d.refine_residual_rel = -1.0;
+        for (int src = 0; src < S; ++src) {
+            d.refine_iterations = std::max(d.refine_iterations, iters[(size_t)b * S + src]);
+            d.refine_residual_rel = std::max(d.refine_residual_rel, resid[(size_t)b * S + src]);
+        }
         d.dense_solve_condition_1norm = anorm[b];
         d.dense_solve_rcond = rcond[b];
     }
@@ -844,20 +870,24 @@ void check_options(const hbem_solve_options &o) {
     if (o.refine_max_iterations < 0) fail(HBEM_EINVAL, "refine_max_iterations must be >= 0");
     if (!(o.refine_tolerance >= 0.0)) fail(HBEM_EINVAL, "refine_tolerance must be >= 0");
     if (o.n_tags < 0 || (o.n_tags > 0 && !o.tags)) fail(HBEM_EINVAL, "tags invalid");
+    if (o.n_sources < 0 || o.n_sources > 4096) fail(HBEM_EINVAL, "n_sources out of range");
 }
 
 // All-device batch pipeline.  Inputs: k_real/field_k [F], k_imag [F] or
-// null, q [F][T] interleaved complex, obs [3][n_obs].  Outputs: field
-// [F][n_obs] interleaved complex (may be null when n_obs == 0), p [F][n]
-// interleaved complex or null, tag_avg [F][1 + n_tags][2] double (impedance
-// tag first).  Frequencies are processed in resident chunks.
+// null, q [F][S][T] interleaved complex, beta [F][T] interleaved complex or
+// null, obs [3][n_obs].  Outputs: field [F][S][n_obs] interleaved complex
+// (may be null when n_obs == 0), p [F][S][n] interleaved complex or null,
+// tag_avg [F][S][1 + n_tags][2] double (impedance tag first).  Frequencies
+// are processed in resident chunks; the S right-hand sides of a frequency
+// share its matrix and LU.
 void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
-                const float *fk_d, const float *q_d, int n_obs, const float *obs_d,
-                const hbem_solve_options &opts, float *field_d, float *p_d, double *tag_d,
-                std::vector<hbem_case_diag> *diag_out) {
+                const float *fk_d, const float *q_d, const float *beta_d, int n_obs,
+                const float *obs_d, const hbem_solve_options &opts, float *field_d, float *p_d,
+                double *tag_d, std::vector<hbem_case_diag> *diag_out) {
     hipStream_t st = s->stream;
     const int n = s->plan.n_dofs;
     const int T = s->plan.n_triangles;
+    const int S = std::max(1, opts.n_sources);
     const int N = (int)round_up(n, hbem::kLuPad);
     const int64_t lda = N, a_stride = (int64_t)N * N;
     const int Bmax = batch_size_for(s, F, N, opts);
@@ -867,26 +897,35 @@ void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
     const int ntag = (int)tag_list.size();
     upload(s->tags, tag_list, st);
     diag_out->assign(F, hbem_case_diag{});
-    if (n_obs > 0) hbem::launch_fill_zero(field_d, (size_t)2 * F * n_obs, st);
+    if (n_obs > 0) hbem::launch_fill_zero(field_d, (size_t)2 * F * S * n_obs, st);
     for (int f0 = 0; f0 < F; f0 += Bmax) {
-        const int B = std::min(Bmax, F - f0);
+        const int B = std::min(Bmax, F - f0), V = B * S;
+        const float *qc = q_d + (size_t)f0 * S * T * 2;
+        const float *bc = beta_d ? beta_d + (size_t)f0 * T * 2 : nullptr;
         Timer ta, treg, tsing, tf;
         ta.start(st);
-        assemble_device(s, B, kre_d + f0, kim_d ? kim_d + f0 : nullptr, q_d + (size_t)f0 * T * 2,
-                        lda, a_stride, N, &treg, &tsing);
+        assemble_device(s, B, S, kre_d + f0, kim_d ? kim_d + f0 : nullptr, qc, bc, lda, a_stride,
+                        N, &treg, &tsing);
         ta.stop(st);
         std::vector<hbem_case_diag> dg;
         double solve_s = 0.0;
-        solve_resident(s, B, n, N, opts, &dg, &solve_s);
+        solve_resident(s, B, S, n, N, opts, &dg, &solve_s);
         tf.start(st);
+        const float *qf = qc;
+        if (bc) {
+            s->qt.reserve(sizeof(float) * 2 * (size_t)V * T);
+            hbem::launch_robin_neumann(s->dg, V, S, kre_d + f0, kim_d ? kim_d + f0 : nullptr, qc,
+                                       bc, s->x.as<float>(), N, s->qt.as<float>(), st);
+            qf = s->qt.as<float>();
+        }
         if (n_obs > 0)
-            hbem::launch_field(s->dg, B, fk_d + f0, s->x.as<float>(), N, q_d + (size_t)f0 * T * 2,
-                               n_obs, obs_d, field_d + (size_t)f0 * n_obs * 2, st);
-        hbem::launch_tag_average(s->dg, B, s->x.as<float>(), N, ntag, s->tags.as<int32_t>(),
-                                 tag_d + (size_t)f0 * ntag * 2, st);
+            hbem::launch_field(s->dg, V, fk_d + f0, S, s->x.as<float>(), N, qf, n_obs, obs_d,
+                               field_d + (size_t)f0 * S * n_obs * 2, st);
+        hbem::launch_tag_average(s->dg, V, s->x.as<float>(), N, ntag, s->tags.as<int32_t>(),
+                                 tag_d + (size_t)f0 * S * ntag * 2, st);
         if (p_d)
-            hip_check(hipMemcpy2DAsync(p_d + (size_t)f0 * n * 2, sizeof(float) * 2 * n,
-                                       s->x.ptr, sizeof(float) * 2 * N, sizeof(float) * 2 * n, B,
+            hip_check(hipMemcpy2DAsync(p_d + (size_t)f0 * S * n * 2, sizeof(float) * 2 * n,
+                                       s->x.ptr, sizeof(float) * 2 * N, sizeof(float) * 2 * n, V,
                                        hipMemcpyDeviceToDevice, st),
                       "p copy");
         hip_check(hipGetLastError(), "field launch");
@@ -935,6 +974,7 @@ int32_t hbem_solve_batch(hbem_session *s, int32_t F, const float *k_real, const 
         hipStream_t st = s->stream;
         const int n = s->plan.n_dofs;
         const int T = s->plan.n_triangles;
+        const int S = std::max(1, opts->n_sources);
         const bool complex_k = any_nonzero(k_imag, (size_t)F);
         std::vector<float> kr(k_real, k_real + F), fk(field_k, field_k + F);
         upload(s->io_k, kr, st);
@@ -943,30 +983,36 @@ int32_t hbem_solve_batch(hbem_session *s, int32_t F, const float *k_real, const 
             std::vector<float> ki(k_imag, k_imag + F);
             upload(s->io_kim, ki, st);
         }
-        upload(s->io_q, interleave(q_re, q_im, (size_t)F * T), st);
+        upload(s->io_q, interleave(q_re, q_im, (size_t)F * S * T), st);
+        const bool robin = any_nonzero(opts->robin_beta, (size_t)2 * F * T);
+        if (robin)
+            upload(s->io_beta, std::vector<float>(opts->robin_beta, opts->robin_beta + (size_t)2 * F * T),
+                   st);
         if (n_obs > 0) upload(s->io_obs, std::vector<float>(obs_3xn, obs_3xn + 3 * (size_t)n_obs), st);
         const int ntag = 1 + opts->n_tags;
-        s->io_field.reserve(sizeof(float) * 2 * std::max<size_t>(1, (size_t)F * n_obs));
-        s->io_tag.reserve(sizeof(double) * 2 * (size_t)F * ntag);
+        const size_t FS = (size_t)F * S;
+        s->io_field.reserve(sizeof(float) * 2 * std::max<size_t>(1, FS * n_obs));
+        s->io_tag.reserve(sizeof(double) * 2 * FS * ntag);
         const bool want_p = p_re && p_im;
-        if (want_p) s->io_p.reserve(sizeof(float) * 2 * (size_t)F * n);
+        if (want_p) s->io_p.reserve(sizeof(float) * 2 * FS * n);
         std::vector<hbem_case_diag> dg;
         solve_core(s, F, s->io_k.as<float>(), complex_k ? s->io_kim.as<float>() : nullptr,
-                   s->io_fk.as<float>(), s->io_q.as<float>(), n_obs, s->io_obs.as<float>(), *opts,
+                   s->io_fk.as<float>(), s->io_q.as<float>(),
+                   robin ? s->io_beta.as<float>() : nullptr, n_obs, s->io_obs.as<float>(), *opts,
                    s->io_field.as<float>(), want_p ? s->io_p.as<float>() : nullptr,
                    s->io_tag.as<double>(), &dg);
-        std::vector<float> fh((size_t)2 * F * std::max(n_obs, 0));
+        std::vector<float> fh((size_t)2 * FS * std::max(n_obs, 0));
         if (n_obs > 0)
             hip_check(hipMemcpyAsync(fh.data(), s->io_field.ptr, sizeof(float) * fh.size(),
                                      hipMemcpyDeviceToHost, st),
                       "field D2H");
-        std::vector<double> th((size_t)2 * F * ntag);
+        std::vector<double> th((size_t)2 * FS * ntag);
         hip_check(hipMemcpyAsync(th.data(), s->io_tag.ptr, sizeof(double) * th.size(),
                                  hipMemcpyDeviceToHost, st),
                   "tags D2H");
         std::vector<float> ph;
         if (want_p) {
-            ph.resize((size_t)2 * F * n);
+            ph.resize((size_t)2 * FS * n);
             hip_check(hipMemcpyAsync(ph.data(), s->io_p.ptr, sizeof(float) * ph.size(),
                                      hipMemcpyDeviceToHost, st),
                       "p D2H");
@@ -976,26 +1022,27 @@ int32_t hbem_solve_batch(hbem_session *s, int32_t F, const float *k_real, const 
             if (dg[f].lapack_info != 0 && !diag)
                 fail(HBEM_ESINGULAR, "LU found an exactly zero pivot (lapack_info=" +
                                          std::to_string(dg[f].lapack_info) + ")");
+            if (diag) diag[f] = dg[f];
+        }
+        for (size_t v = 0; v < FS; ++v) {
             for (int o = 0; o < n_obs; ++o) {
-                field_re[(size_t)f * n_obs + o] = fh[((size_t)f * n_obs + o) * 2];
-                field_im[(size_t)f * n_obs + o] = fh[((size_t)f * n_obs + o) * 2 + 1];
+                field_re[v * n_obs + o] = fh[(v * n_obs + o) * 2];
+                field_im[v * n_obs + o] = fh[(v * n_obs + o) * 2 + 1];
             }
             if (want_p)
                 for (int i = 0; i < n; ++i) {
-                    p_re[(size_t)f * n + i] = ph[((size_t)f * n + i) * 2];
-                    p_im[(size_t)f * n + i] = ph[((size_t)f * n + i) * 2 + 1];
+                    p_re[v * n + i] = ph[(v * n + i) * 2];
+                    p_im[v * n + i] = ph[(v * n + i) * 2 + 1];
                 }
             if (impedance) {
-                impedance[2 * f] = th[((size_t)f * ntag) * 2];
-                impedance[2 * f + 1] = th[((size_t)f * ntag) * 2 + 1];
+                impedance[2 * v] = th[(v * ntag) * 2];
+                impedance[2 * v + 1] = th[(v * ntag) * 2 + 1];
             }
             if (pavg)
                 for (int t = 0; t < opts->n_tags; ++t) {
-                    pavg[((size_t)f * opts->n_tags + t) * 2] = th[((size_t)f * ntag + 1 + t) * 2];
-                    pavg[((size_t)f * opts->n_tags + t) * 2 + 1] =
-                        th[((size_t)f * ntag + 1 + t) * 2 + 1];
+                    pavg[(v * opts->n_tags + t) * 2] = th[(v * ntag + 1 + t) * 2];
+                    pavg[(v * opts->n_tags + t) * 2 + 1] = th[(v * ntag + 1 + t) * 2 + 1];
                 }
-            if (diag) diag[f] = dg[f];
         }
     });
 }
@@ -1016,8 +1063,8 @@ int32_t hbem_solve_batch_device(hbem_session *s, int32_t F, const float *k_real,
         std::lock_guard<std::mutex> lock(s->mu);
         set_device(s);
         std::vector<hbem_case_diag> dg;
-        solve_core(s, F, k_real, k_imag, field_k, q, n_obs, obs_3xn, *opts, field, p, tag_avg,
-                   &dg);
+        solve_core(s, F, k_real, k_imag, field_k, q, opts->robin_beta, n_obs, obs_3xn, *opts,
+                   field, p, tag_avg, &dg);
         hip_check(hipStreamSynchronize(s->stream), "batch sync");
         for (int f = 0; f < F; ++f) {
             if (dg[f].lapack_info != 0 && !diag)
@@ -1049,7 +1096,7 @@ int32_t hbem_evaluate_field(hbem_session *s, int32_t F, const float *field_k, co
         upload(s->obs, obs, st);
         s->field.reserve(sizeof(float) * 2 * (size_t)F * n_obs);
         hbem::launch_fill_zero(s->field.as<float>(), (size_t)2 * F * n_obs, st);
-        hbem::launch_field(s->dg, F, s->fk.as<float>(), s->x.as<float>(), n, s->q.as<float>(),
+        hbem::launch_field(s->dg, F, s->fk.as<float>(), 1, s->x.as<float>(), n, s->q.as<float>(),
                            n_obs, s->obs.as<float>(), s->field.as<float>(), st);
         hip_check(hipGetLastError(), "field launch");
         std::vector<float> fh((size_t)2 * F * n_obs);
@@ -1103,7 +1150,7 @@ int32_t hbem_dense_solve(int32_t n_sys, int32_t n, const float *A_re, const floa
         upload(s.rhs, rh, s.stream);
         std::vector<hbem_case_diag> dg;
         double solve_s = 0.0;
-        solve_resident(&s, B, n, N, *opts, &dg, &solve_s);
+        solve_resident(&s, B, 1, n, N, *opts, &dg, &solve_s);
         std::vector<float> xh((size_t)2 * B * N);
         hip_check(hipMemcpyAsync(xh.data(), s.x.ptr, sizeof(float) * xh.size(),
                                  hipMemcpyDeviceToHost, s.stream),

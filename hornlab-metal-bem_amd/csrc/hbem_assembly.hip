@@ -234,8 +234,9 @@ __device__ __forceinline__ void regular_images(const float *__restrict__ tf, int
 template <bool kComplexK>
 __global__ __launch_bounds__(256) void k_assemble_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
-    const float *__restrict__ q, float *__restrict__ A, int64_t lda, int64_t a_stride,
-    float *__restrict__ rhs, int64_t rhs_stride) {
+    const float *__restrict__ q, int n_src, const float *__restrict__ beta,
+    float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
+    int64_t rhs_stride) {
     extern __shared__ float acc[];  // F: [S dofs][T dofs][2], then R: [T dofs][S dofs][2]
     const int T = g.n_triangles;
     const int lane = threadIdx.x & 63;
@@ -280,10 +281,21 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         for (int l = 0; l < 3; ++l)
             skid[m][l] = (m < M) ? g.tri_i_soa[(size_t)(6 + 3 * m + l) * T + tr] : -1;
     const int orig = g.tri_i_soa[(size_t)19 * T + tr];
-    const float qre = trial_ok ? q[((size_t)b * T + orig) * 2] : 0.0f;
-    const float qim = trial_ok ? q[((size_t)b * T + orig) * 2 + 1] : 0.0f;
-    const bool wave_has_source = __any(qre != 0.0f || qim != 0.0f);
-    float *rb = rhs + (size_t)b * rhs_stride * 2;
+    // Neumann data of the lane's triangle for every source (q [B][S][T])
+    bool lane_q = false;
+    for (int src = 0; src < n_src; ++src) {
+        const size_t qi = (((size_t)b * n_src + src) * T + orig) * 2;
+        lane_q |= trial_ok && (q[qi] != 0.0f || q[qi + 1] != 0.0f);
+    }
+    // Robin admittance of the lane's triangle: its single layer enters the
+    // matrix as -(i k beta / 3) SLP_i in every column of the triangle
+    // (main.swift:4101-4110); i k = (-k_im, k_re)
+    const float bre = (beta && trial_ok) ? beta[((size_t)b * T + orig) * 2] : 0.0f;
+    const float bim = (beta && trial_ok) ? beta[((size_t)b * T + orig) * 2 + 1] : 0.0f;
+    const float cFre = (-kim * bre - k * bim) * (1.0f / 3.0f);
+    const float cFim = (-kim * bim + k * bre) * (1.0f / 3.0f);
+    const bool lane_robin = bre != 0.0f || bim != 0.0f;
+    const bool wave_has_source = __any(lane_q || lane_robin);
     // -4 pi and 4 pi folded out of the kernels, jac = (2 A_test)(2 A_trial)
     const float dscale = -kInv4Pi * jac_s * g.row_weight;
     const float gscale = kInv4Pi * jac_s * g.row_weight;
@@ -297,9 +309,15 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         const int32_t *tii = g.tri_i_aos + (size_t)test * kTriI;
         const float jt = tf[22];
         const int torig = tii[19];
-        const float tqre = q[((size_t)b * T + torig) * 2];
-        const float tqim = q[((size_t)b * T + torig) * 2 + 1];
-        const bool test_source = (tqre != 0.0f || tqim != 0.0f);
+        bool test_q = false;
+        for (int src = 0; src < n_src; ++src) {
+            const size_t qi = (((size_t)b * n_src + src) * T + torig) * 2;
+            test_q |= q[qi] != 0.0f || q[qi + 1] != 0.0f;
+        }
+        const float tbre = beta ? beta[((size_t)b * T + torig) * 2] : 0.0f;
+        const float tbim = beta ? beta[((size_t)b * T + torig) * 2 + 1] : 0.0f;
+        const bool test_robin = tbre != 0.0f || tbim != 0.0f;
+        const bool test_source = test_q || test_robin;
         // lanes and directions of this (t, s) pair
         const bool fwd = trial_ok && (!diag || lane >= lt);
         const bool rev = trial_ok && (!diag || lane > lt);
@@ -329,6 +347,36 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         else
             regular_images<kComplexK, false>(tf, M, packed_masks, skip, yp, v0s, ns, k, krev,
                                              kim, dF, dR, sF, sR);
+        if (src) {
+            // Robin fold in the units of dF / dR: A gets sc * d, and
+            // -(i k beta / 3) SLP = sc * (+(i k beta / 3) s)
+            if (lane_robin) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float are = cFre * sF[i][0] - cFim * sF[i][1];
+                    const float aim = cFre * sF[i][1] + cFim * sF[i][0];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        dF[3 * i + j][0] += are;
+                        dF[3 * i + j][1] += aim;
+                    }
+                }
+            }
+            if (test_robin) {
+                const float cRre = (-kim * tbre - k * tbim) * (1.0f / 3.0f);
+                const float cRim = (-kim * tbim + k * tbre) * (1.0f / 3.0f);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float are = cRre * sR[i][0] - cRim * sR[i][1];
+                    const float aim = cRre * sR[i][1] + cRim * sR[i][0];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        dR[3 * i + j][0] += are;
+                        dR[3 * i + j][1] += aim;
+                    }
+                }
+            }
+        }
         const float sc = dscale * jt;
         if (fwd) {
 #pragma unroll
@@ -356,29 +404,36 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         }
         if (src) {
             const float gsc = gscale * jt;
-            if (wave_has_source) {
-                // rhs_t += 2^planes SLP(t, s) q_s, reduced over the wave
+            for (int sv = 0; sv < n_src; ++sv) {
+                float *rb = rhs + ((size_t)b * n_src + sv) * rhs_stride * 2;
+                const size_t lq = (((size_t)b * n_src + sv) * T + orig) * 2;
+                const float qre = trial_ok ? q[lq] : 0.0f, qim = trial_ok ? q[lq + 1] : 0.0f;
+                if (__any(qre != 0.0f || qim != 0.0f)) {
+                    // rhs_t += 2^planes SLP(t, s) q_s, reduced over the wave
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    float cre = fwd ? (sF[i][0] * qre - sF[i][1] * qim) * gsc : 0.0f;
-                    float cim = fwd ? (sF[i][0] * qim + sF[i][1] * qre) * gsc : 0.0f;
+                    for (int i = 0; i < 3; ++i) {
+                        float cre = fwd ? (sF[i][0] * qre - sF[i][1] * qim) * gsc : 0.0f;
+                        float cim = fwd ? (sF[i][0] * qim + sF[i][1] * qre) * gsc : 0.0f;
 #pragma unroll
-                    for (int off = 32; off >= 1; off >>= 1) {
-                        cre += __shfl_xor(cre, off);
-                        cim += __shfl_xor(cim, off);
-                    }
-                    if (lane == 0) {
-                        atomicAdd(rb + (size_t)tii[i] * 2, cre);
-                        atomicAdd(rb + (size_t)tii[i] * 2 + 1, cim);
+                        for (int off = 32; off >= 1; off >>= 1) {
+                            cre += __shfl_xor(cre, off);
+                            cim += __shfl_xor(cim, off);
+                        }
+                        if (lane == 0) {
+                            atomicAdd(rb + (size_t)tii[i] * 2, cre);
+                            atomicAdd(rb + (size_t)tii[i] * 2 + 1, cim);
+                        }
                     }
                 }
-            }
-            if (test_source && rev) {
-                // rhs_s += 2^planes SLP(s, t) q_t
+                const size_t tq = (((size_t)b * n_src + sv) * T + torig) * 2;
+                const float tqre = q[tq], tqim = q[tq + 1];
+                if ((tqre != 0.0f || tqim != 0.0f) && rev) {
+                    // rhs_s += 2^planes SLP(s, t) q_t
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    atomicAdd(rb + (size_t)sdof[i] * 2, (sR[i][0] * tqre - sR[i][1] * tqim) * gsc);
-                    atomicAdd(rb + (size_t)sdof[i] * 2 + 1, (sR[i][0] * tqim + sR[i][1] * tqre) * gsc);
+                    for (int i = 0; i < 3; ++i) {
+                        atomicAdd(rb + (size_t)sdof[i] * 2, (sR[i][0] * tqre - sR[i][1] * tqim) * gsc);
+                        atomicAdd(rb + (size_t)sdof[i] * 2 + 1, (sR[i][0] * tqim + sR[i][1] * tqre) * gsc);
+                    }
                 }
             }
         }
@@ -423,8 +478,9 @@ template <bool kComplexK, int kFB>
 __global__ __launch_bounds__(256) void k_assemble_singular(
     DeviceGeometry g, const float *__restrict__ rule, int n_batch,
     const float *__restrict__ k_real, const float *__restrict__ k_imag,
-    const float *__restrict__ q, float *__restrict__ A, int64_t lda, int64_t a_stride,
-    float *__restrict__ rhs, int64_t rhs_stride) {
+    const float *__restrict__ q, int n_src, const float *__restrict__ beta,
+    float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
+    int64_t rhs_stride) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     const int w1 = g.sg_waves[0], w2 = w1 + g.sg_waves[1], w3 = w2 + g.sg_waves[2];
@@ -570,21 +626,32 @@ __global__ __launch_bounds__(256) void k_assemble_singular(
         const int b = b0 + f;
         if (b >= n_batch) break;
         float *Ab = A + (size_t)b * a_stride * 2;
+        // Robin fold -(i k beta / 3) SLP_a of the trial triangle
+        double cre = 0.0, cim = 0.0;
+        if (beta) {
+            const double br = beta[((size_t)b * T + orig) * 2];
+            const double bi = beta[((size_t)b * T + orig) * 2 + 1];
+            cre = (-(double)kim[f] * br - (double)kk[f] * bi) / 3.0;
+            cim = (-(double)kim[f] * bi + (double)kk[f] * br) / 3.0;
+        }
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb) {
-                float vre = (float)(dd[f][3 * a + bb][0] * scale);
-                const float vim = (float)(dd[f][3 * a + bb][1] * scale);
+                const double rre = cre * ds[f][a][0] - cim * ds[f][a][1];
+                const double rim = cre * ds[f][a][1] + cim * ds[f][a][0];
+                float vre = (float)((dd[f][3 * a + bb][0] - rre) * scale);
+                const float vim = (float)((dd[f][3 * a + bb][1] - rim) * scale);
                 if (mc != 0.0f) vre += mc * (rows[a] == cols[bb] ? (1.0f / 6.0f) : (1.0f / 12.0f));
                 const size_t idx = ((size_t)cols[bb] * lda + rows[a]) * 2;
                 atomicAdd(Ab + idx, vre);
                 atomicAdd(Ab + idx + 1, vim);
             }
-        const float qre = q[((size_t)b * T + orig) * 2];
-        const float qim = q[((size_t)b * T + orig) * 2 + 1];
-        if (qre != 0.0f || qim != 0.0f) {
-            float *rb = rhs + (size_t)b * rhs_stride * 2;
+        for (int sv = 0; sv < n_src; ++sv) {
+            const size_t qi = (((size_t)b * n_src + sv) * T + orig) * 2;
+            const float qre = q[qi], qim = q[qi + 1];
+            if (qre == 0.0f && qim == 0.0f) continue;
+            float *rb = rhs + ((size_t)b * n_src + sv) * rhs_stride * 2;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const float sre = (float)(ds[f][a][0] * scale);
@@ -597,6 +664,44 @@ __global__ __launch_bounds__(256) void k_assemble_singular(
 }
 
 // ---------------------------------------------------------------------------
+// Neumann data seen by the field with Robin walls (neumannWithRobin,
+// main.swift:1646-1674): qt = q + i k beta (p0 + p1 + p2) / 3 per triangle
+// (caller order), i k = (-k_im, k_re); one thread per (triangle, vector),
+// vector v of frequency v / vpk.
+// ---------------------------------------------------------------------------
+
+__global__ void k_robin_neumann(const int32_t *__restrict__ l2g, int T, int vpk,
+                                const float *__restrict__ k_real,
+                                const float *__restrict__ k_imag, const float *__restrict__ q,
+                                const float *__restrict__ beta, const float *__restrict__ p,
+                                int64_t p_stride, float *__restrict__ qt) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int v = blockIdx.y;
+    if (t >= T) return;
+    const int f = v / vpk;
+    const size_t qi = ((size_t)v * T + t) * 2;
+    float re = q[qi], im = q[qi + 1];
+    const float br = beta[((size_t)f * T + t) * 2], bi = beta[((size_t)f * T + t) * 2 + 1];
+    if (br != 0.0f || bi != 0.0f) {
+        const float *pv = p + (size_t)v * p_stride * 2;
+        float pr = 0.0f, pim = 0.0f;
+        for (int l = 0; l < 3; ++l) {
+            const int d = l2g[3 * t + l];
+            pr += pv[2 * d];
+            pim += pv[2 * d + 1];
+        }
+        pr *= (1.0f / 3.0f);
+        pim *= (1.0f / 3.0f);
+        const float ikr = k_imag ? -k_imag[f] : 0.0f, iki = k_real[f];
+        const float cr = ikr * br - iki * bi, ci = ikr * bi + iki * br;  // i k beta
+        re += cr * pr - ci * pim;
+        im += cr * pim + ci * pr;
+    }
+    qt[qi] = re;
+    qt[qi + 1] = im;
+}
+
+// ---------------------------------------------------------------------------
 // Exterior field.  Lanes own observation points; a wave walks a chunk of
 // triangles (wave-uniform records).  grid = (obs blocks, triangle chunks, B).
 // ---------------------------------------------------------------------------
@@ -605,17 +710,17 @@ constexpr int kFieldChunk = 64;
 
 template <int kMasks>
 __global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__restrict__ field_k,
-                                               const float *__restrict__ p, int64_t p_stride,
-                                               const float *__restrict__ q, int n_obs,
-                                               const float *__restrict__ obs,
+                                               int vpk, const float *__restrict__ p,
+                                               int64_t p_stride, const float *__restrict__ q,
+                                               int n_obs, const float *__restrict__ obs,
                                                float *__restrict__ out) {
     const int T = g.n_triangles;
     const int o = blockIdx.x * 256 + threadIdx.x;
-    const int b = blockIdx.z;
+    const int b = blockIdx.z;  // vector; its wavenumber is field_k[b / vpk]
     const bool ok = o < n_obs;
     const int oo = ok ? o : 0;
     const float ox = obs[oo], oy = obs[n_obs + oo], oz = obs[2 * n_obs + oo];
-    const float k = field_k[b];
+    const float k = field_k[b / vpk];
     const float krev = k * kInv2Pi;
     const float *pb = p + (size_t)b * p_stride * 2;
     const float *qb = q + (size_t)b * T * 2;
@@ -740,55 +845,63 @@ void launch_fill_zero_f64(double *ptr, size_t n, hipStream_t stream) {
 }
 
 void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *k_real,
-                             const float *k_imag, const float *q, float *A, int64_t lda,
-                             int64_t a_stride, float *rhs, int64_t rhs_stride,
-                             hipStream_t stream) {
+                             const float *k_imag, const float *q, int n_src, const float *beta,
+                             float *A, int64_t lda, int64_t a_stride, float *rhs,
+                             int64_t rhs_stride, hipStream_t stream) {
     if (g.n_tile_pairs == 0 || n_batch <= 0) return;
     dim3 grid(g.n_tile_pairs, n_batch);
     // two tables of (max tile dofs)^2 complex floats
     const size_t shm = sizeof(float) * 4 * (size_t)g.max_tile_dofs * g.max_tile_dofs;
     if (k_imag != nullptr)
         hipLaunchKernelGGL((k_assemble_regular<true>), grid, dim3(256), shm, stream, g, k_real,
-                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
+                           k_imag, q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
     else
         hipLaunchKernelGGL((k_assemble_regular<false>), grid, dim3(256), shm, stream, g, k_real,
-                           k_imag, q, A, lda, a_stride, rhs, rhs_stride);
+                           k_imag, q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
 }
 
 void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float *k_real,
-                              const float *k_imag, const float *q, float *A, int64_t lda,
-                              int64_t a_stride, float *rhs, int64_t rhs_stride,
-                              hipStream_t stream) {
+                              const float *k_imag, const float *q, int n_src, const float *beta,
+                              float *A, int64_t lda, int64_t a_stride, float *rhs,
+                              int64_t rhs_stride, hipStream_t stream) {
     const int waves = g.sg_waves[0] + g.sg_waves[1] + g.sg_waves[2];
     if (waves == 0 || n_batch <= 0) return;
     constexpr int kFB = 2;
     dim3 grid((waves + 3) / 4, (n_batch + kFB - 1) / kFB);
     if (k_imag != nullptr)
         hipLaunchKernelGGL((k_assemble_singular<true, kFB>), grid, dim3(256), 0, stream, g,
-                           g.sg_rule, n_batch, k_real, k_imag, q, A, lda, a_stride, rhs,
+                           g.sg_rule, n_batch, k_real, k_imag, q, n_src, beta, A, lda, a_stride, rhs,
                            rhs_stride);
     else
         hipLaunchKernelGGL((k_assemble_singular<false, kFB>), grid, dim3(256), 0, stream, g,
-                           g.sg_rule, n_batch, k_real, k_imag, q, A, lda, a_stride, rhs,
+                           g.sg_rule, n_batch, k_real, k_imag, q, n_src, beta, A, lda, a_stride, rhs,
                            rhs_stride);
 }
 
-void launch_field(const DeviceGeometry &g, int n_batch, const float *field_k, const float *p,
-                  int64_t p_stride, const float *q, int n_obs, const float *obs_3xn, float *out,
-                  hipStream_t stream) {
+void launch_robin_neumann(const DeviceGeometry &g, int n_vec, int vpk, const float *k_real,
+                          const float *k_imag, const float *q, const float *beta, const float *p,
+                          int64_t p_stride, float *qt, hipStream_t stream) {
+    hipLaunchKernelGGL(k_robin_neumann, dim3((g.n_triangles + 255) / 256, n_vec), dim3(256), 0,
+                       stream, g.l2g, g.n_triangles, vpk, k_real, k_imag, q, beta, p, p_stride,
+                       qt);
+}
+
+void launch_field(const DeviceGeometry &g, int n_batch, const float *field_k, int vpk,
+                  const float *p, int64_t p_stride, const float *q, int n_obs,
+                  const float *obs_3xn, float *out, hipStream_t stream) {
     if (n_obs == 0) return;
     dim3 grid((n_obs + 255) / 256, (g.n_triangles + kFieldChunk - 1) / kFieldChunk, n_batch);
     switch (g.n_masks) {
     case 1:
-        hipLaunchKernelGGL((k_field<1>), grid, dim3(256), 0, stream, g, field_k, p, p_stride, q,
+        hipLaunchKernelGGL((k_field<1>), grid, dim3(256), 0, stream, g, field_k, vpk, p, p_stride, q,
                            n_obs, obs_3xn, out);
         break;
     case 2:
-        hipLaunchKernelGGL((k_field<2>), grid, dim3(256), 0, stream, g, field_k, p, p_stride, q,
+        hipLaunchKernelGGL((k_field<2>), grid, dim3(256), 0, stream, g, field_k, vpk, p, p_stride, q,
                            n_obs, obs_3xn, out);
         break;
     default:
-        hipLaunchKernelGGL((k_field<4>), grid, dim3(256), 0, stream, g, field_k, p, p_stride, q,
+        hipLaunchKernelGGL((k_field<4>), grid, dim3(256), 0, stream, g, field_k, vpk, p, p_stride, q,
                            n_obs, obs_3xn, out);
         break;
     }

@@ -165,6 +165,7 @@ struct DeviceGeometry {
     const int32_t *tile_ndofs;
     const int32_t *tile_dofs;
     const int32_t *tile_pairs;
+    const int32_t *l2g;  // [T][3] P1 dofs in caller triangle order
     int n_tiles;
     int n_tile_pairs;
     int max_tile_dofs;
@@ -176,20 +177,29 @@ void launch_fill_zero_f64(double *ptr, size_t n, hipStream_t stream);
 
 // Regular (non-touching) pair quadrature of all pairs, all frequencies of a
 // batch, accumulated into A (column-major, interleaved complex, lda) and
-// rhs (interleaved complex, stride ldr per frequency).
+// rhs (interleaved complex, stride rhs_stride per vector, n_src vectors per
+// frequency).  q [B][n_src][T] complex; beta [B][T] complex Robin admittance
+// or null.
 void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *k_real,
-                             const float *k_imag, const float *q /* [B][T][2] */, float *A,
-                             int64_t lda, int64_t a_stride, float *rhs, int64_t rhs_stride,
-                             hipStream_t stream);
+                             const float *k_imag, const float *q, int n_src, const float *beta,
+                             float *A, int64_t lda, int64_t a_stride, float *rhs,
+                             int64_t rhs_stride, hipStream_t stream);
 
 // Singular (Sauter-Schwab) pair blocks plus the -M/2 mass term.
 void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float *k_real,
-                              const float *k_imag, const float *q, float *A, int64_t lda,
-                              int64_t a_stride, float *rhs, int64_t rhs_stride,
-                              hipStream_t stream);
+                              const float *k_imag, const float *q, int n_src, const float *beta,
+                              float *A, int64_t lda, int64_t a_stride, float *rhs,
+                              int64_t rhs_stride, hipStream_t stream);
 
-// Exterior field p(x_o) for each frequency of a batch.
-void launch_field(const DeviceGeometry &g, int n_batch, const float *field_k,
+// qt = q + i k beta mean(p) per triangle for n_vec vectors (vpk per
+// frequency): the Neumann data of the field with Robin walls.
+void launch_robin_neumann(const DeviceGeometry &g, int n_vec, int vpk, const float *k_real,
+                          const float *k_imag, const float *q, const float *beta, const float *p,
+                          int64_t p_stride, float *qt, hipStream_t stream);
+
+// Exterior field p(x_o) for n_batch vectors (vector v at wavenumber
+// field_k[v / vpk]).
+void launch_field(const DeviceGeometry &g, int n_batch, const float *field_k, int vpk,
                   const float *p /* [B][p_stride] complex */, int64_t p_stride,
                   const float *q /* [B][T] complex */, int n_obs, const float *obs_3xn,
                   float *out /* [B][n_obs] complex */, hipStream_t stream);
@@ -212,11 +222,18 @@ struct LuWorkspace {
     int32_t *ipiv;     // [B][N]
     int32_t *info;     // [B]
     int32_t *gl;       // [B][N / 32][kLuGather] composed row exchanges per step
+    int32_t *scratch_i;  // [B][kLuMaxOrder / 32] tournament candidates
+    float *scratch_f;    // [B][32][32] + [B][64][32] complex: top block, gathered rows
 };
 
 constexpr int kLuPad = 64;
 constexpr int kLuOuter = 128;   // outer block of the two-level LU (exchange granularity)
 constexpr int kLuGather = 132;  // ints per step of LuWorkspace::gl
+constexpr int kLuMaxOrder = 32768;  // tournament panels nominate from <= 32 chunks
+
+// sizes of LuWorkspace::scratch_i (ints) and scratch_f (floats) per system
+constexpr size_t kLuScratchI = kLuMaxOrder / 32;
+constexpr size_t kLuScratchF = (32 * 32 + 64 * 32) * 2;
 
 // ints of LuWorkspace::gl per system
 inline size_t lu_gather_ints(int N) { return (size_t)(N / 32) * kLuGather; }
@@ -238,15 +255,18 @@ void launch_pow2_scale(const double *amax, float *scale, int B, hipStream_t stre
 void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStream_t stream);
 
 // Solves A x = b (trans = 0) or A^H x = b (trans = 2) in place on x
-// [B][ldx] interleaved complex, using the LU factors and pivots.
-void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream);
+// [B * vpm][ldx] interleaved complex (vector v against matrix v / vpm), using
+// the LU factors and pivots.
+void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream,
+              int vpm = 1);
 
 // Residual r = b - A x in FP64 with A the FP32 operator (column-major
 // interleaved), x and r FP64 interleaved.  Also writes per-system
 // ||r||_inf and ||b||_inf into norms[2*b], norms[2*b+1].
+// (B vectors, vector v against matrix v / vpm)
 void launch_residual_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,
                          const float *b, int64_t ldb, const double *x, int64_t ldx, double *r,
-                         double *norms, hipStream_t stream);
+                         double *norms, hipStream_t stream, int vpm = 1);
 
 // Helpers for refinement and rcond estimation.
 void launch_copy_f32_to_f64(const float *src, double *dst, int64_t ld_src, int64_t ld_dst, int n,
@@ -260,7 +280,8 @@ void launch_axpy_f32_to_f64(const float *d, double *x, int64_t ld_d, int64_t ld_
 // ld per system; a basis V holds vector i of system b at (b * vstride +
 // i * ldv) complex elements).
 void launch_gemv_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B, const double *x,
-                     int64_t ldx, double *y, int64_t ldy, hipStream_t stream);  // y = A x
+                     int64_t ldx, double *y, int64_t ldy, hipStream_t stream,
+                     int vpm = 1);  // y = A x, B vectors, matrix v / vpm
 void launch_dots_f64(const double *V, int64_t ldv, int64_t vstride, int m, const double *w,
                      int64_t ldw, int n, int B, double *h, int64_t ldh,
                      hipStream_t stream);  // h_i = V_i^H w, i < m

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 #include <type_traits>
 
 #include "hbem_internal.h"
@@ -157,69 +158,211 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long ke
     return key;
 }
 
-// General panel (any N): one 1024-thread block per system factors columns
-// [k, k + kNB) over rows [k, N) in global memory.
-__global__ __launch_bounds__(1024) void k_lu_panel(float *A, int64_t lda, int64_t a_stride, int N,
-                                                   int k, int32_t *ipiv, int32_t *info,
-                                                   int32_t *gl, int64_t gl_stride) {
-    const int b = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    float *Ab = A + (size_t)b * a_stride * 2;
-    __shared__ unsigned long long red[16];
-    __shared__ int piv_s[kNB];
-    __shared__ c32 urow[kNB];
-    for (int j = 0; j < kNB; ++j) {
-        const int col = k + j;
-        unsigned long long key = 0ull;
-        for (int r = col + tid; r < N; r += 1024) {
-            const unsigned long long kr = pivot_key(ld(Ab, (size_t)col * lda + r), r);
-            key = kr > key ? kr : key;
-        }
-        key = wave_max_u64(key);
-        if (lane == 0) red[wave] = key;
-        __syncthreads();
-        unsigned long long best = red[0];
-#pragma unroll
-        for (int w = 1; w < 16; ++w) best = red[w] > best ? red[w] : best;
-        const int p = static_cast<int>(0xffffffffu - static_cast<unsigned>(best));
-        if (tid == 0) {
-            piv_s[j] = p;
-            ipiv[(size_t)b * N + col] = p;
-            if (!(__uint_as_float(static_cast<unsigned>(best >> 32)) > 0.0f) && info[b] == 0)
-                info[b] = col + 1;
-        }
-        if (p != col && tid < kNB) {
-            const size_t cc = (size_t)(k + tid) * lda;
-            const c32 x = ld(Ab, cc + col), y = ld(Ab, cc + p);
-            st(Ab, cc + col, y);
-            st(Ab, cc + p, x);
-        }
-        __syncthreads();
-        const c32 piv = ld(Ab, (size_t)col * lda + col);
-        if (tid < kNB - j - 1) urow[tid] = ld(Ab, (size_t)(col + 1 + tid) * lda + col);
-        __syncthreads();
-        const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
-        const c32 inv = nz ? cinv(piv) : c32{0.0f, 0.0f};
-        const int rest = kNB - j - 1;
-        for (int r = col + 1 + tid; r < N; r += 1024) {
-            c32 l = ld(Ab, (size_t)col * lda + r);
-            if (nz) {
-                l = cmul(l, inv);
-                st(Ab, (size_t)col * lda + r, l);
-            }
-            for (int c = 0; c < rest; ++c) {
-                const size_t idx = (size_t)(col + 1 + c) * lda + r;
-                const c32 u = urow[c];
-                c32 a = ld(Ab, idx);
-                a.re -= l.re * u.re - l.im * u.im;
-                a.im -= l.re * u.im + l.im * u.re;
-                st(Ab, idx, a);
-            }
-        }
-        __syncthreads();
+// Partial-pivoting elimination of the kNB panel columns over the rows held
+// by the block, one row per thread: the pivot of column j is the row not yet
+// chosen with the largest |re| + |im| (ties: the smallest id), and every
+// other unchosen row is eliminated against it.  Returns the column the
+// thread's row became the pivot of (or -1); v ends as that row's factored
+// values (multipliers left of its pivot column, U entries from it).  win_s[j]
+// is the id of column j's pivot (-1 without candidates), mag_s[j] its
+// |re| + |im|.
+__device__ int eliminate_rows(c32 (&v)[kNB], bool own, unsigned id,
+                              unsigned long long *best_s, c32 *prow, int *win_s,
+                              float *mag_s) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    int rank = -1;
+    if (tid == 0) {
+        best_s[0] = 0ull;
+        best_s[1] = 0ull;
     }
-    if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+        unsigned long long key = (own && rank < 0) ? pivot_key(v[j], (int)id) : 0ull;
+        key = wave_max_u64(key);
+        if (lane == 0) atomicMax(&best_s[j & 1], key);
+        __syncthreads();
+        const unsigned long long best = best_s[j & 1];
+        const unsigned wid = 0xffffffffu - static_cast<unsigned>(best);
+        const bool holder = best != 0ull && own && rank < 0 && id == wid;
+        if (holder) {
+            rank = j;
+#pragma unroll
+            for (int c = j; c < kNB; ++c) prow[c] = v[c];
+        }
+        if (tid == 0) {
+            best_s[(j + 1) & 1] = 0ull;
+            win_s[j] = best != 0ull ? static_cast<int>(wid) : -1;
+            mag_s[j] = __uint_as_float(static_cast<unsigned>(best >> 32));
+        }
+        __syncthreads();
+        if (best != 0ull && own && rank < 0) {
+            const c32 piv = prow[j];
+            const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
+            c32 l = v[j];
+            if (nz) l = cmul(l, cinv(piv));
+            v[j] = l;
+#pragma unroll
+            for (int c = j + 1; c < kNB; ++c) {
+                const c32 u = prow[c];
+                v[c].re = fmaf(-l.re, u.re, fmaf(l.im, u.im, v[c].re));
+                v[c].im = fmaf(-l.re, u.im, fmaf(-l.im, u.re, v[c].im));
+            }
+        }
+    }
+    return rank;
+}
+
+// Panels with more than 1024 rows: tournament pivoting (communication-
+// avoiding LU).  Step 1: every 1024-row chunk runs the elimination on its
+// rows and nominates its kNB pivot rows.  cand[b][g * kNB + j].
+__global__ __launch_bounds__(1024) void k_lu_panel_nominate(const float *A, int64_t lda,
+                                                            int64_t a_stride, int N, int k,
+                                                            int32_t *cand) {
+    const int g = blockIdx.x, b = blockIdx.y, G = gridDim.x;
+    const float *Ab = A + (size_t)b * a_stride * 2;
+    const int r = k + g * 1024 + (int)threadIdx.x;
+    const bool own = r < N;
+    c32 v[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) v[c] = own ? ld(Ab, (size_t)(k + c) * lda + r) : c32{0.f, 0.f};
+    __shared__ unsigned long long best_s[2];
+    __shared__ c32 prow[kNB];
+    __shared__ int win_s[kNB];
+    __shared__ float mag_s[kNB];
+    (void)eliminate_rows(v, own, (unsigned)r, best_s, prow, win_s, mag_s);
+    if (threadIdx.x < kNB) cand[(size_t)b * G * kNB + g * kNB + threadIdx.x] = win_s[threadIdx.x];
+}
+
+// Step 2, one block per system: the elimination over the G * kNB nominated
+// rows (original values) picks the panel's kNB pivot rows and factors the
+// top block.  Writes ipiv (the row exchanges in order), the gather list, the
+// factored top block top[b][kNB][kNB] (row j = pivot j) and the original
+// panel values of the gathered source rows gsrc[b][2 kNB][kNB].
+__global__ __launch_bounds__(1024) void k_lu_panel_tournament(
+    const float *A, int64_t lda, int64_t a_stride, int N, int k, int G, const int32_t *cand,
+    int32_t *ipiv, int32_t *info, int32_t *gl, int64_t gl_stride, float *top, float *gsrc) {
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const float *Ab = A + (size_t)b * a_stride * 2;
+    const int ncand = G * kNB;
+    const int id = tid < ncand ? cand[(size_t)b * ncand + tid] : -1;
+    const bool own = id >= 0;
+    c32 v[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) v[c] = own ? ld(Ab, (size_t)(k + c) * lda + id) : c32{0.f, 0.f};
+    __shared__ unsigned long long best_s[2];
+    __shared__ c32 prow[kNB];
+    __shared__ int win_s[kNB];
+    __shared__ float mag_s[kNB];
+    __shared__ int piv_s[kNB];
+    __shared__ int posr[2 * kNB], posc[2 * kNB];
+    const int rank = eliminate_rows(v, own, (unsigned)id, best_s, prow, win_s, mag_s);
+    float *topb = top + (size_t)b * kNB * kNB * 2;
+    if (rank >= 0) {
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) st(topb, (size_t)rank * kNB + c, v[c]);
+    }
+    if (tid == 0) {
+        // exchanges in order: the winner of column j moves to row k + j
+        int nt = 0;  // touched positions: posr[i] holds row posc[i]
+        auto content = [&](int p) {
+            for (int i = 0; i < nt; ++i)
+                if (posr[i] == p) return posc[i];
+            return p;
+        };
+        auto set = [&](int p, int row) {
+            for (int i = 0; i < nt; ++i)
+                if (posr[i] == p) {
+                    posc[i] = row;
+                    return;
+                }
+            posr[nt] = p;
+            posc[nt] = row;
+            ++nt;
+        };
+        int bad = 0;
+        for (int j = 0; j < kNB; ++j) {
+            const int w = win_s[j] >= 0 ? win_s[j] : k + j;
+            int p = w;  // current position of row w
+            for (int i = 0; i < nt; ++i)
+                if (posc[i] == w) p = posr[i];
+            const int cj = content(k + j), cp = content(p);
+            set(k + j, cp);
+            set(p, cj);
+            piv_s[j] = p;
+            ipiv[(size_t)b * N + k + j] = p;
+            if (!(mag_s[j] > 0.0f) && bad == 0) bad = k + j + 1;
+        }
+        if (bad != 0 && info[b] == 0) info[b] = bad;
+    }
+    __syncthreads();
+    int32_t *g = gl + (size_t)b * gl_stride;
+    if (wave == 0) build_gather_list(piv_s, k, g);
+    __syncthreads();
+    const int cnt = g[0];
+    float *gs = gsrc + (size_t)b * 2 * kNB * kNB * 2;
+    for (int e = tid; e < cnt * kNB; e += 1024) {
+        const int m = e / kNB, c = e % kNB;
+        st(gs, (size_t)m * kNB + c, ld(Ab, (size_t)(k + c) * lda + g[1 + 2 * kNB + m]));
+    }
+}
+
+// Step 3: the panel rows in their new order: rows k..k+kNB-1 get the
+// factored top block, every row below gets its multipliers
+// l = a U11^{-1} (a row-vector triangular solve), where a is the row's
+// panel values after the exchanges.  One thread per row.
+__global__ __launch_bounds__(256) void k_lu_panel_apply(float *A, int64_t lda, int64_t a_stride,
+                                                        int N, int k, const int32_t *gl,
+                                                        int64_t gl_stride, const float *top,
+                                                        const float *gsrc) {
+    const int b = blockIdx.y;
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const float *topb = top + (size_t)b * kNB * kNB * 2;
+    const float *gs = gsrc + (size_t)b * 2 * kNB * kNB * 2;
+    const int32_t *g = gl + (size_t)b * gl_stride;
+    __shared__ c32 U[kNB][kNB + 1];
+    __shared__ c32 rdiag[kNB];
+    __shared__ int trow[2 * kNB];
+    __shared__ int cnt_s;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kNB * kNB; e += 256) U[e / kNB][e % kNB] = ld(topb, e);
+    if (tid == 0) cnt_s = g[0];
+    __syncthreads();
+    const int cnt = cnt_s;
+    if (tid < cnt) trow[tid] = g[1 + tid];
+    if (tid < kNB) {
+        const c32 d = U[tid][tid];
+        rdiag[tid] = (d.re != 0.0f || d.im != 0.0f) ? cinv(d) : c32{0.0f, 0.0f};
+    }
+    __syncthreads();
+    const int r = k + blockIdx.x * 256 + tid;
+    if (r >= N) return;
+    if (r < k + kNB) {
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) st(Ab, (size_t)(k + c) * lda + r, U[r - k][c]);
+        return;
+    }
+    int m = -1;
+    for (int i = 0; i < cnt; ++i)
+        if (trow[i] == r) m = i;
+    c32 x[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+        x[c] = m >= 0 ? ld(gs, (size_t)m * kNB + c) : ld(Ab, (size_t)(k + c) * lda + r);
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) {
+        c32 acc = x[c];
+#pragma unroll
+        for (int i = 0; i < c; ++i) {
+            const c32 u = U[i][c];
+            acc.re = fmaf(-x[i].re, u.re, fmaf(x[i].im, u.im, acc.re));
+            acc.im = fmaf(-x[i].re, u.im, fmaf(-x[i].im, u.re, acc.im));
+        }
+        x[c] = cmul(acc, rdiag[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) st(Ab, (size_t)(k + c) * lda + r, x[c]);
 }
 
 // Register panel for N - k <= 1024: thread t holds the kNB panel values of
@@ -229,7 +372,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel(float *A, int64_t lda, int64_
 // through LDS and every thread writes its values back to its final logical
 // row.  The pivot is reduced per wave with shuffles and across waves with
 // one LDS atomic max per wave into a double-buffered slot.  Two block
-// barriers per column; same pivot rule as k_lu_panel.
+// barriers per column.
 __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, int64_t a_stride,
                                                        int N, int k, int32_t *ipiv,
                                                        int32_t *info, int32_t *gl,
@@ -625,16 +768,16 @@ __device__ __forceinline__ void block_gemv_h(const float *Ab, int64_t lda, int k
 
 __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, int64_t a_stride,
                                                    int N, const int32_t *ipiv, float *X,
-                                                   int64_t ldx, int trans) {
+                                                   int64_t ldx, int trans, int vpm) {
     extern __shared__ c32 xs[];
     __shared__ c32 D[kNB][kNB + 1];
     __shared__ c32 part[kNB][kNB + 1];
     __shared__ int pv_s[kLuOuter];
-    const int b = blockIdx.x;
+    const int b = blockIdx.x;  // vector; its matrix is b / vpm
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const float *Ab = A + (size_t)b * a_stride * 2;
-    const int32_t *pv = ipiv + (size_t)b * N;
+    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
+    const int32_t *pv = ipiv + (size_t)(b / vpm) * N;
     float *xb = X + (size_t)b * ldx * 2;
     for (int r = tid; r < N; r += 1024) xs[r] = ld(xb, r);
     if (trans == 0) {
@@ -771,6 +914,90 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
 }
 
 // ---------------------------------------------------------------------------
+// A x = b for large orders: the same sweeps as k_lu_solve, one launch per
+// step so that the coupling updates run on many workgroups: exchanges of an
+// outer block (one thread per system), the kNB x kNB diagonal solve (one
+// wave per system), the update of the remaining rows (256 rows per block).
+// ---------------------------------------------------------------------------
+
+__global__ void k_solve_exchange(float *X, int64_t ldx, const int32_t *ipiv, int N, int kb,
+                                 int w, int vpm) {
+    const int b = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    float *xb = X + (size_t)b * ldx * 2;
+    const int32_t *pv = ipiv + (size_t)(b / vpm) * N;
+    for (int j = 0; j < w; ++j) {
+        const int p = pv[kb + j];
+        if (p != kb + j) {
+            const c32 t = ld(xb, kb + j);
+            st(xb, kb + j, ld(xb, p));
+            st(xb, p, t);
+        }
+    }
+}
+
+// upper == 0: unit lower block; 1: upper block with its diagonal.
+__global__ __launch_bounds__(64) void k_solve_diag(const float *A, int64_t lda, int64_t a_stride,
+                                                   float *X, int64_t ldx, int kb, int upper,
+                                                   int vpm) {
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
+    float *xb = X + (size_t)b * ldx * 2;
+    __shared__ c32 D[kNB][kNB + 1];
+    for (int e = lane; e < kNB * kNB; e += 64) {
+        const int i = e % kNB, j = e / kNB;
+        D[i][j] = ld(Ab, (size_t)(kb + j) * lda + kb + i);
+    }
+    __syncthreads();
+    c32 x = lane < kNB ? ld(xb, kb + lane) : c32{0.0f, 0.0f};
+    if (!upper) {
+        for (int j = 0; j < kNB - 1; ++j) {
+            const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+            if (lane > j && lane < kNB) {
+                const c32 l = D[lane][j];
+                x.re -= l.re * xr - l.im * xi;
+                x.im -= l.re * xi + l.im * xr;
+            }
+        }
+    } else {
+        for (int j = kNB - 1; j >= 0; --j) {
+            if (lane == j) x = cmul(x, cinv(D[j][j]));
+            const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+            if (lane < j) {
+                const c32 u = D[lane][j];
+                x.re -= u.re * xr - u.im * xi;
+                x.im -= u.re * xi + u.im * xr;
+            }
+        }
+    }
+    if (lane < kNB) st(xb, kb + lane, x);
+}
+
+// x[r] -= sum_j A[r][kb + j] x[kb + j] for r in [r0, r1).
+__global__ __launch_bounds__(256) void k_solve_update(const float *A, int64_t lda,
+                                                      int64_t a_stride, float *X, int64_t ldx,
+                                                      int kb, int r0, int r1, int vpm) {
+    const int b = blockIdx.y;
+    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
+    float *xb = X + (size_t)b * ldx * 2;
+    __shared__ c32 xk[kNB];
+    if (threadIdx.x < kNB) xk[threadIdx.x] = ld(xb, kb + threadIdx.x);
+    __syncthreads();
+    const int r = r0 + blockIdx.x * 256 + threadIdx.x;
+    if (r >= r1) return;
+    c32 acc = ld(xb, r);
+#pragma unroll 8
+    for (int j = 0; j < kNB; ++j) {
+        const c32 a = ld(Ab, (size_t)(kb + j) * lda + r);
+        const c32 xj = xk[j];
+        acc.re -= a.re * xj.re - a.im * xj.im;
+        acc.im -= a.re * xj.im + a.im * xj.re;
+    }
+    st(xb, r, acc);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 residual r = b - A x over the true n x n block.
 // Block = 256 rows of one system; x staged through LDS in chunks.
 // ---------------------------------------------------------------------------
@@ -778,10 +1005,10 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
 __global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t lda,
                                                       int64_t a_stride, int n, const float *bvec,
                                                       int64_t ldb, const double *x, int64_t ldx,
-                                                      double *r, double *norms) {
+                                                      double *r, double *norms, int vpm) {
     const int b = blockIdx.y;
     const int row = blockIdx.x * 256 + threadIdx.x;
-    const float *Ab = A + (size_t)b * a_stride * 2;
+    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
     const double *xb = x + (size_t)b * ldx * 2;
     __shared__ double xsh[2][256];
     double ar = 0.0, ai = 0.0;
@@ -833,10 +1060,10 @@ __global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t ld
 // y = A x, A the FP32 operator (true n x n block), x and y FP64.
 __global__ __launch_bounds__(256) void k_gemv_f64(const float *A, int64_t lda, int64_t a_stride,
                                                   int n, const double *x, int64_t ldx, double *y,
-                                                  int64_t ldy) {
+                                                  int64_t ldy, int vpm) {
     const int b = blockIdx.y;
     const int row = blockIdx.x * 256 + threadIdx.x;
-    const float *Ab = A + (size_t)b * a_stride * 2;
+    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
     const double *xb = x + (size_t)b * ldx * 2;
     __shared__ double xsh[2][256];
     double ar = 0.0, ai = 0.0;
@@ -1007,6 +1234,7 @@ void launch_pow2_scale(const double *amax, float *scale, int B, hipStream_t stre
 // rank-kLuOuter update of the trailing matrix in the requested precision.
 void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStream_t stream) {
     static_assert(kLuPad % kNB == 0 && kLuOuter % kNB == 0, "whole panels");
+    if (w.N > kLuMaxOrder) throw std::runtime_error("LU order above the supported maximum");
     // int32 zero has the bit pattern of float zero
     launch_fill_zero(reinterpret_cast<float *>(w.info), (size_t)w.B, stream);
     const int N = w.N, B = w.B;
@@ -1039,12 +1267,20 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
     for (int K0 = 0; K0 < N; K0 += kLuOuter) {
         const int K1 = std::min(N, K0 + kLuOuter);
         for (int k = K0; k < K1; k += kNB) {
-            if (N - k <= 1024)
+            if (N - k <= 1024) {
                 hipLaunchKernelGGL(k_lu_panel_reg, dim3(B), dim3(1024), 0, stream, w.A, w.lda,
                                    w.a_stride, N, k, w.ipiv, w.info, gl_of(k), gls);
-            else
-                hipLaunchKernelGGL(k_lu_panel, dim3(B), dim3(1024), 0, stream, w.A, w.lda,
-                                   w.a_stride, N, k, w.ipiv, w.info, gl_of(k), gls);
+            } else {
+                const int G = (N - k + 1023) / 1024;
+                hipLaunchKernelGGL(k_lu_panel_nominate, dim3(G, B), dim3(1024), 0, stream, w.A,
+                                   w.lda, w.a_stride, N, k, w.scratch_i);
+                hipLaunchKernelGGL(k_lu_panel_tournament, dim3(B), dim3(1024), 0, stream, w.A,
+                                   w.lda, w.a_stride, N, k, G, w.scratch_i, w.ipiv, w.info,
+                                   gl_of(k), gls, w.scratch_f, w.scratch_f + (size_t)B * kNB * kNB * 2);
+                hipLaunchKernelGGL(k_lu_panel_apply, dim3((N - k + 255) / 256, B), dim3(256), 0,
+                                   stream, w.A, w.lda, w.a_stride, N, k, gl_of(k), gls,
+                                   w.scratch_f, w.scratch_f + (size_t)B * kNB * kNB * 2);
+            }
             swap_trsm(k, 1, K0, k, 0, k + kNB, K1, 1);
             gemm(0, k + kNB, N, k + kNB, K1, k, k + kNB);
         }
@@ -1059,24 +1295,47 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
     }
 }
 
-void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream) {
-    const size_t shm = sizeof(float) * 2 * (size_t)w.N;
-    hipLaunchKernelGGL(k_lu_solve, dim3(w.B), dim3(1024), shm, stream, w.A, w.lda, w.a_stride,
-                       w.N, w.ipiv, x, ldx, trans);
+void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream,
+              int vpm) {
+    const int N = w.N, V = w.B * vpm;
+    if (trans != 0 || N <= 2048) {
+        const size_t shm = sizeof(float) * 2 * (size_t)N;
+        hipLaunchKernelGGL(k_lu_solve, dim3(V), dim3(1024), shm, stream, w.A, w.lda, w.a_stride,
+                           N, w.ipiv, x, ldx, trans, vpm);
+        return;
+    }
+    for (int kb = 0; kb < N; kb += kNB) {
+        if (kb % kLuOuter == 0)
+            hipLaunchKernelGGL(k_solve_exchange, dim3(V), dim3(64), 0, stream, x, ldx, w.ipiv, N,
+                               kb, std::min(kLuOuter, N - kb), vpm);
+        hipLaunchKernelGGL(k_solve_diag, dim3(V), dim3(64), 0, stream, w.A, w.lda, w.a_stride, x,
+                           ldx, kb, 0, vpm);
+        const int r0 = kb + kNB;
+        if (r0 < N)
+            hipLaunchKernelGGL(k_solve_update, dim3((N - r0 + 255) / 256, V), dim3(256), 0,
+                               stream, w.A, w.lda, w.a_stride, x, ldx, kb, r0, N, vpm);
+    }
+    for (int kb = N - kNB; kb >= 0; kb -= kNB) {
+        hipLaunchKernelGGL(k_solve_diag, dim3(V), dim3(64), 0, stream, w.A, w.lda, w.a_stride, x,
+                           ldx, kb, 1, vpm);
+        if (kb > 0)
+            hipLaunchKernelGGL(k_solve_update, dim3((kb + 255) / 256, V), dim3(256), 0, stream,
+                               w.A, w.lda, w.a_stride, x, ldx, kb, 0, kb, vpm);
+    }
 }
 
 void launch_residual_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,
                          const float *b, int64_t ldb, const double *x, int64_t ldx, double *r,
-                         double *norms, hipStream_t stream) {
+                         double *norms, hipStream_t stream, int vpm) {
     launch_fill_zero_f64(norms, (size_t)2 * B, stream);
     hipLaunchKernelGGL(k_residual_f64, dim3((n + 255) / 256, B), dim3(256), 0, stream, A, lda,
-                       a_stride, n, b, ldb, x, ldx, r, norms);
+                       a_stride, n, b, ldb, x, ldx, r, norms, vpm);
 }
 
 void launch_gemv_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B, const double *x,
-                     int64_t ldx, double *y, int64_t ldy, hipStream_t stream) {
+                     int64_t ldx, double *y, int64_t ldy, hipStream_t stream, int vpm) {
     hipLaunchKernelGGL(k_gemv_f64, dim3((n + 255) / 256, B), dim3(256), 0, stream, A, lda, a_stride,
-                       n, x, ldx, y, ldy);
+                       n, x, ldx, y, ldy, vpm);
 }
 
 void launch_dots_f64(const double *V, int64_t ldv, int64_t vstride, int m, const double *w,

@@ -24,7 +24,7 @@ from numpy.typing import NDArray
 from .backends import AssemblyBackendUnavailable
 from .geometry import SYMMETRY_CODES, GeometryBuffers, validate_native_symmetry_plane
 
-HBEM_ABI_VERSION = 1
+HBEM_ABI_VERSION = 2
 _LIB_NAME = "libhbem.so"
 
 SOLVE_MODES = {"fp32": 0, "fp32_ir": 1, "mixed_fp16": 2, "mixed_bf16": 3}
@@ -71,7 +71,9 @@ class _SolveOptions(C.Structure):
         ("n_tags", C.c_int32),
         ("tags", C.POINTER(C.c_int32)),
         ("impedance_tag", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("robin_beta", C.c_void_p),
+        ("n_sources", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -169,7 +171,7 @@ def load_library() -> C.CDLL:
         lib.hbem_session_get_info.argtypes = [C.c_void_p, C.POINTER(_SessionInfo)]
         lib.hbem_session_get_info.restype = C.c_int32
         fp = C.POINTER(C.c_float)
-        lib.hbem_assemble.argtypes = [C.c_void_p, C.c_float, C.c_float, fp, fp, fp, fp, fp, fp]
+        lib.hbem_assemble.argtypes = [C.c_void_p, C.c_float, C.c_float, fp, fp, fp, fp, fp, fp, fp, fp]
         lib.hbem_assemble.restype = C.c_int32
         lib.hbem_solve_batch.argtypes = [
             C.c_void_p, C.c_int32, fp, fp, fp, fp, fp, C.c_int32, fp,
@@ -239,6 +241,22 @@ def _complex_out(re: NDArray[np.float32], im: NDArray[np.float32]) -> NDArray[np
 
 
 @dataclass
+class ExtraSourceSolveResult:
+    """Outputs of one additional right-hand side of a multi-source case,
+    sharing the case's assembled and factored operator (the reference's
+    ExtraSourceSolveResult, hornlab_metal_bem/metal/session.py:329-351, as
+    in-memory arrays)."""
+
+    source_index: int
+    pressure: NDArray[np.complex64] | None
+    field: NDArray[np.complex64]
+    field_s: float = 0.0
+    field_row_index: int | None = None
+    impedance: complex | None = None
+    surface_pressure_avg: dict[int, complex] | None = None
+
+
+@dataclass
 class DenseSolveFieldResult:
     """Per-frequency solve + field result (the fields of the reference's
     DenseSolveFieldResult, hornlab_metal_bem/metal/session.py:353-376, as
@@ -255,6 +273,7 @@ class DenseSolveFieldResult:
     surface_pressure_avg: dict[int, complex] | None
     diagnostics: dict[str, Any] = field(default_factory=dict)
     field_row_index: int | None = None
+    extra_sources: tuple[ExtraSourceSolveResult, ...] = ()
 
 
 class HipBemSession:
@@ -350,6 +369,24 @@ class HipBemSession:
         }
 
     # -- operations ---------------------------------------------------------
+    def _beta_rows(self, payloads: list[dict[int, complex] | None]) -> NDArray[np.complex64] | None:
+        """Per-triangle Robin admittance rows [len(payloads), T] from {tag:
+        beta} dicts (the reference's impedance payload, native.py:870-902);
+        None when no case has a Robin wall."""
+        if not any(payloads):
+            return None
+        tags = np.asarray(self.geometry_buffers.physical_tags_i32, np.int64)
+        rows = np.zeros((len(payloads), tags.size), np.complex64)
+        for i, d in enumerate(payloads):
+            for tag, beta in (d or {}).items():
+                b = complex(beta)
+                if int(tag) < 0:
+                    raise ValueError("impedance_sources tags must be non-negative")
+                if not (np.isfinite(b.real) and np.isfinite(b.imag)):
+                    raise ValueError("impedance_sources values must be finite")
+                rows[i, tags == int(tag)] = np.complex64(b)
+        return rows
+
     def assemble_standard_neumann(
         self,
         frequency_hz: float,
@@ -357,9 +394,11 @@ class HipBemSession:
         neumann_dp0: NDArray[Any],
         *,
         k_imag: float = 0.0,
+        impedance_sources: dict[int, complex] | None = None,
         operation_id: str | None = None,
     ) -> tuple[NDArray[np.complex64], NDArray[np.complex64]]:
-        """Corrected system at one wavenumber: (A (n, n) row-major, rhs (n,))."""
+        """Corrected system at one wavenumber: (A (n, n) row-major, rhs (n,)).
+        ``impedance_sources`` {tag: beta} adds the Robin fold."""
         self._ensure_open()
         q = np.asarray(neumann_dp0)
         T, n = self.geometry_buffers.n_triangles, self.geometry_buffers.p1_dof_count
@@ -369,6 +408,9 @@ class HipBemSession:
             raise ValueError("neumann_dp0 must contain only finite values")
         qre = np.ascontiguousarray(q.real, dtype=np.float32)
         qim = np.ascontiguousarray(q.imag, dtype=np.float32)
+        beta = self._beta_rows([impedance_sources])
+        bre = np.ascontiguousarray(beta[0].real, np.float32) if beta is not None else None
+        bim = np.ascontiguousarray(beta[0].imag, np.float32) if beta is not None else None
         are = np.empty((n, n), np.float32)
         aim = np.empty((n, n), np.float32)
         rre = np.empty(n, np.float32)
@@ -376,7 +418,7 @@ class HipBemSession:
         _check(
             load_library().hbem_assemble(
                 self._handle, float(np.float32(k_real)), float(np.float32(k_imag)), _f32p(qre),
-                _f32p(qim), _f32p(are), _f32p(aim), _f32p(rre), _f32p(rim),
+                _f32p(qim), _f32p(bre), _f32p(bim), _f32p(are), _f32p(aim), _f32p(rre), _f32p(rim),
             ),
             "assemble_standard_neumann",
         )
@@ -401,11 +443,17 @@ class HipBemSession:
         estimate_rcond: bool = True,
         on_case_result: Callable[[int, DenseSolveFieldResult], bool | None] | None = None,
         dense_solve_dtype: str | None = None,
+        impedance_sources: dict[int, complex] | list[dict[int, complex]] | None = None,
+        extra_neumann_dp0: NDArray[Any] | None = None,
+        extra_impedance_source_tags: list[int | None] | tuple[int | None, ...] | None = None,
         **_ignored: Any,
     ) -> list[DenseSolveFieldResult]:
         """Assemble, solve and evaluate every frequency; returns one result per
         frequency in order.  ``on_case_result(i, result)`` is called in order;
-        returning exactly False stops after that case."""
+        returning exactly False stops after that case.  ``impedance_sources``
+        ({tag: beta}, or one dict per frequency) adds Robin walls;
+        ``extra_neumann_dp0`` (n_extra, F, T) adds right-hand sides that share
+        each case's operator and LU (multi-RHS, native.py:775-783)."""
         self._ensure_open()
         freqs = np.asarray(frequency_hz, dtype=np.float64).reshape(-1)
         k = np.ascontiguousarray(np.asarray(k_real, dtype=np.float32).reshape(-1))
@@ -432,6 +480,34 @@ class HipBemSession:
             raise ValueError("neumann_dp0 must be complex")
         if not np.all(np.isfinite(q)):
             raise ValueError("neumann_dp0 must contain only finite values")
+        n_extra = 0
+        if extra_neumann_dp0 is not None:
+            xq = np.asarray(extra_neumann_dp0)
+            if xq.ndim != 3 or xq.shape[1:] != (F, T):
+                raise ValueError(f"extra_neumann_dp0 must have shape (n_extra, {F}, {T}), got {xq.shape}")
+            if xq.shape[0] == 0:
+                raise ValueError("extra_neumann_dp0 must be None or non-empty")
+            if not np.iscomplexobj(xq):
+                raise ValueError("extra_neumann_dp0 must be complex")
+            if not np.all(np.isfinite(xq)):
+                raise ValueError("extra_neumann_dp0 must contain only finite values")
+            n_extra = int(xq.shape[0])
+            q = np.concatenate([q[:, None, :], np.transpose(xq, (1, 0, 2))], axis=1)  # (F, S, T)
+        else:
+            q = q[:, None, :]
+        if extra_impedance_source_tags is not None:
+            if n_extra == 0:
+                raise ValueError("extra_impedance_source_tags requires extra_neumann_dp0")
+            if len(extra_impedance_source_tags) != n_extra:
+                raise ValueError("extra_impedance_source_tags must have one entry per extra source")
+        S = 1 + n_extra
+        if isinstance(impedance_sources, list):
+            if len(impedance_sources) != F:
+                raise ValueError(f"impedance_sources list must have one dict per frequency ({F})")
+            payloads = list(impedance_sources)
+        else:
+            payloads = [impedance_sources] * F
+        beta = self._beta_rows(payloads)
         obs = np.asarray(observation_points, dtype=np.float32)
         if obs.ndim != 2 or 3 not in obs.shape:
             raise ValueError("observation_points must have shape (3, n_obs) or (n_obs, 3)")
@@ -443,10 +519,13 @@ class HipBemSession:
             solve_mode = "fp32_ir"
         if solve_mode not in SOLVE_MODES:
             raise ValueError(f"solve_mode must be one of {sorted(SOLVE_MODES)}")
-        tags = np.ascontiguousarray(list(source_tags or []), dtype=np.int32)
+        user_tags = [int(t) for t in (source_tags or [])]
         imp_tag = int(impedance_source_tag) if impedance_source_tag is not None else (
-            int(min(tags)) if tags.size else 2
+            int(min(user_tags)) if user_tags else 2
         )
+        extra_tags = [int(t) for t in (extra_impedance_source_tags or []) if t is not None]
+        all_tags = list(dict.fromkeys(user_tags + extra_tags))
+        tags = np.ascontiguousarray(all_tags, dtype=np.int32)
         opts = _SolveOptions()
         opts.solve_mode = SOLVE_MODES[solve_mode]
         opts.refine_max_iterations = int(refine_max_iterations)
@@ -456,14 +535,19 @@ class HipBemSession:
         opts.n_tags = int(tags.size)
         opts.tags = _i32p(tags) if tags.size else None
         opts.impedance_tag = imp_tag
+        opts.n_sources = S
+        beta_f = None
+        if beta is not None:
+            beta_f = np.ascontiguousarray(beta.astype(np.complex64)).view(np.float32)
+            opts.robin_beta = beta_f.ctypes.data
         qre = np.ascontiguousarray(q.real, dtype=np.float32)
         qim = np.ascontiguousarray(q.imag, dtype=np.float32)
-        fre = np.zeros((F, n_obs), np.float32)
-        fim = np.zeros((F, n_obs), np.float32)
-        pre = np.zeros((F, n), np.float32)
-        pim = np.zeros((F, n), np.float32)
-        imp = np.zeros((F, 2), np.float64)
-        pav = np.zeros((F, max(1, tags.size), 2), np.float64)
+        fre = np.zeros((F, S, n_obs), np.float32)
+        fim = np.zeros((F, S, n_obs), np.float32)
+        pre = np.zeros((F, S, n), np.float32)
+        pim = np.zeros((F, S, n), np.float32)
+        imp = np.zeros((F, S, 2), np.float64)
+        pav = np.zeros((F, S, max(1, tags.size), 2), np.float64)
         diag = (_CaseDiag * F)()
         _check(
             load_library().hbem_solve_batch(
@@ -489,6 +573,9 @@ class HipBemSession:
                 "symmetry_plane": self.symmetry_plane,
                 "pressure_shape": [n],
                 "field_shape": [n_obs],
+                "robin_boundary": bool(payloads[i]),
+                "multi_source": n_extra > 0,
+                "n_sources": S,
             }
             diagnostics["regular_assembly_seconds"] = float(d.regular_assembly_s)
             diagnostics["singular_assembly_seconds"] = float(d.singular_assembly_s)
@@ -497,22 +584,45 @@ class HipBemSession:
                 diagnostics["dense_solve_rcond"] = float(d.dense_solve_rcond)
             if d.refine_residual_rel >= 0:
                 diagnostics["dense_solve_refine_residual_rel"] = float(d.refine_residual_rel)
+            def avg(src: int) -> dict[int, complex]:
+                return {int(t): complex(pav[i, src, j, 0], pav[i, src, j, 1]) for j, t in enumerate(tags)}
+
+            extras = []
+            for e in range(n_extra):
+                etag = None
+                if extra_impedance_source_tags is not None:
+                    etag = extra_impedance_source_tags[e]
+                e_imp = (
+                    complex(imp[i, 1 + e, 0], imp[i, 1 + e, 1])
+                    if etag is None
+                    else avg(1 + e).get(int(etag))
+                )
+                extras.append(
+                    ExtraSourceSolveResult(
+                        source_index=e,
+                        pressure=_complex_out(pre[i, 1 + e], pim[i, 1 + e]) if write_surface_pressure else None,
+                        field=_complex_out(fre[i, 1 + e], fim[i, 1 + e]),
+                        field_s=float(d.field_s) / S,
+                        field_row_index=i,
+                        impedance=e_imp,
+                        surface_pressure_avg={t: v for t, v in avg(1 + e).items() if t in user_tags} or None,
+                    )
+                )
             res = DenseSolveFieldResult(
                 frequency_hz=float(freqs[i]),
-                pressure=_complex_out(pre[i], pim[i]) if write_surface_pressure else None,
-                field=_complex_out(fre[i], fim[i]),
+                pressure=_complex_out(pre[i, 0], pim[i, 0]) if write_surface_pressure else None,
+                field=_complex_out(fre[i, 0], fim[i, 0]),
                 assembly_s=float(d.assembly_s),
                 dense_solve_s=float(d.dense_solve_s),
                 field_s=float(d.field_s),
                 lapack_info=int(d.lapack_info),
-                impedance=complex(imp[i, 0], imp[i, 1]),
+                impedance=complex(imp[i, 0, 0], imp[i, 0, 1]),
                 surface_pressure_avg=(
-                    {int(t): complex(pav[i, j, 0], pav[i, j, 1]) for j, t in enumerate(tags)}
-                    if tags.size
-                    else None
+                    {t: v for t, v in avg(0).items() if t in user_tags} if user_tags else None
                 ),
                 diagnostics=diagnostics,
                 field_row_index=i,
+                extra_sources=tuple(extras),
             )
             results.append(res)
             if on_case_result is not None and on_case_result(i, res) is False:
@@ -539,10 +649,13 @@ class HipBemSession:
         refine_tolerance: float = 1e-7,
         batch_frequencies: int | None = None,
         estimate_rcond: bool = False,
+        robin_beta_ptr: int | None = None,
+        n_sources: int = 1,
     ) -> list[dict[str, Any]]:
         """Device-resident batch (hbem_solve_batch_device): every pointer is
         a device address on the session's GPU (e.g. ``tensor.data_ptr()``).
-        Returns the per-case diagnostics."""
+        q is [F][n_sources][T]; robin_beta_ptr an interleaved complex [F][T]
+        admittance or None.  Returns the per-case diagnostics."""
         self._ensure_open()
         tags = np.ascontiguousarray(list(source_tags), dtype=np.int32)
         opts = _SolveOptions()
@@ -554,6 +667,8 @@ class HipBemSession:
         opts.n_tags = int(tags.size)
         opts.tags = _i32p(tags) if tags.size else None
         opts.impedance_tag = int(impedance_source_tag)
+        opts.robin_beta = robin_beta_ptr
+        opts.n_sources = int(n_sources)
         diag = (_CaseDiag * int(n_freq))()
         _check(
             load_library().hbem_solve_batch_device(

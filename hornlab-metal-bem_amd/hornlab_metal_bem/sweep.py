@@ -44,8 +44,6 @@ def should_route_native_metal(config: SolveConfig) -> bool:
         raise AssemblyBackendUnavailable("chief_points (CHIEF least squares) is not built on the HIP path")
     if config.aperture_tag is not None:
         raise AssemblyBackendUnavailable("aperture_tag (coupled infinite baffle) is not built on the HIP path")
-    if config.impedance_sources or config.impedance_source_callback is not None:
-        raise AssemblyBackendUnavailable("impedance_sources (Robin boundary) is not built on the HIP path")
     return True
 
 
@@ -63,14 +61,63 @@ def _k_values_for_native(frequencies, config: SolveConfig):
     return k_real, k_imag
 
 
-def _build_neumann_rows(dp0_space, physical_tags, frequencies, config, source_face_scale=None):
-    rows = [
-        _build_driver_neumann_coeffs(
-            dp0_space, physical_tags, 2.0 * np.pi * float(f), config, np.complex64,
-            impedance_tags=set(), source_face_scale=source_face_scale,
+def _active_impedance_sources(physical_tags, config: SolveConfig) -> dict[int, complex]:
+    """Static Robin walls present in the mesh (reference: sweep.py:164-182)."""
+    active: dict[int, complex] = {}
+    if not config.impedance_sources:
+        return active
+    mesh_tags = {int(t) for t in np.unique(physical_tags)}
+    for tag, beta in config.impedance_sources.items():
+        if int(tag) not in mesh_tags:
+            logger.warning("impedance_sources references tag %d but mesh has no elements with that tag; skipping", int(tag))
+            continue
+        active[int(tag)] = complex(beta)
+    return active
+
+
+def _impedance_sources_for_frequencies(physical_tags, frequencies, config: SolveConfig):
+    """Per-frequency {tag: beta} dicts with the callback, else one static
+    dict; the callback overrides and extends the static walls, passivity
+    Re(beta) >= 0 is enforced (reference: sweep.py:185-235)."""
+    static = _active_impedance_sources(physical_tags, config)
+    if config.impedance_source_callback is None:
+        return static
+    mesh_tags = {int(t) for t in np.unique(physical_tags)}
+    per_case = []
+    for freq in frequencies:
+        f = float(freq)
+        merged = dict(static)
+        for tag, beta in config.impedance_source_callback(f).items():
+            t = int(tag)
+            if t not in mesh_tags:
+                logger.warning("impedance_source_callback returned tag %d not in mesh; skipping", t)
+                continue
+            b = complex(beta)
+            if not (math.isfinite(b.real) and math.isfinite(b.imag)):
+                raise ValueError(f"impedance_source_callback({f:.3f}) returned non-finite beta for tag {t}")
+            if b.real < 0.0:
+                raise ValueError(
+                    f"impedance_source_callback({f:.3f}) returned Re(beta)={b.real:.4g} < 0 for tag {t}; "
+                    "admittance must be passive (Re(beta) >= 0)"
+                )
+            merged[t] = b
+        per_case.append(merged)
+    return per_case
+
+
+def _build_neumann_rows(dp0_space, physical_tags, frequencies, config, source_face_scale=None,
+                        impedance=None):
+    """q rows [F][T]; Robin tags carry no prescribed velocity (reference:
+    sweep.py:238-285)."""
+    rows = []
+    for i, f in enumerate(frequencies):
+        payload = impedance[i] if isinstance(impedance, list) else (impedance or {})
+        rows.append(
+            _build_driver_neumann_coeffs(
+                dp0_space, physical_tags, 2.0 * np.pi * float(f), config, np.complex64,
+                impedance_tags={int(t) for t in payload}, source_face_scale=source_face_scale,
+            )
         )
-        for f in frequencies
-    ]
     return np.stack(rows, axis=0)
 
 
@@ -154,10 +201,16 @@ def _prepare(mesh: LoadedMesh, frame: ObservationFrame, config: SolveConfig):
     return obs_points, angles, p1, dp0, buffers, face_scale
 
 
-def run_sweep_native_metal(
-    mesh: LoadedMesh, frequencies: NDArray[np.float64], frame: ObservationFrame, config: SolveConfig
-) -> SolveResult:
-    """Standard-Neumann sweep on the GPU (reference: sweep.py:612-926)."""
+def _native_sweep(
+    mesh: LoadedMesh, frequencies, frame: ObservationFrame, config: SolveConfig,
+    sources: list[dict] | None,
+) -> list[SolveResult]:
+    """Standard-Neumann sweep on the GPU (reference: sweep.py:612-926, and
+    the multi-RHS sweep sweep.py:957-1301 when ``sources`` is given: one
+    assembled and factored operator per frequency, one right-hand side per
+    source dict).  Returns one SolveResult per source."""
+    from dataclasses import replace
+
     from .native import HipBemSession
 
     should_route_native_metal(config)
@@ -166,8 +219,14 @@ def run_sweep_native_metal(
         raise ValueError("frequencies must contain at least one value")
     t_total = time.time()
     obs_points, angles, p1, dp0, buffers, face_scale = _prepare(mesh, frame, config)
-    source_tags = list(config.velocity_sources.keys())
-    impedance_tag = min(config.velocity_sources.keys(), default=2)
+    src_dicts = [dict(config.velocity_sources)] if sources is None else [dict(s) for s in sources]
+    S = len(src_dicts)
+    multi = sources is not None
+    # reductions: each source's own tags, or the union of every source's tags
+    # in every result (multi-source, reference sweep.py:972-975)
+    union = sorted({int(t) for d in src_dicts for t in d})
+    source_tags = union if multi else list(config.velocity_sources.keys())
+    imp_tags = [min((int(t) for t in d), default=2) for d in src_dicts]
     on_axis = int(np.argmin(np.abs(angles)))
     P, N, _ = obs_points.shape
     pts = _field_points_3xn(obs_points)
@@ -178,10 +237,16 @@ def run_sweep_native_metal(
         n_sphere = int(sph.shape[1])
     max_edge = _mesh_max_edge_m(mesh)
     k_real, k_imag = _k_values_for_native(freqs, config)
-    q_rows = _build_neumann_rows(dp0, mesh.physical_tags, freqs, config, face_scale)
-
-    pressure_rows, spl_rows, imp_rows, surf_rows, diag_rows, log = [], [], [], [], [], []
-    pavg: dict[int, list[complex]] = {t: [] for t in source_tags}
+    impedance = _impedance_sources_for_frequencies(mesh.physical_tags, freqs, config)
+    q_rows = [
+        _build_neumann_rows(dp0, mesh.physical_tags, freqs, replace(config, velocity_sources=d),
+                            face_scale, impedance)
+        for d in src_dicts
+    ]
+    per_src = [
+        dict(pressure=[], spl=[], imp=[], surf=[], diag=[], log=[], pavg={t: [] for t in source_tags})
+        for _ in range(S)
+    ]
     done: list[float] = []
     stream = config.on_frequency_result is not None
     chunk = (config.hip_batch_frequencies or 8) if stream else freqs.size
@@ -192,56 +257,66 @@ def run_sweep_native_metal(
     ) as session:
         for c0 in range(0, freqs.size, chunk):
             sl = slice(c0, min(freqs.size, c0 + chunk))
+            imp_sl = impedance[sl] if isinstance(impedance, list) else impedance
             results = session.assemble_solve_evaluate_standard_neumann_batch(
-                freqs[sl], k_real[sl], q_rows[sl], pts,
+                freqs[sl], k_real[sl], q_rows[0][sl], pts,
                 k_imag_f32=k_imag[sl], field_k_real=k_real[sl], source_tags=source_tags,
-                impedance_source_tag=impedance_tag,
+                impedance_source_tag=imp_tags[0],
                 write_surface_pressure=config.return_surface_pressure,
                 solve_mode=config.effective_hip_solve_mode,
                 refine_max_iterations=config.hip_refine_max_iterations,
                 refine_tolerance=config.hip_refine_tolerance,
                 batch_frequencies=config.hip_batch_frequencies,
                 estimate_rcond=config.hip_estimate_rcond,
+                impedance_sources=imp_sl if imp_sl else None,
+                extra_neumann_dp0=np.stack([q[sl] for q in q_rows[1:]], axis=0) if S > 1 else None,
+                extra_impedance_source_tags=imp_tags[1:] if S > 1 else None,
             )
             for j, r in enumerate(results):
                 i = c0 + j
                 f = float(freqs[i])
-                flat = np.asarray(r.field, np.complex128)
-                pressure = flat[: P * N].reshape(P, N)
-                sphere = flat[P * N : P * N + n_sphere] if n_sphere else None
-                directivity = _directivity_from_pressure(pressure, on_axis)
                 diag = dict(r.diagnostics)
                 _apply_dense_solve_policy(diag, threshold=config.dense_solve_rcond_warning_threshold)
                 _apply_mesh_resolution_policy(
                     diag, frequency_hz=f, mesh_max_edge_m=max_edge,
                     elements_per_wavelength_min=config.mesh_elements_per_wavelength_min,
                 )
-                entry = {
-                    "frequency_hz": f,
-                    "iterations": None,
-                    "timing_s": r.assembly_s + r.dense_solve_s + r.field_s,
-                    "backend": "hip_gfx950_resident_assembly_lu_field_batch",
-                    "assembly_s": r.assembly_s,
-                    "dense_solve_s": r.dense_solve_s,
-                    "field_s": r.field_s,
-                    "lapack_info": r.lapack_info,
-                    "impedance": r.impedance,
-                    "native_diagnostics": diag,
-                    "observation_sphere_pressure_complex": sphere,
-                }
-                for t in source_tags:
-                    pavg[t].append((r.surface_pressure_avg or {}).get(int(t), 0j))
+                entries = []
+                for sidx in range(S):
+                    rr = r if sidx == 0 else r.extra_sources[sidx - 1]
+                    flat = np.asarray(rr.field, np.complex128)
+                    pressure = flat[: P * N].reshape(P, N)
+                    sphere = flat[P * N : P * N + n_sphere] if n_sphere else None
+                    directivity = _directivity_from_pressure(pressure, on_axis)
+                    acc = per_src[sidx]
+                    entry = {
+                        "frequency_hz": f,
+                        "iterations": None,
+                        "timing_s": r.assembly_s + r.dense_solve_s + r.field_s,
+                        "backend": "hip_gfx950_resident_assembly_lu_field_batch",
+                        "assembly_s": r.assembly_s,
+                        "dense_solve_s": r.dense_solve_s,
+                        "field_s": rr.field_s,
+                        "lapack_info": r.lapack_info,
+                        "impedance": rr.impedance,
+                        "native_diagnostics": diag,
+                        "observation_sphere_pressure_complex": sphere,
+                    }
+                    for t in source_tags:
+                        acc["pavg"][t].append((rr.surface_pressure_avg or {}).get(int(t), 0j))
+                    acc["pressure"].append(pressure)
+                    acc["spl"].append(directivity)
+                    acc["imp"].append(rr.impedance)
+                    if config.return_surface_pressure:
+                        acc["surf"].append(np.asarray(rr.pressure, np.complex128))
+                    acc["diag"].append(diag)
+                    acc["log"].append(entry)
+                    entries.append((entry, pressure, directivity))
                 done.append(f)
-                pressure_rows.append(pressure)
-                spl_rows.append(directivity)
-                imp_rows.append(r.impedance)
-                if config.return_surface_pressure:
-                    surf_rows.append(np.asarray(r.pressure, np.complex128))
-                diag_rows.append(diag)
-                log.append(entry)
                 if config.progress_callback is not None:
                     config.progress_callback(i, freqs.size, f)
                 if config.on_frequency_result is not None:
+                    entry, pressure, directivity = entries[0]
                     cb = {
                         **entry,
                         "observation_pressure_complex": pressure,
@@ -249,36 +324,55 @@ def run_sweep_native_metal(
                         "observation_angles_deg": angles,
                         "observation_planes": config.observation.planes,
                     }
+                    if multi:
+                        cb["source_results"] = [e for e, _, _ in entries]
                     if config.on_frequency_result(i, f, cb) is False:
                         logger.info("Early stop requested after %.1f Hz", f)
                         stop = True
                         break
             if stop:
                 break
-    timings = {
-        "solve_s": sum(e["timing_s"] for e in log),
-        "assembly_s": sum(e["assembly_s"] for e in log),
-        "dense_solve_s": sum(e["dense_solve_s"] for e in log),
-        "directivity_s": sum(e["field_s"] for e in log),
-        "total_s": time.time() - t_total,
-    }
-    sp_avg = {t: np.array(v, dtype=np.complex128) for t, v in pavg.items()}
-    return SolveResult(
-        frequencies_hz=np.array(done, dtype=np.float64),
-        pressure_complex=np.stack(pressure_rows, axis=0),
-        directivity_db=np.stack(spl_rows, axis=0),
-        impedance=np.array(imp_rows, dtype=np.complex128),
-        observation_angles_deg=angles,
-        observation_points=obs_points,
-        observation_planes=config.observation.planes,
-        config=config,
-        mesh_info=mesh.info,
-        timings=timings,
-        solver_log=log,
-        surface_pressure_avg=sp_avg if sp_avg else None,
-        surface_pressure_complex=np.stack(surf_rows, axis=0) if config.return_surface_pressure else None,
-        native_diagnostics=diag_rows,
-    )
+    out = []
+    for sidx in range(S):
+        acc = per_src[sidx]
+        log = acc["log"]
+        timings = {
+            "solve_s": sum(e["timing_s"] for e in log),
+            "assembly_s": sum(e["assembly_s"] for e in log),
+            "dense_solve_s": sum(e["dense_solve_s"] for e in log),
+            "directivity_s": sum(e["field_s"] for e in log),
+            "total_s": time.time() - t_total,
+        }
+        sp_avg = {t: np.array(v, dtype=np.complex128) for t, v in acc["pavg"].items()}
+        cfg = replace(config, velocity_sources=src_dicts[sidx]) if multi else config
+        out.append(
+            SolveResult(
+                frequencies_hz=np.array(done, dtype=np.float64),
+                pressure_complex=np.stack(acc["pressure"], axis=0),
+                directivity_db=np.stack(acc["spl"], axis=0),
+                impedance=np.array(acc["imp"], dtype=np.complex128),
+                observation_angles_deg=angles,
+                observation_points=obs_points,
+                observation_planes=config.observation.planes,
+                config=cfg,
+                mesh_info=mesh.info,
+                timings=timings,
+                solver_log=log,
+                surface_pressure_avg=sp_avg if sp_avg else None,
+                surface_pressure_complex=(
+                    np.stack(acc["surf"], axis=0) if config.return_surface_pressure else None
+                ),
+                native_diagnostics=acc["diag"],
+            )
+        )
+    return out
+
+
+def run_sweep_native_metal(
+    mesh: LoadedMesh, frequencies: NDArray[np.float64], frame: ObservationFrame, config: SolveConfig
+) -> SolveResult:
+    """Standard-Neumann sweep on the GPU (reference: sweep.py:612-926)."""
+    return _native_sweep(mesh, frequencies, frame, config, None)[0]
 
 
 run_sweep = run_sweep_native_metal
@@ -287,15 +381,24 @@ run_sweep = run_sweep_native_metal
 def run_sweep_native_metal_multi_source(
     mesh: LoadedMesh, frequencies, frame: ObservationFrame, config: SolveConfig, sources: list[dict]
 ) -> list[SolveResult]:
-    """One SolveResult per velocity-source dict (reference: sweep.py:957-1301).
-    Each source runs through the same resident path; ``surface_pressure_avg``
-    covers the union of every source's tags."""
-    from dataclasses import replace
+    """One SolveResult per velocity-source dict, all sources sharing each
+    frequency's assembled and factored operator (reference: sweep.py:957-1301).
+    ``surface_pressure_avg`` covers the union of every source's tags."""
+    if not sources:
+        raise ValueError("sources must contain at least one velocity dict")
+    if config.velocity_source_callback is not None:
+        raise ValueError("multi-source solves do not support velocity_source_callback")
+    mesh_tags = {int(t) for t in np.unique(mesh.physical_tags)}
+    for index, source in enumerate(sources):
+        missing = sorted(set(int(t) for t in source) - mesh_tags)
+        if missing:
+            raise ValueError(
+                f"sources[{index}] tags {missing} are not present in the mesh; "
+                f"available physical tags: {sorted(mesh_tags)}"
+            )
+    if len(sources) == 1:
+        from dataclasses import replace
 
-    all_tags = sorted({int(t) for src in sources for t in src})
-    out = []
-    for src in sources:
-        vs = {t: src.get(t, 0.0) for t in all_tags}
-        cfg = replace(config, velocity_sources=vs)
-        out.append(run_sweep_native_metal(mesh, frequencies, frame, cfg))
-    return out
+        return [run_sweep_native_metal(mesh, frequencies, frame, replace(config, velocity_sources=dict(sources[0])))]
+    return _native_sweep(mesh, frequencies, frame, config, sources)
+

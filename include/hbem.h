@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HBEM_ABI_VERSION 1
+#define HBEM_ABI_VERSION 2
 
 #define HBEM_OK 0
 #define HBEM_EINVAL -1      /* bad argument / contract violation */
@@ -92,7 +92,17 @@ typedef struct hbem_solve_options {
     int32_t n_tags;                /* tags reduced into surface_pressure_avg */
     const int32_t *tags;           /* [n_tags] */
     int32_t impedance_tag;         /* tag of the impedance reduction */
-    int32_t reserved[8];
+    /* Robin walls (impedance_sources, config.py:174-196): normalised surface
+     * admittance beta = rho c / Z_s per triangle, interleaved complex float
+     * [F][n_triangles] (host memory for hbem_solve_batch, device memory for
+     * hbem_solve_batch_device), or NULL for none.  The matrix gets
+     * -(i k beta / 3) V on the triangle's three columns (main.swift:4101-4110)
+     * and the field uses q + i k beta mean(p) (main.swift:1646-1674). */
+    const float *robin_beta;
+    /* right-hand sides per frequency sharing one matrix and one LU
+     * (solve_multi_source, __init__.py:205-252); 0 or 1: one */
+    int32_t n_sources;
+    int32_t reserved[7];
 } hbem_solve_options;
 
 /* Per-frequency diagnostics: the case fields the reference returns
@@ -136,25 +146,28 @@ int32_t hbem_session_destroy(hbem_session *session);
 int32_t hbem_session_get_info(hbem_session *session, hbem_session_info *info);
 
 /* Assemble the corrected standard-Neumann system at one wavenumber:
- * A = (K - M/2) 2^planes (row-major n x n, split re/im float32) and
- * rhs = 2^planes V q.  Replaces assemble_standard_neumann
+ * A = (K - M/2 - (i k beta / 3) V) 2^planes (row-major n x n, split re/im
+ * float32) and rhs = 2^planes V q.  Replaces assemble_standard_neumann
  * (native.py:388-480; helper op MS:8128-8310 in "corrected" mode).
- * q_re/q_im: DP0 Neumann data [n_triangles]. */
+ * q_re/q_im: DP0 Neumann data [n_triangles]; beta_re/beta_im: Robin
+ * admittance [n_triangles] or NULL. */
 int32_t hbem_assemble(hbem_session *session, float k_real, float k_imag, const float *q_re,
-                      const float *q_im, float *A_re, float *A_im, float *rhs_re, float *rhs_im);
+                      const float *q_im, const float *beta_re, const float *beta_im, float *A_re,
+                      float *A_im, float *rhs_re, float *rhs_im);
 
 /* Assemble, solve and evaluate the exterior field for n_freq frequencies.
  * Replaces assemble_solve_evaluate_standard_neumann_batch
- * (native.py:737-1203; helper MS:8869-9878).
+ * (native.py:737-1203; helper MS:8869-9878).  With S = opts->n_sources
+ * (>= 1) right-hand sides per frequency:
  *   k_real/k_imag [F]: assembly wavenumber (k_imag >= 0 for complex_k);
  *   field_k [F]: real wavenumber of the field kernel;
- *   q_re/q_im [F][n_triangles]: DP0 Neumann rows;
+ *   q_re/q_im [F][S][n_triangles]: DP0 Neumann rows;
  *   obs_3xn [3][n_obs]: observation points;
  * outputs (host):
- *   field_re/im [F][n_obs];
- *   p_re/im [F][p1_dof_count] or NULL (write_surface_pressure);
- *   impedance [F][2] or NULL (area-mean pressure on opts->impedance_tag);
- *   pavg [F][n_tags][2] or NULL;
+ *   field_re/im [F][S][n_obs];
+ *   p_re/im [F][S][p1_dof_count] or NULL (write_surface_pressure);
+ *   impedance [F][S][2] or NULL (area-mean pressure on opts->impedance_tag);
+ *   pavg [F][S][n_tags][2] or NULL;
  *   diag [F] or NULL. */
 int32_t hbem_solve_batch(hbem_session *session, int32_t n_freq, const float *k_real,
                          const float *k_imag, const float *field_k, const float *q_re,
@@ -165,13 +178,14 @@ int32_t hbem_solve_batch(hbem_session *session, int32_t n_freq, const float *k_r
 
 /* hbem_solve_batch with every array in device memory of the session's GPU
  * (for callers that keep data resident, e.g. a PyTorch wrapper).  No
- * counterpart in the reference, whose helper only reads files.
+ * counterpart in the reference, whose helper only reads files.  With
+ * S = opts->n_sources (>= 1):
  *   k_real, field_k [F], k_imag [F] or NULL (real k);
- *   q [F][n_triangles] interleaved complex float;
+ *   q [F][S][n_triangles] interleaved complex float;
  *   obs_3xn [3][n_obs];
- *   field [F][n_obs] interleaved complex float (overwritten);
- *   p [F][p1_dof_count] interleaved complex float or NULL;
- *   tag_avg [F][1 + n_tags][2] double: impedance tag, then opts->tags;
+ *   field [F][S][n_obs] interleaved complex float (overwritten);
+ *   p [F][S][p1_dof_count] interleaved complex float or NULL;
+ *   tag_avg [F][S][1 + n_tags][2] double: impedance tag, then opts->tags;
  *   diag [F] host array or NULL.
  * Returns after the results are complete on the device. */
 int32_t hbem_solve_batch_device(hbem_session *session, int32_t n_freq, const float *k_real,

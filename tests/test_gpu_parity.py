@@ -258,3 +258,114 @@ def test_gmres_refinement_converges_beyond_classical_range(require_gpu, mode):
     x, info = dense_solve(A, b, solve_mode=mode, refine_max_iterations=20, refine_tolerance=1e-13)
     ref = np.linalg.solve(A[0].astype(np.complex128), b[0].astype(np.complex128))
     assert rel_l2(x[0], ref) < 1e-7, (mode, rel_l2(x[0], ref), info[0])
+
+
+def test_dense_solve_tournament_panels_and_stepped_solve(require_gpu):
+    """n = 2200: panels with more than 1024 rows use tournament pivoting and
+    the solve runs step by step on many workgroups; against numpy, and the
+    residual of the FP32 solution is at the FP32 backward-error level."""
+    from hornlab_metal_bem.native import dense_solve
+
+    rng = np.random.default_rng(17)
+    n = 2200
+    A = ((rng.standard_normal((1, n, n)) + 1j * rng.standard_normal((1, n, n))) / np.sqrt(n)).astype(np.complex64)
+    A += (1.5 * np.eye(n, dtype=np.complex64))[None]
+    b = (rng.standard_normal((1, n)) + 1j * rng.standard_normal((1, n))).astype(np.complex64)
+    x, info = dense_solve(A, b)
+    A64 = A[0].astype(np.complex128)
+    ref = np.linalg.solve(A64, b[0].astype(np.complex128))
+    assert info[0]["lapack_info"] == 0
+    assert rel_l2(x[0], ref) < 2e-5, rel_l2(x[0], ref)
+    res = np.linalg.norm(A64 @ x[0] - b[0]) / (np.linalg.norm(A64, 2) * np.linalg.norm(x[0]))
+    assert res < 1e-6, res
+    x2, info2 = dense_solve(A, b, solve_mode="mixed_fp16", refine_tolerance=1e-12, refine_max_iterations=12)
+    assert rel_l2(x2[0], ref) < 1e-8, rel_l2(x2[0], ref)
+
+
+def test_robin_assembly_matches_oracle(require_gpu):
+    """Robin walls on the horn (tag 1, including images and touching
+    pairs): -(i k beta / 3) V folded into the matrix as the oracle does."""
+    b = horn_buffers(edge_length=0.03, quarter=True)
+    f = 3000.0
+    k = float(np.float32(2 * math.pi * f / C0))
+    tags = b.physical_tags_i32
+    beta = 0.35 + 0.2j
+    q = np.where(tags == 2, 1j * 1.2041 * 2 * math.pi * f, 0.0).astype(np.complex128)
+    with _session(b, "yz+xz") as s:
+        A, rhs = s.assemble_standard_neumann(f, k, q.astype(np.complex64), impedance_sources={1: beta})
+        A0, _ = s.assemble_standard_neumann(f, k, q.astype(np.complex64))
+    g = O.Geometry.from_buffers(b, "yz+xz")
+    bt = np.where(tags == 1, beta, 0.0)
+    Ao, ro, _ = O.assemble(g, k, q, beta=bt)
+    assert rel_l2(A, Ao) < 1e-5, rel_l2(A, Ao)
+    assert rel_l2(rhs, ro) < 1e-5, rel_l2(rhs, ro)
+    assert rel_l2(A0, Ao) > 1e-2  # the fold is in use
+
+
+def test_robin_sphere_solve_matches_closed_form_and_oracle(require_gpu):
+    """Whole sphere Robin with uniform q: p(a) = a q / (ika - 1 - ik beta a);
+    the field uses q + i k beta mean(p)."""
+    b = sphere_buffers(0.1, 3)
+    a, rho, U = 0.1, 1.21, 1.0
+    beta = 0.4 + 0.3j
+    freqs = np.array([300.0, 600.0, 1200.0])
+    k = (2 * np.pi * freqs / C0).astype(np.float32)
+    q = np.stack([np.full(b.n_triangles, 1j * rho * 2 * np.pi * f * U) for f in freqs])
+    ang = np.deg2rad(np.linspace(0, 180, 19))
+    obs = np.stack([2.0 * np.sin(ang), np.zeros_like(ang), 2.0 * np.cos(ang)])
+    with _session(b) as s:
+        res = s.assemble_solve_evaluate_standard_neumann_batch(
+            freqs, k, q.astype(np.complex64), obs.astype(np.float32), source_tags=[2],
+            impedance_source_tag=2, impedance_sources={2: beta},
+        )
+    g = O.Geometry.from_buffers(b)
+    l2g = np.asarray(b.p1_local2global_i32).reshape(-1, 3)
+    for i, f in enumerate(freqs):
+        kk = float(k[i])
+        bt = np.full(b.n_triangles, beta)
+        A, rhs, _ = O.assemble(g, kk, q[i], beta=bt)
+        p = O.solve(A, rhs)
+        fo = O.field(g, kk, p, q[i] + 1j * kk * bt * p[l2g].mean(axis=1), obs)
+        r = res[i]
+        assert r.diagnostics["robin_boundary"] is True
+        assert rel_l2(r.pressure, p) < 1e-4, (f, rel_l2(r.pressure, p))
+        assert rel_l2(r.field, fo) < 1e-4, (f, rel_l2(r.field, fo))
+        pa = a * q[i][0] / (1j * kk * a - 1 - 1j * kk * beta * a)
+        exact = pa * (a / 2.0) * np.exp(1j * kk * (2.0 - a))
+        assert np.max(np.abs(r.field - exact)) / abs(exact) < 0.03, f
+
+
+def test_multi_source_shares_operator(require_gpu):
+    """Three right-hand sides on one LU per frequency equal three separate
+    solves (solve_multi_source)."""
+    b = horn_buffers(edge_length=0.03, quarter=True)
+    tags = b.physical_tags_i32
+    freqs = np.array([800.0, 2500.0, 6000.0])
+    k = (2 * np.pi * freqs / C0).astype(np.float32)
+    rng = np.random.default_rng(4)
+    qs = []
+    for s_ in range(3):
+        w = rng.standard_normal(b.n_triangles) + 1j * rng.standard_normal(b.n_triangles)
+        qs.append(np.stack([np.where(tags == (2 if s_ == 0 else 1), w * f, 0.0) for f in freqs]).astype(np.complex64))
+    ang = np.deg2rad(np.linspace(0, 180, 13))
+    obs = np.stack([2.0 * np.sin(ang), np.zeros_like(ang), 2.0 * np.cos(ang)]).astype(np.float32)
+    with _session(b, "yz+xz") as s:
+        multi = s.assemble_solve_evaluate_standard_neumann_batch(
+            freqs, k, qs[0], obs, source_tags=[1, 2], impedance_source_tag=2,
+            extra_neumann_dp0=np.stack(qs[1:]), extra_impedance_source_tags=[1, None],
+        )
+        single = [
+            s.assemble_solve_evaluate_standard_neumann_batch(
+                freqs, k, q, obs, source_tags=[1, 2], impedance_source_tag=2)
+            for q in qs
+        ]
+    for i in range(freqs.size):
+        assert len(multi[i].extra_sources) == 2
+        assert rel_l2(multi[i].pressure, single[0][i].pressure) < 1e-6
+        assert rel_l2(multi[i].field, single[0][i].field) < 1e-6
+        for e in range(2):
+            x = multi[i].extra_sources[e]
+            assert rel_l2(x.pressure, single[e + 1][i].pressure) < 1e-6, (i, e)
+            assert rel_l2(x.field, single[e + 1][i].field) < 1e-6, (i, e)
+        assert abs(multi[i].extra_sources[0].impedance
+                   - single[1][i].surface_pressure_avg[1]) <= 1e-5 * abs(single[1][i].surface_pressure_avg[1])

@@ -228,7 +228,11 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     assert sorted(EXPORTED_SYMBOLS) == declared
-    assert lib.hbem_abi_version() == 1
+    from hornlab_metal_bem.native import HBEM_ABI_VERSION
+
+    header = (ROOT / "include" / "hbem.h").read_text()
+    assert f"#define HBEM_ABI_VERSION {HBEM_ABI_VERSION}" in header
+    assert lib.hbem_abi_version() == HBEM_ABI_VERSION
 
 
 def test_no_gpu_fails_loudly():

@@ -345,3 +345,38 @@ def test_singular_mirror_classes_are_invariant():
             cross[kind] = max(cross[kind], e)
     assert max(same.values()) < 1e-10, same
     assert cross[1] > 1e-8 and cross[2] > 1e-8, cross
+
+
+def test_robin_sphere_matches_closed_form():
+    """Uniform Neumann data q with a Robin wall on the whole sphere:
+    dp/dn = q + i k beta p (neumannWithRobin, main.swift:1646-1674).  The
+    monopole relation p(a) = a / (i k a - 1) dp/dn gives
+    p(a) = a q / (i k a - 1 - i k beta a), and p(r) = p(a) (a / r) e^{ik(r-a)}.
+    The oracle's Robin fold -(i k beta / 3) V and the field's
+    q + i k beta mean(p) must reproduce it (beta = 0 is the monopole test)."""
+    a, rho, c, U = 0.1, 1.21, 343.0, 1.0
+    beta = 0.4 + 0.3j
+    ang = np.deg2rad(np.linspace(0, 180, 7))
+    obs = np.stack([2 * np.sin(ang), 0 * ang, 2 * np.cos(ang)]).astype(np.float32)
+    errs = []
+    for sub in (2, 3):
+        b = sphere_buffers(a, sub)
+        g = O.Geometry.from_buffers(b)
+        f = 600.0
+        k = 2 * np.pi * f / c
+        q = np.full(b.n_triangles, 1j * rho * 2 * np.pi * f * U)
+        bt = np.full(b.n_triangles, beta)
+        A, rhs, _ = O.assemble(g, k, q, beta=bt)
+        p = O.solve(A, rhs)
+        l2g = np.asarray(b.p1_local2global_i32).reshape(-1, 3)
+        q_tot = q + 1j * k * bt * p[l2g].mean(axis=1)
+        fo = O.field(g, k, p, q_tot, obs)
+        pa = a * q[0] / (1j * k * a - 1 - 1j * k * beta * a)
+        exact = pa * (a / 2.0) * np.exp(1j * k * (2.0 - a))
+        errs.append(np.max(np.abs(fo - exact)) / abs(exact))
+        avg, _ = O.surface_average(g, p, 2)
+        assert abs(avg - pa) / abs(pa) < 0.05
+        # the wall changes the answer: beta = 0 differs by far more than the error
+        pa0 = a * q[0] / (1j * k * a - 1)
+        assert abs(pa - pa0) / abs(pa0) > 0.2
+    assert errs[1] < 0.02 and errs[1] < errs[0]
