@@ -135,7 +135,7 @@ struct hbem_session {
     hbem::Plan plan;
     hbem::DeviceGeometry dg{};
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
-        tile_count, tile_ndofs, tile_dofs, tile_pairs;
+        tile_count, tile_ndofs, tile_dofs, tile_pairs, nc_f, nc_i, nc_rule;
     // per-call workspaces
     DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, lu_scr_i, lu_scr_f, amax,
         lu_scale, kv, kh, kn, kf,
@@ -144,12 +144,13 @@ struct hbem_session {
     // staging of host-input batches
     DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag, io_beta, qt, l2g;
     double setup_seconds = 0.0;
+    int64_t near_pairs = 0;
     std::string device_name;
     int cu_count = 0;
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag, &io_beta, &qt, &l2g})
@@ -187,7 +188,7 @@ bool any_nonzero(const float *v, size_t n) {
 // rhs vector b * S + src), beta [B][T] Robin admittance or null.
 void assemble_device(hbem_session *s, int B, int S, const float *kre_d, const float *kim_d,
                      const float *q_d, const float *beta_d, int64_t lda, int64_t a_stride,
-                     int64_t rhs_stride, Timer *t_reg, Timer *t_sing) {
+                     int64_t rhs_stride, Timer *t_reg, Timer *t_sing, Timer *t_near = nullptr) {
     const size_t a_floats = (size_t)B * a_stride * 2;
     s->A.reserve(a_floats * sizeof(float));
     s->rhs.reserve((size_t)B * S * rhs_stride * 2 * sizeof(float));
@@ -201,6 +202,10 @@ void assemble_device(hbem_session *s, int B, int S, const float *kre_d, const fl
     hbem::launch_assemble_singular(s->dg, B, kre_d, kim_d, q_d, S, beta_d, s->A.as<float>(), lda,
                                    a_stride, s->rhs.as<float>(), rhs_stride, s->stream);
     if (t_sing) t_sing->stop(s->stream);
+    if (t_near) t_near->start(s->stream);
+    hbem::launch_assemble_near(s->dg, B, kre_d, kim_d, q_d, S, beta_d, s->A.as<float>(), lda,
+                               a_stride, s->rhs.as<float>(), rhs_stride, s->stream);
+    if (t_near) t_near->stop(s->stream);
     hip_check(hipGetLastError(), "assembly launch");
 }
 
@@ -467,6 +472,43 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         hip_check(hipStreamSynchronize(s->stream), "session upload");
         s->setup_seconds = now_s() - t0;
         *out = s.release();
+    });
+}
+
+int32_t hbem_session_set_near_field(hbem_session *s, int32_t level, double threshold,
+                                    int64_t *pair_count) {
+    return guarded([&] {
+        if (!s) fail(HBEM_EINVAL, "hbem_session_set_near_field: null session");
+        if (level < 0 || level > 2)
+            fail(HBEM_EINVAL, "hbem_session_set_near_field: level must be 0, 1 or 2");
+        if (level > 0 && !(std::isfinite(threshold) && threshold > 0.0))
+            fail(HBEM_EINVAL, "hbem_session_set_near_field: threshold must be finite and positive");
+        std::lock_guard<std::mutex> lock(s->mu);
+        set_device(s);
+        hbem::DeviceGeometry &g = s->dg;
+        if (level == 0) {
+            g.nc_np = 0;
+            g.nc_sub_points = 0;
+            g.nc_f = nullptr;
+            g.nc_i = nullptr;
+            g.nc_rule = nullptr;
+            s->near_pairs = 0;
+        } else if (s->plan.nc_level != level || s->plan.nc_threshold != threshold || g.nc_rule == nullptr) {
+            hbem::Plan &P = s->plan;
+            hbem::build_near_classes(&P, threshold, level);
+            hip_check(hipStreamSynchronize(s->stream), "near-field upload");
+            upload(s->nc_f, P.nc_f, s->stream);
+            upload(s->nc_i, P.nc_i, s->stream);
+            upload(s->nc_rule, P.nc_rule, s->stream);
+            hip_check(hipStreamSynchronize(s->stream), "near-field upload");
+            g.nc_f = s->nc_f.as<float>();
+            g.nc_i = s->nc_i.as<int32_t>();
+            g.nc_rule = s->nc_rule.as<float>();
+            g.nc_np = P.nc_np;
+            g.nc_sub_points = P.nc_sub_points;
+            s->near_pairs = P.nc_pairs;
+        }
+        if (pair_count) *pair_count = s->near_pairs;
     });
 }
 
@@ -902,10 +944,10 @@ void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
         const int B = std::min(Bmax, F - f0), V = B * S;
         const float *qc = q_d + (size_t)f0 * S * T * 2;
         const float *bc = beta_d ? beta_d + (size_t)f0 * T * 2 : nullptr;
-        Timer ta, treg, tsing, tf;
+        Timer ta, treg, tsing, tnear, tf;
         ta.start(st);
         assemble_device(s, B, S, kre_d + f0, kim_d ? kim_d + f0 : nullptr, qc, bc, lda, a_stride,
-                        N, &treg, &tsing);
+                        N, &treg, &tsing, &tnear);
         ta.stop(st);
         std::vector<hbem_case_diag> dg;
         double solve_s = 0.0;
@@ -931,6 +973,7 @@ void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
         hip_check(hipGetLastError(), "field launch");
         tf.stop(st);
         const double asm_s = ta.seconds(), reg_s = treg.seconds(), sing_s = tsing.seconds();
+        const double near_s = tnear.seconds();
         const double field_s = tf.seconds();
         for (int b = 0; b < B; ++b) {
             hbem_case_diag &d = (*diag_out)[f0 + b];
@@ -938,6 +981,8 @@ void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
             d.assembly_s = asm_s / B;
             d.regular_assembly_s = reg_s / B;
             d.singular_assembly_s = sing_s / B;
+            d.near_assembly_s = near_s / B;
+            d.near_pairs = s->near_pairs;
             d.dense_solve_s = solve_s / B;
             d.field_s = field_s / B;
             d.batch_index = f0 / Bmax;

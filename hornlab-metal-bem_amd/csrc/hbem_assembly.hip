@@ -10,6 +10,9 @@
 //                          plus the -M/2 mass term (reference:
 //                          duffy_delta_blocks main.swift:4246-4431 folded by
 //                          reduceDuffyDeltaBlocks main.swift:6498-6561).
+//   k_assemble_near     -- optional near-field deltas (subdivided minus
+//                          6x6 rule) of close non-touching pairs
+//                          (applyNearFieldCorrectionsCPU, main.swift:3072-3175).
 //   k_field             -- exterior potential (evaluate_field_regular,
 //                          main.swift:4135-4203).
 //   k_tag_average       -- area-weighted tag means
@@ -664,6 +667,135 @@ __global__ __launch_bounds__(256) void k_assemble_singular(
 }
 
 // ---------------------------------------------------------------------------
+// Near-field corrections (applyNearFieldCorrectionsCPU,
+// main.swift:3072-3175, with subdividedPairBlocks, main.swift:2840-2910):
+// for every near class, delta = (level-L subdivided rule) - (6 x 6 rule) of
+// the (test, trial image) block, added to A (and the Robin fold) and to the
+// right-hand sides.  One lane per class, grid.y = frequency.  Both rules are
+// invariant under the swap of the reference coordinates, so the image
+// parity of the test side drops out and test points use the identity map.
+// Per test point the trial sums are FP32; across test points FP64.
+// ---------------------------------------------------------------------------
+template <bool kComplexK>
+__global__ __launch_bounds__(256) void k_assemble_near(
+    DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
+    const float *__restrict__ q, int n_src, const float *__restrict__ beta,
+    float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
+    int64_t rhs_stride) {
+    const int np = g.nc_np;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= np) return;
+    const int orig = g.nc_i[(size_t)6 * np + c];
+    if (orig < 0) return;
+    const float *__restrict__ F = g.nc_f;
+    float gf[18];
+#pragma unroll
+    for (int j = 0; j < 18; ++j) gf[j] = F[(size_t)j * np + c];
+    const int b = blockIdx.y;
+    const float kk = k_real[b];
+    const float krev = kk * kInv2Pi;
+    const float kim = kComplexK ? k_imag[b] : 0.0f;
+    const float *__restrict__ rule = g.nc_rule;
+
+    // sums[0]: subdivided rule, sums[1]: plain rule
+    double ds[2][3][2], dd[2][9][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ds[h][i][0] = ds[h][i][1] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dd[h][i][0] = dd[h][i][1] = 0.0;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float *__restrict__ rr = h == 0 ? rule : rule + (size_t)g.nc_sub_points * 3;
+        const int n_pts = h == 0 ? g.nc_sub_points : 6;
+        for (int a = 0; a < n_pts; ++a) {
+            const float xa = rr[3 * a], ya = rr[3 * a + 1], wa = rr[3 * a + 2];
+            // R0 - x E1 - y E2
+            const float bx = fmaf(-ya, gf[6], fmaf(-xa, gf[3], gf[0]));
+            const float by = fmaf(-ya, gf[7], fmaf(-xa, gf[4], gf[1]));
+            const float bz = fmaf(-ya, gf[8], fmaf(-xa, gf[5], gf[2]));
+            float sg[2] = {0.0f, 0.0f}, sd[3][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
+            for (int bb = 0; bb < n_pts; ++bb) {
+                const float xs = rr[3 * bb], ys = rr[3 * bb + 1], wb = rr[3 * bb + 2];
+                const float dx = fmaf(ys, gf[12], fmaf(xs, gf[9], bx));
+                const float dy = fmaf(ys, gf[13], fmaf(xs, gf[10], by));
+                const float dz = fmaf(ys, gf[14], fmaf(xs, gf[11], bz));
+                const GD v = kernel_gd<kComplexK>(dx, dy, dz, gf[15], gf[16], gf[17], kk, krev, kim);
+                sg[0] = fmaf(wb, v.gre, sg[0]);
+                sg[1] = fmaf(wb, v.gim, sg[1]);
+                const float s0 = wb * (1.0f - xs - ys), s1 = wb * xs, s2 = wb * ys;
+                sd[0][0] = fmaf(s0, v.dre, sd[0][0]);
+                sd[0][1] = fmaf(s0, v.dim, sd[0][1]);
+                sd[1][0] = fmaf(s1, v.dre, sd[1][0]);
+                sd[1][1] = fmaf(s1, v.dim, sd[1][1]);
+                sd[2][0] = fmaf(s2, v.dre, sd[2][0]);
+                sd[2][1] = fmaf(s2, v.dim, sd[2][1]);
+            }
+            const double t[3] = {(double)(wa * (1.0f - xa - ya)), (double)(wa * xa), (double)(wa * ya)};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                ds[h][i][0] += t[i] * (double)sg[0];
+                ds[h][i][1] += t[i] * (double)sg[1];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    dd[h][3 * i + j][0] += t[i] * (double)sd[j][0];
+                    dd[h][3 * i + j][1] += t[i] * (double)sd[j][1];
+                }
+            }
+        }
+    }
+
+    const int T = g.n_triangles;
+    const double scale = (double)F[(size_t)18 * np + c];
+    int rows[3], cols[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        rows[a] = g.nc_i[(size_t)a * np + c];
+        cols[a] = g.nc_i[(size_t)(3 + a) * np + c];
+    }
+    double sl[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        sl[i][0] = (ds[0][i][0] - ds[1][i][0]) * scale;
+        sl[i][1] = (ds[0][i][1] - ds[1][i][1]) * scale;
+    }
+    float *Ab = A + (size_t)b * a_stride * 2;
+    double cre = 0.0, cim = 0.0;
+    if (beta) {
+        const double br = beta[((size_t)b * T + orig) * 2];
+        const double bi = beta[((size_t)b * T + orig) * 2 + 1];
+        cre = (-(double)kim * br - (double)kk * bi) / 3.0;
+        cim = (-(double)kim * bi + (double)kk * br) / 3.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double vre = (dd[0][3 * i + j][0] - dd[1][3 * i + j][0]) * scale -
+                               (cre * sl[i][0] - cim * sl[i][1]);
+            const double vim = (dd[0][3 * i + j][1] - dd[1][3 * i + j][1]) * scale -
+                               (cre * sl[i][1] + cim * sl[i][0]);
+            const size_t idx = ((size_t)cols[j] * lda + rows[i]) * 2;
+            atomicAdd(Ab + idx, (float)vre);
+            atomicAdd(Ab + idx + 1, (float)vim);
+        }
+    for (int sv = 0; sv < n_src; ++sv) {
+        const size_t qi = (((size_t)b * n_src + sv) * T + orig) * 2;
+        const float qre = q[qi], qim = q[qi + 1];
+        if (qre == 0.0f && qim == 0.0f) continue;
+        float *rb = rhs + ((size_t)b * n_src + sv) * rhs_stride * 2;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float sre = (float)sl[i][0], sim = (float)sl[i][1];
+            atomicAdd(rb + (size_t)rows[i] * 2, sre * qre - sim * qim);
+            atomicAdd(rb + (size_t)rows[i] * 2 + 1, sre * qim + sim * qre);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Neumann data seen by the field with Robin walls (neumannWithRobin,
 // main.swift:1646-1674): qt = q + i k beta (p0 + p1 + p2) / 3 per triangle
 // (caller order), i k = (-k_im, k_re); one thread per (triangle, vector),
@@ -876,6 +1008,20 @@ void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float 
         hipLaunchKernelGGL((k_assemble_singular<false, kFB>), grid, dim3(256), 0, stream, g,
                            g.sg_rule, n_batch, k_real, k_imag, q, n_src, beta, A, lda, a_stride, rhs,
                            rhs_stride);
+}
+
+void launch_assemble_near(const DeviceGeometry &g, int n_batch, const float *k_real,
+                          const float *k_imag, const float *q, int n_src, const float *beta,
+                          float *A, int64_t lda, int64_t a_stride, float *rhs,
+                          int64_t rhs_stride, hipStream_t stream) {
+    if (g.nc_np == 0 || g.nc_rule == nullptr || n_batch <= 0) return;
+    dim3 grid((g.nc_np + 255) / 256, n_batch);
+    if (k_imag != nullptr)
+        hipLaunchKernelGGL((k_assemble_near<true>), grid, dim3(256), 0, stream, g, k_real, k_imag,
+                           q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
+    else
+        hipLaunchKernelGGL((k_assemble_near<false>), grid, dim3(256), 0, stream, g, k_real, k_imag,
+                           q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
 }
 
 void launch_robin_neumann(const DeviceGeometry &g, int n_vec, int vpk, const float *k_real,

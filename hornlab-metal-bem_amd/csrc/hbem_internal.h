@@ -114,6 +114,28 @@ struct Plan {
     std::vector<float> areas;        // [T]
     std::vector<int32_t> tags;       // [T]
     std::vector<int32_t> l2g;        // [T][3]
+    std::vector<float> vtx;          // [3][nv] caller vertices
+    std::vector<int32_t> tri;        // [3][T] caller triangles
+    std::vector<float> normals;      // [3][T]
+
+    // Near-field classes (built on demand for a threshold and a level):
+    // non-touching (test, trial image) combinations whose mirrored triangles
+    // lie within threshold x the longer longest edge (bounding-box gap and
+    // centroid distance), grouped like the singular classes by
+    // (test, trial, tm^sm) with the count as weight.  Float fields
+    // [kNcF][nc_np]: [0..2] R0 = Q0 - P0, [3..5] E1, [6..8] E2 (test, identity),
+    // [9..11] F1, [12..14] F2 (trial image), [15..17] trial image normal,
+    // [18] (2 A_t)(2 A_s) weight.  Int fields [kNcI][nc_np]: [0..2] row dofs,
+    // [3..5] column dofs, [6] trial caller index (-1 padding).  Rule rows
+    // (x, y, w): the level-L subdivided 6-point rule, then the 6-point rule.
+    int nc_level = 0;
+    double nc_threshold = 0.0;
+    std::vector<float> nc_f;
+    std::vector<int32_t> nc_i;
+    int nc_np = 0;
+    int64_t nc_pairs = 0;     // near (test image, trial image) combinations
+    std::vector<float> nc_rule;
+    int nc_sub_points = 0;
 
     // Tiles of device-order triangles.
     std::vector<int32_t> tile_start;   // [n_tiles]
@@ -129,6 +151,13 @@ std::string build_plan(int n_vertices, int n_triangles, int n_dofs, int symmetry
                        const float *vertices_3xn, const int32_t *triangles_3xm,
                        const int32_t *l2g, const float *areas, const float *normals_3xm,
                        const int32_t *tags, Plan *plan);
+
+constexpr int kNcF = 19;
+constexpr int kNcI = 7;
+
+// Builds the near-field classes of Plan for (threshold, level); returns the
+// number of classes.
+int64_t build_near_classes(Plan *P, double threshold, int level);
 
 constexpr int kSgF = 20;
 constexpr int kSgI = 7;
@@ -166,6 +195,11 @@ struct DeviceGeometry {
     const int32_t *tile_dofs;
     const int32_t *tile_pairs;
     const int32_t *l2g;  // [T][3] P1 dofs in caller triangle order
+    const float *nc_f;
+    const int32_t *nc_i;
+    const float *nc_rule;
+    int nc_np;
+    int nc_sub_points;
     int n_tiles;
     int n_tile_pairs;
     int max_tile_dofs;
@@ -190,6 +224,12 @@ void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float 
                               const float *k_imag, const float *q, int n_src, const float *beta,
                               float *A, int64_t lda, int64_t a_stride, float *rhs,
                               int64_t rhs_stride, hipStream_t stream);
+
+// Near-field deltas (subdivided minus 6x6 rule) of the near classes.
+void launch_assemble_near(const DeviceGeometry &g, int n_batch, const float *k_real,
+                          const float *k_imag, const float *q, int n_src, const float *beta,
+                          float *A, int64_t lda, int64_t a_stride, float *rhs,
+                          int64_t rhs_stride, hipStream_t stream);
 
 // qt = q + i k beta mean(p) per triangle for n_vec vectors (vpk per
 // frequency): the Neumann data of the field with Robin walls.

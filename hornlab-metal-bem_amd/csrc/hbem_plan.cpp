@@ -10,6 +10,7 @@
 //     (buildSymmetryDuffyPairList main.swift:2301-2380,
 //      imageSharedLocalIds main.swift:2152-2193).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -590,11 +591,174 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
     }
     P->unique_slots = static_cast<int64_t>(slots.size());
     build_singular_classes(nv, T, vtx, tri, l2g, areas, normals, pairs, P);
+    P->vtx.assign(vtx, vtx + 3 * (size_t)nv);
+    P->tri.assign(tri, tri + 3 * (size_t)T);
+    P->normals.assign(normals, normals + 3 * (size_t)T);
     P->areas.assign(areas, areas + T);
     P->tags.assign(T, 0);
     if (tags) P->tags.assign(tags, tags + T);
     P->l2g.assign(l2g, l2g + 3 * static_cast<size_t>(T));
     return std::string();
+}
+
+}  // namespace hbem
+
+namespace hbem {
+
+// Near-field pair selection (buildNearPairList, main.swift:2471-2558) and
+// the level-L subdivided rule (subdividedPairBlocks, main.swift:2840-2910).
+int64_t build_near_classes(Plan *P, double threshold, int level) {
+    const int T = P->n_triangles, nv = P->n_vertices, M = P->n_masks;
+    const float *vtx = P->vtx.data();
+    const int32_t *tri = P->tri.data();
+    struct Metric {
+        double c[3], lo[3], hi[3], edge;
+        int key[3];
+    };
+    // coordinate keys per vertex and mask for the touching test
+    auto key_of = [&](int v, int mask, int c) {
+        double x = vtx[c * nv + v];
+        if (mask & (1 << c)) x = -x;
+        return std::llround(x / 1.0e-6);
+    };
+    std::vector<Metric> met(static_cast<size_t>(M) * T);
+    for (int m = 0; m < M; ++m)
+        for (int t = 0; t < T; ++t) {
+            Metric &mt = met[static_cast<size_t>(m) * T + t];
+            double p[3][3];
+            for (int l = 0; l < 3; ++l)
+                for (int c = 0; c < 3; ++c) {
+                    const double x = vtx[c * nv + tri[l * T + t]];
+                    p[l][c] = (P->masks[m] & (1 << c)) ? -x : x;
+                }
+            for (int c = 0; c < 3; ++c) {
+                mt.c[c] = (p[0][c] + p[1][c] + p[2][c]) / 3.0;
+                mt.lo[c] = std::min({p[0][c], p[1][c], p[2][c]});
+                mt.hi[c] = std::max({p[0][c], p[1][c], p[2][c]});
+            }
+            double e = 0.0;
+            for (int l = 0; l < 3; ++l) {
+                const int o = (l + 1) % 3;
+                double d2 = 0.0;
+                for (int c = 0; c < 3; ++c) d2 += (p[l][c] - p[o][c]) * (p[l][c] - p[o][c]);
+                e = std::max(e, d2);
+            }
+            mt.edge = std::sqrt(e);
+        }
+    auto gap = [](double amin, double amax, double bmin, double bmax) {
+        if (amax < bmin) return bmin - amax;
+        if (bmax < amin) return amin - bmax;
+        return 0.0;
+    };
+    auto touching = [&](int test, int trial, int tm, int sm) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const int vt = tri[i * T + test], vs = tri[j * T + trial];
+                bool same = true;
+                for (int c = 0; c < 3 && same; ++c) same = key_of(vt, tm, c) == key_of(vs, sm, c);
+                if (same) return true;
+            }
+        return false;
+    };
+    std::unordered_map<uint64_t, int> index;
+    std::vector<std::array<int, 4>> cls;  // test, trial, smask (tm = 0), weight
+    int64_t pairs = 0;
+    for (int trial = 0; trial < T; ++trial)
+        for (int smi = 0; smi < M; ++smi)
+            for (int tmi = 0; tmi < M; ++tmi) {
+                const Metric &b = met[static_cast<size_t>(smi) * T + trial];
+                for (int test = 0; test < T; ++test) {
+                    const Metric &a = met[static_cast<size_t>(tmi) * T + test];
+                    const double cut = threshold * std::max(a.edge, b.edge), cut2 = cut * cut;
+                    const double dx = gap(a.lo[0], a.hi[0], b.lo[0], b.hi[0]);
+                    const double dy = gap(a.lo[1], a.hi[1], b.lo[1], b.hi[1]);
+                    const double dz = gap(a.lo[2], a.hi[2], b.lo[2], b.hi[2]);
+                    if (dx * dx + dy * dy + dz * dz > cut2) continue;
+                    const double cx = a.c[0] - b.c[0], cy = a.c[1] - b.c[1], cz = a.c[2] - b.c[2];
+                    if (cx * cx + cy * cy + cz * cz >= cut2) continue;
+                    const int tm = P->masks[tmi], sm = P->masks[smi];
+                    if (touching(test, trial, tm, sm)) continue;
+                    ++pairs;
+                    const uint64_t key = (static_cast<uint64_t>(test) << 36) |
+                                         (static_cast<uint64_t>(trial) << 8) |
+                                         static_cast<uint64_t>(tm ^ sm);
+                    auto it = index.find(key);
+                    if (it == index.end()) {
+                        index.emplace(key, static_cast<int>(cls.size()));
+                        cls.push_back({test, trial, tm ^ sm, 1});
+                    } else {
+                        cls[it->second][3]++;
+                    }
+                }
+            }
+    const int nc = static_cast<int>(cls.size());
+    const int np = ((nc + 63) / 64) * 64;
+    P->nc_np = np;
+    P->nc_pairs = pairs;
+    P->nc_f.assign(static_cast<size_t>(kNcF) * std::max(np, 1), 0.0f);
+    P->nc_i.assign(static_cast<size_t>(kNcI) * std::max(np, 1), 0);
+    for (int c = 0; c < np; ++c) P->nc_i[static_cast<size_t>(6) * np + c] = -1;
+    for (int c = 0; c < nc; ++c) {
+        const int test = cls[c][0], trial = cls[c][1], sm = cls[c][2], w = cls[c][3];
+        double v[3][3], u[3][3];
+        for (int l = 0; l < 3; ++l)
+            for (int d = 0; d < 3; ++d) {
+                v[l][d] = vtx[d * nv + tri[l * T + test]];
+                const double y = vtx[d * nv + tri[l * T + trial]];
+                u[l][d] = (sm & (1 << d)) ? -y : y;
+            }
+        auto put = [&](int field, double val) { P->nc_f[static_cast<size_t>(field) * np + c] = static_cast<float>(val); };
+        for (int d = 0; d < 3; ++d) {
+            put(0 + d, u[0][d] - v[0][d]);
+            put(3 + d, v[1][d] - v[0][d]);
+            put(6 + d, v[2][d] - v[0][d]);
+            put(9 + d, u[1][d] - u[0][d]);
+            put(12 + d, u[2][d] - u[0][d]);
+            const double nrm = P->normals[d * T + trial];
+            put(15 + d, (sm & (1 << d)) ? -nrm : nrm);
+        }
+        put(18, 4.0 * static_cast<double>(P->areas[test]) * static_cast<double>(P->areas[trial]) * w);
+        for (int a = 0; a < 3; ++a) {
+            P->nc_i[static_cast<size_t>(a) * np + c] = P->l2g[test * 3 + a];
+            P->nc_i[static_cast<size_t>(3 + a) * np + c] = P->l2g[trial * 3 + a];
+        }
+        P->nc_i[static_cast<size_t>(6) * np + c] = trial;
+    }
+    // rules: level-L subdivision (midpoint split into 4, recursively), then
+    // the plain rule; weights include the sub-triangle area ratio
+    std::vector<std::array<double, 6>> sub{{0, 0, 1, 0, 0, 1}};
+    for (int l = 0; l < level; ++l) {
+        std::vector<std::array<double, 6>> next;
+        for (const auto &t : sub) {
+            const double ax = t[0], ay = t[1], bx = t[2], by = t[3], cx = t[4], cy = t[5];
+            const double abx = (ax + bx) * 0.5, aby = (ay + by) * 0.5;
+            const double bcx = (bx + cx) * 0.5, bcy = (by + cy) * 0.5;
+            const double cax = (cx + ax) * 0.5, cay = (cy + ay) * 0.5;
+            next.push_back({ax, ay, abx, aby, cax, cay});
+            next.push_back({abx, aby, bx, by, bcx, bcy});
+            next.push_back({cax, cay, bcx, bcy, cx, cy});
+            next.push_back({abx, aby, bcx, bcy, cax, cay});
+        }
+        sub.swap(next);
+    }
+    P->nc_rule.clear();
+    for (const auto &t : sub) {
+        const double det = std::fabs((t[2] - t[0]) * (t[5] - t[1]) - (t[3] - t[1]) * (t[4] - t[0]));
+        for (int a = 0; a < 6; ++a) {
+            P->nc_rule.push_back(static_cast<float>(t[0] + kRule6X[a] * (t[2] - t[0]) + kRule6Y[a] * (t[4] - t[0])));
+            P->nc_rule.push_back(static_cast<float>(t[1] + kRule6X[a] * (t[3] - t[1]) + kRule6Y[a] * (t[5] - t[1])));
+            P->nc_rule.push_back(static_cast<float>(kRule6W[a] * det));
+        }
+    }
+    P->nc_sub_points = static_cast<int>(sub.size()) * 6;
+    for (int a = 0; a < 6; ++a) {
+        P->nc_rule.push_back(static_cast<float>(kRule6X[a]));
+        P->nc_rule.push_back(static_cast<float>(kRule6Y[a]));
+        P->nc_rule.push_back(static_cast<float>(kRule6W[a]));
+    }
+    P->nc_level = level;
+    P->nc_threshold = threshold;
+    return nc;
 }
 
 }  // namespace hbem

@@ -24,7 +24,7 @@ from numpy.typing import NDArray
 from .backends import AssemblyBackendUnavailable
 from .geometry import SYMMETRY_CODES, GeometryBuffers, validate_native_symmetry_plane
 
-HBEM_ABI_VERSION = 2
+HBEM_ABI_VERSION = 3
 _LIB_NAME = "libhbem.so"
 
 SOLVE_MODES = {"fp32": 0, "fp32_ir": 1, "mixed_fp16": 2, "mixed_bf16": 3}
@@ -91,6 +91,8 @@ class _CaseDiag(C.Structure):
         ("singular_assembly_s", C.c_double),
         ("batch_index", C.c_int32),
         ("batch_size", C.c_int32),
+        ("near_assembly_s", C.c_double),
+        ("near_pairs", C.c_int64),
     ]
 
 
@@ -102,6 +104,7 @@ EXPORTED_SYMBOLS = (
     "hbem_session_create",
     "hbem_session_destroy",
     "hbem_session_get_info",
+    "hbem_session_set_near_field",
     "hbem_assemble",
     "hbem_solve_batch",
     "hbem_solve_batch_device",
@@ -124,6 +127,7 @@ def diag_to_dict(d: "_CaseDiag") -> dict[str, Any]:
         "dense_solve_refine_residual_rel": float(d.refine_residual_rel),
         "batch_index": int(d.batch_index),
         "batch_size": int(d.batch_size),
+        "near_assembly_seconds": float(d.near_assembly_s),
     }
 
 _lock = threading.Lock()
@@ -170,6 +174,9 @@ def load_library() -> C.CDLL:
         lib.hbem_session_destroy.restype = C.c_int32
         lib.hbem_session_get_info.argtypes = [C.c_void_p, C.POINTER(_SessionInfo)]
         lib.hbem_session_get_info.restype = C.c_int32
+        lib.hbem_session_set_near_field.argtypes = [
+            C.c_void_p, C.c_int32, C.c_double, C.POINTER(C.c_int64)]
+        lib.hbem_session_set_near_field.restype = C.c_int32
         fp = C.POINTER(C.c_float)
         lib.hbem_assemble.argtypes = [C.c_void_p, C.c_float, C.c_float, fp, fp, fp, fp, fp, fp, fp, fp]
         lib.hbem_assemble.restype = C.c_int32
@@ -200,6 +207,45 @@ def load_library() -> C.CDLL:
             )
         _lib = lib
         return lib
+
+
+NEAR_QUADRATURE_ENV = "HORNLAB_METAL_BEM_NATIVE_NEAR_QUADRATURE"
+
+
+def near_quadrature_from_env(environ: Any = None) -> tuple[int, float] | None:
+    """(level, threshold) of the near-field quadrature requested through
+    HORNLAB_METAL_BEM_NATIVE_NEAR_QUADRATURE, or None when it is unset, empty
+    or "0" (requestedNearQuadratureConfig, main.swift:5024-5055): "1"/"2"
+    give that level at threshold 1.5, "<level>:<threshold>" a finite positive
+    threshold; anything else is an error naming the variable."""
+    env = os.environ if environ is None else environ
+    raw = env.get(NEAR_QUADRATURE_ENV)
+    if raw is None or raw == "" or raw == "0":
+        return None
+
+    def level_of(text: str) -> int | None:
+        try:
+            level = int(text)
+        except ValueError:
+            return None
+        return level if 1 <= level <= 2 and text.strip() == text else None
+
+    level = level_of(raw)
+    if level is not None:
+        return level, 1.5
+    parts = raw.split(":")
+    if len(parts) == 2:
+        level = level_of(parts[0])
+        try:
+            threshold = float(parts[1])
+        except ValueError:
+            threshold = float("nan")
+        if level is not None and np.isfinite(threshold) and threshold > 0.0:
+            return level, threshold
+    raise RuntimeError(
+        f"{NEAR_QUADRATURE_ENV} must be unset, '0', '1', '2', "
+        "or '<level>:<positive threshold>' with level 1...2"
+    )
 
 
 def _check(rc: int, op: str) -> None:
@@ -285,6 +331,8 @@ class HipBemSession:
         self.geometry_buffers = buffers
         self.symmetry_plane = symmetry_plane
         self._closed = False
+        self._near: tuple[int, float] | None = None
+        self._near_pairs = 0
 
     @classmethod
     def create_session(
@@ -369,6 +417,39 @@ class HipBemSession:
         }
 
     # -- operations ---------------------------------------------------------
+    def set_near_quadrature(self, level: int, threshold: float = 1.5) -> int:
+        """Turns the near-field correction on (level 1 or 2) or off (level 0)
+        for later assemblies; returns the number of near pairs."""
+        self._ensure_open()
+        count = C.c_int64(0)
+        _check(
+            load_library().hbem_session_set_near_field(
+                self._handle, int(level), float(threshold), C.byref(count)),
+            "set_near_quadrature",
+        )
+        self._near = (int(level), float(threshold)) if level > 0 else None
+        self._near_pairs = int(count.value)
+        return self._near_pairs
+
+    def _sync_near_quadrature(self) -> None:
+        """Applies the environment's near-field setting, read per operation as
+        the reference helper does; with the variable unset the session keeps
+        the setting of set_near_quadrature (off by default)."""
+        if NEAR_QUADRATURE_ENV not in os.environ:
+            return
+        want = near_quadrature_from_env()
+        if want != self._near:
+            if want is None:
+                self.set_near_quadrature(0)
+            else:
+                self.set_near_quadrature(*want)
+
+    def near_quadrature_diagnostics(self, seconds: float = 0.0) -> dict[str, Any] | None:
+        if self._near is None:
+            return None
+        return {"level": self._near[0], "threshold": self._near[1],
+                "pair_count": self._near_pairs, "seconds": float(seconds)}
+
     def _beta_rows(self, payloads: list[dict[int, complex] | None]) -> NDArray[np.complex64] | None:
         """Per-triangle Robin admittance rows [len(payloads), T] from {tag:
         beta} dicts (the reference's impedance payload, native.py:870-902);
@@ -400,6 +481,7 @@ class HipBemSession:
         """Corrected system at one wavenumber: (A (n, n) row-major, rhs (n,)).
         ``impedance_sources`` {tag: beta} adds the Robin fold."""
         self._ensure_open()
+        self._sync_near_quadrature()
         q = np.asarray(neumann_dp0)
         T, n = self.geometry_buffers.n_triangles, self.geometry_buffers.p1_dof_count
         if q.shape != (T,):
@@ -455,6 +537,7 @@ class HipBemSession:
         ``extra_neumann_dp0`` (n_extra, F, T) adds right-hand sides that share
         each case's operator and LU (multi-RHS, native.py:775-783)."""
         self._ensure_open()
+        self._sync_near_quadrature()
         freqs = np.asarray(frequency_hz, dtype=np.float64).reshape(-1)
         k = np.ascontiguousarray(np.asarray(k_real, dtype=np.float32).reshape(-1))
         F = int(freqs.size)
@@ -579,6 +662,9 @@ class HipBemSession:
             }
             diagnostics["regular_assembly_seconds"] = float(d.regular_assembly_s)
             diagnostics["singular_assembly_seconds"] = float(d.singular_assembly_s)
+            near = self.near_quadrature_diagnostics(float(d.near_assembly_s))
+            if near is not None:
+                diagnostics["near_quadrature"] = near
             diagnostics["resident_batch"] = {"index": int(d.batch_index), "size": int(d.batch_size)}
             if d.dense_solve_rcond >= 0:
                 diagnostics["dense_solve_rcond"] = float(d.dense_solve_rcond)
@@ -657,6 +743,7 @@ class HipBemSession:
         q is [F][n_sources][T]; robin_beta_ptr an interleaved complex [F][T]
         admittance or None.  Returns the per-case diagnostics."""
         self._ensure_open()
+        self._sync_near_quadrature()
         tags = np.ascontiguousarray(list(source_tags), dtype=np.int32)
         opts = _SolveOptions()
         opts.solve_mode = SOLVE_MODES[solve_mode]

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HBEM_ABI_VERSION 2
+#define HBEM_ABI_VERSION 3
 
 #define HBEM_OK 0
 #define HBEM_EINVAL -1      /* bad argument / contract violation */
@@ -120,6 +120,8 @@ typedef struct hbem_case_diag {
     double singular_assembly_s;          /* singular-pair kernel alone (HIP events) */
     int32_t batch_index;                 /* resident chunk the case ran in */
     int32_t batch_size;                  /* cases in that chunk (times are chunk / size) */
+    double near_assembly_s;              /* near-field kernel alone (0 when off) */
+    int64_t near_pairs;                  /* near (test image, trial image) pairs corrected */
 } hbem_case_diag;
 
 /* ABI version of the loaded library. */
@@ -142,6 +144,20 @@ int32_t hbem_session_create(const hbem_geometry *geometry, hbem_session **out);
 /* Release every device buffer of the session.  Replaces close()
  * (native.py:1709-1716). */
 int32_t hbem_session_destroy(hbem_session *session);
+
+/* Near-field quadrature of close non-touching pairs for every later
+ * assembly of the session: level 1 or 2 subdivides both triangles into
+ * 4^level sub-triangles with the 6-point rule on each and adds
+ * (subdivided - 6x6) blocks; pairs are those whose mirrored triangles have a
+ * bounding-box gap and a centroid distance below threshold x the longer
+ * longest edge.  level 0 turns it off.  Replaces the
+ * HORNLAB_METAL_BEM_NATIVE_NEAR_QUADRATURE setting read by the helper
+ * (requestedNearQuadratureConfig, main.swift:5024-5055; pair list
+ * buildNearPairList main.swift:2471-2558; corrections
+ * applyNearFieldCorrectionsCPU main.swift:3072-3175).  pair_count (may be
+ * NULL) receives the number of near pairs. */
+int32_t hbem_session_set_near_field(hbem_session *session, int32_t level, double threshold,
+                                    int64_t *pair_count);
 
 int32_t hbem_session_get_info(hbem_session *session, hbem_session_info *info);
 

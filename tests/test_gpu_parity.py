@@ -369,3 +369,77 @@ def test_multi_source_shares_operator(require_gpu):
             assert rel_l2(x.field, single[e + 1][i].field) < 1e-6, (i, e)
         assert abs(multi[i].extra_sources[0].impedance
                    - single[1][i].surface_pressure_avg[1]) <= 1e-5 * abs(single[1][i].surface_pressure_avg[1])
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_near_field_assembly_matches_oracle(require_gpu, level):
+    """Near-field deltas (subdivided minus 6x6 rule) of the quarter horn's
+    close non-touching pairs, images included, with complex k, Robin walls
+    and a Neumann source (applyNearFieldCorrectionsCPU,
+    main.swift:3072-3175)."""
+    b = horn_buffers(edge_length=0.03, quarter=True)
+    f = 2500.0
+    k = float(np.float32(2 * math.pi * f / C0))
+    kim = 0.2
+    tags = b.physical_tags_i32
+    beta = 0.3 - 0.1j
+    q = np.where(tags == 2, 1j * 1.2041 * 2 * math.pi * f, 0.0).astype(np.complex128)
+    with _session(b, "yz+xz") as s:
+        n_pairs = s.set_near_quadrature(level, 1.5)
+        A, rhs = s.assemble_standard_neumann(f, k, q.astype(np.complex64), k_imag=kim,
+                                             impedance_sources={1: beta})
+        s.set_near_quadrature(0)
+        A0, r0 = s.assemble_standard_neumann(f, k, q.astype(np.complex64), k_imag=kim,
+                                             impedance_sources={1: beta})
+    g = O.Geometry.from_buffers(b, "yz+xz")
+    bt = np.where(tags == 1, beta, 0.0)
+    assert n_pairs == len(O.near_pairs(g, 1.5)) > 0
+    Ao, ro, st = O.assemble(g, k, q, kim=kim, beta=bt, near_level=level, near_threshold=1.5)
+    Ab, rb, _ = O.assemble(g, k, q, kim=kim, beta=bt)
+    assert st["near_pairs"] == n_pairs
+    assert rel_l2(A, Ao) < 1e-5, rel_l2(A, Ao)
+    assert rel_l2(rhs, ro) < 1e-5, rel_l2(rhs, ro)
+    # the correction itself, not only the total
+    assert rel_l2(A - A0, Ao - Ab) < 2e-3, rel_l2(A - A0, Ao - Ab)
+    assert rel_l2(rhs - r0, ro - rb) < 2e-3, rel_l2(rhs - r0, ro - rb)
+
+
+def test_near_field_env_moves_close_pair_toward_fine_reference(require_gpu, monkeypatch):
+    """HORNLAB_METAL_BEM_NATIVE_NEAR_QUADRATURE: unset and "0" leave the
+    assembly as it is; "2:1.5" corrects two parallel triangles 0.05 apart
+    toward a level-4 subdivided block and reports the pass
+    (test_metal_native.py:1824-1949)."""
+    from meshes import mesh_buffers
+
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 0.05], [1, 0, 0.05], [0, 1, 0.05]], float)
+    Fc = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    b = mesh_buffers(V, Fc, np.array([1, 2], np.int32))
+    k = 1.8318326
+    q = np.array([1.0, 0.0], np.complex64)
+    obs = np.array([[0.0], [0.0], [2.0]], np.float32)
+
+    def run(value):
+        if value is None:
+            monkeypatch.delenv("HORNLAB_METAL_BEM_NATIVE_NEAR_QUADRATURE", raising=False)
+        else:
+            monkeypatch.setenv("HORNLAB_METAL_BEM_NATIVE_NEAR_QUADRATURE", value)
+        with _session(b, check_open_edges=False) as s:
+            A, _ = s.assemble_standard_neumann(100.0, k, q)
+            res = s.assemble_solve_evaluate_standard_neumann_batch(
+                np.array([100.0]), np.array([k], np.float32), q[None, :], obs, source_tags=[1])
+        return A, res[0].diagnostics
+
+    A_unset, d_unset = run(None)
+    A_zero, d_zero = run("0")
+    A_near, d_near = run("2:1.5")
+    assert "near_quadrature" not in d_unset and "near_quadrature" not in d_zero
+    assert rel_l2(A_zero, A_unset) < 1e-6
+    nq = d_near["near_quadrature"]
+    assert nq["level"] == 2 and nq["threshold"] == pytest.approx(1.5)
+    assert nq["pair_count"] >= 1 and nq["seconds"] >= 0.0
+    g = O.Geometry.from_buffers(b)
+    _, high = O.pair_block(g, 0, 1, mode=2, order=4, k=k)
+    base = A_unset[np.ix_([0, 1, 2], [3, 4, 5])]
+    near = A_near[np.ix_([0, 1, 2], [3, 4, 5])]
+    assert np.linalg.norm(near - base) > 0.0
+    assert np.linalg.norm(near - high) < np.linalg.norm(base - high)

@@ -242,3 +242,19 @@ def test_no_gpu_fails_loudly():
         pytest.skip("a GPU is visible")
     with pytest.raises(AssemblyBackendUnavailable):
         HipBemSession.create_session(tiny_two_triangles())
+
+
+def test_near_quadrature_env_parsing():
+    """HORNLAB_METAL_BEM_NATIVE_NEAR_QUADRATURE as the reference helper reads
+    it (main.swift:5024-5055; test_metal_native.py:1952-1977 for junk)."""
+    from hornlab_metal_bem.native import NEAR_QUADRATURE_ENV, near_quadrature_from_env as parse
+
+    assert parse({}) is None
+    assert parse({NEAR_QUADRATURE_ENV: ""}) is None
+    assert parse({NEAR_QUADRATURE_ENV: "0"}) is None
+    assert parse({NEAR_QUADRATURE_ENV: "1"}) == (1, 1.5)
+    assert parse({NEAR_QUADRATURE_ENV: "2"}) == (2, 1.5)
+    assert parse({NEAR_QUADRATURE_ENV: "2:0.75"}) == (2, 0.75)
+    for junk in ("junk", "3", "1:0", "1:-2", "1:inf", "1:nan", "0:1.5", "1:", "1:2:3", " 1"):
+        with pytest.raises(RuntimeError, match=NEAR_QUADRATURE_ENV):
+            parse({NEAR_QUADRATURE_ENV: junk})
