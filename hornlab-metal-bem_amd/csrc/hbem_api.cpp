@@ -135,7 +135,15 @@ struct hbem_session {
     hbem::Plan plan;
     hbem::DeviceGeometry dg{};
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
-        tile_count, tile_ndofs, tile_dofs, tile_pairs, nc_f, nc_i, nc_rule;
+        tile_count, tile_ndofs, tile_dofs, tile_pairs, nc_f, nc_i, nc_rule, sg_adj, vtx_d, triv_d,
+        normals_d, jac_d;
+    // adjoint store (hbem_adjoint_forward -> hbem_adjoint_backward)
+    DevBuf adj_k, adj_kim, adj_fk, adj_q, adj_beta, adj_obs, adj_c, adj_ft, adj_sig, adj_gN,
+        adj_l32, adj_e64, adj_l64, adj_r64, adj_act;
+    bool adj_keep = false;   // forward in progress: one resident batch, A kept
+    bool adj_valid = false;  // the store matches the session's workspaces
+    int adj_F = 0, adj_nobs = 0;
+    bool adj_complex = false, adj_robin = false;
     // per-call workspaces
     DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, lu_scr_i, lu_scr_f, amax,
         lu_scale, kv, kh, kn, kf,
@@ -150,7 +158,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag, &io_beta, &qt, &l2g})
@@ -440,6 +448,16 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         upload(s->tile_dofs, P.tile_dofs, s->stream);
         upload(s->tile_pairs, P.tile_pairs, s->stream);
         upload(s->l2g, P.l2g, s->stream);
+        upload(s->sg_adj, P.sg_adj, s->stream);
+        upload(s->vtx_d, P.vtx, s->stream);
+        upload(s->triv_d, P.tri, s->stream);
+        upload(s->normals_d, P.normals, s->stream);
+        {
+            std::vector<float> jac(P.areas.size());
+            for (size_t t = 0; t < jac.size(); ++t) jac[t] = 2.0f * P.areas[t];
+            upload(s->jac_d, jac, s->stream);
+            hip_check(hipStreamSynchronize(s->stream), "session upload");
+        }
         hbem::DeviceGeometry &g = s->dg;
         g.tri_f_aos = s->tri_f_aos.as<float>();
         g.tri_i_aos = s->tri_i_aos.as<int32_t>();
@@ -466,6 +484,12 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         g.tile_dofs = s->tile_dofs.as<int32_t>();
         g.tile_pairs = s->tile_pairs.as<int32_t>();
         g.l2g = s->l2g.as<int32_t>();
+        g.sg_adj = s->sg_adj.as<int32_t>();
+        g.vtx = s->vtx_d.as<float>();
+        g.tri_v = s->triv_d.as<int32_t>();
+        g.normals = s->normals_d.as<float>();
+        g.jac_c = s->jac_d.as<float>();
+        g.n_vertices = P.n_vertices;
         g.n_tile_pairs = (int)(P.tile_pairs.size() / 2);
         g.n_tiles = (int)P.tile_start.size();
         g.max_tile_dofs = P.max_tile_dofs;
@@ -484,6 +508,7 @@ int32_t hbem_session_set_near_field(hbem_session *s, int32_t level, double thres
         if (level > 0 && !(std::isfinite(threshold) && threshold > 0.0))
             fail(HBEM_EINVAL, "hbem_session_set_near_field: threshold must be finite and positive");
         std::lock_guard<std::mutex> lock(s->mu);
+        s->adj_valid = false;
         set_device(s);
         hbem::DeviceGeometry &g = s->dg;
         if (level == 0) {
@@ -549,6 +574,7 @@ int32_t hbem_assemble(hbem_session *s, float k_real, float k_imag, const float *
         if (!std::isfinite(k_real) || !std::isfinite(k_imag) || k_imag < 0.0f)
             fail(HBEM_EINVAL, "hbem_assemble: k must be finite with k_imag >= 0");
         std::lock_guard<std::mutex> lock(s->mu);
+        s->adj_valid = false;
         set_device(s);
         const int n = s->plan.n_dofs;
         const int64_t lda = n, a_stride = (int64_t)n * n, rhs_stride = n;
@@ -774,7 +800,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
     hbem::launch_norm1(s->A.as<float>(), lda, a_stride, n, B, s->anorm.as<double>(),
                        s->amax.as<double>(), st);
     hbem::launch_pow2_scale(s->amax.as<double>(), s->lu_scale.as<float>(), B, st);
-    if (refine) {
+    if (refine || s->adj_keep) {
         s->A_orig.reserve(sizeof(float) * 2 * (size_t)B * a_stride);
         hip_check(hipMemcpyAsync(s->A_orig.ptr, s->A.ptr, sizeof(float) * 2 * (size_t)B * a_stride,
                                  hipMemcpyDeviceToDevice, st),
@@ -894,6 +920,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
 }
 
 int batch_size_for(const hbem_session *s, int F, int N, const hbem_solve_options &opt) {
+    if (s->adj_keep) return F;
     if (opt.batch_frequencies > 0) return std::min(F, opt.batch_frequencies);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)16 << 30;
@@ -901,7 +928,6 @@ int batch_size_for(const hbem_session *s, int F, int N, const hbem_solve_options
     const double per = (double)N * N * 8.0 * (refine ? 2.0 : 1.0) + (double)N * 64.0;
     const double budget = std::min(0.5 * (double)free_b, 96.0 * (1ull << 30));
     int b = (int)std::max(1.0, std::floor(budget / per));
-    (void)s;
     return std::max(1, std::min({F, b, 64}));
 }
 
@@ -1015,6 +1041,7 @@ int32_t hbem_solve_batch(hbem_session *s, int32_t F, const float *k_real, const 
                 fail(HBEM_EINVAL, "hbem_solve_batch: k_imag must be finite and >= 0");
         }
         std::lock_guard<std::mutex> lock(s->mu);
+        s->adj_valid = false;
         set_device(s);
         hipStream_t st = s->stream;
         const int n = s->plan.n_dofs;
@@ -1106,6 +1133,7 @@ int32_t hbem_solve_batch_device(hbem_session *s, int32_t F, const float *k_real,
             fail(HBEM_EINVAL, "hbem_solve_batch_device: observation arrays invalid");
         check_options(*opts);
         std::lock_guard<std::mutex> lock(s->mu);
+        s->adj_valid = false;
         set_device(s);
         std::vector<hbem_case_diag> dg;
         solve_core(s, F, k_real, k_imag, field_k, q, opts->robin_beta, n_obs, obs_3xn, *opts,
@@ -1119,6 +1147,137 @@ int32_t hbem_solve_batch_device(hbem_session *s, int32_t F, const float *k_real,
     });
 }
 
+int32_t hbem_adjoint_forward(hbem_session *s, int32_t F, const float *k_real,
+                             const float *k_imag, const float *field_k, const float *q,
+                             int32_t n_obs, const float *obs_3xn, const hbem_solve_options *opts,
+                             float *field, float *p, double *tag_avg, hbem_case_diag *diag) {
+    return guarded([&] {
+        if (!s || !opts) fail(HBEM_EINVAL, "hbem_adjoint_forward: null argument");
+        if (F <= 0) fail(HBEM_EINVAL, "hbem_adjoint_forward: n_freq must be positive");
+        if (!k_real || !field_k || !q || !tag_avg || n_obs <= 0 || !obs_3xn || !field)
+            fail(HBEM_EINVAL, "hbem_adjoint_forward: k_real, field_k, q, tag_avg, obs, field required");
+        check_options(*opts);
+        if (opts->n_sources > 1)
+            fail(HBEM_EINVAL, "hbem_adjoint_forward: one source per frequency");
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->adj_valid = false;
+        if (s->dg.nc_np > 0)
+            fail(HBEM_EINVAL, "hbem_adjoint_forward: near-field quadrature must be off");
+        set_device(s);
+        hipStream_t st = s->stream;
+        const int T = s->plan.n_triangles;
+        auto keep = [&](DevBuf &d, const float *src, size_t count) {
+            d.reserve(sizeof(float) * count);
+            hip_check(hipMemcpyAsync(d.ptr, src, sizeof(float) * count, hipMemcpyDeviceToDevice, st),
+                      "adjoint input copy");
+        };
+        keep(s->adj_k, k_real, F);
+        keep(s->adj_fk, field_k, F);
+        if (k_imag) keep(s->adj_kim, k_imag, F);
+        keep(s->adj_q, q, (size_t)2 * F * T);
+        keep(s->adj_obs, obs_3xn, (size_t)3 * n_obs);
+        if (opts->robin_beta) keep(s->adj_beta, opts->robin_beta, (size_t)2 * F * T);
+        hbem_solve_options o = *opts;
+        o.n_sources = 1;
+        std::vector<hbem_case_diag> dg;
+        s->adj_keep = true;
+        try {
+            solve_core(s, F, s->adj_k.as<float>(), k_imag ? s->adj_kim.as<float>() : nullptr,
+                       s->adj_fk.as<float>(), s->adj_q.as<float>(),
+                       opts->robin_beta ? s->adj_beta.as<float>() : nullptr, n_obs,
+                       s->adj_obs.as<float>(), o, field, p, tag_avg, &dg);
+        } catch (...) {
+            s->adj_keep = false;
+            throw;
+        }
+        s->adj_keep = false;
+        hip_check(hipStreamSynchronize(st), "adjoint forward sync");
+        for (int f = 0; f < F; ++f) {
+            if (dg[f].lapack_info != 0 && !diag)
+                fail(HBEM_ESINGULAR, "LU found an exactly zero pivot");
+            if (diag) diag[f] = dg[f];
+        }
+        s->adj_F = F;
+        s->adj_nobs = n_obs;
+        s->adj_complex = k_imag != nullptr;
+        s->adj_robin = opts->robin_beta != nullptr;
+        s->adj_valid = true;
+    });
+}
+
+int32_t hbem_adjoint_backward(hbem_session *s, const float *grad_field, double *grad_vertices,
+                              float *grad_q, int32_t refine_iterations) {
+    return guarded([&] {
+        if (!s || !grad_field || !grad_vertices)
+            fail(HBEM_EINVAL, "hbem_adjoint_backward: null argument");
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!s->adj_valid)
+            fail(HBEM_EINVAL,
+                 "hbem_adjoint_backward: no adjoint forward is resident on this session");
+        set_device(s);
+        hipStream_t st = s->stream;
+        const int B = s->adj_F, n = s->plan.n_dofs, T = s->plan.n_triangles;
+        const int nv = s->plan.n_vertices, n_obs = s->adj_nobs;
+        const int N = (int)round_up(n, hbem::kLuPad);
+        const int64_t lda = N, a_stride = (int64_t)N * N;
+        const float *kre = s->adj_k.as<float>();
+        const float *kim = s->adj_complex ? s->adj_kim.as<float>() : nullptr;
+        const float *qt = s->adj_robin ? s->qt.as<float>() : s->adj_q.as<float>();
+        const float *pv = s->x.as<float>();
+        s->adj_c.reserve(sizeof(double) * 2 * (size_t)B * N);
+        s->adj_ft.reserve(sizeof(double) * 2 * (size_t)B * T);
+        s->adj_sig.reserve(sizeof(double) * 2 * (size_t)B * T);
+        s->adj_gN.reserve(sizeof(double) * 3 * (size_t)T);
+        s->adj_l32.reserve(sizeof(float) * 2 * (size_t)B * N);
+        s->adj_e64.reserve(sizeof(double) * 2 * (size_t)B * N);
+        s->adj_l64.reserve(sizeof(double) * 2 * (size_t)B * N);
+        s->adj_r64.reserve(sizeof(double) * 2 * (size_t)B * N);
+        s->norms.reserve(sizeof(double) * 2 * B);
+        hbem::launch_fill_zero_f64(grad_vertices, (size_t)3 * nv, st);
+        hbem::launch_fill_zero_f64(s->adj_c.as<double>(), (size_t)2 * B * N, st);
+        hbem::launch_fill_zero_f64(s->adj_ft.as<double>(), (size_t)2 * B * T, st);
+        hbem::launch_fill_zero_f64(s->adj_sig.as<double>(), (size_t)2 * B * T, st);
+        hbem::launch_fill_zero_f64(s->adj_gN.as<double>(), (size_t)3 * T, st);
+        hbem::launch_field_adjoint(s->dg, B, s->adj_fk.as<float>(), pv, N, qt, n_obs,
+                                   s->adj_obs.as<float>(), grad_field, s->adj_c.as<double>(), N,
+                                   s->adj_ft.as<double>(), grad_vertices, s->adj_gN.as<double>(), st);
+        hbem::launch_adjoint_rhs(s->dg, B, kre, kim, s->adj_robin ? s->adj_beta.as<float>() : nullptr,
+                                 s->adj_ft.as<double>(), s->adj_c.as<double>(), N, n, N,
+                                 s->adj_l32.as<float>(), s->adj_e64.as<double>(), st);
+        // lambda = A^{-H} e with the resident LU, refined against the FP64
+        // residual e - A^H lambda of the kept matrix.
+        hbem::LuWorkspace w{s->A.as<float>(), lda, a_stride, N, n, B, s->ipiv.as<int32_t>(),
+                            s->info.as<int32_t>(), s->gl.as<int32_t>(), s->lu_scr_i.as<int32_t>(),
+                            s->lu_scr_f.as<float>()};
+        hbem::lu_solve(w, s->adj_l32.as<float>(), N, 2, st);
+        hbem::launch_fill_zero_f64(s->adj_l64.as<double>(), (size_t)2 * B * N, st);
+        hbem::launch_axpy_f32_to_f64(s->adj_l32.as<float>(), s->adj_l64.as<double>(), N, N, n, B,
+                                     nullptr, st);
+        for (int it = 0; it < refine_iterations; ++it) {
+            hbem::launch_residual_h_f64(s->A_orig.as<float>(), lda, a_stride, n, B,
+                                        s->adj_e64.as<double>(), s->adj_l64.as<double>(), N,
+                                        s->adj_r64.as<double>(), s->norms.as<double>(), st);
+            hbem::launch_fill_zero(s->adj_l32.as<float>(), (size_t)2 * B * N, st);
+            hbem::launch_copy_f64_to_f32(s->adj_r64.as<double>(), s->adj_l32.as<float>(), N, N, n,
+                                         B, st);
+            hbem::lu_solve(w, s->adj_l32.as<float>(), N, 2, st);
+            hbem::launch_axpy_f32_to_f64(s->adj_l32.as<float>(), s->adj_l64.as<double>(), N, N, n,
+                                         B, nullptr, st);
+        }
+        hbem::launch_fill_zero(s->adj_l32.as<float>(), (size_t)2 * B * N, st);
+        hbem::launch_copy_f64_to_f32(s->adj_l64.as<double>(), s->adj_l32.as<float>(), N, N, n, B,
+                                     st);
+        hbem::launch_assembly_adjoint(s->dg, B, kre, kim, s->adj_l32.as<float>(), pv, N, qt,
+                                      grad_vertices, s->adj_gN.as<double>(),
+                                      s->adj_sig.as<double>(), st);
+        hbem::launch_adjoint_finish(s->dg, B, s->adj_l32.as<float>(), pv, N,
+                                    s->adj_gN.as<double>(), grad_vertices, s->adj_ft.as<double>(),
+                                    s->adj_sig.as<double>(), grad_q, st);
+        hip_check(hipGetLastError(), "adjoint launch");
+        hip_check(hipStreamSynchronize(st), "adjoint sync");
+    });
+}
+
 int32_t hbem_evaluate_field(hbem_session *s, int32_t F, const float *field_k, const float *p_re,
                             const float *p_im, const float *q_re, const float *q_im, int32_t n_obs,
                             const float *obs_3xn, float *out_re, float *out_im) {
@@ -1128,6 +1287,7 @@ int32_t hbem_evaluate_field(hbem_session *s, int32_t F, const float *field_k, co
         if (!field_k || !p_re || !p_im || !q_re || !q_im || !obs_3xn || !out_re || !out_im)
             fail(HBEM_EINVAL, "hbem_evaluate_field: null array");
         std::lock_guard<std::mutex> lock(s->mu);
+        s->adj_valid = false;
         set_device(s);
         hipStream_t st = s->stream;
         const int n = s->plan.n_dofs, T = s->plan.n_triangles;

@@ -100,6 +100,10 @@ struct Plan {
     //   [6] trial original index (-1 on padding)
     std::vector<float> sg_f;
     std::vector<int32_t> sg_i;
+    // Adjoint fields [kSgAdj][sg_np]: [0..2] test vertex (caller index) of
+    // the remapped test corners, [3..5] the same for the trial, [6] test mask
+    // | trial mask << 4, [7] test caller index.
+    std::vector<int32_t> sg_adj;
     int sg_np = 0;                 // padded class count
     int sg_waves[3] = {0, 0, 0};   // 64-lane groups per kind
     int64_t sg_classes = 0;        // unpadded class count
@@ -159,6 +163,7 @@ constexpr int kNcI = 7;
 // number of classes.
 int64_t build_near_classes(Plan *P, double threshold, int level);
 
+constexpr int kSgAdj = 8;
 constexpr int kSgF = 20;
 constexpr int kSgI = 7;
 constexpr int kSgRule = 16;
@@ -195,6 +200,12 @@ struct DeviceGeometry {
     const int32_t *tile_dofs;
     const int32_t *tile_pairs;
     const int32_t *l2g;  // [T][3] P1 dofs in caller triangle order
+    const int32_t *sg_adj;
+    const float *vtx;      // [3][nv] caller vertices
+    const int32_t *tri_v;  // [3][T] caller triangles
+    const float *normals;  // [3][T] caller normals
+    const float *jac_c;    // [T] 2 * area, caller order
+    int n_vertices;
     const float *nc_f;
     const int32_t *nc_i;
     const float *nc_rule;
@@ -230,6 +241,33 @@ void launch_assemble_near(const DeviceGeometry &g, int n_batch, const float *k_r
                           const float *k_imag, const float *q, int n_src, const float *beta,
                           float *A, int64_t lda, int64_t a_stride, float *rhs,
                           int64_t rhs_stride, hipStream_t stream);
+
+// Adjoint (shape derivative) stages, all frequencies of a batch per launch.
+// Cotangent grad_field [B][n_obs] complex float; c [B][c_stride] and ft
+// [B][T] complex double receive E^H g and F^H g before conjugation; gV
+// [3][nv] and gN [3][T] (double) the vertex and normal gradients.
+void launch_field_adjoint(const DeviceGeometry &g, int n_batch, const float *field_k,
+                          const float *p, int64_t p_stride, const float *qt, int n_obs,
+                          const float *obs, const float *grad_field, double *c, int64_t c_stride,
+                          double *ft, double *gV, double *gN, hipStream_t stream);
+// e = conj(c) (+ the Robin term from ft), as float and double, zero-padded
+// to N per system.
+void launch_adjoint_rhs(const DeviceGeometry &g, int n_batch, const float *k_real,
+                        const float *k_imag, const float *beta, const double *ft, double *c,
+                        int64_t ld, int n, int N, float *e32, double *e64, hipStream_t stream);
+// r = e - A^H x in FP64, norms[2 b] = max|r|, norms[2 b + 1] = max|e|.
+void launch_residual_h_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,
+                           const double *e, const double *x, int64_t ldx, double *r,
+                           double *norms, hipStream_t stream);
+// Regular and singular pair derivatives of Re(lambda^H (V qt - K p)).
+void launch_assembly_adjoint(const DeviceGeometry &g, int n_batch, const float *k_real,
+                             const float *k_imag, const float *lam, const float *p, int64_t ld,
+                             const float *qt, double *gV, double *gN, double *sig,
+                             hipStream_t stream);
+// Mass term, normal -> vertex gradients, grad_q = conj(ft + sig).
+void launch_adjoint_finish(const DeviceGeometry &g, int n_batch, const float *lam, const float *p,
+                           int64_t ld, const double *gN, double *gV, const double *ft,
+                           const double *sig, float *grad_q, hipStream_t stream);
 
 // qt = q + i k beta mean(p) per triangle for n_vec vectors (vpk per
 // frequency): the Neumann data of the field with Robin walls.

@@ -237,6 +237,7 @@ void build_singular_classes(int nv, int T, const float *vtx, const int32_t *tri,
     P->sg_classes = nclasses;
     P->sg_f.assign(static_cast<size_t>(kSgF) * np, 0.0f);
     P->sg_i.assign(static_cast<size_t>(kSgI) * np, 0);
+    P->sg_adj.assign(static_cast<size_t>(kSgAdj) * np, 0);
     for (int p = 0; p < np; ++p) P->sg_i[static_cast<size_t>(6) * np + p] = -1;
     int base = 0;
     for (int k = 0; k < 3; ++k) {
@@ -266,6 +267,13 @@ void build_singular_classes(int nv, int T, const float *vtx, const int32_t *tri,
                 ii[static_cast<size_t>(3 + a) * np + slot] = l2g[pr.trial * 3 + smap.loc[a]];
             }
             ii[static_cast<size_t>(6) * np + slot] = pr.trial;
+            int32_t *aj = P->sg_adj.data();
+            for (int a = 0; a < 3; ++a) {
+                aj[static_cast<size_t>(a) * np + slot] = tri[tmap.loc[a] * T + pr.test];
+                aj[static_cast<size_t>(3 + a) * np + slot] = tri[smap.loc[a] * T + pr.trial];
+            }
+            aj[static_cast<size_t>(6) * np + slot] = pr.tm | (pr.sm << 4);
+            aj[static_cast<size_t>(7) * np + slot] = pr.test;
         }
         base += 64 * P->sg_waves[k];
     }

@@ -110,6 +110,8 @@ EXPORTED_SYMBOLS = (
     "hbem_solve_batch_device",
     "hbem_evaluate_field",
     "hbem_dense_solve",
+    "hbem_adjoint_forward",
+    "hbem_adjoint_backward",
 )
 
 
@@ -192,6 +194,13 @@ def load_library() -> C.CDLL:
             vp, vp, vp, C.POINTER(_CaseDiag),
         ]
         lib.hbem_solve_batch_device.restype = C.c_int32
+        lib.hbem_adjoint_forward.argtypes = [
+            C.c_void_p, C.c_int32, vp, vp, vp, vp, C.c_int32, vp, C.POINTER(_SolveOptions),
+            vp, vp, vp, C.POINTER(_CaseDiag),
+        ]
+        lib.hbem_adjoint_forward.restype = C.c_int32
+        lib.hbem_adjoint_backward.argtypes = [C.c_void_p, vp, vp, vp, C.c_int32]
+        lib.hbem_adjoint_backward.restype = C.c_int32
         lib.hbem_evaluate_field.argtypes = [
             C.c_void_p, C.c_int32, fp, fp, fp, fp, fp, C.c_int32, fp, fp, fp,
         ]
@@ -766,6 +775,71 @@ class HipBemSession:
         )
         return [diag_to_dict(d) for d in diag]
 
+
+    def _device_options(self, solve_mode, refine_max_iterations, refine_tolerance, estimate_rcond,
+                        source_tags, impedance_source_tag, robin_beta_ptr, n_sources, keep):
+        tags = np.ascontiguousarray(list(source_tags), dtype=np.int32)
+        keep.append(tags)
+        opts = _SolveOptions()
+        opts.solve_mode = SOLVE_MODES[solve_mode]
+        opts.refine_max_iterations = int(refine_max_iterations)
+        opts.refine_tolerance = float(refine_tolerance)
+        opts.batch_frequencies = 0
+        opts.estimate_rcond = 1 if estimate_rcond else 0
+        opts.n_tags = int(tags.size)
+        opts.tags = _i32p(tags) if tags.size else None
+        opts.impedance_tag = int(impedance_source_tag)
+        opts.robin_beta = robin_beta_ptr
+        opts.n_sources = int(n_sources)
+        return opts
+
+    def adjoint_forward_device(
+        self,
+        n_freq: int,
+        k_real_ptr: int,
+        field_k_ptr: int,
+        q_ptr: int,
+        n_obs: int,
+        obs_ptr: int,
+        field_ptr: int,
+        tag_avg_ptr: int,
+        *,
+        k_imag_ptr: int | None = None,
+        p_ptr: int | None = None,
+        robin_beta_ptr: int | None = None,
+        source_tags: list[int] | tuple[int, ...] = (),
+        impedance_source_tag: int = 0,
+        solve_mode: str = "fp32_ir",
+        refine_max_iterations: int = 8,
+        refine_tolerance: float = 1e-7,
+    ) -> list[dict[str, Any]]:
+        """hbem_adjoint_forward: solve_batch_device that keeps every
+        frequency's operator, LU and solution resident for
+        adjoint_backward_device."""
+        self._ensure_open()
+        keep: list[Any] = []
+        opts = self._device_options(solve_mode, refine_max_iterations, refine_tolerance, False,
+                                    source_tags, impedance_source_tag, robin_beta_ptr, 1, keep)
+        diag = (_CaseDiag * int(n_freq))()
+        _check(
+            load_library().hbem_adjoint_forward(
+                self._handle, int(n_freq), k_real_ptr, k_imag_ptr, field_k_ptr, q_ptr, int(n_obs),
+                obs_ptr, C.byref(opts), field_ptr, p_ptr, tag_avg_ptr, diag,
+            ),
+            "adjoint_forward",
+        )
+        return [diag_to_dict(d) for d in diag]
+
+    def adjoint_backward_device(self, grad_field_ptr: int, grad_vertices_ptr: int,
+                                grad_q_ptr: int | None = None, refine_iterations: int = 3) -> None:
+        """hbem_adjoint_backward: vertex (and Neumann) gradients of a real
+        loss whose field cotangent is at grad_field_ptr."""
+        self._ensure_open()
+        _check(
+            load_library().hbem_adjoint_backward(
+                self._handle, grad_field_ptr, grad_vertices_ptr, grad_q_ptr, int(refine_iterations)),
+            "adjoint_backward",
+        )
     def evaluate_standard_exterior_batch(
         self,
         frequency_hz: NDArray[Any],

@@ -210,6 +210,28 @@ int32_t hbem_solve_batch_device(hbem_session *session, int32_t n_freq, const flo
                                 const hbem_solve_options *opts, float *field, float *p,
                                 double *tag_avg, hbem_case_diag *diag);
 
+/* Forward pass of the shape adjoint: hbem_solve_batch_device (device
+ * pointers, one source per frequency, near-field off) that keeps every
+ * frequency's matrix, LU factors, surface solution and Neumann data resident
+ * in the session until the next call on it.  No counterpart in the reference
+ * (whose solver has no derivatives); it serves the torch.autograd wrapper of
+ * waveguide shape optimisation. */
+int32_t hbem_adjoint_forward(hbem_session *session, int32_t n_freq, const float *k_real,
+                             const float *k_imag, const float *field_k, const float *q,
+                             int32_t n_obs, const float *obs_3xn,
+                             const hbem_solve_options *options, float *field, float *p,
+                             double *tag_avg, hbem_case_diag *diag);
+
+/* Backward pass after hbem_adjoint_forward.  grad_field [F][n_obs]
+ * interleaved complex float (device) holds dL/dRe f + i dL/dIm f of a real
+ * loss L of the field.  Writes grad_vertices [3][n_vertices] double (device,
+ * dL/d vertex xyz, overwritten) and, when non-NULL, grad_q [F][T] interleaved
+ * complex float (device, dL/dRe q + i dL/dIm q).  The transpose solve reuses
+ * the forward LU factors and takes refine_iterations FP64-residual
+ * refinement steps. */
+int32_t hbem_adjoint_backward(hbem_session *session, const float *grad_field,
+                              double *grad_vertices, float *grad_q, int32_t refine_iterations);
+
 /* Exterior field from given surface data.  Replaces
  * evaluate_standard_exterior_batch (native.py:1552-1707; kernel
  * MS:4135-4203).  p [F][p1_dof_count], q [F][n_triangles], out [F][n_obs]. */

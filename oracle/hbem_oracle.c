@@ -63,11 +63,11 @@ static inline cx cx_scale(cx a, real s) { return cx_make(a.re * s, a.im * s); }
 
 /* Geometry view (layout of geometry.py:36-65). */
 typedef struct {
-    const float *vertices;   /* [3][n_vertices] */
+    const real *vertices;     /* [3][n_vertices] */
     const int32_t *triangles; /* [3][n_triangles] */
     const int32_t *l2g;       /* [n_triangles][3] */
-    const float *areas;       /* [n_triangles] */
-    const float *normals;     /* [3][n_triangles] */
+    const real *areas;        /* [n_triangles] */
+    const real *normals;      /* [3][n_triangles] */
     int32_t n_vertices;
     int32_t n_triangles;
     int32_t n_dofs;

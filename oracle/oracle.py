@@ -75,7 +75,9 @@ SYMMETRY = {None: 0, "yz": 1, "xz": 2, "yz+xz": 3, "xy": 4}
 
 @dataclass
 class Geometry:
-    """Keeps the float32/int32 buffers alive while the C side reads them."""
+    """Mesh arrays for the C side.  Vertices, areas and normals are kept in
+    float64 (the caller's float32 buffers convert exactly) and handed to the
+    64-bit build as they are, to the 32-bit build rounded to float32."""
 
     vertices: np.ndarray
     triangles: np.ndarray
@@ -89,24 +91,44 @@ class Geometry:
     @classmethod
     def from_buffers(cls, b, symmetry_plane: str | None = None) -> "Geometry":
         return cls(
-            np.ascontiguousarray(b.vertices_3xn_f32, np.float32),
+            np.ascontiguousarray(b.vertices_3xn_f32, np.float64),
             np.ascontiguousarray(b.triangles_3xm_i32, np.int32),
             np.ascontiguousarray(b.p1_local2global_i32, np.int32),
-            np.ascontiguousarray(b.triangle_areas_f32, np.float32),
-            np.ascontiguousarray(b.triangle_normals_3xm_f32, np.float32),
+            np.ascontiguousarray(b.triangle_areas_f32, np.float64),
+            np.ascontiguousarray(b.triangle_normals_3xm_f32, np.float64),
             int(b.p1_dof_count),
             SYMMETRY[symmetry_plane],
             np.ascontiguousarray(b.physical_tags_i32, np.int32),
         )
 
+    @classmethod
+    def from_vertices(cls, vertices, like: "Geometry") -> "Geometry":
+        """Same topology as ``like`` with float64 vertex positions; areas and
+        unit normals recomputed in float64 from the (v1 - v0) x (v2 - v0)
+        winding (the geometry-buffer convention)."""
+        v = np.ascontiguousarray(vertices, np.float64)
+        tri = like.triangles
+        p0, p1, p2 = v[:, tri[0]], v[:, tri[1]], v[:, tri[2]]
+        cr = np.cross((p1 - p0).T, (p2 - p0).T).T
+        tw = np.linalg.norm(cr, axis=0)
+        return cls(v, like.triangles, like.l2g, np.ascontiguousarray(0.5 * tw),
+                   np.ascontiguousarray(cr / tw), like.n_dofs, like.symmetry, like.tags)
+
     @property
     def n_triangles(self) -> int:
         return int(self.triangles.shape[1])
 
-    def c(self) -> _Geom:
+    def c(self, bits: int = 64) -> _Geom:
+        if bits == 64:
+            arrs = (self.vertices, self.areas, self.normals)
+        else:
+            arrs = (np.ascontiguousarray(self.vertices, np.float32),
+                    np.ascontiguousarray(self.areas, np.float32),
+                    np.ascontiguousarray(self.normals, np.float32))
+        self._keep = arrs
         return _Geom(
-            self.vertices.ctypes.data, self.triangles.ctypes.data, self.l2g.ctypes.data,
-            self.areas.ctypes.data, self.normals.ctypes.data, int(self.vertices.shape[1]),
+            arrs[0].ctypes.data, self.triangles.ctypes.data, self.l2g.ctypes.data,
+            arrs[1].ctypes.data, arrs[2].ctypes.data, int(self.vertices.shape[1]),
             self.n_triangles, self.n_dofs, self.symmetry,
         )
 
@@ -149,7 +171,7 @@ def kernel(d, n, k, kim=0.0, bits=64):
 
 def duffy_pairs(geom: Geometry):
     counts = np.zeros(3, np.int64)
-    g = geom.c()
+    g = geom.c(64)
     n = lib(64).hbo_duffy_pairs(C.byref(g), None, 0, _p(counts))
     pairs = np.zeros((max(n, 1), 9), np.int32)
     lib(64).hbo_duffy_pairs(C.byref(g), _p(pairs), n, _p(counts))
@@ -157,7 +179,7 @@ def duffy_pairs(geom: Geometry):
 
 
 def near_pairs(geom: Geometry, threshold: float):
-    g = geom.c()
+    g = geom.c(64)
     n = lib(64).hbo_near_pairs(C.byref(g), float(threshold), None, 0)
     pairs = np.zeros((max(n, 1), 9), np.int32)
     lib(64).hbo_near_pairs(C.byref(g), float(threshold), _p(pairs), n)
@@ -167,7 +189,7 @@ def near_pairs(geom: Geometry, threshold: float):
 def pair_block(geom: Geometry, test, trial, tm=0, sm=0, mode=0, order=4, k=1.0, kim=0.0, bits=64):
     d = _dt(bits)
     slp, dlp = np.zeros(6, d), np.zeros(18, d)
-    g = geom.c()
+    g = geom.c(bits)
     rc = lib(bits).hbo_pair_block(C.byref(g), test, trial, tm, sm, mode, order, float(k), float(kim), _p(slp), _p(dlp))
     if rc != 0:
         raise ValueError("pair_block: invalid pair/mode")
@@ -193,7 +215,7 @@ def assemble(geom: Geometry, k: float, q=None, *, kim: float = 0.0, beta=None,
     rre, rim = np.zeros(n, d), np.zeros(n, d)
     qre, qim = _split(q if q is not None else np.zeros(T), T, d)
     bre, bim = _split(beta, T, d)
-    g = geom.c()
+    g = geom.c(bits)
     L = lib(bits)
     L.hbo_assemble_regular(C.byref(g), k, kim, _p(qre), _p(qim), _p(bre), _p(bim), _p(Are), _p(Aim), _p(rre), _p(rim))
     stats = {}
@@ -217,7 +239,7 @@ def field(geom: Geometry, k: float, p, q, obs_3xn, bits: int = 64):
     obs = np.ascontiguousarray(obs_3xn, np.float32)
     m = int(obs.shape[1])
     ore, oim = np.zeros(m, d), np.zeros(m, d)
-    g = geom.c()
+    g = geom.c(bits)
     lib(bits).hbo_field(C.byref(g), k, _p(pre), _p(pim), _p(qre), _p(qim), m, _p(obs), _p(ore), _p(oim))
     return ore + 1j * oim
 
@@ -226,7 +248,7 @@ def surface_average(geom: Geometry, p, tag: int, bits: int = 64):
     d = _dt(bits)
     pre, pim = _split(p, geom.n_dofs, d)
     out = np.zeros(3, np.float64)
-    g = geom.c()
+    g = geom.c(bits)
     lib(bits).hbo_surface_average(C.byref(g), _p(geom.tags), int(tag), _p(pre), _p(pim), _p(out))
     return complex(out[0], out[1]), float(out[2])
 
