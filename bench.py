@@ -41,7 +41,234 @@ WORKLOADS = {
     # full horn, no symmetry, ~8000 dofs, 64 log frequencies 200 Hz - 16 kHz, 3 x 181 points
     "c3": dict(quarter=False, symmetry=None, dofs=8000, fmin=200.0, fmax=16000.0, nf=64,
                angles=181),
+    # adjoint shape gradient: quarter exponential horn ~3000 dofs, 24 log frequencies 1-12 kHz
+    "c5": dict(quarter=True, symmetry="yz+xz", dofs=3000, fmin=1000.0, fmax=12000.0, nf=24,
+               angles=37),
+    # 16 seeded quarter-horn variants, n ~ U(600, 4000), 32 log frequencies 300 Hz - 18 kHz
+    "c4": dict(quarter=True, symmetry="yz+xz", dofs=None, fmin=300.0, fmax=18000.0, nf=32,
+               angles=37, variants=16),
 }
+
+
+def c4_variants(seed: int = 4):
+    """The batched uneven sweep: 16 horn variants (throat 10-25 mm, mouth
+    80-200 mm, length 80-250 mm, conical / exponential / tractrix-like
+    flares), dof targets uniform in 600-4000, 32 log frequencies each."""
+    from hornlab_metal_bem import ObservationConfig, native_config
+    from hornlab_metal_bem.mesh import loaded_mesh_from_arrays
+    from hornlab_metal_bem.procedural import HornSpec, edge_length_for_dofs, horn_body
+
+    w = WORKLOADS["c4"]
+    rng = np.random.default_rng(seed)
+    flares = ("conical", "exponential", "tractrix")
+    out = []
+    for i in range(w["variants"]):
+        spec = HornSpec(float(rng.uniform(0.010, 0.025)), float(rng.uniform(0.080, 0.200)),
+                        float(rng.uniform(0.080, 0.250)), 0.020, flares[i % 3])
+        target = int(rng.integers(600, 4001))
+        # n ~ h^-2: one coarse probe brackets the bisection
+        V0, _, _ = horn_body(spec, edge_length=0.03, quarter=True, jitter=0.0, seed=i)
+        h0 = 0.03 * math.sqrt(V0.shape[0] / target)
+        h = edge_length_for_dofs(spec, target, quarter=True, seed=i, rtol=0.02,
+                                 bracket=(h0 / 1.5, h0 * 1.5))
+        V, F, T = horn_body(spec, edge_length=h, quarter=True, jitter=0.05, seed=i)
+        mesh = loaded_mesh_from_arrays(V, F, T, native_symmetry_plane="yz+xz")
+        config = native_config(
+            freq_min_hz=w["fmin"], freq_max_hz=w["fmax"], freq_count=w["nf"], freq_spacing="log",
+            native_symmetry_plane="yz+xz", source_motion="axial", velocity_sources={2: 1.0},
+            observation=ObservationConfig(planes=["horizontal", "vertical", "diagonal"],
+                                          distance_m=2.0, angle_count=w["angles"]),
+        )
+        out.append((spec, target, mesh, config))
+    return out
+
+
+def c5_problem():
+    """Inputs of the adjoint workload: mesh buffers, wavenumbers, Neumann
+    rows, observation points (3 planes x 37) and the 50 seeded off-plane
+    vertices the finite-difference check samples."""
+    spec, h, mesh, config = workload(name="c5")
+    freqs, k, q, pts, buffers = prepare_inputs(mesh, config)
+    V = np.asarray(buffers.vertices_3xn_f32, np.float64)
+    cand = np.flatnonzero((np.abs(V[0]) > 1e-3) & (np.abs(V[1]) > 1e-3))
+    pick = np.sort(np.random.default_rng(5).choice(cand, 50, replace=False))
+    return freqs, k, q, pts, buffers, pick
+
+
+def run_c5(args) -> int:
+    """Forward field + directivity/ripple loss + hand-written adjoint
+    backward (vertex gradients) per step; the session is rebuilt from the
+    vertices every step as in a shape-optimisation loop."""
+    import torch
+
+    from hornlab_metal_bem.adjoint import FieldProblem, directivity_ripple_loss, exterior_field
+
+    freqs, k, q, pts, buffers, pick = c5_problem()
+    F, n, T = freqs.size, buffers.p1_dof_count, buffers.n_triangles
+    solve_mode = args.solve_mode or "fp32_ir"
+    prob = FieldProblem.from_buffers(buffers, k.astype(np.float32), pts, symmetry_plane="yz+xz",
+                                     solve_mode=solve_mode, refine_tolerance=1e-7,
+                                     source_tags=(2,), impedance_source_tag=2)
+    dev = torch.device("cuda:0")
+    V = torch.tensor(np.asarray(buffers.vertices_3xn_f32, np.float64), device=dev, requires_grad=True)
+    qd = torch.tensor(q.astype(np.complex64), device=dev)
+
+    def step():
+        V.grad = None
+        f = exterior_field(V, qd, prob)
+        loss = directivity_ripple_loss(f, 3, pts.shape[1] // 3)
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("c5: warm", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    with torch.no_grad():
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        f = exterior_field(V.detach(), qd, prob)
+        torch.cuda.synchronize()
+        fwd = time.perf_counter() - t1
+    grad = V.grad.cpu().numpy()
+    if args.save_gradient:
+        np.savez(args.save_gradient, vertices=pick, grad=grad[:, pick], loss=float(loss),
+                 field=f.cpu().numpy())
+    out = {
+        "metric": "frequencies solved/sec (assemble+LU+field) at 1k/8k dofs; rel-L2 vs FP64",
+        "value": round(F * args.steps / dt, 3),
+        "unit": "frequencies/s (forward + adjoint backward)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"adjoint shape gradient: quarter exponential horn yz+xz, n={n} P1 dofs, "
+            f"T={T} triangles, {F} log frequencies 1-12 kHz, 3x37 directivity dB + on-axis ripple loss",
+            "n_dofs": n, "n_triangles": T, "frequencies": F, "solve_mode": solve_mode,
+        },
+        "forward_only_ms": round(1e3 * fwd, 3),
+        "loss": float(loss),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def run_c4(args) -> int:
+    """Batched uneven sweep: every variant's session on its own HIP stream,
+    driven from its own host thread, all 512 systems per step; per-system
+    latency is the GPU time of the resident batch the system ran in."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+
+    from hornlab_metal_bem.native import HipBemSession
+
+    variants = c4_variants()
+    print(f"c4: {len(variants)} variants meshed", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    jobs = []
+    for spec, target, mesh, config in variants:
+        freqs, k, q, pts, buffers = prepare_inputs(mesh, config)
+        F, n, n_obs = freqs.size, buffers.p1_dof_count, pts.shape[1]
+        job = dict(
+            spec=spec, n=n, T=buffers.n_triangles, F=F, n_obs=n_obs,
+            k=torch.from_numpy(k.astype(np.float32)).to(dev),
+            q=torch.from_numpy(np.ascontiguousarray(q.astype(np.complex64)).view(np.float32)).to(dev),
+            obs=torch.from_numpy(np.ascontiguousarray(pts, np.float32)).to(dev),
+            field=torch.zeros(F * n_obs * 2, dtype=torch.float32, device=dev),
+            tag=torch.zeros(F * 2 * 2, dtype=torch.float64, device=dev),
+            session=HipBemSession.create_session(buffers, symmetry_plane="yz+xz"),
+        )
+        jobs.append(job)
+    torch.cuda.synchronize()
+    solve_mode = args.solve_mode or "fp32"
+
+    def run(job, batch):
+        return job["session"].solve_batch_device(
+            job["F"], job["k"].data_ptr(), job["k"].data_ptr(), job["q"].data_ptr(), job["n_obs"],
+            job["obs"].data_ptr(), job["field"].data_ptr(), job["tag"].data_ptr(), source_tags=[2],
+            impedance_source_tag=2, solve_mode=solve_mode, estimate_rcond=False,
+            batch_frequencies=batch)
+
+    def sweep(batch, workers):
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            return list(ex.map(lambda j: run(j, batch), jobs))
+
+    # autotune: resident batch size and host concurrency, one timed sweep each
+    best, trials = None, []
+    for batch in (8, 16, 32):
+        for workers in (4, 8, 16):
+            sweep(batch, workers)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sweep(batch, workers)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            trials.append({"batch_frequencies": batch, "host_threads": workers, "s": round(dt, 4)})
+            print(f"c4 autotune {trials[-1]}", file=sys.stderr, flush=True)
+            if best is None or dt < best[0]:
+                best = (dt, batch, workers)
+    _, batch, workers = best
+    for _ in range(args.warmup):
+        sweep(batch, workers)
+    torch.cuda.synchronize()
+    diags = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        diags.append(sweep(batch, workers))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    systems = sum(j["F"] for j in jobs)
+    lat = []
+    for step in diags:
+        for per_job in step:
+            for d in per_job:
+                lat.append(d["batch_size"] * (d["assembly_seconds"] + d["dense_solve_seconds"]
+                                              + d["field_seconds"]))
+    lat = np.asarray(lat) * 1e3
+    for j in jobs:
+        j["session"].close()
+    ns = [j["n"] for j in jobs]
+    out = {
+        "metric": "frequencies solved/sec (assemble+LU+field) at 1k/8k dofs; rel-L2 vs FP64",
+        "value": round(systems * args.steps / dt, 3),
+        "unit": "frequencies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched uneven sweep: {len(jobs)} quarter-horn variants, n {min(ns)}-{max(ns)} "
+            f"P1 dofs, {WORKLOADS['c4']['nf']} log frequencies each = {systems} systems",
+            "n_dofs": ns,
+            "solve_mode": solve_mode,
+            "autotuned": {"batch_frequencies": batch, "host_threads": workers, "trials": trials},
+        },
+        "latency_ms_per_system": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                  "p99": round(float(np.percentile(lat, 99)), 3)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
 
 
 def workload(target_dofs: int | None = None, seed: int = 0, name: str = "c2"):
@@ -122,7 +349,12 @@ def main() -> int:
                     help="default: fp32 for c2, mixed_fp16 (GMRES-IR to 1e-6) for c3")
     ap.add_argument("--refine-tol", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--save-gradient", default=None, help="c5: write the sampled gradient (.npz)")
     args = ap.parse_args()
+    if args.gpus == 1 and args.workload == "c5":
+        return run_c5(args)
+    if args.gpus == 1 and args.workload == "c4":
+        return run_c4(args)
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: frequencies shard independently across GPUs with no "

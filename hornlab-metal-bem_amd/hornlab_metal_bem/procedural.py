@@ -259,11 +259,13 @@ def horn_mesh(
 
 
 def edge_length_for_dofs(
-    spec: HornSpec, target_dofs: int, *, quarter: bool = True, seed: int = 0
+    spec: HornSpec, target_dofs: int, *, quarter: bool = True, seed: int = 0, rtol: float = 0.002,
+    bracket: tuple[float, float] = (1e-4, 0.2),
 ) -> float:
     """Edge length whose mesh has about ``target_dofs`` P1 dofs (bisection in
-    log h; deterministic)."""
-    lo, hi = 1e-4, 0.2
+    log h over ``bracket`` down to a relative width of ``rtol``;
+    deterministic)."""
+    lo, hi = bracket
     for _ in range(40):
         mid = math.sqrt(lo * hi)
         V, _, _ = horn_body(spec, edge_length=mid, quarter=quarter, jitter=0.0, seed=seed)
@@ -271,7 +273,7 @@ def edge_length_for_dofs(
             lo = mid
         else:
             hi = mid
-        if hi / lo < 1.002:
+        if hi / lo < 1.0 + rtol:
             break
     return math.sqrt(lo * hi)
 

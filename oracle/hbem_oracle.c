@@ -391,9 +391,10 @@ static int color_triangles(const hbo_geom *g, int *color) {
 
 /* A (row-major n x n) and rhs are ADDED to; beta (optional, per triangle)
  * is the Robin admittance folded as in MS:1745-1820. */
-static void assemble_regular(const hbo_geom *g, real k, real kim, const real *q_re,
-                             const real *q_im, const real *beta_re, const real *beta_im,
-                             real *A_re, real *A_im, real *rhs_re, real *rhs_im) {
+static void assemble_regular(const hbo_geom *g, const unsigned char *sel, real k, real kim,
+                             const real *q_re, const real *q_im, const real *beta_re,
+                             const real *beta_im, real *A_re, real *A_im, real *rhs_re,
+                             real *rhs_im) {
     int T = g->n_triangles, n = g->n_dofs;
     int masks[3];
     int nm = image_masks(g->symmetry, masks);
@@ -420,6 +421,7 @@ static void assemble_regular(const hbo_geom *g, real k, real kim, const real *q_
             real tp[6][3];
             for (int a = 0; a < 6; ++a) point_on_tri(g, test, qx[a], qy[a], tp[a]);
             for (int trial = 0; trial < T; ++trial) {
+                if (sel && !sel[test] && !sel[trial]) continue;
                 real nnx = tri_normal(g, trial, 0), nny = tri_normal(g, trial, 1),
                      nnz = tri_normal(g, trial, 2);
                 real jac = (2 * (real)g->areas[test]) * (2 * (real)g->areas[trial]);
@@ -498,7 +500,7 @@ static void assemble_regular(const hbo_geom *g, real k, real kim, const real *q_
 HBO_EXPORT void hbo_assemble_regular(const hbo_geom *g, real k, real kim, const real *q_re,
                                      const real *q_im, const real *beta_re, const real *beta_im,
                                      real *A_re, real *A_im, real *rhs_re, real *rhs_im) {
-    assemble_regular(g, k, kim, q_re, q_im, beta_re, beta_im, A_re, A_im, rhs_re, rhs_im);
+    assemble_regular(g, NULL, k, kim, q_re, q_im, beta_re, beta_im, A_re, A_im, rhs_re, rhs_im);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1232,4 +1234,32 @@ HBO_EXPORT int hbo_set_threads(int n) {
     (void)n;
     return 1;
 #endif
+}
+
+/* Contributions of every (test, trial, masks) pair that has its test or its
+ * trial triangle in sel (regular rule, mass term, Sauter-Schwab deltas),
+ * added to A and rhs: the part of the corrected system that moves with the
+ * vertices of the selected triangles (local finite differences).  Returns
+ * the number of touching pairs corrected. */
+HBO_EXPORT int64_t hbo_assemble_subset(const hbo_geom *g, const unsigned char *sel, int order,
+                                       real k, real kim, const real *q_re, const real *q_im,
+                                       const real *beta_re, const real *beta_im, real *A_re,
+                                       real *A_im, real *rhs_re, real *rhs_im) {
+    assemble_regular(g, sel, k, kim, q_re, q_im, beta_re, beta_im, A_re, A_im, rhs_re, rhs_im);
+    int64_t counts[3] = {0, 0, 0};
+    int np = hbo_duffy_pairs(g, NULL, 0, counts);
+    int32_t *pairs = (int32_t *)malloc(sizeof(int32_t) * (size_t)(np > 0 ? np : 1) * PAIR_W);
+    hbo_duffy_pairs(g, pairs, np, counts);
+    int nk = 0;
+    for (int i = 0; i < np; ++i) {
+        const int32_t *p = pairs + (size_t)i * PAIR_W;
+        if (sel[p[0]] || sel[p[1]]) {
+            if (nk != i) memmove(pairs + (size_t)nk * PAIR_W, p, sizeof(int32_t) * PAIR_W);
+            nk++;
+        }
+    }
+    apply_pair_deltas(g, pairs, nk, 1, order, k, kim, q_re, q_im, beta_re, beta_im, A_re, A_im,
+                      rhs_re, rhs_im);
+    free(pairs);
+    return nk;
 }

@@ -58,6 +58,8 @@ def lib(bits: int = 64) -> C.CDLL:
         L.hbo_kernel.argtypes = [real] * 8 + [C.c_void_p]
         L.hbo_assemble_regular.argtypes = [C.c_void_p, real, real] + [C.c_void_p] * 8
         L.hbo_apply_duffy.argtypes = [C.c_void_p, C.c_int, real, real] + [C.c_void_p] * 9
+        L.hbo_assemble_subset.argtypes = [C.c_void_p, C.c_void_p, C.c_int, real, real] + [C.c_void_p] * 8
+        L.hbo_assemble_subset.restype = C.c_int64
         L.hbo_apply_near.argtypes = [C.c_void_p, C.c_int, C.c_double, real, real] + [C.c_void_p] * 8
         L.hbo_field.argtypes = [C.c_void_p, real] + [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 3
         L.hbo_pair_block.argtypes = [C.c_void_p] + [C.c_int] * 6 + [real, real, C.c_void_p, C.c_void_p]
@@ -229,6 +231,24 @@ def assemble(geom: Geometry, k: float, q=None, *, kim: float = 0.0, beta=None,
         stats["near_pairs"] = int(L.hbo_apply_near(C.byref(g), near_level, near_threshold, k, kim, _p(qre), _p(qim),
                                                    _p(bre), _p(bim), _p(Are), _p(Aim), _p(rre), _p(rim)))
     return Are + 1j * Aim, rre + 1j * rim, stats
+
+
+def assemble_subset(geom: Geometry, sel, k: float, q=None, *, kim: float = 0.0, beta=None,
+                    duffy_order: int = 4, bits: int = 64):
+    """Part of the corrected system from pairs with the test or the trial
+    triangle selected (``sel`` boolean per triangle): the entries that move
+    with the selected triangles' vertices.  Returns (A, rhs) like assemble."""
+    d = _dt(bits)
+    n, T = geom.n_dofs, geom.n_triangles
+    Are, Aim = np.zeros((n, n), d), np.zeros((n, n), d)
+    rre, rim = np.zeros(n, d), np.zeros(n, d)
+    qre, qim = _split(q if q is not None else np.zeros(T), T, d)
+    bre, bim = _split(beta, T, d)
+    s = np.ascontiguousarray(np.asarray(sel, bool), np.uint8)
+    g = geom.c(bits)
+    lib(bits).hbo_assemble_subset(C.byref(g), _p(s), duffy_order, k, kim, _p(qre), _p(qim), _p(bre),
+                                  _p(bim), _p(Are), _p(Aim), _p(rre), _p(rim))
+    return Are + 1j * Aim, rre + 1j * rim
 
 
 def field(geom: Geometry, k: float, p, q, obs_3xn, bits: int = 64):

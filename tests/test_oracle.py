@@ -380,3 +380,26 @@ def test_robin_sphere_matches_closed_form():
         pa0 = a * q[0] / (1j * k * a - 1)
         assert abs(pa - pa0) / abs(pa0) > 0.2
     assert errs[1] < 0.02 and errs[1] < errs[0]
+
+
+def test_subset_assembly_is_the_part_that_moves_with_a_vertex():
+    """assemble_subset (pairs touching the selected triangles) changes under a
+    vertex move exactly as the full corrected assembly does: the local
+    finite differences of the shape-gradient check rest on this."""
+    b = horn_buffers(edge_length=0.04, quarter=True)
+    g0 = O.Geometry.from_vertices(np.asarray(b.vertices_3xn_f32, np.float64),
+                                  O.Geometry.from_buffers(b, "yz+xz"))
+    k = 40.0
+    q = np.where(b.physical_tags_i32 == 2, 1j * 100.0, 0.0)
+    A, r, _ = O.assemble(g0, k, q)
+    for v in (7, 33):
+        sel = np.any(g0.triangles == v, axis=0)
+        V = g0.vertices.copy()
+        V[:, v] += [1e-4, -2e-4, 3e-4]
+        g1 = O.Geometry.from_vertices(V, g0)
+        A1, r1, _ = O.assemble(g1, k, q)
+        As0, rs0 = O.assemble_subset(g0, sel, k, q)
+        As1, rs1 = O.assemble_subset(g1, sel, k, q)
+        assert np.abs((A1 - A) - (As1 - As0)).max() <= 1e-12 * np.abs(A1 - A).max()
+        assert np.abs((r1 - r) - (rs1 - rs0)).max() <= 1e-12 * np.abs(r1 - r).max()
+        assert np.abs(A1 - A).max() > 0
