@@ -73,7 +73,8 @@ class _SolveOptions(C.Structure):
         ("impedance_tag", C.c_int32),
         ("robin_beta", C.c_void_p),
         ("n_sources", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("lu_graph", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -537,6 +538,7 @@ class HipBemSession:
         impedance_sources: dict[int, complex] | list[dict[int, complex]] | None = None,
         extra_neumann_dp0: NDArray[Any] | None = None,
         extra_impedance_source_tags: list[int | None] | tuple[int | None, ...] | None = None,
+        lu_graph: bool = False,
         **_ignored: Any,
     ) -> list[DenseSolveFieldResult]:
         """Assemble, solve and evaluate every frequency; returns one result per
@@ -628,6 +630,7 @@ class HipBemSession:
         opts.tags = _i32p(tags) if tags.size else None
         opts.impedance_tag = imp_tag
         opts.n_sources = S
+        opts.lu_graph = 1 if lu_graph else 0
         beta_f = None
         if beta is not None:
             beta_f = np.ascontiguousarray(beta.astype(np.complex64)).view(np.float32)
@@ -640,18 +643,42 @@ class HipBemSession:
         pim = np.zeros((F, S, n), np.float32)
         imp = np.zeros((F, S, 2), np.float64)
         pav = np.zeros((F, S, max(1, tags.size), 2), np.float64)
-        diag = (_CaseDiag * F)()
+        # With a streaming callback the sweep runs in resident chunks and the
+        # callback sees each chunk's cases before the next chunk is solved
+        # (per-case streaming and early stop, native.py:1362-1447).
+        chunk = F if on_case_result is None else max(1, int(batch_frequencies or 8))
+        results: list[DenseSolveFieldResult] = []
+        for f0 in range(0, F, chunk):
+            f1 = min(F, f0 + chunk)
+            if not self._solve_chunk(f0, f1, k, kim, fk, qre, qim, obs, opts, beta_f, fre, fim,
+                                     pre, pim, imp, pav, tags, freqs, solve_mode, payloads, n_extra,
+                                     S, n, n_obs, user_tags, extra_impedance_source_tags,
+                                     write_surface_pressure, on_case_result, results):
+                break
+        return results
+
+    def _solve_chunk(self, f0, f1, k, kim, fk, qre, qim, obs, opts, beta_f, fre, fim, pre, pim,
+                     imp, pav, tags, freqs, solve_mode, payloads, n_extra, S, n, n_obs, user_tags,
+                     extra_impedance_source_tags, write_surface_pressure, on_case_result,
+                     results) -> bool:
+        """Solves frequencies f0..f1-1 (one library call) and appends their
+        results; False when the callback asked to stop."""
+        Fc = f1 - f0
+        if beta_f is not None:
+            opts.robin_beta = beta_f[f0:f1].ctypes.data
+        diag = (_CaseDiag * Fc)()
+        kc, kimc = np.ascontiguousarray(k[f0:f1]), np.ascontiguousarray(kim[f0:f1])
         _check(
             load_library().hbem_solve_batch(
-                self._handle, F, _f32p(k), _f32p(kim) if np.any(kim != 0) else None, _f32p(fk),
-                _f32p(qre), _f32p(qim), n_obs, _f32p(obs), C.byref(opts), _f32p(fre), _f32p(fim),
-                _f32p(pre), _f32p(pim), _f64p(imp), _f64p(pav) if tags.size else None, diag,
+                self._handle, Fc, _f32p(kc), _f32p(kimc) if np.any(kimc != 0) else None,
+                _f32p(np.ascontiguousarray(fk[f0:f1])), _f32p(qre[f0:f1]), _f32p(qim[f0:f1]), n_obs,
+                _f32p(obs), C.byref(opts), _f32p(fre[f0:f1]), _f32p(fim[f0:f1]), _f32p(pre[f0:f1]),
+                _f32p(pim[f0:f1]), _f64p(imp[f0:f1]), _f64p(pav[f0:f1]) if tags.size else None, diag,
             ),
             "assemble_solve_evaluate_standard_neumann_batch",
         )
-        results: list[DenseSolveFieldResult] = []
-        for i in range(F):
-            d = diag[i]
+        for i in range(f0, f1):
+            d = diag[i - f0]
             diagnostics: dict[str, Any] = {
                 "implementation": "hip_gfx950_resident_assembly_lu_field_batch",
                 "assembly_mode": "corrected",
@@ -721,8 +748,8 @@ class HipBemSession:
             )
             results.append(res)
             if on_case_result is not None and on_case_result(i, res) is False:
-                break
-        return results
+                return False
+        return True
 
     def solve_batch_device(
         self,
@@ -746,6 +773,7 @@ class HipBemSession:
         estimate_rcond: bool = False,
         robin_beta_ptr: int | None = None,
         n_sources: int = 1,
+        lu_graph: bool = False,
     ) -> list[dict[str, Any]]:
         """Device-resident batch (hbem_solve_batch_device): every pointer is
         a device address on the session's GPU (e.g. ``tensor.data_ptr()``).
@@ -765,6 +793,7 @@ class HipBemSession:
         opts.impedance_tag = int(impedance_source_tag)
         opts.robin_beta = robin_beta_ptr
         opts.n_sources = int(n_sources)
+        opts.lu_graph = 1 if lu_graph else 0
         diag = (_CaseDiag * int(n_freq))()
         _check(
             load_library().hbem_solve_batch_device(

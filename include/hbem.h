@@ -102,7 +102,11 @@ typedef struct hbem_solve_options {
     /* right-hand sides per frequency sharing one matrix and one LU
      * (solve_multi_source, __init__.py:205-252); 0 or 1: one */
     int32_t n_sources;
-    int32_t reserved[7];
+    /* 1: capture the padding / norm / LU / first-solve launches of a resident
+     * batch in a hipGraph and replay it for later batches of the same shape
+     * and buffers (launch-bound small systems); 0: plain stream launches */
+    int32_t lu_graph;
+    int32_t reserved[6];
 } hbem_solve_options;
 
 /* Per-frequency diagnostics: the case fields the reference returns

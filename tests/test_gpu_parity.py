@@ -443,3 +443,58 @@ def test_near_field_env_moves_close_pair_toward_fine_reference(require_gpu, monk
     near = A_near[np.ix_([0, 1, 2], [3, 4, 5])]
     assert np.linalg.norm(near - base) > 0.0
     assert np.linalg.norm(near - high) < np.linalg.norm(base - high)
+
+
+def test_streaming_callback_stops_early(require_gpu):
+    """on_case_result streams cases in order chunk by chunk; returning False
+    stops the sweep (native.py:1362-1447).  Streamed cases equal a full run."""
+    b = horn_buffers(edge_length=0.03, quarter=True)
+    freqs = np.array([500.0, 900.0, 1600.0, 2800.0, 5000.0, 9000.0])
+    k = (2 * np.pi * freqs / C0).astype(np.float32)
+    tags = b.physical_tags_i32
+    q = np.stack([np.where(tags == 2, 1j * 1.2041 * 2 * np.pi * f, 0.0) for f in freqs]).astype(np.complex64)
+    ang = np.deg2rad(np.linspace(0, 180, 7))
+    obs = np.stack([2.0 * np.sin(ang), np.zeros_like(ang), 2.0 * np.cos(ang)]).astype(np.float32)
+    seen = []
+
+    def cb(i, res):
+        seen.append(i)
+        return False if i == 2 else None
+
+    with _session(b, "yz+xz") as s:
+        full = s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, source_tags=[2],
+                                                                impedance_source_tag=2)
+        part = s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, source_tags=[2],
+                                                                impedance_source_tag=2,
+                                                                batch_frequencies=2, on_case_result=cb)
+    assert seen == [0, 1, 2]
+    assert len(part) == 3
+    for i in range(3):
+        assert part[i].frequency_hz == full[i].frequency_hz
+        assert rel_l2(part[i].field, full[i].field) < 1e-6
+        assert part[i].diagnostics["resident_batch"]["size"] == 2
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed_fp16"])
+def test_lu_graph_replays_match_eager(require_gpu, mode):
+    """The captured pad/norm/LU/solve graph is replayed for new wavenumbers
+    (two equal chunks per call, three calls) with eager calls and host
+    synchronisations in between; every replay matches the eager result."""
+    b = horn_buffers(edge_length=0.03, quarter=True)
+    tags = b.physical_tags_i32
+    ang = np.deg2rad(np.linspace(0, 180, 9))
+    obs = np.stack([2.0 * np.sin(ang), np.zeros_like(ang), 2.0 * np.cos(ang)]).astype(np.float32)
+    sets = [np.array([600.0, 1200.0, 2400.0, 4800.0, 700.0, 1500.0, 3000.0, 6000.0]),
+            np.array([550.0, 1100.0, 2200.0, 4400.0, 800.0, 1600.0, 3200.0, 6400.0])]
+    with _session(b, "yz+xz") as s:
+        for rep, freqs in enumerate(sets + sets[:1]):
+            k = (2 * np.pi * freqs / C0).astype(np.float32)
+            q = np.stack([np.where(tags == 2, 1j * 1.2041 * 2 * np.pi * f, 0.0) for f in freqs]).astype(np.complex64)
+            kw = dict(source_tags=[2], impedance_source_tag=2, batch_frequencies=4, solve_mode=mode,
+                      refine_tolerance=1e-7)
+            g = s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, lu_graph=True, **kw)
+            e = s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, lu_graph=False, **kw)
+            for i in range(freqs.size):
+                assert rel_l2(g[i].pressure, e[i].pressure) < 1e-5, (rep, i)
+                assert rel_l2(g[i].field, e[i].field) < 1e-5, (rep, i)
+                assert g[i].lapack_info == 0
