@@ -167,6 +167,14 @@ def load_library() -> C.CDLL:
             raise AssemblyBackendUnavailable(
                 f"libhbem is not built ({path} missing); run __graft_entry__.build() or make"
             )
+        # One HIP runtime per process: when PyTorch is installed its bundled
+        # libamdhip64 is loaded first, so libhbem binds to it and device
+        # pointers move between torch tensors and the library.  Loading the
+        # system runtime first would leave torch unable to initialise the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(str(path))
         lib.hbem_abi_version.restype = C.c_int32
         lib.hbem_last_error.restype = C.c_char_p

@@ -8,6 +8,13 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+# PyTorch's HIP runtime must be the process's runtime before libhbem loads
+# (the library then binds to it and shares device pointers with tensors).
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = Path(__file__).resolve().parents[1]
 PKG = ROOT / "hornlab-metal-bem_amd"
 for p in (str(PKG), str(ROOT / "oracle"), str(ROOT)):
