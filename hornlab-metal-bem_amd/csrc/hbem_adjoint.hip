@@ -266,12 +266,12 @@ __global__ __launch_bounds__(256) void k_residual_h_f64(const float *__restrict_
                                                         const double *__restrict__ e,
                                                         const double *__restrict__ x, int64_t ldx,
                                                         double *__restrict__ r,
-                                                        double *__restrict__ norms) {
+                                                        double *__restrict__ norms, int vpm) {
     const int lane = threadIdx.x & 63;
     const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int b = blockIdx.y;
+    const int b = blockIdx.y;  // vector; its matrix is b / vpm
     if (j >= n) return;
-    const float *col = A + ((size_t)b * a_stride + (size_t)j * lda) * 2;
+    const float *col = A + ((size_t)(b / vpm) * a_stride + (size_t)j * lda) * 2;
     const double *xb = x + (size_t)b * ldx * 2;
     double sr = 0.0, si = 0.0;
     for (int i = lane; i < n; i += 64) {
@@ -744,10 +744,10 @@ void launch_adjoint_rhs(const DeviceGeometry &g, int n_batch, const float *k_rea
 
 void launch_residual_h_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,
                            const double *e, const double *x, int64_t ldx, double *r,
-                           double *norms, hipStream_t stream) {
+                           double *norms, hipStream_t stream, int vpm) {
     launch_fill_zero_f64(norms, (size_t)2 * B, stream);
     hipLaunchKernelGGL(k_residual_h_f64, dim3((n + 3) / 4, B), dim3(256), 0, stream, A, lda,
-                       a_stride, n, e, x, ldx, r, norms);
+                       a_stride, n, e, x, ldx, r, norms, vpm);
 }
 
 void launch_assembly_adjoint(const DeviceGeometry &g, int n_batch, const float *k_real,

@@ -137,6 +137,11 @@ struct hbem_session {
     DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
         tile_count, tile_ndofs, tile_dofs, tile_pairs, nc_f, nc_i, nc_rule, sg_adj, vtx_d, triv_d,
         normals_d, jac_d;
+    // CHIEF rows and solves of the current resident batch
+    DevBuf chief_C, chief_d, chief_pts, chief_z, chief_e, chief_z64, chief_r, chief_ainf;
+    const float *chief_pts_d = nullptr;
+    std::vector<float> chief_b;       // original right-hand sides of the batch
+    std::vector<double> chief_scale;  // s per matrix
     // adjoint store (hbem_adjoint_forward -> hbem_adjoint_backward)
     DevBuf adj_k, adj_kim, adj_fk, adj_q, adj_beta, adj_obs, adj_c, adj_ft, adj_sig, adj_gN,
         adj_l32, adj_e64, adj_l64, adj_r64, adj_act;
@@ -161,7 +166,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &chief_C, &chief_d, &chief_pts, &chief_z, &chief_e, &chief_z64, &chief_r, &chief_ainf, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag, &io_beta, &qt, &l2g})
@@ -784,14 +789,203 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int vpm, i
 // s->rhs (vector v against matrix v / S).  Solutions (FP32, interleaved,
 // stride N) end in s->x.  Fills per-matrix diagnostics (refinement figures
 // are the worst over the matrix's right-hand sides).
+// CHIEF least squares on the resident batch (solveDenseLeastSquaresZgels,
+// main.swift:7220-7416, solved differently): with the square system's FP32
+// LU and FP64-refined solves, min |A p - b|^2 + |s (C p - d)|^2 has the
+// solution p = A^{-1} y with
+//   Z = A^{-H} C^H (m refined transposed solves per matrix),
+//   y = b - s Z (I + s^2 Z^H Z)^{-1} s (C p0 - d),   p0 = A^{-1} b
+// (the normal equations in y = A p, reduced by the push-through identity to
+// an m x m Hermitian system).  Writes y into s->rhs; the caller solves
+// A p = y again.  chief_res is left for chief_residuals.
+namespace {
+using cdbl = std::complex<double>;
+
+// Gaussian elimination with partial pivoting, m x m complex (row-major),
+// one right-hand side.
+void small_solve(std::vector<cdbl> K, std::vector<cdbl> &x, int m) {
+    for (int c = 0; c < m; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < m; ++r)
+            if (std::abs(K[(size_t)r * m + c]) > std::abs(K[(size_t)piv * m + c])) piv = r;
+        if (piv != c) {
+            for (int j = 0; j < m; ++j) std::swap(K[(size_t)c * m + j], K[(size_t)piv * m + j]);
+            std::swap(x[c], x[piv]);
+        }
+        for (int r = c + 1; r < m; ++r) {
+            const cdbl f = K[(size_t)r * m + c] / K[(size_t)c * m + c];
+            for (int j = c; j < m; ++j) K[(size_t)r * m + j] -= f * K[(size_t)c * m + j];
+            x[r] -= f * x[c];
+        }
+    }
+    for (int r = m - 1; r >= 0; --r) {
+        cdbl acc = x[r];
+        for (int j = r + 1; j < m; ++j) acc -= K[(size_t)r * m + j] * x[j];
+        x[r] = acc / K[(size_t)r * m + r];
+    }
+}
+}  // namespace
+
+void chief_solve(hbem_session *s, const hbem::LuWorkspace &w, int B, int S, int n, int N,
+                 const hbem_solve_options &opt, std::vector<double> *chief_res) {
+    hipStream_t st = s->stream;
+    const int m = opt.n_chief, V = B * S, BM = B * m;
+    const int64_t lda = N, a_stride = (int64_t)N * N;
+    const double wgt = opt.chief_weight > 0.0 ? opt.chief_weight : 1.0;
+    std::vector<float> Ch((size_t)2 * BM * n), dh((size_t)2 * V * m);
+    std::vector<double> ainf(B);
+    s->chief_ainf.reserve(sizeof(double) * B);
+    hbem::launch_row_abs_sum(s->A_orig.as<float>(), lda, a_stride, n, B, s->chief_ainf.as<double>(),
+                             st);
+    hip_check(hipMemcpyAsync(Ch.data(), s->chief_C.ptr, sizeof(float) * Ch.size(),
+                             hipMemcpyDeviceToHost, st), "C D2H");
+    hip_check(hipMemcpyAsync(dh.data(), s->chief_d.ptr, sizeof(float) * dh.size(),
+                             hipMemcpyDeviceToHost, st), "d D2H");
+    hip_check(hipMemcpyAsync(ainf.data(), s->chief_ainf.ptr, sizeof(double) * B,
+                             hipMemcpyDeviceToHost, st), "norm D2H");
+    hip_check(hipStreamSynchronize(st), "chief sync");
+    // Z = A^{-H} C^H: right-hand sides conj(C) rows, float for the LU and
+    // double for the refinement target
+    std::vector<float> zr((size_t)2 * BM * N, 0.0f);
+    std::vector<double> ze((size_t)2 * BM * N, 0.0);
+    for (int bm = 0; bm < BM; ++bm)
+        for (int j = 0; j < n; ++j) {
+            const float re = Ch[((size_t)bm * n + j) * 2], im = Ch[((size_t)bm * n + j) * 2 + 1];
+            zr[((size_t)bm * N + j) * 2] = re;
+            zr[((size_t)bm * N + j) * 2 + 1] = -im;
+            ze[((size_t)bm * N + j) * 2] = re;
+            ze[((size_t)bm * N + j) * 2 + 1] = -im;
+        }
+    upload(s->chief_z, zr, st);
+    upload(s->chief_e, ze, st);
+    s->chief_z64.reserve(sizeof(double) * ze.size());
+    s->chief_r.reserve(sizeof(double) * ze.size());
+    s->norms.reserve(sizeof(double) * 2 * std::max(BM, V));
+    hbem::lu_solve(w, s->chief_z.as<float>(), N, 2, st, m);
+    hbem::launch_fill_zero_f64(s->chief_z64.as<double>(), ze.size(), st);
+    hbem::launch_axpy_f32_to_f64(s->chief_z.as<float>(), s->chief_z64.as<double>(), N, N, n, BM,
+                                 nullptr, st);
+    for (int it = 0; it < 3; ++it) {
+        hbem::launch_residual_h_f64(s->A_orig.as<float>(), lda, a_stride, n, BM,
+                                    s->chief_e.as<double>(), s->chief_z64.as<double>(), N,
+                                    s->chief_r.as<double>(), s->norms.as<double>(), st, m);
+        hbem::launch_fill_zero(s->chief_z.as<float>(), zr.size(), st);
+        hbem::launch_copy_f64_to_f32(s->chief_r.as<double>(), s->chief_z.as<float>(), N, N, n, BM,
+                                     st);
+        hbem::lu_solve(w, s->chief_z.as<float>(), N, 2, st, m);
+        hbem::launch_axpy_f32_to_f64(s->chief_z.as<float>(), s->chief_z64.as<double>(), N, N, n,
+                                     BM, nullptr, st);
+    }
+    std::vector<double> Z(ze.size()), x0((size_t)2 * V * N);
+    std::vector<float> bh((size_t)2 * V * N);
+    hip_check(hipMemcpyAsync(Z.data(), s->chief_z64.ptr, sizeof(double) * Z.size(),
+                             hipMemcpyDeviceToHost, st), "Z D2H");
+    hip_check(hipMemcpyAsync(x0.data(), s->x64.ptr, sizeof(double) * x0.size(),
+                             hipMemcpyDeviceToHost, st), "x D2H");
+    hip_check(hipMemcpyAsync(bh.data(), s->rhs.ptr, sizeof(float) * bh.size(),
+                             hipMemcpyDeviceToHost, st), "rhs D2H");
+    hip_check(hipStreamSynchronize(st), "chief sync");
+    s->chief_b = bh;
+    s->chief_scale.assign(B, 1.0);
+    std::vector<float> y(bh);
+    for (int b = 0; b < B; ++b) {
+        double cinf = 0.0;
+        for (int i = 0; i < m; ++i) {
+            double row = 0.0;
+            for (int j = 0; j < n; ++j) {
+                const size_t o = (((size_t)b * m + i) * n + j) * 2;
+                row += std::hypot((double)Ch[o], (double)Ch[o + 1]);
+            }
+            cinf = std::max(cinf, row);
+        }
+        const double sc = wgt * (cinf > 0.0 ? ainf[b] / cinf : 1.0);
+        s->chief_scale[b] = sc;
+        auto zat = [&](int i, int j) {
+            const size_t o = (((size_t)b * m + i) * N + j) * 2;
+            return cdbl(Z[o], Z[o + 1]);
+        };
+        // K = I + s^2 Z^H Z
+        std::vector<cdbl> K((size_t)m * m);
+        for (int i = 0; i < m; ++i)
+            for (int k = 0; k < m; ++k) {
+                cdbl acc = 0.0;
+                for (int j = 0; j < n; ++j) acc += std::conj(zat(i, j)) * zat(k, j);
+                K[(size_t)i * m + k] = sc * sc * acc + (i == k ? 1.0 : 0.0);
+            }
+        for (int src = 0; src < S; ++src) {
+            const int v = b * S + src;
+            std::vector<cdbl> r(m);
+            for (int i = 0; i < m; ++i) {
+                cdbl acc = 0.0;
+                for (int j = 0; j < n; ++j) {
+                    const size_t oc = (((size_t)b * m + i) * n + j) * 2;
+                    const size_t ox = ((size_t)v * N + j) * 2;
+                    acc += cdbl(Ch[oc], Ch[oc + 1]) * cdbl(x0[ox], x0[ox + 1]);
+                }
+                const size_t od = ((size_t)v * m + i) * 2;
+                r[i] = sc * (acc - cdbl(dh[od], dh[od + 1]));
+            }
+            small_solve(K, r, m);
+            for (int j = 0; j < n; ++j) {
+                cdbl acc = 0.0;
+                for (int i = 0; i < m; ++i) acc += zat(i, j) * r[i];
+                const size_t o = ((size_t)v * N + j) * 2;
+                y[o] = (float)((double)bh[o] - sc * acc.real());
+                y[o + 1] = (float)((double)bh[o + 1] - sc * acc.imag());
+            }
+        }
+    }
+    hip_check(hipMemcpyAsync(s->rhs.ptr, y.data(), sizeof(float) * y.size(), hipMemcpyHostToDevice,
+                             st), "rhs H2D");
+    hip_check(hipStreamSynchronize(st), "chief sync");
+    (void)chief_res;
+}
+
+// |s (C p - d)|_2 / |b|_2 per vector with the final solution (s->x64).
+void chief_residuals(hbem_session *s, int B, int S, int n, int N, const hbem_solve_options &opt,
+                     std::vector<double> *chief_res) {
+    hipStream_t st = s->stream;
+    const int m = opt.n_chief, V = B * S;
+    std::vector<float> Ch((size_t)2 * B * m * n), dh((size_t)2 * V * m);
+    std::vector<double> x((size_t)2 * V * N);
+    hip_check(hipMemcpyAsync(Ch.data(), s->chief_C.ptr, sizeof(float) * Ch.size(),
+                             hipMemcpyDeviceToHost, st), "C D2H");
+    hip_check(hipMemcpyAsync(dh.data(), s->chief_d.ptr, sizeof(float) * dh.size(),
+                             hipMemcpyDeviceToHost, st), "d D2H");
+    hip_check(hipMemcpyAsync(x.data(), s->x64.ptr, sizeof(double) * x.size(),
+                             hipMemcpyDeviceToHost, st), "x D2H");
+    hip_check(hipStreamSynchronize(st), "chief sync");
+    for (int v = 0; v < V; ++v) {
+        const int b = v / S;
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < m; ++i) {
+            cdbl acc = 0.0;
+            for (int j = 0; j < n; ++j) {
+                const size_t oc = (((size_t)b * m + i) * n + j) * 2;
+                const size_t ox = ((size_t)v * N + j) * 2;
+                acc += cdbl(Ch[oc], Ch[oc + 1]) * cdbl(x[ox], x[ox + 1]);
+            }
+            const size_t od = ((size_t)v * m + i) * 2;
+            num += std::norm(s->chief_scale[b] * (acc - cdbl(dh[od], dh[od + 1])));
+        }
+        for (int j = 0; j < n; ++j) {
+            const size_t o = ((size_t)v * N + j) * 2;
+            den += (double)s->chief_b[o] * s->chief_b[o] + (double)s->chief_b[o + 1] * s->chief_b[o + 1];
+        }
+        (*chief_res)[v] = den > 0.0 ? std::sqrt(num / den) : std::sqrt(num);
+    }
+}
+
 void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solve_options &opt,
                     std::vector<hbem_case_diag> *diag, double *solve_seconds) {
     hipStream_t st = s->stream;
     const int64_t lda = N, a_stride = (int64_t)N * N;
     const int V = B * S;
+    // CHIEF solves in the FP64-refined mode, as the reference's least squares
+    // runs in FP64 (sweep.py:123)
     const bool refine = opt.solve_mode == HBEM_SOLVE_F32_IR ||
                         opt.solve_mode == HBEM_SOLVE_MIXED_F16 ||
-                        opt.solve_mode == HBEM_SOLVE_MIXED_BF16;
+                        opt.solve_mode == HBEM_SOLVE_MIXED_BF16 || opt.n_chief > 0;
     const int precision = opt.solve_mode == HBEM_SOLVE_MIXED_F16    ? 1
                           : opt.solve_mode == HBEM_SOLVE_MIXED_BF16 ? 2
                                                                     : 0;
@@ -874,6 +1068,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
     hip_check(hipStreamSynchronize(st), "solve sync");
     std::vector<int32_t> iters(V, 0);
     std::vector<double> resid(V, -1.0);
+    auto refine_all = [&]() {
     if (refine && opt.refine_max_iterations > 0) {
         s->x64.reserve(sizeof(double) * 2 * (size_t)V * N);
         s->r64.reserve(sizeof(double) * 2 * (size_t)V * N);
@@ -943,6 +1138,18 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         hbem::launch_copy_f64_to_f32(s->x64.as<double>(), s->x.as<float>(), N, N, n, V, st);
         for (int v = 0; v < V; ++v) resid[v] = best[v];
     }
+    };
+    refine_all();
+    std::vector<double> chief_res(V, -1.0);
+    if (opt.n_chief > 0) {
+        chief_solve(s, w, B, S, n, N, opt, &chief_res);
+        hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)V * N,
+                                 hipMemcpyDeviceToDevice, st),
+                  "rhs copy");
+        hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
+        refine_all();
+        chief_residuals(s, B, S, n, N, opt, &chief_res);
+    }
     std::vector<double> rcond(B, -1.0);
     if (opt.estimate_rcond) estimate_rcond(s, w, anorm, &rcond);
     t.stop(st);
@@ -959,6 +1166,9 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         }
         d.dense_solve_condition_1norm = anorm[b];
         d.dense_solve_rcond = rcond[b];
+        d.chief_residual_rel = -1.0;
+        for (int src = 0; src < S; ++src)
+            d.chief_residual_rel = std::max(d.chief_residual_rel, chief_res[(size_t)b * S + src]);
     }
 }
 
@@ -982,6 +1192,14 @@ void check_options(const hbem_solve_options &o) {
     if (!(o.refine_tolerance >= 0.0)) fail(HBEM_EINVAL, "refine_tolerance must be >= 0");
     if (o.n_tags < 0 || (o.n_tags > 0 && !o.tags)) fail(HBEM_EINVAL, "tags invalid");
     if (o.n_sources < 0 || o.n_sources > 4096) fail(HBEM_EINVAL, "n_sources out of range");
+    if (o.n_chief < 0 || o.n_chief > 4096) fail(HBEM_EINVAL, "n_chief out of range");
+    if (o.n_chief > 0) {
+        if (!o.chief_points) fail(HBEM_EINVAL, "chief_points must be given with n_chief > 0");
+        if (!(std::isfinite(o.chief_weight) && o.chief_weight >= 0.0))
+            fail(HBEM_EINVAL, "chief_weight must be finite and positive");
+        if (o.refine_max_iterations <= 0)
+            fail(HBEM_EINVAL, "CHIEF needs refine_max_iterations > 0 (FP64-refined solves)");
+    }
 }
 
 // All-device batch pipeline.  Inputs: k_real/field_k [F], k_imag [F] or
@@ -1017,6 +1235,16 @@ void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
         ta.start(st);
         assemble_device(s, B, S, kre_d + f0, kim_d ? kim_d + f0 : nullptr, qc, bc, lda, a_stride,
                         N, &treg, &tsing, &tnear);
+        if (opts.n_chief > 0) {
+            const int m = opts.n_chief;
+            s->chief_C.reserve(sizeof(float) * 2 * (size_t)B * m * n);
+            s->chief_d.reserve(sizeof(float) * 2 * (size_t)V * m);
+            hbem::launch_fill_zero(s->chief_C.as<float>(), (size_t)2 * B * m * n, st);
+            hbem::launch_fill_zero(s->chief_d.as<float>(), (size_t)2 * V * m, st);
+            hbem::launch_chief_rows(s->dg, B, m, s->chief_pts_d, kre_d + f0,
+                                    kim_d ? kim_d + f0 : nullptr, S, qc, bc, s->chief_C.as<float>(),
+                                    s->chief_d.as<float>(), st);
+        }
         ta.stop(st);
         std::vector<hbem_case_diag> dg;
         double solve_s = 0.0;
@@ -1111,6 +1339,12 @@ int32_t hbem_solve_batch(hbem_session *s, int32_t F, const float *k_real, const 
         const bool want_p = p_re && p_im;
         if (want_p) s->io_p.reserve(sizeof(float) * 2 * FS * n);
         std::vector<hbem_case_diag> dg;
+        if (opts->n_chief > 0) {
+            upload(s->chief_pts, std::vector<float>(opts->chief_points,
+                                                    opts->chief_points + 3 * (size_t)opts->n_chief),
+                   st);
+            s->chief_pts_d = s->chief_pts.as<float>();
+        }
         solve_core(s, F, s->io_k.as<float>(), complex_k ? s->io_kim.as<float>() : nullptr,
                    s->io_fk.as<float>(), s->io_q.as<float>(),
                    robin ? s->io_beta.as<float>() : nullptr, n_obs, s->io_obs.as<float>(), *opts,
@@ -1179,6 +1413,7 @@ int32_t hbem_solve_batch_device(hbem_session *s, int32_t F, const float *k_real,
         s->adj_valid = false;
         set_device(s);
         std::vector<hbem_case_diag> dg;
+        s->chief_pts_d = opts->chief_points;
         solve_core(s, F, k_real, k_imag, field_k, q, opts->robin_beta, n_obs, obs_3xn, *opts,
                    field, p, tag_avg, &dg);
         hip_check(hipStreamSynchronize(s->stream), "batch sync");

@@ -255,10 +255,20 @@ void launch_field_adjoint(const DeviceGeometry &g, int n_batch, const float *fie
 void launch_adjoint_rhs(const DeviceGeometry &g, int n_batch, const float *k_real,
                         const float *k_imag, const float *beta, const double *ft, double *c,
                         int64_t ld, int n, int N, float *e32, double *e64, hipStream_t stream);
-// r = e - A^H x in FP64, norms[2 b] = max|r|, norms[2 b + 1] = max|e|.
+// r = e - A^H x in FP64 for B vectors (vector v uses matrix v / vpm),
+// norms[2 v] = max|r|, norms[2 v + 1] = max|e|.
 void launch_residual_h_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,
                            const double *e, const double *x, int64_t ldx, double *r,
-                           double *norms, hipStream_t stream);
+                           double *norms, hipStream_t stream, int vpm = 1);
+
+// CHIEF rows C [B][m][n] and d [B][n_src][m] (interleaved complex float,
+// accumulated: zero them first) of m interior points pts [3][m].
+void launch_chief_rows(const DeviceGeometry &g, int n_batch, int m, const float *pts,
+                       const float *k_real, const float *k_imag, int n_src, const float *q,
+                       const float *beta, float *C, float *d, hipStream_t stream);
+// ||A||_inf per matrix (FP64).
+void launch_row_abs_sum(const float *A, int64_t lda, int64_t a_stride, int n, int B, double *out,
+                        hipStream_t stream);
 // Regular and singular pair derivatives of Re(lambda^H (V qt - K p)).
 void launch_assembly_adjoint(const DeviceGeometry &g, int n_batch, const float *k_real,
                              const float *k_imag, const float *lam, const float *p, int64_t ld,

@@ -24,7 +24,7 @@ from numpy.typing import NDArray
 from .backends import AssemblyBackendUnavailable
 from .geometry import SYMMETRY_CODES, GeometryBuffers, validate_native_symmetry_plane
 
-HBEM_ABI_VERSION = 3
+HBEM_ABI_VERSION = 4
 _LIB_NAME = "libhbem.so"
 
 SOLVE_MODES = {"fp32": 0, "fp32_ir": 1, "mixed_fp16": 2, "mixed_bf16": 3}
@@ -74,7 +74,11 @@ class _SolveOptions(C.Structure):
         ("robin_beta", C.c_void_p),
         ("n_sources", C.c_int32),
         ("lu_graph", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("chief_points", C.c_void_p),
+        ("n_chief", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("chief_weight", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -94,6 +98,7 @@ class _CaseDiag(C.Structure):
         ("batch_size", C.c_int32),
         ("near_assembly_s", C.c_double),
         ("near_pairs", C.c_int64),
+        ("chief_residual_rel", C.c_double),
     ]
 
 
@@ -547,6 +552,8 @@ class HipBemSession:
         extra_neumann_dp0: NDArray[Any] | None = None,
         extra_impedance_source_tags: list[int | None] | tuple[int | None, ...] | None = None,
         lu_graph: bool = False,
+        chief_points: NDArray[Any] | None = None,
+        chief_weight: float = 1.0,
         **_ignored: Any,
     ) -> list[DenseSolveFieldResult]:
         """Assemble, solve and evaluate every frequency; returns one result per
@@ -639,6 +646,24 @@ class HipBemSession:
         opts.impedance_tag = imp_tag
         opts.n_sources = S
         opts.lu_graph = 1 if lu_graph else 0
+        chief = None
+        if chief_points is not None:
+            chief = np.asarray(chief_points, dtype=np.float32)
+            if chief.ndim != 2 or chief.shape[0] != 3:
+                raise ValueError("chief_points must have shape (3, m)")
+            if chief.shape[1] == 0:
+                raise ValueError("chief_points must be non-empty when set")
+            if not np.all(np.isfinite(chief)):
+                raise ValueError("chief_points must be finite")
+            if not (np.isfinite(chief_weight) and float(chief_weight) > 0):
+                raise ValueError("chief_weight must be finite and positive")
+            chief = np.ascontiguousarray(chief)
+            opts.chief_points = chief.ctypes.data
+            opts.n_chief = int(chief.shape[1])
+            opts.chief_weight = float(chief_weight)
+            if solve_mode == "fp32":
+                solve_mode = "fp32_ir"
+                opts.solve_mode = SOLVE_MODES[solve_mode]
         beta_f = None
         if beta is not None:
             beta_f = np.ascontiguousarray(beta.astype(np.complex64)).view(np.float32)
@@ -710,6 +735,10 @@ class HipBemSession:
             if near is not None:
                 diagnostics["near_quadrature"] = near
             diagnostics["resident_batch"] = {"index": int(d.batch_index), "size": int(d.batch_size)}
+            if opts.n_chief > 0:
+                diagnostics["chief_points"] = True
+                diagnostics["chief_points_count"] = int(opts.n_chief)
+                diagnostics["chief_residual_rel"] = float(d.chief_residual_rel)
             if d.dense_solve_rcond >= 0:
                 diagnostics["dense_solve_rcond"] = float(d.dense_solve_rcond)
             if d.refine_residual_rel >= 0:

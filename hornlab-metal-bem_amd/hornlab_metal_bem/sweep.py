@@ -40,8 +40,6 @@ def should_route_native_metal(config: SolveConfig) -> bool:
             "native_symmetry_plane must be None or one of "
             + ", ".join(repr(p) for p in NATIVE_SYMMETRY_PLANES)
         )
-    if config.chief_points is not None:
-        raise AssemblyBackendUnavailable("chief_points (CHIEF least squares) is not built on the HIP path")
     if config.aperture_tag is not None:
         raise AssemblyBackendUnavailable("aperture_tag (coupled infinite baffle) is not built on the HIP path")
     return True
@@ -271,6 +269,9 @@ def _native_sweep(
                 impedance_sources=imp_sl if imp_sl else None,
                 extra_neumann_dp0=np.stack([q[sl] for q in q_rows[1:]], axis=0) if S > 1 else None,
                 extra_impedance_source_tags=imp_tags[1:] if S > 1 else None,
+                chief_points=(None if config.chief_points is None else
+                              np.asarray(config.chief_points, np.float64).reshape(-1, 3).T),
+                chief_weight=config.chief_weight,
             )
             for j, r in enumerate(results):
                 i = c0 + j

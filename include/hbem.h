@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HBEM_ABI_VERSION 3
+#define HBEM_ABI_VERSION 4
 
 #define HBEM_OK 0
 #define HBEM_EINVAL -1      /* bad argument / contract violation */
@@ -106,7 +106,17 @@ typedef struct hbem_solve_options {
      * batch in a hipGraph and replay it for later batches of the same shape
      * and buffers (launch-bound small systems); 0: plain stream launches */
     int32_t lu_graph;
-    int32_t reserved[6];
+    /* CHIEF interior points (chief_points, config.py:212-217): [3][n_chief]
+     * float (host memory for hbem_solve_batch, device memory for
+     * hbem_solve_batch_device) or NULL.  Each frequency is then solved as the
+     * least-squares problem min |A p - b|^2 + |s (C p - d)|^2 with
+     * s = chief_weight |A|_inf / |C|_inf (solveDenseLeastSquaresZgels,
+     * main.swift:7220-7416), with FP64-residual refinement. */
+    const float *chief_points;
+    int32_t n_chief;
+    int32_t reserved0;
+    double chief_weight;  /* > 0 (0 means 1) */
+    int32_t reserved[4];
 } hbem_solve_options;
 
 /* Per-frequency diagnostics: the case fields the reference returns
@@ -126,6 +136,7 @@ typedef struct hbem_case_diag {
     int32_t batch_size;                  /* cases in that chunk (times are chunk / size) */
     double near_assembly_s;              /* near-field kernel alone (0 when off) */
     int64_t near_pairs;                  /* near (test image, trial image) pairs corrected */
+    double chief_residual_rel;           /* max over sources |s (C p - d)|_2 / |b|_2; -1 without CHIEF */
 } hbem_case_diag;
 
 /* ABI version of the loaded library. */

@@ -1197,6 +1197,67 @@ HBO_EXPORT void hbo_field(const hbo_geom *g, real k, const real *p_re, const rea
     }
 }
 
+/* CHIEF rows: MS:1930-2036 (assembleChiefRows).  For interior point i and
+ * P1 column j: C[i][j] = sum over triangles and images of the 6-point
+ * integral of dG/dn_y (real k) times the basis, minus the Robin fold
+ * (i k beta / 3) int G; d[s][i] = sum_t int G * q_s[t].  C is row-major m x n
+ * complex, d is [n_src][m] complex. */
+HBO_EXPORT void hbo_chief_rows(const hbo_geom *g, int m, const float *pts3xm, real k, real kim,
+                               int n_src, const real *q_re, const real *q_im, const real *beta_re,
+                               const real *beta_im, real *C_re, real *C_im, real *d_re,
+                               real *d_im) {
+    int T = g->n_triangles, n = g->n_dofs;
+    int masks[3];
+    int nm = image_masks(g->symmetry, masks);
+    real qx[6], qy[6], qw[6];
+    hbo_rule6(qx, qy, qw);
+    cx iK = cx_make(-kim, k);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ci = 0; ci < m; ++ci) {
+        real ox = (real)pts3xm[ci], oy = (real)pts3xm[m + ci], oz = (real)pts3xm[2 * m + ci];
+        for (int tri = 0; tri < T; ++tri) {
+            real nrm[3] = {tri_normal(g, tri, 0), tri_normal(g, tri, 1), tri_normal(g, tri, 2)};
+            real jac = 2 * (real)g->areas[tri];
+            cx dcoef[3] = {cx_make(0, 0), cx_make(0, 0), cx_make(0, 0)};
+            cx gsum = cx_make(0, 0);
+            for (int a = 0; a < 6; ++a) {
+                real sp[3];
+                point_on_tri(g, tri, qx[a], qy[a], sp);
+                real b[3] = {1 - qx[a] - qy[a], qx[a], qy[a]};
+                real w = qw[a] * jac;
+                for (int mi = -1; mi < nm; ++mi) {
+                    real ip[3] = {sp[0], sp[1], sp[2]};
+                    real in[3] = {nrm[0], nrm[1], nrm[2]};
+                    if (mi >= 0) {
+                        mirror3(ip, masks[mi]);
+                        mirror3(in, masks[mi]);
+                    }
+                    cx gg, dd;
+                    kern_g_dlp(ip[0] - ox, ip[1] - oy, ip[2] - oz, in[0], in[1], in[2], k, 0, &gg,
+                               &dd);
+                    for (int l = 0; l < 3; ++l) dcoef[l] = cx_add(dcoef[l], cx_scale(dd, b[l] * w));
+                    gsum = cx_add(gsum, cx_scale(gg, w));
+                }
+            }
+            for (int s = 0; s < n_src; ++s) {
+                cx q = cx_make(q_re[(size_t)s * T + tri], q_im[(size_t)s * T + tri]);
+                cx v = cx_mul(gsum, q);
+                d_re[(size_t)s * m + ci] += v.re;
+                d_im[(size_t)s * m + ci] += v.im;
+            }
+            cx beta = cx_make(beta_re ? beta_re[tri] : 0, beta_im ? beta_im[tri] : 0);
+            int has_robin = beta.re != 0 || beta.im != 0;
+            cx fold = cx_scale(cx_mul(gsum, cx_mul(iK, beta)), (real)(1.0 / 3.0));
+            for (int l = 0; l < 3; ++l) {
+                int col = p1_dof(g, tri, l);
+                cx term = has_robin ? cx_sub(dcoef[l], fold) : dcoef[l];
+                C_re[(size_t)ci * n + col] += term.re;
+                C_im[(size_t)ci * n + col] += term.im;
+            }
+        }
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Surface reductions: MS:807-841 (averageSurfacePressureForTag)           */
 /* ---------------------------------------------------------------------- */

@@ -60,6 +60,7 @@ def lib(bits: int = 64) -> C.CDLL:
         L.hbo_apply_duffy.argtypes = [C.c_void_p, C.c_int, real, real] + [C.c_void_p] * 9
         L.hbo_assemble_subset.argtypes = [C.c_void_p, C.c_void_p, C.c_int, real, real] + [C.c_void_p] * 8
         L.hbo_assemble_subset.restype = C.c_int64
+        L.hbo_chief_rows.argtypes = [C.c_void_p, C.c_int, C.c_void_p, real, real, C.c_int] + [C.c_void_p] * 8
         L.hbo_apply_near.argtypes = [C.c_void_p, C.c_int, C.c_double, real, real] + [C.c_void_p] * 8
         L.hbo_field.argtypes = [C.c_void_p, real] + [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 3
         L.hbo_pair_block.argtypes = [C.c_void_p] + [C.c_int] * 6 + [real, real, C.c_void_p, C.c_void_p]
@@ -249,6 +250,40 @@ def assemble_subset(geom: Geometry, sel, k: float, q=None, *, kim: float = 0.0, 
     lib(bits).hbo_assemble_subset(C.byref(g), _p(s), duffy_order, k, kim, _p(qre), _p(qim), _p(bre),
                                   _p(bim), _p(Are), _p(Aim), _p(rre), _p(rim))
     return Are + 1j * Aim, rre + 1j * rim
+
+
+def chief_rows(geom: Geometry, pts_3xm, k: float, q, *, kim: float = 0.0, beta=None, bits: int = 64):
+    """CHIEF interior rows (C (m, n), d (n_src, m)); q is (T,) or (n_src, T)."""
+    d_ = _dt(bits)
+    n, T = geom.n_dofs, geom.n_triangles
+    pts = np.ascontiguousarray(pts_3xm, np.float32)
+    m = int(pts.shape[1])
+    qq = np.atleast_2d(np.asarray(q, np.complex128))
+    ns = qq.shape[0]
+    qre, qim = np.ascontiguousarray(qq.real, d_), np.ascontiguousarray(qq.imag, d_)
+    bre, bim = _split(beta, T, d_)
+    Cre, Cim = np.zeros((m, n), d_), np.zeros((m, n), d_)
+    dre, dim = np.zeros((ns, m), d_), np.zeros((ns, m), d_)
+    g = geom.c(bits)
+    lib(bits).hbo_chief_rows(C.byref(g), m, _p(pts), k, kim, ns, _p(qre), _p(qim), _p(bre), _p(bim),
+                             _p(Cre), _p(Cim), _p(dre), _p(dim))
+    return Cre + 1j * Cim, dre + 1j * dim
+
+
+def chief_solve(A, rhs, Cm, d, weight: float = 1.0):
+    """Least squares min ||A x - b||^2 + ||s (C x - d)||^2 in FP64 with
+    s = weight ||A||_inf / ||C||_inf (solveDenseLeastSquaresZgels,
+    MS:7220-7416); rhs (n,) or (n_src, n), d (m,) or (n_src, m).  Returns the
+    solutions and the CHIEF residuals ||s (C x - d)||_2 / ||b||_2."""
+    a_inf = np.abs(A).sum(axis=1).max()
+    c_inf = np.abs(Cm).sum(axis=1).max()
+    s = weight * (a_inf / c_inf if c_inf > 0 else 1.0)
+    B = np.atleast_2d(rhs)
+    D = np.atleast_2d(d)
+    M = np.vstack([A, s * Cm])
+    X = np.linalg.lstsq(M, np.vstack([B.T, s * D.T]), rcond=None)[0].T
+    res = np.array([np.linalg.norm(s * (Cm @ x - dd)) / np.linalg.norm(b) for x, dd, b in zip(X, D, B)])
+    return (X[0] if np.ndim(rhs) == 1 else X), res
 
 
 def field(geom: Geometry, k: float, p, q, obs_3xn, bits: int = 64):

@@ -498,3 +498,38 @@ def test_lu_graph_replays_match_eager(require_gpu, mode):
                 assert rel_l2(g[i].pressure, e[i].pressure) < 1e-5, (rep, i)
                 assert rel_l2(g[i].field, e[i].field) < 1e-5, (rep, i)
                 assert g[i].lapack_info == 0
+
+
+def test_chief_least_squares_matches_oracle_and_fixes_resonance(require_gpu):
+    """CHIEF interior points (chief_points, zgels MS:7220-7416): the GPU
+    least-squares solution matches the FP64 oracle's lstsq on [A; s C], and
+    at the sphere's first interior resonance (ka = pi) it is closer to the
+    monopole than the square solve."""
+    a, rho, c, U = 0.1, 1.21, 343.0, 1.0
+    b = sphere_buffers(a, 2)
+    pts = (np.random.default_rng(0).uniform(-0.5, 0.5, (3, 6)) * a).astype(np.float32)
+    ka = np.array([2.0, math.pi])
+    freqs = ka * c / (2 * math.pi * a)
+    k = (2 * np.pi * freqs / c).astype(np.float32)
+    q = np.stack([np.full(b.n_triangles, 1j * rho * 2 * np.pi * f * U) for f in freqs]).astype(np.complex64)
+    obs = np.array([[0.0], [0.0], [2.0]], np.float32)
+    with _session(b) as s:
+        plain = s.assemble_solve_evaluate_standard_neumann_batch(
+            freqs, k, q, obs, source_tags=[2], impedance_source_tag=2, solve_mode="fp32_ir")
+        chief = s.assemble_solve_evaluate_standard_neumann_batch(
+            freqs, k, q, obs, source_tags=[2], impedance_source_tag=2, chief_points=pts)
+    g = O.Geometry.from_buffers(b)
+    for i in range(freqs.size):
+        kk = float(k[i])
+        qd = q[i].astype(np.complex128)
+        A, r, _ = O.assemble(g, kk, qd)
+        Cm, d = O.chief_rows(g, pts, kk, qd)
+        x, res = O.chief_solve(A, r, Cm, d[0])
+        assert rel_l2(chief[i].pressure, x) < 2e-5, (i, rel_l2(chief[i].pressure, x))
+        dg = chief[i].diagnostics
+        assert dg["chief_points"] is True and dg["chief_points_count"] == 6
+        assert abs(dg["chief_residual_rel"] - res[0]) <= 0.05 * res[0] + 1e-7
+        pex = rho * c * U * (1j * kk * a / (1j * kk * a - 1)) * np.ones(b.p1_dof_count)
+        e_plain, e_chief = rel_l2(plain[i].pressure, pex), rel_l2(chief[i].pressure, pex)
+        if i == 1:
+            assert e_chief < e_plain / 2, (e_plain, e_chief)

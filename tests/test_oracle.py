@@ -403,3 +403,34 @@ def test_subset_assembly_is_the_part_that_moves_with_a_vertex():
         assert np.abs((A1 - A) - (As1 - As0)).max() <= 1e-12 * np.abs(A1 - A).max()
         assert np.abs((r1 - r) - (rs1 - rs0)).max() <= 1e-12 * np.abs(r1 - r).max()
         assert np.abs(A1 - A).max() > 0
+
+
+def test_chief_rows_and_least_squares_on_the_pulsating_sphere():
+    """CHIEF (assembleChiefRows MS:1930-2036, zgels MS:7220-7416): the rows
+    are the field operator at interior points (C p - d = field(p, q) for any
+    p); the exact monopole satisfies the interior identity C p - d = 0 up to
+    discretisation error; at the first interior resonance ka = pi the
+    least-squares solution is closer to the monopole than the square solve,
+    and away from it CHIEF changes nothing of note."""
+    a, rho, c, U = 0.1, 1.21, 343.0, 1.0
+    b = sphere_buffers(a, 2)
+    g = O.Geometry.from_buffers(b)
+    pts = (np.random.default_rng(0).uniform(-0.5, 0.5, (3, 6)) * a).astype(np.float32)
+    for ka, gain in ((2.0, None), (math.pi, 2.0)):
+        k = ka / a
+        q = np.full(b.n_triangles, 1j * rho * k * c * U)
+        pex = rho * c * U * (1j * ka / (1j * ka - 1)) * np.ones(b.p1_dof_count)
+        A, r, _ = O.assemble(g, k, q)
+        Cm, d = O.chief_rows(g, pts, k, q)
+        pr = np.random.default_rng(1).standard_normal(b.p1_dof_count) + 0j
+        f = O.field(g, k, pr, q, pts)
+        assert rel_l2(Cm @ pr - d[0], f) < 1e-12
+        assert np.linalg.norm(Cm @ pex - d[0]) < 1e-2 * np.linalg.norm(d[0])
+        p = np.linalg.solve(A, r)
+        x, res = O.chief_solve(A, r, Cm, d[0])
+        e0, e1 = rel_l2(p, pex), rel_l2(x, pex)
+        if gain is None:
+            assert abs(e1 - e0) < 0.01 * e0
+        else:
+            assert e1 < e0 / gain, (e0, e1)
+        assert res[0] < 1e-2
