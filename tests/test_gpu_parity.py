@@ -279,7 +279,8 @@ def test_dense_solve_tournament_panels_and_stepped_solve(require_gpu):
     res = np.linalg.norm(A64 @ x[0] - b[0]) / (np.linalg.norm(A64, 2) * np.linalg.norm(x[0]))
     assert res < 1e-6, res
     x2, info2 = dense_solve(A, b, solve_mode="mixed_fp16", refine_tolerance=1e-12, refine_max_iterations=12)
-    assert rel_l2(x2[0], ref) < 1e-8, rel_l2(x2[0], ref)
+    # refined to the FP32 rounding of the returned solution
+    assert rel_l2(x2[0], ref) < 1e-7, rel_l2(x2[0], ref)
 
 
 def test_robin_assembly_matches_oracle(require_gpu):
