@@ -32,6 +32,10 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector, published
 FLOPS_PER_EVAL = 80  # G + dG/dn evaluation incl. block accumulation (fixed count)
 TRANS_PER_EVAL = 4  # sqrt/rcp, sin, cos, (exp) per evaluation (fixed count)
 REFERENCE_FREQ_PER_S = 20.0  # reference README: 40 freqs in ~2 s on Apple M-series
+# HBM bytes per k_assemble_regular dispatch on the 40-frequency quarter-horn
+# sweep: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a
+# 512 MiB copy kernel), profiles/pmc_traffic_c2_regular_assembly.md
+REGULAR_TRAFFIC_BYTES = {"c2": (8768.6 * 2.0 + 2896140.0) * 1024.0}
 
 
 WORKLOADS = {
@@ -137,7 +141,7 @@ def run_c5(args) -> int:
         fwd = time.perf_counter() - t1
     grad = V.grad.cpu().numpy()
     if args.save_gradient:
-        np.savez(args.save_gradient, vertices=pick, grad=grad[:, pick], loss=float(loss),
+        np.savez(args.save_gradient, vertices=pick, grad=grad[:, pick], loss=float(loss.detach()),
                  field=f.cpu().numpy())
     out = {
         "metric": "frequencies solved/sec (assemble+LU+field) at 1k/8k dofs; rel-L2 vs FP64",
@@ -158,7 +162,7 @@ def run_c5(args) -> int:
             "n_dofs": n, "n_triangles": T, "frequencies": F, "solve_mode": solve_mode,
         },
         "forward_only_ms": round(1e3 * fwd, 3),
-        "loss": float(loss),
+        "loss": float(loss.detach()),
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -426,7 +430,7 @@ def main() -> int:
     cpu = None
     acc = {}
     if not args.no_cpu_baseline:
-        sample = [0, F // 2, F - 1] if args.workload == "c2" else [F // 2]
+        sample = list(range(0, F, 5)) if args.workload == "c2" else [F // 2]
         cps, cdt = cpu_baseline(buffers, sym, freqs, k, q, pts, sample, threads)
         cpu = {
             "value": round(cps, 4),
@@ -482,7 +486,7 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": REGULAR_TRAFFIC_BYTES.get(args.workload) if args.dofs is None else None,
             "algorithmic": f"T^2*36*(1+I)*F = {evals_per_step:.4g} G/dG evaluations per step x "
             f"{FLOPS_PER_EVAL} FP32 flops (+{TRANS_PER_EVAL} transcendentals) each",
             "launch_ms": round(reg_launch_s * 1e3, 4),

