@@ -29,6 +29,8 @@
 // before the hardware sin/cos, which take revolutions.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hbem_internal.h"
 
 namespace hbem {
@@ -192,16 +194,50 @@ __device__ __forceinline__ void regular_both(const float (&xa_m)[18], const floa
     }
 }
 
-// All images of one (t, s): tf is t's float record (uniform), yp / v0s /
-// ns the lane's points, vertex 0 and normal.
-template <bool kComplexK, bool kSource>
-__device__ __forceinline__ void regular_images(const float *__restrict__ tf, int n_masks,
+// The fields of a test triangle's records the regular kernel uses, held in
+// registers (loaded one test ahead so their latency overlaps the previous
+// test's evaluations).
+struct TestRec {
+    float x[18];   // quadrature points
+    float n[3];    // unit normal
+    float v0[3];   // vertex 0
+    float jac;     // 2 * area
+    int vid[3];    // vertex ids
+    int key[3];    // coordinate-key ids under the identity
+    int slot[3];   // tile dof slots
+    int dof[3];    // P1 dofs
+    int orig;      // caller index
+};
+
+__device__ __forceinline__ void load_test(const float *__restrict__ tf,
+                                          const int32_t *__restrict__ tii, TestRec &r) {
+#pragma unroll
+    for (int j = 0; j < 18; ++j) r.x[j] = tf[j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r.n[c] = tf[18 + c];
+        r.v0[c] = tf[23 + c];
+        r.dof[c] = tii[c];
+        r.vid[c] = tii[3 + c];
+        r.key[c] = tii[6 + c];
+        r.slot[c] = tii[20 + c];
+    }
+    r.jac = tf[22];
+    r.orig = tii[19];
+}
+
+// All images of one (t, s): tr is t's record (uniform), yp / v0s / ns the
+// lane's points, vertex 0 and normal.
+template <bool kComplexK, bool kSource, int kM>
+__device__ __forceinline__ void regular_images(const TestRec &tr, int n_masks,
                                                int packed_masks, unsigned skip,
                                                const float (&yp)[18], const float (&v0s)[3],
                                                const float (&ns)[3], float k, float krev,
                                                float kim, float (&dF)[9][2], float (&dR)[9][2],
                                                float (&sF)[3][2], float (&sR)[3][2]) {
-    for (int m = 0; m < n_masks; ++m) {
+    // one or two images: unrolled; four: a runtime loop (code size)
+    const int nm = kM < 4 ? kM : n_masks;
+    for (int m = 0; m < nm; ++m) {
         if ((skip >> m) & 1u) continue;
         const int mask = (packed_masks >> (3 * m)) & 7;
         const float sg[3] = {(mask & 1) ? -1.0f : 1.0f, (mask & 2) ? -1.0f : 1.0f,
@@ -212,7 +248,7 @@ __device__ __forceinline__ void regular_images(const float *__restrict__ tf, int
             float h = 0.0f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                xm[3 * a + c] = sg[c] * tf[3 * a + c];
+                xm[3 * a + c] = sg[c] * tr.x[3 * a + c];
                 h = fmaf(v0s[c] - xm[3 * a + c], ns[c], h);
             }
             hF[a] = h;
@@ -220,8 +256,8 @@ __device__ __forceinline__ void regular_images(const float *__restrict__ tf, int
         float nt[3], v0t[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            nt[c] = sg[c] * tf[18 + c];
-            v0t[c] = sg[c] * tf[23 + c];
+            nt[c] = sg[c] * tr.n[c];
+            v0t[c] = sg[c] * tr.v0[c];
         }
 #pragma unroll
         for (int bq = 0; bq < 6; ++bq) {
@@ -234,7 +270,7 @@ __device__ __forceinline__ void regular_images(const float *__restrict__ tf, int
     }
 }
 
-template <bool kComplexK>
+template <bool kComplexK, int kM>
 __global__ __launch_bounds__(256) void k_assemble_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, int n_src, const float *__restrict__ beta,
@@ -256,7 +292,7 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     const float k = k_real[b];
     const float kim = kComplexK ? k_imag[b] : 0.0f;
     const float krev = k * kInv2Pi;
-    const int M = g.n_masks;
+    constexpr int M = kM;  // 1 + image masks (the session's n_masks)
     const int packed_masks = g.masks[0] | (g.masks[1] << 3) | (g.masks[2] << 6) | (g.masks[3] << 9);
 
     const bool trial_ok = lane < scount;
@@ -283,6 +319,7 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
 #pragma unroll
         for (int l = 0; l < 3; ++l)
             skid[m][l] = (m < M) ? g.tri_i_soa[(size_t)(6 + 3 * m + l) * T + tr] : -1;
+    (void)skid;
     const int orig = g.tri_i_soa[(size_t)19 * T + tr];
     // Neumann data of the lane's triangle for every source (q [B][S][T])
     bool lane_q = false;
@@ -304,21 +341,38 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     const float gscale = kInv4Pi * jac_s * g.row_weight;
     __syncthreads();
 
-    for (int ti = 0; ti < kTestsPerWave; ++ti) {
-        const int lt = wave * kTestsPerWave + ti;
-        if (lt >= tcount) break;
-        const int test = t0 + lt;
-        const float *tf = g.tri_f_aos + (size_t)test * kTriF;
-        const int32_t *tii = g.tri_i_aos + (size_t)test * kTriI;
-        const float jt = tf[22];
-        const int torig = tii[19];
-        bool test_q = false;
+    const int n_tests = max(0, min(kTestsPerWave, tcount - wave * kTestsPerWave));
+    // Per-test source flags and Robin admittances of this wave's tests, one
+    // lane per test, read back with v_readlane inside the loop (no memory
+    // access on the per-test path).
+    bool tq_lane = false;
+    float tbre_lane = 0.0f, tbim_lane = 0.0f;
+    if (lane < n_tests) {
+        const int torig_l = g.tri_i_soa[(size_t)19 * T + t0 + wave * kTestsPerWave + lane];
         for (int src = 0; src < n_src; ++src) {
-            const size_t qi = (((size_t)b * n_src + src) * T + torig) * 2;
-            test_q |= q[qi] != 0.0f || q[qi + 1] != 0.0f;
+            const size_t qi = (((size_t)b * n_src + src) * T + torig_l) * 2;
+            tq_lane |= q[qi] != 0.0f || q[qi + 1] != 0.0f;
         }
-        const float tbre = beta ? beta[((size_t)b * T + torig) * 2] : 0.0f;
-        const float tbim = beta ? beta[((size_t)b * T + torig) * 2 + 1] : 0.0f;
+        if (beta) {
+            tbre_lane = beta[((size_t)b * T + torig_l) * 2];
+            tbim_lane = beta[((size_t)b * T + torig_l) * 2 + 1];
+        }
+    }
+    const unsigned long long tq_mask = __ballot(tq_lane);
+    TestRec cur, nxt;
+    if (n_tests > 0)
+        load_test(g.tri_f_aos + (size_t)(t0 + wave * kTestsPerWave) * kTriF,
+                  g.tri_i_aos + (size_t)(t0 + wave * kTestsPerWave) * kTriI, cur);
+    for (int ti = 0; ti < n_tests; ++ti) {
+        const int lt = wave * kTestsPerWave + ti;
+        // next test's record in flight while this one is evaluated
+        const int nt_next = t0 + lt + (ti + 1 < n_tests ? 1 : 0);
+        load_test(g.tri_f_aos + (size_t)nt_next * kTriF, g.tri_i_aos + (size_t)nt_next * kTriI, nxt);
+        const float jt = cur.jac;
+        const int torig = cur.orig;
+        const bool test_q = (tq_mask >> ti) & 1ull;
+        const float tbre = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tbre_lane), ti));
+        const float tbim = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tbim_lane), ti));
         const bool test_robin = tbre != 0.0f || tbim != 0.0f;
         const bool test_source = test_q || test_robin;
         // lanes and directions of this (t, s) pair
@@ -329,13 +383,13 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         // mask m); s touches t^m exactly when t touches s^m
         unsigned skip = fwd ? 0u : 0xfu;
 #pragma unroll
-        for (int m = 0; m < kMaxMasks; ++m) {
+        for (int m = 0; m < kM; ++m) {
             bool touch = false;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
-                    touch |= (m == 0) ? (tii[3 + i] == svid[j]) : (tii[6 + i] == skid[m][j]);
+                    touch |= (m == 0) ? (cur.vid[i] == svid[j]) : (cur.key[i] == skid[m][j]);
             if (m < M && touch) skip |= 1u << m;
         }
         float dF[9][2], dR[9][2], sF[3][2], sR[3][2];
@@ -345,11 +399,11 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         for (int i = 0; i < 3; ++i) sF[i][0] = sF[i][1] = sR[i][0] = sR[i][1] = 0.0f;
         const bool src = wave_has_source || test_source;
         if (src)
-            regular_images<kComplexK, true>(tf, M, packed_masks, skip, yp, v0s, ns, k, krev, kim,
-                                            dF, dR, sF, sR);
+            regular_images<kComplexK, true, kM>(cur, g.n_masks, packed_masks, skip, yp, v0s, ns, k,
+                                                krev, kim, dF, dR, sF, sR);
         else
-            regular_images<kComplexK, false>(tf, M, packed_masks, skip, yp, v0s, ns, k, krev,
-                                             kim, dF, dR, sF, sR);
+            regular_images<kComplexK, false, kM>(cur, g.n_masks, packed_masks, skip, yp, v0s, ns,
+                                                 k, krev, kim, dF, dR, sF, sR);
         if (src) {
             // Robin fold in the units of dF / dR: A gets sc * d, and
             // -(i k beta / 3) SLP = sc * (+(i k beta / 3) s)
@@ -381,27 +435,32 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
             }
         }
         const float sc = dscale * jt;
+        // LDS table adds: relaxed, workgroup scope (no ordering among them)
         if (fwd) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const int lr = tii[20 + i];
+                const int lr = cur.slot[i];
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     float *cell = accF + 2 * (slc[j] * tnd + lr);
-                    atomicAdd(cell, dF[3 * i + j][0] * sc);
-                    atomicAdd(cell + 1, dF[3 * i + j][1] * sc);
+                    __hip_atomic_fetch_add(cell, dF[3 * i + j][0] * sc, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(cell + 1, dF[3 * i + j][1] * sc, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
         }
         if (rev) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const int lc = tii[20 + j];
+                const int lc = cur.slot[j];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     float *cell = accR + 2 * (lc * snd + slc[i]);
-                    atomicAdd(cell, dR[3 * i + j][0] * sc);
-                    atomicAdd(cell + 1, dR[3 * i + j][1] * sc);
+                    __hip_atomic_fetch_add(cell, dR[3 * i + j][0] * sc, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(cell + 1, dR[3 * i + j][1] * sc, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
         }
@@ -423,8 +482,8 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
                             cim += __shfl_xor(cim, off);
                         }
                         if (lane == 0) {
-                            atomicAdd(rb + (size_t)tii[i] * 2, cre);
-                            atomicAdd(rb + (size_t)tii[i] * 2 + 1, cim);
+                            atomicAdd(rb + (size_t)cur.dof[i] * 2, cre);
+                            atomicAdd(rb + (size_t)cur.dof[i] * 2 + 1, cim);
                         }
                     }
                 }
@@ -440,6 +499,7 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
                 }
             }
         }
+        cur = nxt;
     }
     __syncthreads();
     float *Ab = A + (size_t)b * a_stride * 2;
@@ -984,12 +1044,23 @@ void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *
     dim3 grid(g.n_tile_pairs, n_batch);
     // two tables of (max tile dofs)^2 complex floats
     const size_t shm = sizeof(float) * 4 * (size_t)g.max_tile_dofs * g.max_tile_dofs;
+    auto go = [&](auto complex_k, auto masks) {
+        constexpr bool kC = decltype(complex_k)::value;
+        constexpr int kM = decltype(masks)::value;
+        hipLaunchKernelGGL((k_assemble_regular<kC, kM>), grid, dim3(256), shm, stream, g, k_real,
+                           k_imag, q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
+    };
+    auto by_masks = [&](auto complex_k) {
+        switch (g.n_masks) {
+        case 1: go(complex_k, std::integral_constant<int, 1>{}); break;
+        case 2: go(complex_k, std::integral_constant<int, 2>{}); break;
+        default: go(complex_k, std::integral_constant<int, 4>{}); break;
+        }
+    };
     if (k_imag != nullptr)
-        hipLaunchKernelGGL((k_assemble_regular<true>), grid, dim3(256), shm, stream, g, k_real,
-                           k_imag, q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
+        by_masks(std::true_type{});
     else
-        hipLaunchKernelGGL((k_assemble_regular<false>), grid, dim3(256), shm, stream, g, k_real,
-                           k_imag, q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
+        by_masks(std::false_type{});
 }
 
 void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float *k_real,
