@@ -353,6 +353,8 @@ def main() -> int:
                     help="default: fp32 for c2, mixed_fp16 (GMRES-IR to 1e-6) for c3")
     ap.add_argument("--refine-tol", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lu-graph", type=int, default=1, choices=[0, 1],
+                    help="replay the LU launch sequence from a captured hipGraph (c2/c3)")
     ap.add_argument("--save-gradient", default=None, help="c5: write the sampled gradient (.npz)")
     args = ap.parse_args()
     if args.gpus == 1 and args.workload == "c5":
@@ -399,7 +401,7 @@ def main() -> int:
                 F, k_d.data_ptr(), k_d.data_ptr(), q_d.data_ptr(), n_obs, obs_d.data_ptr(),
                 field_d.data_ptr(), tag_d.data_ptr(), p_ptr=p_d.data_ptr(), source_tags=[2],
                 impedance_source_tag=2, solve_mode=solve_mode, estimate_rcond=False,
-                refine_tolerance=refine_tol,
+                refine_tolerance=refine_tol, lu_graph=bool(args.lu_graph),
             )
 
         for _ in range(args.warmup):
@@ -470,6 +472,7 @@ def main() -> int:
             "edge_length_m": round(h, 5),
             "solve_mode": solve_mode,
             "refine_tolerance": refine_tol if solve_mode != "fp32" else None,
+            "lu_graph": bool(args.lu_graph),
             "duffy_pairs": info["planned_pairs"],
         },
         "stages_ms_per_step": {

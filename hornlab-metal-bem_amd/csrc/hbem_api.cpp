@@ -134,7 +134,7 @@ struct hbem_session {
     hipStream_t stream = nullptr;
     hbem::Plan plan;
     hbem::DeviceGeometry dg{};
-    DevBuf tri_f_aos, tri_i_aos, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
+    DevBuf tri_f_aos, tri_i_aos, tri_t_f, tri_t_i, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
         tile_count, tile_ndofs, tile_dofs, tile_pairs, nc_f, nc_i, nc_rule, sg_adj, vtx_d, triv_d,
         normals_d, jac_d;
     // CHIEF rows and solves of the current resident batch
@@ -165,7 +165,7 @@ struct hbem_session {
     int cu_count = 0;
 
     ~hbem_session() {
-        for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
+        for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_t_f, &tri_t_i, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
                           &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &chief_C, &chief_d, &chief_pts, &chief_z, &chief_e, &chief_z64, &chief_r, &chief_ainf, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
@@ -446,6 +446,8 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         const hbem::Plan &P = s->plan;
         upload(s->tri_f_aos, P.tri_f_aos, s->stream);
         upload(s->tri_i_aos, P.tri_i_aos, s->stream);
+        upload(s->tri_t_f, P.tri_t_f, s->stream);
+        upload(s->tri_t_i, P.tri_t_i, s->stream);
         upload(s->tri_f_soa, P.tri_f_soa, s->stream);
         upload(s->tri_i_soa, P.tri_i_soa, s->stream);
         upload(s->sg_f, P.sg_f, s->stream);
@@ -470,6 +472,8 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         hbem::DeviceGeometry &g = s->dg;
         g.tri_f_aos = s->tri_f_aos.as<float>();
         g.tri_i_aos = s->tri_i_aos.as<int32_t>();
+        g.tri_t_f = s->tri_t_f.as<float>();
+        g.tri_t_i = s->tri_t_i.as<int32_t>();
         g.tri_f_soa = s->tri_f_soa.as<float>();
         g.tri_i_soa = s->tri_i_soa.as<int32_t>();
         g.sg_f = s->sg_f.as<float>();

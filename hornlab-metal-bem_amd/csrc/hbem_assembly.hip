@@ -95,10 +95,11 @@ __device__ __forceinline__ float mask_sign(int mask, int bit) { return (mask & b
 
 constexpr int kTestsPerWave = kTileTris / 4;
 
-// Both directions of one (t, s, image) combination over the 6 x 6 rule.
-// dF[i][j] (test basis i on t, trial basis j on s) and dR[i][j] (test basis
-// i on s, trial basis j on t) hold -4 pi DLP / jac; sF[i] (basis on t) and
-// sR[i] (basis on s) hold 4 pi SLP / jac with the source's unit density.
+// Both directions of one (t, s, image) combination over the 6 x 6 rule,
+// added to dF[i][j] (test basis i on t, trial basis j on s) and dR[i][j]
+// (test basis i on s, trial basis j on t) as -4 pi DLP / jac_s; sF[i]
+// (basis on t) and sR[i] (basis on s) get 4 pi SLP / jac_s with the
+// source's unit density.  tsc = 2 area(t) scales t's weights.
 // Both triangles are flat, so the normal projections of d = y_b - S_m x_a
 // depend on one point only:
 //   d . n_s         = (v0_s - S_m x_a) . n_s          = hF[a]
@@ -107,7 +108,8 @@ constexpr int kTestsPerWave = kTileTris / 4;
 template <bool kComplexK, bool kSource>
 __device__ __forceinline__ void regular_both(const float (&xa_m)[18], const float (&hF)[6],
                                              const float (&hR)[6], const float (&yp)[18],
-                                             float k, float krev, float kim, float (&dF)[9][2],
+                                             float k, float krev, float kim, float tsc,
+                                             float (&dF)[9][2],
                                              float (&dR)[9][2], float (&sF)[3][2],
                                              float (&sR)[3][2]) {
 #pragma unroll
@@ -172,7 +174,7 @@ __device__ __forceinline__ void regular_both(const float (&xa_m)[18], const floa
                 gR[2][1] = fmaf(gs, w2, gR[2][1]);
             }
         }
-        const float wa = (float)kRule6W[a];
+        const float wa = (float)kRule6W[a] * tsc;
         const float ta[3] = {wa * (float)(1.0 - kRule6X[a] - kRule6Y[a]), wa * (float)kRule6X[a],
                              wa * (float)kRule6Y[a]};
 #pragma unroll
@@ -194,9 +196,12 @@ __device__ __forceinline__ void regular_both(const float (&xa_m)[18], const floa
     }
 }
 
-// The fields of a test triangle's records the regular kernel uses, held in
-// registers (loaded one test ahead so their latency overlaps the previous
-// test's evaluations).
+// The fields of a test triangle's records the regular kernel uses.  They are
+// wave-uniform (scalar registers).  The next test's records are fetched one
+// test ahead as one vector load spread over the lanes (lane l < 32: float
+// word l, lane 32 + l: int word l) and moved to scalar registers with
+// v_readlane, so waiting for them never waits for the LDS adds in flight
+// (scalar loads share their counter with LDS operations).
 struct TestRec {
     float x[18];   // quadrature points
     float n[3];    // unit normal
@@ -207,34 +212,48 @@ struct TestRec {
     int slot[3];   // tile dof slots
     int dof[3];    // P1 dofs
     int orig;      // caller index
+    int dev;       // device index
 };
 
-__device__ __forceinline__ void load_test(const float *__restrict__ tf,
-                                          const int32_t *__restrict__ tii, TestRec &r) {
+static_assert(kTriF <= 32 && kTriI <= 32, "records must fit one wave");
+
+__device__ __forceinline__ int rec_word(const float *__restrict__ tf,
+                                        const int32_t *__restrict__ tii, int lane) {
+    if (lane < kTriF) return __float_as_int(tf[lane]);
+    if (lane >= 32 && lane < 32 + kTriI) return tii[lane - 32];
+    return 0;
+}
+
+__device__ __forceinline__ void unpack_test(int w, TestRec &r) {
+    auto f = [w](int j) { return __int_as_float(__builtin_amdgcn_readlane(w, j)); };
+    auto i = [w](int j) { return __builtin_amdgcn_readlane(w, 32 + j); };
 #pragma unroll
-    for (int j = 0; j < 18; ++j) r.x[j] = tf[j];
+    for (int j = 0; j < 18; ++j) r.x[j] = f(j);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        r.n[c] = tf[18 + c];
-        r.v0[c] = tf[23 + c];
-        r.dof[c] = tii[c];
-        r.vid[c] = tii[3 + c];
-        r.key[c] = tii[6 + c];
-        r.slot[c] = tii[20 + c];
+        r.n[c] = f(18 + c);
+        r.v0[c] = f(23 + c);
+        r.dof[c] = i(c);
+        r.vid[c] = i(3 + c);
+        r.key[c] = i(6 + c);
+        r.slot[c] = i(20 + c);
     }
-    r.jac = tf[22];
-    r.orig = tii[19];
+    r.jac = f(22);
+    r.orig = i(19);
+    r.dev = i(23);
 }
 
 // All images of one (t, s): tr is t's record (uniform), yp / v0s / ns the
-// lane's points, vertex 0 and normal.
+// lane's points, vertex 0 and normal; rf (0 or 1) keeps or drops the (s, t)
+// double-layer block.
 template <bool kComplexK, bool kSource, int kM>
 __device__ __forceinline__ void regular_images(const TestRec &tr, int n_masks,
                                                int packed_masks, unsigned skip,
                                                const float (&yp)[18], const float (&v0s)[3],
                                                const float (&ns)[3], float k, float krev,
-                                               float kim, float (&dF)[9][2], float (&dR)[9][2],
-                                               float (&sF)[3][2], float (&sR)[3][2]) {
+                                               float kim, float rf, float (&dF)[9][2],
+                                               float (&dR)[9][2], float (&sF)[3][2],
+                                               float (&sR)[3][2]) {
     // one or two images: unrolled; four: a runtime loop (code size)
     const int nm = kM < 4 ? kM : n_masks;
     for (int m = 0; m < nm; ++m) {
@@ -264,9 +283,9 @@ __device__ __forceinline__ void regular_images(const TestRec &tr, int n_masks,
             float h = 0.0f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) h = fmaf(yp[3 * bq + c] - v0t[c], nt[c], h);
-            hR[bq] = h;
+            hR[bq] = h * rf;  // rf = 0 drops the (s, t) double-layer block
         }
-        regular_both<kComplexK, kSource>(xm, hF, hR, yp, k, krev, kim, dF, dR, sF, sR);
+        regular_both<kComplexK, kSource>(xm, hF, hR, yp, k, krev, kim, tr.jac, dF, dR, sF, sR);
     }
 }
 
@@ -276,17 +295,22 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     const float *__restrict__ q, int n_src, const float *__restrict__ beta,
     float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
     int64_t rhs_stride) {
-    extern __shared__ float acc[];  // F: [S dofs][T dofs][2], then R: [T dofs][S dofs][2]
+    // LDS tables, each [T dofs][S dofs] with real and imaginary planes:
+    // F (rows of t, columns of s) and R (rows of s, columns of t), indexed
+    // (test slot) * snd + (trial slot), so a wave's adds for one entry hit
+    // consecutive words (one bank per distinct trial slot).
+    extern __shared__ float acc[];
     const int T = g.n_triangles;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s_tile = g.tile_pairs[2 * blockIdx.x], t_tile = g.tile_pairs[2 * blockIdx.x + 1];
     const int b = blockIdx.y;
     const bool diag = s_tile == t_tile;
     const int s0 = g.tile_start[s_tile], scount = g.tile_count[s_tile], snd = g.tile_ndofs[s_tile];
     const int t0 = g.tile_start[t_tile], tcount = g.tile_count[t_tile], tnd = g.tile_ndofs[t_tile];
-    float *accF = acc;
-    float *accR = acc + 2 * snd * tnd;
+    const int plane = snd * tnd;
+    float *accF = acc;              // re, then im at + plane
+    float *accR = acc + 2 * plane;  // re, then im at + plane
     for (int e = threadIdx.x; e < 4 * snd * tnd; e += 256) acc[e] = 0.0f;
 
     const float k = k_real[b];
@@ -347,8 +371,9 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     // access on the per-test path).
     bool tq_lane = false;
     float tbre_lane = 0.0f, tbim_lane = 0.0f;
+    const int rbase = t0 + wave * kTestsPerWave;  // this wave's first test-side record
     if (lane < n_tests) {
-        const int torig_l = g.tri_i_soa[(size_t)19 * T + t0 + wave * kTestsPerWave + lane];
+        const int torig_l = g.tri_t_i[(size_t)(rbase + lane) * kTriI + 19];
         for (int src = 0; src < n_src; ++src) {
             const size_t qi = (((size_t)b * n_src + src) * T + torig_l) * 2;
             tq_lane |= q[qi] != 0.0f || q[qi + 1] != 0.0f;
@@ -359,16 +384,52 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         }
     }
     const unsigned long long tq_mask = __ballot(tq_lane);
-    TestRec cur, nxt;
+    // dF[3 r + j] / dR[3 i + r]: sums for the dof at local position r of the
+    // current test, kept across tests while that position's dof is
+    // unchanged (the test-side records keep shared dofs in place), and added
+    // to the LDS tables only when it changes and after the last test.
+    float dF[9][2], dR[9][2];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dF[i][0] = dF[i][1] = dR[i][0] = dR[i][1] = 0.0f;
+    int held[3] = {-1, -1, -1};  // tile dof slot of each position
+    auto flush = [&](int r) {
+        if (trial_ok) {
+            float *cF = accF + held[r] * snd;
+            float *cR = accR + held[r] * snd;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                // LDS table adds: relaxed, workgroup scope (no ordering among them)
+                __hip_atomic_fetch_add(cF + slc[j], dF[3 * r + j][0] * dscale, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(cF + slc[j] + plane, dF[3 * r + j][1] * dscale,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(cR + slc[j], dR[3 * j + r][0] * dscale, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(cR + slc[j] + plane, dR[3 * j + r][1] * dscale,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            dF[3 * r + j][0] = dF[3 * r + j][1] = dR[3 * j + r][0] = dR[3 * j + r][1] = 0.0f;
+    };
+    int rec = 0;
     if (n_tests > 0)
-        load_test(g.tri_f_aos + (size_t)(t0 + wave * kTestsPerWave) * kTriF,
-                  g.tri_i_aos + (size_t)(t0 + wave * kTestsPerWave) * kTriI, cur);
+        rec = rec_word(g.tri_t_f + (size_t)rbase * kTriF, g.tri_t_i + (size_t)rbase * kTriI, lane);
     for (int ti = 0; ti < n_tests; ++ti) {
-        const int lt = wave * kTestsPerWave + ti;
+        TestRec cur;
+        unpack_test(rec, cur);
         // next test's record in flight while this one is evaluated
-        const int nt_next = t0 + lt + (ti + 1 < n_tests ? 1 : 0);
-        load_test(g.tri_f_aos + (size_t)nt_next * kTriF, g.tri_i_aos + (size_t)nt_next * kTriI, nxt);
-        const float jt = cur.jac;
+        const int nt_next = rbase + ti + (ti + 1 < n_tests ? 1 : 0);
+        rec = rec_word(g.tri_t_f + (size_t)nt_next * kTriF, g.tri_t_i + (size_t)nt_next * kTriI,
+                       lane);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            if (held[r] != cur.slot[r]) {
+                if (held[r] >= 0) flush(r);
+                held[r] = cur.slot[r];
+            }
+        const int lt = cur.dev - t0;  // the test's position in its tile
         const int torig = cur.orig;
         const bool test_q = (tq_mask >> ti) & 1ull;
         const float tbre = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tbre_lane), ti));
@@ -392,21 +453,20 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
                     touch |= (m == 0) ? (cur.vid[i] == svid[j]) : (cur.key[i] == skid[m][j]);
             if (m < M && touch) skip |= 1u << m;
         }
-        float dF[9][2], dR[9][2], sF[3][2], sR[3][2];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) dF[i][0] = dF[i][1] = dR[i][0] = dR[i][1] = 0.0f;
+        float sF[3][2], sR[3][2];
 #pragma unroll
         for (int i = 0; i < 3; ++i) sF[i][0] = sF[i][1] = sR[i][0] = sR[i][1] = 0.0f;
+        const float rf = rev ? 1.0f : 0.0f;
         const bool src = wave_has_source || test_source;
         if (src)
             regular_images<kComplexK, true, kM>(cur, g.n_masks, packed_masks, skip, yp, v0s, ns, k,
-                                                krev, kim, dF, dR, sF, sR);
+                                                krev, kim, rf, dF, dR, sF, sR);
         else
             regular_images<kComplexK, false, kM>(cur, g.n_masks, packed_masks, skip, yp, v0s, ns,
-                                                 k, krev, kim, dF, dR, sF, sR);
+                                                 k, krev, kim, rf, dF, dR, sF, sR);
         if (src) {
-            // Robin fold in the units of dF / dR: A gets sc * d, and
-            // -(i k beta / 3) SLP = sc * (+(i k beta / 3) s)
+            // Robin fold in the units of dF / dR (scaled by dscale when they
+            // reach the tables): -(i k beta / 3) SLP = dscale * (+(i k beta / 3) s)
             if (lane_robin) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
@@ -419,7 +479,7 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
                     }
                 }
             }
-            if (test_robin) {
+            if (test_robin && rev) {
                 const float cRre = (-kim * tbre - k * tbim) * (1.0f / 3.0f);
                 const float cRim = (-kim * tbim + k * tbre) * (1.0f / 3.0f);
 #pragma unroll
@@ -434,38 +494,8 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
                 }
             }
         }
-        const float sc = dscale * jt;
-        // LDS table adds: relaxed, workgroup scope (no ordering among them)
-        if (fwd) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const int lr = cur.slot[i];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    float *cell = accF + 2 * (slc[j] * tnd + lr);
-                    __hip_atomic_fetch_add(cell, dF[3 * i + j][0] * sc, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(cell + 1, dF[3 * i + j][1] * sc, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
-            }
-        }
-        if (rev) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int lc = cur.slot[j];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    float *cell = accR + 2 * (lc * snd + slc[i]);
-                    __hip_atomic_fetch_add(cell, dR[3 * i + j][0] * sc, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(cell + 1, dR[3 * i + j][1] * sc, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
-            }
-        }
         if (src) {
-            const float gsc = gscale * jt;
+            const float gsc = gscale;  // sF / sR carry 2 area(t)
             for (int sv = 0; sv < n_src; ++sv) {
                 float *rb = rhs + ((size_t)b * n_src + sv) * rhs_stride * 2;
                 const size_t lq = (((size_t)b * n_src + sv) * T + orig) * 2;
@@ -499,27 +529,28 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
                 }
             }
         }
-        cur = nxt;
     }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        if (held[r] >= 0) flush(r);
     __syncthreads();
     float *Ab = A + (size_t)b * a_stride * 2;
     const int32_t *tdofs = g.tile_dofs + (size_t)t_tile * kTileMaxDofs;
     const int32_t *sdofs = g.tile_dofs + (size_t)s_tile * kTileMaxDofs;
-    for (int e = threadIdx.x; e < snd * tnd; e += 256) {
-        const float vre = accF[2 * e], vim = accF[2 * e + 1];
-        if (vre == 0.0f && vim == 0.0f) continue;
-        const int lc = e / tnd, lr = e - lc * tnd;
-        const size_t idx = ((size_t)sdofs[lc] * lda + tdofs[lr]) * 2;
-        atomicAdd(Ab + idx, vre);
-        atomicAdd(Ab + idx + 1, vim);
-    }
-    for (int e = threadIdx.x; e < snd * tnd; e += 256) {
-        const float vre = accR[2 * e], vim = accR[2 * e + 1];
-        if (vre == 0.0f && vim == 0.0f) continue;
-        const int lc = e / snd, lr = e - lc * snd;
-        const size_t idx = ((size_t)tdofs[lc] * lda + sdofs[lr]) * 2;
-        atomicAdd(Ab + idx, vre);
-        atomicAdd(Ab + idx + 1, vim);
+    for (int e = threadIdx.x; e < plane; e += 256) {
+        const int lt_ = e / snd, ls = e - lt_ * snd;
+        const float fre = accF[e], fim = accF[e + plane];
+        if (fre != 0.0f || fim != 0.0f) {  // A[t dof][s dof]
+            const size_t idx = ((size_t)sdofs[ls] * lda + tdofs[lt_]) * 2;
+            atomicAdd(Ab + idx, fre);
+            atomicAdd(Ab + idx + 1, fim);
+        }
+        const float rre = accR[e], rim = accR[e + plane];
+        if (rre != 0.0f || rim != 0.0f) {  // A[s dof][t dof]
+            const size_t idx = ((size_t)tdofs[lt_] * lda + sdofs[ls]) * 2;
+            atomicAdd(Ab + idx, rre);
+            atomicAdd(Ab + idx + 1, rim);
+        }
     }
 }
 

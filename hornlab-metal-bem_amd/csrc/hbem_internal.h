@@ -45,7 +45,7 @@ constexpr int kTriF = 32;
 //   [18]     physical tag
 //   [19]     original (caller) triangle index
 //   [20..22] position of each dof in the triangle's tile dof list
-//   [23]     pad
+//   [23]     device index (test-side records), else -1
 constexpr int kTriI = 24;
 
 // Triangles are stored in Morton order of their centroids and grouped into
@@ -69,6 +69,13 @@ struct Plan {
     std::vector<int32_t> tri_i_aos;  // [T][kTriI]
     std::vector<float> tri_f_soa;    // [kTriF][T]
     std::vector<int32_t> tri_i_soa;  // [kTriI][T]
+    // Test-side records of the regular kernel ([T][kTriF], [T][kTriI]):
+    // position t0 + 16 w + i of a tile holds the i-th test of wave w, the
+    // wave's tests ordered as a path through shared edges, each record's
+    // vertices permuted so that a dof shared with the previous test keeps
+    // its local position; int word 23 is the test's device index.
+    std::vector<float> tri_t_f;
+    std::vector<int32_t> tri_t_i;
 
     // Counts of the full singular pair list (every touching (test image,
     // trial image) combination), as the reference reports them.
@@ -179,6 +186,8 @@ int duffy_rule(int kind, std::vector<double> *out);
 struct DeviceGeometry {
     const float *tri_f_aos;
     const int32_t *tri_i_aos;
+    const float *tri_t_f;
+    const int32_t *tri_t_i;
     const float *tri_f_soa;
     const int32_t *tri_i_soa;
     const float *sg_f;

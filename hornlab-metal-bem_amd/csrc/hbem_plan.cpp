@@ -454,16 +454,13 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
             }
     }
 
-    // Triangle records, in device order.
-    P->tri_f_aos.assign(static_cast<size_t>(T) * kTriF, 0.0f);
-    P->tri_i_aos.assign(static_cast<size_t>(T) * kTriI, -1);
-    for (int p = 0; p < T; ++p) {
+    // Triangle records of device triangle p with its local vertices taken in
+    // the order o[0], o[1], o[2].
+    auto fill_record = [&](int p, const int o[3], float *f, int32_t *ii) {
         const int t = perm[p];
-        float *f = &P->tri_f_aos[static_cast<size_t>(p) * kTriF];
-        int32_t *ii = &P->tri_i_aos[static_cast<size_t>(p) * kTriI];
         ii[19] = t;
-        for (int l = 0; l < 3; ++l) ii[20 + l] = ldof[3 * static_cast<size_t>(p) + l];
-        const int v0 = tri[t], v1 = tri[T + t], v2 = tri[2 * T + t];
+        for (int l = 0; l < 3; ++l) ii[20 + l] = ldof[3 * static_cast<size_t>(p) + o[l]];
+        const int v0 = tri[o[0] * T + t], v1 = tri[o[1] * T + t], v2 = tri[o[2] * T + t];
         for (int a = 0; a < 6; ++a) {
             const double xi = kRule6X[a], eta = kRule6Y[a], b0 = 1.0 - xi - eta;
             for (int c = 0; c < 3; ++c)
@@ -479,12 +476,92 @@ std::string build_plan(int nv, int T, int n, int symmetry, const float *vtx, con
             f[29 + c] = vtx[c * nv + v2];
         }
         for (int l = 0; l < 3; ++l) {
-            ii[l] = l2g[t * 3 + l];
-            ii[3 + l] = tri[l * T + t];
+            ii[l] = l2g[t * 3 + o[l]];
+            ii[3 + l] = tri[o[l] * T + t];
         }
         for (int m = 0; m < M; ++m)
-            for (int l = 0; l < 3; ++l) ii[6 + 3 * m + l] = key_id(m, tri[l * T + t]);
+            for (int l = 0; l < 3; ++l) ii[6 + 3 * m + l] = key_id(m, tri[o[l] * T + t]);
         ii[18] = tags ? tags[t] : 0;
+    };
+    // Records in device order.
+    P->tri_f_aos.assign(static_cast<size_t>(T) * kTriF, 0.0f);
+    P->tri_i_aos.assign(static_cast<size_t>(T) * kTriI, -1);
+    {
+        const int natural[3] = {0, 1, 2};
+        for (int p = 0; p < T; ++p)
+            fill_record(p, natural, &P->tri_f_aos[static_cast<size_t>(p) * kTriF],
+                        &P->tri_i_aos[static_cast<size_t>(p) * kTriI]);
+    }
+    // Test-side records: each wave's run of tile positions ordered as a path
+    // (the next test shares the most dofs with the current one, ties to the
+    // one with fewer unvisited edge neighbours, then the lower position), and
+    // each test's vertices placed so that dofs shared with the previous test
+    // keep their local positions.
+    P->tri_t_f.assign(static_cast<size_t>(T) * kTriF, 0.0f);
+    P->tri_t_i.assign(static_cast<size_t>(T) * kTriI, -1);
+    {
+        constexpr int kRun = kTileTris / 4;  // tests per wave
+        auto slot = [&](int p, int l) { return ldof[3 * static_cast<size_t>(p) + l]; };
+        auto n_shared = [&](int p, int q) {
+            int c = 0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) c += slot(p, a) == slot(q, b);
+            return c;
+        };
+        for (size_t tile = 0; tile < P->tile_start.size(); ++tile) {
+            const int t0 = P->tile_start[tile], tc = P->tile_count[tile];
+            for (int w0 = 0; w0 < tc; w0 += kRun) {
+                const int n = std::min(kRun, tc - w0), base = t0 + w0;
+                std::vector<char> used(static_cast<size_t>(n), 0);
+                auto free_edges = [&](int i) {
+                    int c = 0;
+                    for (int j = 0; j < n; ++j)
+                        if (j != i && !used[j] && n_shared(base + i, base + j) >= 2) ++c;
+                    return c;
+                };
+                std::vector<int> order;
+                int cur = 0;
+                for (int i = 1; i < n; ++i)
+                    if (free_edges(i) < free_edges(cur)) cur = i;
+                for (int k = 0; k < n; ++k) {
+                    order.push_back(cur);
+                    used[cur] = 1;
+                    int best = -1, bs = -1, bf = 0;
+                    for (int j = 0; j < n; ++j) {
+                        if (used[j]) continue;
+                        const int sh = n_shared(base + cur, base + j), fe = free_edges(j);
+                        if (sh > bs || (sh == bs && fe < bf)) {
+                            best = j;
+                            bs = sh;
+                            bf = fe;
+                        }
+                    }
+                    cur = best;
+                }
+                int held[3] = {-1, -1, -1};
+                for (int k = 0; k < n; ++k) {
+                    const int p = base + order[k];
+                    int o[3] = {-1, -1, -1};
+                    bool placed[3] = {false, false, false};
+                    for (int r = 0; r < 3; ++r)
+                        for (int l = 0; l < 3; ++l)
+                            if (!placed[l] && o[r] < 0 && held[r] >= 0 && slot(p, l) == held[r]) {
+                                o[r] = l;
+                                placed[l] = true;
+                            }
+                    for (int l = 0; l < 3; ++l)
+                        for (int r = 0; r < 3 && !placed[l]; ++r)
+                            if (o[r] < 0) {
+                                o[r] = l;
+                                placed[l] = true;
+                            }
+                    const size_t q = static_cast<size_t>(base + k);
+                    fill_record(p, o, &P->tri_t_f[q * kTriF], &P->tri_t_i[q * kTriI]);
+                    P->tri_t_i[q * kTriI + 23] = p;
+                    for (int r = 0; r < 3; ++r) held[r] = slot(p, o[r]);
+                }
+            }
+        }
     }
     P->tri_f_soa.assign(static_cast<size_t>(T) * kTriF, 0.0f);
     P->tri_i_soa.assign(static_cast<size_t>(T) * kTriI, -1);
