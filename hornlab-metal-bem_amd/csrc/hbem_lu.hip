@@ -710,6 +710,162 @@ __global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t 
             }
 }
 
+// Rank-kLuOuter trailing update in FP16 / BF16 on the gfx950
+// v_mfma_f32_16x16x32_{f16,bf16}: 128 x 128 complex outputs per block of
+// 4 waves (2 x 2, 64 x 64 each = 4 x 4 MFMA tiles with separate real and
+// imaginary FP32 accumulators).  As in k_lu_gemm the MFMA computes the
+// transposed tile C^T = U^T L^T, so 16 lanes of an output row group hold 16
+// consecutive rows of one column (coalesced column-major stores).  The inner
+// range is staged 32 k at a time: each thread loads its share of the next
+// stage into registers while the MFMAs of the current stage run, then
+// converts it into the LDS planes [row | col][k] (row stride 40 elements,
+// so the 16-byte operand reads of a wave are conflict-free).  kPrec 1: U is
+// scaled by the power of two scale[b] before rounding to FP16 and the
+// product by 1/scale[b]; kPrec 2: BF16, round to nearest even.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kTrTile = 128;  // output rows and columns per block
+constexpr int kTrK = 32;      // k per stage
+constexpr int kTrS = 40;      // LDS plane row stride (16-bit elements)
+
+struct __attribute__((aligned(16))) TrLds {
+    unsigned short Lr[kTrTile][kTrS], Li[kTrTile][kTrS], Ur[kTrTile][kTrS], Ui[kTrTile][kTrS];
+};
+
+template <int kPrec>
+__device__ __forceinline__ unsigned short tr_round(float x) {
+    if constexpr (kPrec == 1) {
+        const _Float16 h = static_cast<_Float16>(x);
+        return *reinterpret_cast<const unsigned short *>(&h);
+    } else {
+        return f32_to_bf16_rne(x);
+    }
+}
+
+template <int kPrec>
+__global__ __launch_bounds__(256, 2) void k_lu_gemm_tr(float *A, int64_t lda, int64_t a_stride,
+                                                    int r_begin, int r_end, int c_begin, int c_end,
+                                                    int k_begin, int k_end, const float *scale) {
+    const int b = blockIdx.z;
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const int r0 = r_begin + blockIdx.x * kTrTile;
+    const int c0 = c_begin + blockIdx.y * kTrTile;
+    __shared__ TrLds sm;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    const float us = (kPrec == 1) ? scale[b] : 1.0f;
+    // staging of L: thread owns row (tid & 127) and k half (tid >> 7) * 16
+    // (a wave's loads cover 64 consecutive rows of one column); of U: 16
+    // lanes read the 32 k of one column as 16-byte pairs, a wave 4 columns
+    // per load, 8 loads per thread.
+    const int sr = tid & (kTrTile - 1), sk = (tid >> 7) * 16;
+    const int uk = 2 * (tid & 15), uc = tid >> 4;  // k pair, first column
+    const bool lrow_ok = r0 + sr < r_end;
+    float2 lv[16];
+    float4 uv[8];
+    auto load_stage = [&](int k) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            lv[i] = lrow_ok ? *reinterpret_cast<const float2 *>(
+                                  Ab + 2 * ((size_t)(k + sk + i) * lda + r0 + sr))
+                            : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = c0 + uc + 16 * q;
+            uv[q] = c < c_end ? *reinterpret_cast<const float4 *>(
+                                    Ab + 2 * ((size_t)c * lda + k + uk))
+                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    };
+    auto store_stage = [&]() {
+        typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+        typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            u16x8 lr, lim;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                lr[e] = tr_round<kPrec>(lv[8 * h + e].x);
+                lim[e] = tr_round<kPrec>(lv[8 * h + e].y);
+            }
+            *reinterpret_cast<u16x8 *>(&sm.Lr[sr][sk + 8 * h]) = lr;
+            *reinterpret_cast<u16x8 *>(&sm.Li[sr][sk + 8 * h]) = lim;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = uc + 16 * q;
+            const u16x2 ur = {tr_round<kPrec>(uv[q].x * us), tr_round<kPrec>(uv[q].z * us)};
+            const u16x2 ui = {tr_round<kPrec>(uv[q].y * us), tr_round<kPrec>(uv[q].w * us)};
+            *reinterpret_cast<u16x2 *>(&sm.Ur[c][uk]) = ur;
+            *reinterpret_cast<u16x2 *>(&sm.Ui[c][uk]) = ui;
+        }
+    };
+    f32x4 accr[4][4], acci[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            accr[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acci[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    using V = typename std::conditional<kPrec == 1, f16x8, bf16x8>::type;
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    load_stage(k_begin);
+    for (int k = k_begin; k < k_end; k += kTrK) {
+        __syncthreads();  // the previous stage's operand reads are done
+        store_stage();
+        __syncthreads();
+        if (k + kTrK < k_end) load_stage(k + kTrK);  // in flight during the MFMAs
+        V lr[4], lim[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            lr[t] = *reinterpret_cast<const V *>(&sm.Lr[wr + 16 * t + li][lk]);
+            lim[t] = *reinterpret_cast<const V *>(&sm.Li[wr + 16 * t + li][lk]);
+        }
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) {
+            const V ur = *reinterpret_cast<const V *>(&sm.Ur[wc + 16 * ti + li][lk]);
+            const V ui = *reinterpret_cast<const V *>(&sm.Ui[wc + 16 * ti + li][lk]);
+            u32x4 bits = *reinterpret_cast<const u32x4 *>(&ui);
+            bits ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};  // -Ui
+            const V uin = *reinterpret_cast<const V *>(&bits);
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                if constexpr (kPrec == 1) {
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ur, lr[tj], accr[ti][tj], 0, 0, 0);
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(uin, lim[tj], accr[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ui, lr[tj], acci[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ur, lim[tj], acci[ti][tj], 0, 0, 0);
+                } else {
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ur, lr[tj], accr[ti][tj], 0, 0, 0);
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uin, lim[tj], accr[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ui, lr[tj], acci[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ur, lim[tj], acci[ti][tj], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const float inv_us = 1.0f / us;  // exact: us is a power of two
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = c0 + wc + 16 * ti + (lane >> 4) * 4 + v;
+                const int row = r0 + wr + 16 * tj + li;
+                if (row < r_end && col < c_end) {
+                    const size_t idx = (size_t)col * lda + row;
+                    c32 x = ld(Ab, idx);
+                    x.re -= accr[ti][tj][v] * inv_us;
+                    x.im -= acci[ti][tj][v] * inv_us;
+                    st(Ab, idx, x);
+                }
+            }
+}
+
 // Power-of-two scale 2^-e per system with 2^e ~ max |a_ij| (FP16 operand
 // range of the trailing update): scale[b] from the max-abs bits in amax[b].
 __global__ void k_pow2_scale(const double *amax, float *scale, int B) {
@@ -1234,6 +1390,7 @@ void launch_pow2_scale(const double *amax, float *scale, int B, hipStream_t stre
 // rank-kLuOuter update of the trailing matrix in the requested precision.
 void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStream_t stream) {
     static_assert(kLuPad % kNB == 0 && kLuOuter % kNB == 0, "whole panels");
+    static_assert(kLuOuter % kTrK == 0, "whole trailing-update stages");
     if (w.N > kLuMaxOrder) throw std::runtime_error("LU order above the supported maximum");
     // int32 zero has the bit pattern of float zero
     launch_fill_zero(reinterpret_cast<float *>(w.info), (size_t)w.B, stream);
@@ -1254,12 +1411,13 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
     auto gemm = [&](int prec, int r0, int r1, int c0, int c1, int k0, int k1) {
         if (r1 <= r0 || c1 <= c0) return;
         const dim3 grid((r1 - r0 + 63) / 64, (c1 - c0 + 63) / 64, B);
+        const dim3 tgrid((r1 - r0 + kTrTile - 1) / kTrTile, (c1 - c0 + kTrTile - 1) / kTrTile, B);
         if (prec == 1)
-            hipLaunchKernelGGL(k_lu_gemm<1>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride,
-                               r0, r1, c0, c1, k0, k1, scale);
+            hipLaunchKernelGGL(k_lu_gemm_tr<1>, tgrid, dim3(256), 0, stream, w.A, w.lda,
+                               w.a_stride, r0, r1, c0, c1, k0, k1, scale);
         else if (prec == 2)
-            hipLaunchKernelGGL(k_lu_gemm<2>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride,
-                               r0, r1, c0, c1, k0, k1, scale);
+            hipLaunchKernelGGL(k_lu_gemm_tr<2>, tgrid, dim3(256), 0, stream, w.A, w.lda,
+                               w.a_stride, r0, r1, c0, c1, k0, k1, scale);
         else
             hipLaunchKernelGGL(k_lu_gemm<0>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride,
                                r0, r1, c0, c1, k0, k1, scale);
