@@ -324,7 +324,7 @@ struct LuWorkspace {
 };
 
 constexpr int kLuPad = 64;
-constexpr int kLuOuter = 128;   // outer block of the two-level LU (exchange granularity)
+constexpr int kLuOuter = 256;   // outer block of the two-level LU (exchange granularity)
 constexpr int kLuGather = 132;  // ints per step of LuWorkspace::gl
 constexpr int kLuMaxOrder = 32768;  // tournament panels nominate from <= 32 chunks
 
