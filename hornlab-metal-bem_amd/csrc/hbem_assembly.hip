@@ -74,6 +74,39 @@ __device__ __forceinline__ GD kernel_gd(float dx, float dy, float dz, float nx, 
 
 __device__ __forceinline__ float mask_sign(int mask, int bit) { return (mask & bit) ? -1.0f : 1.0f; }
 
+constexpr double kInv2PiD = 0.15915494309189533577;
+
+// G and dG/dn_y at an observation point for d = y - x given in FP64: the
+// distance and the phase k r (in revolutions, reduced to [-1/2, 1/2]) are
+// formed in FP64, so a field point metres away keeps the phase accuracy of
+// the quadrature points; sin, cos and the amplitudes are FP32.
+__device__ __forceinline__ GD kernel_gd_far(double dx, double dy, double dz, float nx, float ny,
+                                            float nz, float k, double krev) {
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const bool ok = r2 > 1.0e-14;
+    const float r2f = ok ? (float)r2 : 1.0f;
+    const float rinv32 = __builtin_amdgcn_rsqf(r2f);
+    const float r32 = r2f * rinv32;
+    // one Newton step from the FP32 root: relative error ~1e-14
+    const double r = (double)r32 + fma(-(double)r32, (double)r32, r2) * (double)(0.5f * rinv32);
+    double t = r * krev;
+    t -= rint(t);
+    const float tf = (float)t;
+    const float s = __builtin_amdgcn_sinf(tf);
+    const float c = __builtin_amdgcn_cosf(tf);
+    const float rinv = ok ? rinv32 : 0.0f;
+    const float scale = kInv4Pi * rinv;
+    const float gre = c * scale, gim = s * scale;
+    const float proj = fmaf((float)dx, nx, fmaf((float)dy, ny, (float)dz * nz)) * rinv;
+    const float fre = -rinv;
+    GD o;
+    o.gre = gre;
+    o.gim = gim;
+    o.dre = (gre * fre - gim * k) * proj;
+    o.dim = (gre * k + gim * fre) * proj;
+    return o;
+}
+
 // ---------------------------------------------------------------------------
 // Regular pairs.
 // One block = one unordered pair {T, S} of tiles of up to 64 triangles, for
@@ -944,12 +977,13 @@ __global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__
     const int oo = ok ? o : 0;
     const float ox = obs[oo], oy = obs[n_obs + oo], oz = obs[2 * n_obs + oo];
     const float k = field_k[b / vpk];
-    const float krev = k * kInv2Pi;
     const float *pb = p + (size_t)b * p_stride * 2;
     const float *qb = q + (size_t)b * T * 2;
     float acc_re = 0.f, acc_im = 0.f;
     const int t_begin = blockIdx.y * kFieldChunk;
     const int t_end = min(T, t_begin + kFieldChunk);
+    const double ox64 = ox, oy64 = oy, oz64 = oz;
+    const double krev64 = (double)k * kInv2PiD;
     for (int t = t_begin; t < t_end; ++t) {
         const float *tf = g.tri_f_aos + (size_t)t * kTriF;
         const int32_t *ti = g.tri_i_aos + (size_t)t * kTriI;
@@ -958,6 +992,14 @@ __global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__
         const float p2r = pb[2 * ti[2]], p2i = pb[2 * ti[2] + 1];
         const float qre = qb[2 * ti[19]], qim = qb[2 * ti[19] + 1];
         const float jac = tf[22];
+        // vertex 0 and the two edges in FP64 (the vertices are exact floats)
+        double v0[3], e1[3], e2[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v0[c] = tf[23 + c];
+            e1[c] = (double)tf[26 + c] - v0[c];
+            e2[c] = (double)tf[29 + c] - v0[c];
+        }
         float tre = 0.f, tim = 0.f;
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
@@ -966,13 +1008,15 @@ __global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__
             const float pr = b0 * p0r + xi * p1r + eta * p2r;
             const float pi = b0 * p0i + xi * p1i + eta * p2i;
             const float w = (float)kRule6W[a];
+            double y[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) y[c] = fma(kRule6X[a], e1[c], fma(kRule6Y[a], e2[c], v0[c]));
 #pragma unroll
             for (int m = 0; m < kMasks; ++m) {
                 const int mask = g.masks[m];
                 const float sx = mask_sign(mask, 1), sy = mask_sign(mask, 2), sz = mask_sign(mask, 4);
-                const GD v = kernel_gd<false>(sx * tf[3 * a] - ox, sy * tf[3 * a + 1] - oy,
-                                              sz * tf[3 * a + 2] - oz, sx * tf[18], sy * tf[19],
-                                              sz * tf[20], k, krev, 0.0f);
+                const GD v = kernel_gd_far(sx * y[0] - ox64, sy * y[1] - oy64, sz * y[2] - oz64,
+                                           sx * tf[18], sy * tf[19], sz * tf[20], k, krev64);
                 // (dG/dn p - G q) w
                 const float cre = (v.dre * pr - v.dim * pi) - (v.gre * qre - v.gim * qim);
                 const float cim = (v.dre * pi + v.dim * pr) - (v.gre * qim + v.gim * qre);
