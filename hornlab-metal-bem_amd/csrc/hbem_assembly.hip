@@ -570,14 +570,19 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     float *Ab = A + (size_t)b * a_stride * 2;
     const int32_t *tdofs = g.tile_dofs + (size_t)t_tile * kTileMaxDofs;
     const int32_t *sdofs = g.tile_dofs + (size_t)s_tile * kTileMaxDofs;
+    // consecutive threads take consecutive rows of one matrix column (the
+    // atomics of a wave share cache lines)
     for (int e = threadIdx.x; e < plane; e += 256) {
-        const int lt_ = e / snd, ls = e - lt_ * snd;
-        const float fre = accF[e], fim = accF[e + plane];
+        const int ls = e / tnd, lt_ = e - ls * tnd;
+        const float fre = accF[lt_ * snd + ls], fim = accF[lt_ * snd + ls + plane];
         if (fre != 0.0f || fim != 0.0f) {  // A[t dof][s dof]
             const size_t idx = ((size_t)sdofs[ls] * lda + tdofs[lt_]) * 2;
             atomicAdd(Ab + idx, fre);
             atomicAdd(Ab + idx + 1, fim);
         }
+    }
+    for (int e = threadIdx.x; e < plane; e += 256) {
+        const int lt_ = e / snd, ls = e - lt_ * snd;
         const float rre = accR[e], rim = accR[e + plane];
         if (rre != 0.0f || rim != 0.0f) {  // A[s dof][t dof]
             const size_t idx = ((size_t)tdofs[lt_] * lda + sdofs[ls]) * 2;
