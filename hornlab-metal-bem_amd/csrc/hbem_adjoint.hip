@@ -69,6 +69,73 @@ __device__ __forceinline__ AdjK adj_kernel(float dx, float dy, float dz, float k
 
 __device__ __forceinline__ float msign(int mask, int bit) { return (mask & bit) ? -1.0f : 1.0f; }
 
+constexpr double kInv2PiD = 0.15915494309189533577;
+
+// adj_kernel for an observation point far from the surface: d given in
+// FP64, distance and phase in FP64, the rest in FP32.
+__device__ __forceinline__ AdjK adj_kernel_far(double dx, double dy, double dz, float k,
+                                               double krev) {
+    const double r2d = fma(dx, dx, fma(dy, dy, dz * dz));
+    const bool ok = r2d > 1.0e-14;
+    const float r2 = ok ? (float)r2d : 1.0f;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r32 = r2 * rinv;
+    const double rd = (double)r32 + fma(-(double)r32, (double)r32, r2d) * (double)(0.5f * rinv);
+    double td = rd * krev;
+    td -= rint(td);
+    const float t = (float)td;
+    const float s = __builtin_amdgcn_sinf(t);
+    const float c = __builtin_amdgcn_cosf(t);
+    const float sc = ok ? kInv4Pi * rinv : 0.0f;
+    AdjK o;
+    o.gre = c * sc;
+    o.gim = s * sc;
+    o.hre = (-o.gre * rinv - o.gim * k) * rinv;
+    o.him = (o.gre * k - o.gim * rinv) * rinv;
+    const float r2inv = rinv * rinv;
+    const float qr = 3.0f * r2inv - k * k, qi = -3.0f * k * rinv;
+    o.qre = (o.gre * qr - o.gim * qi) * r2inv;
+    o.qim = (o.gre * qi + o.gim * qr) * r2inv;
+    return o;
+}
+
+// adj_kernel with the distance and the phase k r formed in FP64 from the
+// FP32 components of d (which the regular adjoint forms without
+// cancellation, from vertex-anchored offsets): the phase error no longer
+// grows with k r.
+template <bool kComplexK>
+__device__ __forceinline__ AdjK adj_kernel_p(float dx, float dy, float dz, float k, double krev,
+                                             float kim) {
+    const double r2d = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * dz));
+    const bool ok = r2d > 1.0e-14;
+    const float r2 = ok ? (float)r2d : 1.0e-14f;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r32 = r2 * rinv;
+    const double rd = (double)r32 + fma(-(double)r32, (double)r32, r2d) * (double)(0.5f * rinv);
+    double td = rd * krev;
+    td -= rint(td);
+    const float t = (float)td;
+    const float r = (float)rd;
+    const float s = __builtin_amdgcn_sinf(t);
+    const float c = __builtin_amdgcn_cosf(t);
+    float sc = kInv4Pi * rinv;
+    if (kComplexK) sc *= __expf(-kim * r);
+    sc = ok ? sc : 0.0f;
+    AdjK o;
+    o.gre = c * sc;
+    o.gim = s * sc;
+    const float fre = kComplexK ? (-kim - rinv) : -rinv;
+    o.hre = (o.gre * fre - o.gim * k) * rinv;
+    o.him = (o.gre * k + o.gim * fre) * rinv;
+    const float r2inv = rinv * rinv;
+    const float qr = kComplexK ? fmaf(kim, kim, -k * k) + 3.0f * kim * rinv + 3.0f * r2inv
+                               : 3.0f * r2inv - k * k;
+    const float qi = kComplexK ? -2.0f * k * kim - 3.0f * k * rinv : -3.0f * k * rinv;
+    o.qre = (o.gre * qr - o.gim * qi) * r2inv;
+    o.qim = (o.gre * qi + o.gim * qr) * r2inv;
+    return o;
+}
+
 __device__ __forceinline__ void atomic_add3(double *g, int stride, int idx, double x, double y,
                                             double z) {
     atomicAdd(g + idx, x);
@@ -99,9 +166,21 @@ __global__ __launch_bounds__(256) void k_field_adjoint(
     const int tdd = live ? td : T - 1;
     const float *tf = g.tri_f_soa;
     const int32_t *ti = g.tri_i_soa;
-    float y[18];
+    // quadrature points in FP64 from the (exact float) vertices
+    double y[18];
+    {
+        double v0[3], e1[3], e2[3];
 #pragma unroll
-    for (int j = 0; j < 18; ++j) y[j] = tf[(size_t)j * T + tdd];
+        for (int c = 0; c < 3; ++c) {
+            v0[c] = tf[(size_t)(23 + c) * T + tdd];
+            e1[c] = (double)tf[(size_t)(26 + c) * T + tdd] - v0[c];
+            e2[c] = (double)tf[(size_t)(29 + c) * T + tdd] - v0[c];
+        }
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) y[3 * a + c] = fma(kRule6X[a], e1[c], fma(kRule6Y[a], e2[c], v0[c]));
+    }
     const float jac = tf[(size_t)22 * T + tdd];
     const float nx = tf[(size_t)18 * T + tdd] * jac, ny = tf[(size_t)19 * T + tdd] * jac,
                 nz = tf[(size_t)20 * T + tdd] * jac;
@@ -127,7 +206,7 @@ __global__ __launch_bounds__(256) void k_field_adjoint(
         Pi[a] = b0 * pim[0] + xi * pim[1] + eta * pim[2];
     }
     const float k = field_k[b];
-    const float krev = k * kInv2Pi;
+    const double krev = (double)k * kInv2PiD;
     const float *gb = grad_field + (size_t)b * n_obs * 2;
 
     double cj[3][2] = {{0, 0}, {0, 0}, {0, 0}}, ftd[2] = {0, 0};
@@ -143,12 +222,13 @@ __global__ __launch_bounds__(256) void k_field_adjoint(
         for (int m = 0; m < g.n_masks; ++m) {
             const int mask = g.masks[m];
             const float sx = msign(mask, 1), sy = msign(mask, 2), sz = msign(mask, 4);
-            const float mx = sx * ox, my = sy * oy, mz = sz * oz;
+            const double mx = sx * ox, my = sy * oy, mz = sz * oz;
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
                 const float w = (float)kRule6W[a];
-                const float rx = y[3 * a] - mx, ry = y[3 * a + 1] - my, rz = y[3 * a + 2] - mz;
-                const AdjK K = adj_kernel<false>(rx, ry, rz, k, krev, 0.0f);
+                const double rxd = y[3 * a] - mx, ryd = y[3 * a + 1] - my, rzd = y[3 * a + 2] - mz;
+                const AdjK K = adj_kernel_far(rxd, ryd, rzd, k, krev);
+                const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
                 const float rn = fmaf(rx, nx, fmaf(ry, ny, rz * nz));
                 // psi = H (r . N);  gamma psi
                 const float psr = K.hre * rn, psi = K.him * rn;
@@ -331,9 +411,24 @@ __global__ __launch_bounds__(256) void k_adjoint_regular(
     const int sdd = live ? sd : T - 1;
     const float *tfs = g.tri_f_soa;
     const int32_t *tis = g.tri_i_soa;
-    float y[18];
+    // trial points as vertex 0 plus offsets X e1 + Y e2 (edges of exact
+    // float vertices): differences of nearby points then round relative to
+    // their own size, not to the coordinates'
+    float v0s[3], ys[18];
+    {
+        float e1[3], e2[3];
 #pragma unroll
-    for (int j = 0; j < 18; ++j) y[j] = tfs[(size_t)j * T + sdd];
+        for (int c = 0; c < 3; ++c) {
+            v0s[c] = tfs[(size_t)(23 + c) * T + sdd];
+            e1[c] = tfs[(size_t)(26 + c) * T + sdd] - v0s[c];
+            e2[c] = tfs[(size_t)(29 + c) * T + sdd] - v0s[c];
+        }
+#pragma unroll
+        for (int b = 0; b < 6; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                ys[3 * b + c] = fmaf((float)kRule6X[b], e1[c], (float)kRule6Y[b] * e2[c]);
+    }
     const float js = tfs[(size_t)22 * T + sdd];
     const float nhx = tfs[(size_t)18 * T + sdd], nhy = tfs[(size_t)19 * T + sdd],
                 nhz = tfs[(size_t)20 * T + sdd];
@@ -364,7 +459,8 @@ __global__ __launch_bounds__(256) void k_adjoint_regular(
         }
     }
     const float qr = qt[((size_t)b * T + s_orig) * 2], qi = qt[((size_t)b * T + s_orig) * 2 + 1];
-    const float k = k_real[b], krev = k * kInv2Pi, kim = kComplexK ? k_imag[b] : 0.0f;
+    const float k = k_real[b], kim = kComplexK ? k_imag[b] : 0.0f;
+    const double krev = (double)k * kInv2PiD;
     const float rw = g.row_weight;
 
     double dy[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, dn[3] = {0, 0, 0}, dnh = 0.0;
@@ -375,6 +471,21 @@ __global__ __launch_bounds__(256) void k_adjoint_regular(
         const float *tf = tfa + (size_t)t * kTriF;
         const int32_t *ti = tia + (size_t)t * kTriI;
         const float jt = tf[22];
+        float v0t[3], xo[18];
+        {
+            float e1[3], e2[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v0t[c] = tf[23 + c];
+                e1[c] = tf[26 + c] - v0t[c];
+                e2[c] = tf[29 + c] - v0t[c];
+            }
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    xo[3 * a + c] = fmaf((float)kRule6X[a], e1[c], (float)kRule6Y[a] * e2[c]);
+        }
         // L_a = rw w_a |N_t| sum_i conj(lambda_i) phi_i(a)
         float lr[3], li[3];
 #pragma unroll
@@ -403,9 +514,13 @@ __global__ __launch_bounds__(256) void k_adjoint_regular(
             if (touch || !live) continue;
             const int mask = g.masks[m];
             const float sx = msign(mask, 1), sy = msign(mask, 2), sz = msign(mask, 4);
+            // y_b - S_m x_a = (v0_s - S_m v0_t) + (o_b - S_m o_a)
+            const float ax = v0s[0] - sx * v0t[0], ay = v0s[1] - sy * v0t[1],
+                        az = v0s[2] - sz * v0t[2];
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
-                const float xx = sx * tf[3 * a], xy = sy * tf[3 * a + 1], xz = sz * tf[3 * a + 2];
+                const float bx = ax - sx * xo[3 * a], by = ay - sy * xo[3 * a + 1],
+                            bz = az - sz * xo[3 * a + 2];
                 const float alr = Lr[a] * qr - Li[a] * qi, ali = Lr[a] * qi + Li[a] * qr;
                 float gxa = 0.0f, gya = 0.0f, gza = 0.0f;
                 float gsr = 0.0f, gsi = 0.0f;  // sum_b w_b G
@@ -413,8 +528,8 @@ __global__ __launch_bounds__(256) void k_adjoint_regular(
 #pragma unroll
                 for (int bb = 0; bb < 6; ++bb) {
                     const float wb = (float)kRule6W[bb];
-                    const float rx = y[3 * bb] - xx, ry = y[3 * bb + 1] - xy, rz = y[3 * bb + 2] - xz;
-                    const AdjK K = adj_kernel<kComplexK>(rx, ry, rz, k, krev, kim);
+                    const float rx = bx + ys[3 * bb], ry = by + ys[3 * bb + 1], rz = bz + ys[3 * bb + 2];
+                    const AdjK K = adj_kernel_p<kComplexK>(rx, ry, rz, k, krev, kim);
                     const float rn = fmaf(rx, nx, fmaf(ry, ny, rz * nz));
                     const float btr = (Lr[a] * Pr[bb] - Li[a] * Pi[bb]) * wb;
                     const float bti = (Lr[a] * Pi[bb] + Li[a] * Pr[bb]) * wb;
