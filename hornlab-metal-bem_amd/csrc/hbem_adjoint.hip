@@ -100,13 +100,12 @@ __device__ __forceinline__ AdjK adj_kernel_far(double dx, double dy, double dz, 
 }
 
 // adj_kernel with the distance and the phase k r formed in FP64 from the
-// FP32 components of d (which the regular adjoint forms without
-// cancellation, from vertex-anchored offsets): the phase error no longer
-// grows with k r.
+// components of d (which the regular adjoint forms without cancellation,
+// from vertex-anchored offsets): the phase error no longer grows with k r.
 template <bool kComplexK>
-__device__ __forceinline__ AdjK adj_kernel_p(float dx, float dy, float dz, float k, double krev,
+__device__ __forceinline__ AdjK adj_kernel_p(double dx, double dy, double dz, float k, double krev,
                                              float kim) {
-    const double r2d = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * dz));
+    const double r2d = fma(dx, dx, fma(dy, dy, dz * dz));
     const bool ok = r2d > 1.0e-14;
     const float r2 = ok ? (float)r2d : 1.0e-14f;
     const float rinv = __builtin_amdgcn_rsqf(r2);

@@ -205,7 +205,8 @@ bool any_nonzero(const float *v, size_t n) {
 // rhs vector b * S + src), beta [B][T] Robin admittance or null.
 void assemble_device(hbem_session *s, int B, int S, const float *kre_d, const float *kim_d,
                      const float *q_d, const float *beta_d, int64_t lda, int64_t a_stride,
-                     int64_t rhs_stride, Timer *t_reg, Timer *t_sing, Timer *t_near = nullptr) {
+                     int64_t rhs_stride, Timer *t_reg, Timer *t_sing, Timer *t_near = nullptr,
+                     bool precise = false) {
     const size_t a_floats = (size_t)B * a_stride * 2;
     s->A.reserve(a_floats * sizeof(float));
     s->rhs.reserve((size_t)B * S * rhs_stride * 2 * sizeof(float));
@@ -213,7 +214,7 @@ void assemble_device(hbem_session *s, int B, int S, const float *kre_d, const fl
     hbem::launch_fill_zero(s->rhs.as<float>(), (size_t)B * S * rhs_stride * 2, s->stream);
     if (t_reg) t_reg->start(s->stream);
     hbem::launch_assemble_regular(s->dg, B, kre_d, kim_d, q_d, S, beta_d, s->A.as<float>(), lda,
-                                  a_stride, s->rhs.as<float>(), rhs_stride, s->stream);
+                                  a_stride, s->rhs.as<float>(), rhs_stride, s->stream, precise);
     if (t_reg) t_reg->stop(s->stream);
     if (t_sing) t_sing->start(s->stream);
     hbem::launch_assemble_singular(s->dg, B, kre_d, kim_d, q_d, S, beta_d, s->A.as<float>(), lda,
@@ -1197,6 +1198,8 @@ void check_options(const hbem_solve_options &o) {
     if (o.n_tags < 0 || (o.n_tags > 0 && !o.tags)) fail(HBEM_EINVAL, "tags invalid");
     if (o.n_sources < 0 || o.n_sources > 4096) fail(HBEM_EINVAL, "n_sources out of range");
     if (o.n_chief < 0 || o.n_chief > 4096) fail(HBEM_EINVAL, "n_chief out of range");
+    if (o.precise_geometry != 0 && o.precise_geometry != 1)
+        fail(HBEM_EINVAL, "precise_geometry must be 0 or 1");
     if (o.n_chief > 0) {
         if (!o.chief_points) fail(HBEM_EINVAL, "chief_points must be given with n_chief > 0");
         if (!(std::isfinite(o.chief_weight) && o.chief_weight >= 0.0))
@@ -1238,7 +1241,7 @@ void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
         Timer ta, treg, tsing, tnear, tf;
         ta.start(st);
         assemble_device(s, B, S, kre_d + f0, kim_d ? kim_d + f0 : nullptr, qc, bc, lda, a_stride,
-                        N, &treg, &tsing, &tnear);
+                        N, &treg, &tsing, &tnear, opts.precise_geometry != 0 || s->adj_keep);
         if (opts.n_chief > 0) {
             const int m = opts.n_chief;
             s->chief_C.reserve(sizeof(float) * 2 * (size_t)B * m * n);

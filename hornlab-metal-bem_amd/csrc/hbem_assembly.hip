@@ -133,37 +133,58 @@ constexpr int kTestsPerWave = kTileTris / 4;
 // (test basis i on s, trial basis j on t) as -4 pi DLP / jac_s; sF[i]
 // (basis on t) and sR[i] (basis on s) get 4 pi SLP / jac_s with the
 // source's unit density.  tsc = 2 area(t) scales t's weights.
+// kPrecise: the points are vertex 0 plus an offset (o = X e1 + Y e2, from
+// exact float vertices), yp holds the trial offsets yo and xa_m the FP64
+//   B_a = (v0_s - S_m v0_t) - S_m xo_a,   d = y_b - S_m x_a = yo_b + B_a,
+// and the distance and the phase are formed in FP64 (every difference is
+// then rounded relative to |d|, and k r keeps its accuracy at large k r).
 // Both triangles are flat, so the normal projections of d = y_b - S_m x_a
 // depend on one point only:
 //   d . n_s         = (v0_s - S_m x_a) . n_s          = hF[a]
 //   d . (S_m n_t)   = (y_b - S_m v0_t) . (S_m n_t)    = hR[b]
 // (differences of nearby points first, so the rounding scales with |d|).
-template <bool kComplexK, bool kSource>
-__device__ __forceinline__ void regular_both(const float (&xa_m)[18], const float (&hF)[6],
+template <bool kComplexK, bool kSource, bool kPrecise, typename XT>
+__device__ __forceinline__ void regular_both(const XT (&xa_m)[18], const float (&hF)[6],
                                              const float (&hR)[6], const float (&yp)[18],
-                                             float k, float krev, float kim, float tsc,
+                                             float k, float krev, double krev64, float kim, float tsc,
                                              float (&dF)[9][2],
                                              float (&dR)[9][2], float (&sF)[3][2],
                                              float (&sR)[3][2]) {
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
-        const float xa = xa_m[3 * a], ya = xa_m[3 * a + 1], za = xa_m[3 * a + 2];
+        const XT xa = xa_m[3 * a], ya = xa_m[3 * a + 1], za = xa_m[3 * a + 2];
         float eF[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
         float eR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
         float gF[2] = {0.f, 0.f};
         float gR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
 #pragma unroll
         for (int bq = 0; bq < 6; ++bq) {
-            const float dx = yp[3 * bq] - xa;
-            const float dy = yp[3 * bq + 1] - ya;
-            const float dz = yp[3 * bq + 2] - za;
-            const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), 1.0e-14f);
-            const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * rinv;
-            // phase in revolutions, reduced to [0, 1) (exact)
-            const float t = __builtin_amdgcn_fractf(r * krev);
-            const float s = __builtin_amdgcn_sinf(t);
-            const float c = __builtin_amdgcn_cosf(t);
+            float r, rinv, s, c;
+            if constexpr (kPrecise) {
+                // d = yo_b + B_a in FP64; distance and phase in FP64
+                const double dxd = (double)yp[3 * bq] + xa, dyd = (double)yp[3 * bq + 1] + ya,
+                             dzd = (double)yp[3 * bq + 2] + za;
+                const double r2d = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
+                const float r2 = fmaxf((float)r2d, 1.0e-14f);
+                rinv = __builtin_amdgcn_rsqf(r2);
+                r = r2 * rinv;
+                const double rd = (double)r + fma(-(double)r, (double)r, r2d) * (double)(0.5f * rinv);
+                double td = rd * krev64;
+                td -= rint(td);
+                s = __builtin_amdgcn_sinf((float)td);
+                c = __builtin_amdgcn_cosf((float)td);
+            } else {
+                const float dx = yp[3 * bq] - xa;
+                const float dy = yp[3 * bq + 1] - ya;
+                const float dz = yp[3 * bq + 2] - za;
+                const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), 1.0e-14f);
+                rinv = __builtin_amdgcn_rsqf(r2);
+                r = r2 * rinv;
+                // phase in revolutions, reduced to [0, 1) (exact)
+                const float t = __builtin_amdgcn_fractf(r * krev);
+                s = __builtin_amdgcn_sinf(t);
+                c = __builtin_amdgcn_cosf(t);
+            }
             float q2 = rinv * rinv;  // 1/r^2 (times the damping)
             float damp = 1.0f;
             if (kComplexK) {
@@ -239,6 +260,8 @@ struct TestRec {
     float x[18];   // quadrature points
     float n[3];    // unit normal
     float v0[3];   // vertex 0
+    float e1[3];   // v1 - v0
+    float e2[3];   // v2 - v0
     float jac;     // 2 * area
     int vid[3];    // vertex ids
     int key[3];    // coordinate-key ids under the identity
@@ -266,6 +289,8 @@ __device__ __forceinline__ void unpack_test(int w, TestRec &r) {
     for (int c = 0; c < 3; ++c) {
         r.n[c] = f(18 + c);
         r.v0[c] = f(23 + c);
+        r.e1[c] = f(26 + c) - r.v0[c];
+        r.e2[c] = f(29 + c) - r.v0[c];
         r.dof[c] = i(c);
         r.vid[c] = i(3 + c);
         r.key[c] = i(6 + c);
@@ -279,12 +304,13 @@ __device__ __forceinline__ void unpack_test(int w, TestRec &r) {
 // All images of one (t, s): tr is t's record (uniform), yp / v0s / ns the
 // lane's points, vertex 0 and normal; rf (0 or 1) keeps or drops the (s, t)
 // double-layer block.
-template <bool kComplexK, bool kSource, int kM>
+template <bool kComplexK, bool kSource, int kM, bool kPrecise>
 __device__ __forceinline__ void regular_images(const TestRec &tr, int n_masks,
                                                int packed_masks, unsigned skip,
                                                const float (&yp)[18], const float (&v0s)[3],
                                                const float (&ns)[3], float k, float krev,
-                                               float kim, float rf, float (&dF)[9][2],
+                                               double krev64, float kim, float rf,
+                                               float (&dF)[9][2],
                                                float (&dR)[9][2], float (&sF)[3][2],
                                                float (&sR)[3][2]) {
     // one or two images: unrolled; four: a runtime loop (code size)
@@ -294,35 +320,66 @@ __device__ __forceinline__ void regular_images(const TestRec &tr, int n_masks,
         const int mask = (packed_masks >> (3 * m)) & 7;
         const float sg[3] = {(mask & 1) ? -1.0f : 1.0f, (mask & 2) ? -1.0f : 1.0f,
                              (mask & 4) ? -1.0f : 1.0f};
-        float xm[18], hF[6], hR[6];
+        float nt[3];
 #pragma unroll
-        for (int a = 0; a < 6; ++a) {
-            float h = 0.0f;
+        for (int c = 0; c < 3; ++c) nt[c] = sg[c] * tr.n[c];
+        using XT = typename std::conditional<kPrecise, double, float>::type;
+        XT xm[18];
+        float hF[6], hR[6];
+        if constexpr (kPrecise) {
+            double av[3];
+            float avf[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                xm[3 * a + c] = sg[c] * tr.x[3 * a + c];
-                h = fmaf(v0s[c] - xm[3 * a + c], ns[c], h);
+                av[c] = (double)v0s[c] - (double)(sg[c] * tr.v0[c]);  // exact
+                avf[c] = (float)av[c];
             }
-            hF[a] = h;
-        }
-        float nt[3], v0t[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            nt[c] = sg[c] * tr.n[c];
-            v0t[c] = sg[c] * tr.v0[c];
-        }
+            for (int a = 0; a < 6; ++a) {
+                float h = 0.0f;
 #pragma unroll
-        for (int bq = 0; bq < 6; ++bq) {
-            float h = 0.0f;
+                for (int c = 0; c < 3; ++c) {
+                    const float xo = fmaf((float)kRule6X[a], tr.e1[c], (float)kRule6Y[a] * tr.e2[c]);
+                    xm[3 * a + c] = av[c] - (double)(sg[c] * xo);
+                    h = fmaf((float)xm[3 * a + c], ns[c], h);
+                }
+                hF[a] = h;
+            }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) h = fmaf(yp[3 * bq + c] - v0t[c], nt[c], h);
-            hR[bq] = h * rf;  // rf = 0 drops the (s, t) double-layer block
+            for (int bq = 0; bq < 6; ++bq) {
+                float h = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) h = fmaf(yp[3 * bq + c] + avf[c], nt[c], h);
+                hR[bq] = h * rf;  // rf = 0 drops the (s, t) double-layer block
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                float h = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    xm[3 * a + c] = sg[c] * tr.x[3 * a + c];
+                    h = fmaf(v0s[c] - xm[3 * a + c], ns[c], h);
+                }
+                hF[a] = h;
+            }
+            float v0t[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v0t[c] = sg[c] * tr.v0[c];
+#pragma unroll
+            for (int bq = 0; bq < 6; ++bq) {
+                float h = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) h = fmaf(yp[3 * bq + c] - v0t[c], nt[c], h);
+                hR[bq] = h * rf;  // rf = 0 drops the (s, t) double-layer block
+            }
         }
-        regular_both<kComplexK, kSource>(xm, hF, hR, yp, k, krev, kim, tr.jac, dF, dR, sF, sR);
+        regular_both<kComplexK, kSource, kPrecise>(xm, hF, hR, yp, k, krev, krev64, kim, tr.jac,
+                                                   dF, dR, sF, sR);
     }
 }
 
-template <bool kComplexK, int kM>
+template <bool kComplexK, int kM, bool kPrecise>
 __global__ __launch_bounds__(256) void k_assemble_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, int n_src, const float *__restrict__ beta,
@@ -349,19 +406,34 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
     const float k = k_real[b];
     const float kim = kComplexK ? k_imag[b] : 0.0f;
     const float krev = k * kInv2Pi;
+    const double krev64 = (double)k * kInv2PiD;
     constexpr int M = kM;  // 1 + image masks (the session's n_masks)
     const int packed_masks = g.masks[0] | (g.masks[1] << 3) | (g.masks[2] << 6) | (g.masks[3] << 9);
 
     const bool trial_ok = lane < scount;
     const int tr = s0 + (trial_ok ? lane : 0);
-    float yp[18];
-#pragma unroll
-    for (int j = 0; j < 18; ++j) yp[j] = g.tri_f_soa[(size_t)j * T + tr];
-    float ns[3], v0s[3];
+    // the lane's points: absolute, or (kPrecise) offsets from vertex 0
+    float yp[18], ns[3], v0s[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         ns[c] = g.tri_f_soa[(size_t)(18 + c) * T + tr];
         v0s[c] = g.tri_f_soa[(size_t)(23 + c) * T + tr];
+    }
+    if constexpr (kPrecise) {
+        float e1[3], e2[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            e1[c] = g.tri_f_soa[(size_t)(26 + c) * T + tr] - v0s[c];
+            e2[c] = g.tri_f_soa[(size_t)(29 + c) * T + tr] - v0s[c];
+        }
+#pragma unroll
+        for (int b = 0; b < 6; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                yp[3 * b + c] = fmaf((float)kRule6X[b], e1[c], (float)kRule6Y[b] * e2[c]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 18; ++j) yp[j] = g.tri_f_soa[(size_t)j * T + tr];
     }
     const float jac_s = g.tri_f_soa[(size_t)22 * T + tr];
     int svid[3], skid[kMaxMasks][3], slc[3], sdof[3];
@@ -492,11 +564,13 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         const float rf = rev ? 1.0f : 0.0f;
         const bool src = wave_has_source || test_source;
         if (src)
-            regular_images<kComplexK, true, kM>(cur, g.n_masks, packed_masks, skip, yp, v0s, ns, k,
-                                                krev, kim, rf, dF, dR, sF, sR);
+            regular_images<kComplexK, true, kM, kPrecise>(cur, g.n_masks, packed_masks, skip, yp,
+                                                          v0s, ns, k, krev, krev64, kim, rf, dF,
+                                                          dR, sF, sR);
         else
-            regular_images<kComplexK, false, kM>(cur, g.n_masks, packed_masks, skip, yp, v0s, ns,
-                                                 k, krev, kim, rf, dF, dR, sF, sR);
+            regular_images<kComplexK, false, kM, kPrecise>(cur, g.n_masks, packed_masks, skip, yp,
+                                                           v0s, ns, k, krev, krev64, kim, rf, dF,
+                                                           dR, sF, sR);
         if (src) {
             // Robin fold in the units of dF / dR (scaled by dscale when they
             // reach the tables): -(i k beta / 3) SLP = dscale * (+(i k beta / 3) s)
@@ -1119,7 +1193,7 @@ void launch_fill_zero_f64(double *ptr, size_t n, hipStream_t stream) {
 void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *k_real,
                              const float *k_imag, const float *q, int n_src, const float *beta,
                              float *A, int64_t lda, int64_t a_stride, float *rhs,
-                             int64_t rhs_stride, hipStream_t stream) {
+                             int64_t rhs_stride, hipStream_t stream, bool precise) {
     if (g.n_tile_pairs == 0 || n_batch <= 0) return;
     dim3 grid(g.n_tile_pairs, n_batch);
     // two tables of (max tile dofs)^2 complex floats
@@ -1127,8 +1201,13 @@ void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *
     auto go = [&](auto complex_k, auto masks) {
         constexpr bool kC = decltype(complex_k)::value;
         constexpr int kM = decltype(masks)::value;
-        hipLaunchKernelGGL((k_assemble_regular<kC, kM>), grid, dim3(256), shm, stream, g, k_real,
-                           k_imag, q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
+        if (precise)
+            hipLaunchKernelGGL((k_assemble_regular<kC, kM, true>), grid, dim3(256), shm, stream, g,
+                               k_real, k_imag, q, n_src, beta, A, lda, a_stride, rhs, rhs_stride);
+        else
+            hipLaunchKernelGGL((k_assemble_regular<kC, kM, false>), grid, dim3(256), shm, stream,
+                               g, k_real, k_imag, q, n_src, beta, A, lda, a_stride, rhs,
+                               rhs_stride);
     };
     auto by_masks = [&](auto complex_k) {
         switch (g.n_masks) {

@@ -237,7 +237,7 @@ void launch_fill_zero_f64(double *ptr, size_t n, hipStream_t stream);
 void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *k_real,
                              const float *k_imag, const float *q, int n_src, const float *beta,
                              float *A, int64_t lda, int64_t a_stride, float *rhs,
-                             int64_t rhs_stride, hipStream_t stream);
+                             int64_t rhs_stride, hipStream_t stream, bool precise = false);
 
 // Singular (Sauter-Schwab) pair blocks plus the -M/2 mass term.
 void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float *k_real,

@@ -175,6 +175,9 @@ class SolveConfig:
     hip_refine_tolerance: float = 1.0e-7
     hip_batch_frequencies: int | None = None
     hip_estimate_rcond: bool = True
+    # Regular-pair distances and phases in FP64 (vertex-anchored offsets):
+    # FP64-grade matrices at about 1.5x the regular assembly time.
+    hip_precise_geometry: bool = False
 
     # Diagnostic policy
     dense_solve_rcond_warning_threshold: float = 1e-6

@@ -24,7 +24,7 @@ from numpy.typing import NDArray
 from .backends import AssemblyBackendUnavailable
 from .geometry import SYMMETRY_CODES, GeometryBuffers, validate_native_symmetry_plane
 
-HBEM_ABI_VERSION = 4
+HBEM_ABI_VERSION = 5
 _LIB_NAME = "libhbem.so"
 
 SOLVE_MODES = {"fp32": 0, "fp32_ir": 1, "mixed_fp16": 2, "mixed_bf16": 3}
@@ -76,7 +76,7 @@ class _SolveOptions(C.Structure):
         ("lu_graph", C.c_int32),
         ("chief_points", C.c_void_p),
         ("n_chief", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("precise_geometry", C.c_int32),
         ("chief_weight", C.c_double),
         ("reserved", C.c_int32 * 4),
     ]
@@ -554,6 +554,7 @@ class HipBemSession:
         lu_graph: bool = False,
         chief_points: NDArray[Any] | None = None,
         chief_weight: float = 1.0,
+        precise_geometry: bool = False,
         **_ignored: Any,
     ) -> list[DenseSolveFieldResult]:
         """Assemble, solve and evaluate every frequency; returns one result per
@@ -646,6 +647,7 @@ class HipBemSession:
         opts.impedance_tag = imp_tag
         opts.n_sources = S
         opts.lu_graph = 1 if lu_graph else 0
+        opts.precise_geometry = 1 if precise_geometry else 0
         chief = None
         if chief_points is not None:
             chief = np.asarray(chief_points, dtype=np.float32)
@@ -811,6 +813,7 @@ class HipBemSession:
         robin_beta_ptr: int | None = None,
         n_sources: int = 1,
         lu_graph: bool = False,
+        precise_geometry: bool = False,
     ) -> list[dict[str, Any]]:
         """Device-resident batch (hbem_solve_batch_device): every pointer is
         a device address on the session's GPU (e.g. ``tensor.data_ptr()``).
@@ -831,6 +834,7 @@ class HipBemSession:
         opts.robin_beta = robin_beta_ptr
         opts.n_sources = int(n_sources)
         opts.lu_graph = 1 if lu_graph else 0
+        opts.precise_geometry = 1 if precise_geometry else 0
         diag = (_CaseDiag * int(n_freq))()
         _check(
             load_library().hbem_solve_batch_device(

@@ -272,6 +272,7 @@ def _native_sweep(
                 chief_points=(None if config.chief_points is None else
                               np.asarray(config.chief_points, np.float64).reshape(-1, 3).T),
                 chief_weight=config.chief_weight,
+                precise_geometry=config.hip_precise_geometry,
             )
             for j, r in enumerate(results):
                 i = c0 + j

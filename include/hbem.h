@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HBEM_ABI_VERSION 4
+#define HBEM_ABI_VERSION 5
 
 #define HBEM_OK 0
 #define HBEM_EINVAL -1      /* bad argument / contract violation */
@@ -114,7 +114,11 @@ typedef struct hbem_solve_options {
      * main.swift:7220-7416), with FP64-residual refinement. */
     const float *chief_points;
     int32_t n_chief;
-    int32_t reserved0;
+    /* 1: regular-pair distances and phases in FP64 from vertex-anchored
+     * offsets (FP64-grade matrices for shape gradients and accuracy
+     * studies, about 1.5x the regular assembly time); 0: FP32.  The shape
+     * adjoint's forward always uses 1.  No reference counterpart. */
+    int32_t precise_geometry;
     double chief_weight;  /* > 0 (0 means 1) */
     int32_t reserved[4];
 } hbem_solve_options;
