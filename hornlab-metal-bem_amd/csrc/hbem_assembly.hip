@@ -379,8 +379,130 @@ __device__ __forceinline__ void regular_images(const TestRec &tr, int n_masks,
     }
 }
 
+// FP32 path of one (t, s) over all images with the test points outer and
+// the images inner: the image sums of a test point share one outer product
+// with t's basis weights (dF / dR / sF / sR are linear in the image sums),
+// so the 3 x 3 block accumulation costs 36 FMAs per test point instead of
+// 36 per test point and image.  hR[m][b] are formed once per pair.
+template <bool kComplexK, bool kSource, int kM>
+__device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks, int packed_masks,
+                                                  unsigned skip, const float (&yp)[18],
+                                                  const float (&v0s)[3], const float (&ns)[3],
+                                                  float k, float krev, float kim, float rf,
+                                                  float (&dF)[9][2], float (&dR)[9][2],
+                                                  float (&sF)[3][2], float (&sR)[3][2]) {
+    const int nm = kM < 4 ? kM : n_masks;
+    float sg[kM][3], hR[kM][6];
+#pragma unroll
+    for (int m = 0; m < kM; ++m) {
+        const int mask = (packed_masks >> (3 * m)) & 7;
+        sg[m][0] = (mask & 1) ? -1.0f : 1.0f;
+        sg[m][1] = (mask & 2) ? -1.0f : 1.0f;
+        sg[m][2] = (mask & 4) ? -1.0f : 1.0f;
+        float nt[3], v0t[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            nt[c] = sg[m][c] * tr.n[c];
+            v0t[c] = sg[m][c] * tr.v0[c];
+        }
+#pragma unroll
+        for (int bq = 0; bq < 6; ++bq) {
+            float h = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) h = fmaf(yp[3 * bq + c] - v0t[c], nt[c], h);
+            hR[m][bq] = h * rf;  // rf = 0 drops the (s, t) double-layer block
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        float eF[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+        float eR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+        float gF[2] = {0.f, 0.f};
+        float gR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+#pragma unroll
+        for (int m = 0; m < kM; ++m) {
+            if (m >= nm || ((skip >> m) & 1u)) continue;
+            const float xa = sg[m][0] * tr.x[3 * a], ya = sg[m][1] * tr.x[3 * a + 1],
+                        za = sg[m][2] * tr.x[3 * a + 2];
+            const float hF = fmaf(v0s[0] - xa, ns[0], fmaf(v0s[1] - ya, ns[1], (v0s[2] - za) * ns[2]));
+#pragma unroll
+            for (int bq = 0; bq < 6; ++bq) {
+                const float dx = yp[3 * bq] - xa;
+                const float dy = yp[3 * bq + 1] - ya;
+                const float dz = yp[3 * bq + 2] - za;
+                const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), 1.0e-14f);
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                const float t = __builtin_amdgcn_fractf(r * krev);  // revolutions, [0, 1)
+                const float sn = __builtin_amdgcn_sinf(t);
+                const float cs = __builtin_amdgcn_cosf(t);
+                float q2 = rinv * rinv;
+                float damp = 1.0f;
+                if (kComplexK) {
+                    damp = __expf(-kim * r);
+                    q2 *= damp;
+                }
+                const float fr = kComplexK ? rinv + kim : rinv;
+                const float p1 = fmaf(sn, k, cs * fr);
+                const float p2 = fmaf(-cs, k, sn * fr);
+                const float uF = hF * q2;
+                const float uR = -hR[m][bq] * q2;
+                const float wb = (float)kRule6W[bq];
+                const float w1 = wb * (float)kRule6X[bq];
+                const float w2 = wb * (float)kRule6Y[bq];
+                const float w0 = wb - w1 - w2;
+                const float f1 = p1 * uF, f2 = p2 * uF;
+                const float r1 = p1 * uR, r2v = p2 * uR;
+                eF[0][0] = fmaf(f1, w0, eF[0][0]);
+                eF[0][1] = fmaf(f2, w0, eF[0][1]);
+                eF[1][0] = fmaf(f1, w1, eF[1][0]);
+                eF[1][1] = fmaf(f2, w1, eF[1][1]);
+                eF[2][0] = fmaf(f1, w2, eF[2][0]);
+                eF[2][1] = fmaf(f2, w2, eF[2][1]);
+                eR[0][0] = fmaf(r1, w0, eR[0][0]);
+                eR[0][1] = fmaf(r2v, w0, eR[0][1]);
+                eR[1][0] = fmaf(r1, w1, eR[1][0]);
+                eR[1][1] = fmaf(r2v, w1, eR[1][1]);
+                eR[2][0] = fmaf(r1, w2, eR[2][0]);
+                eR[2][1] = fmaf(r2v, w2, eR[2][1]);
+                if (kSource) {
+                    const float gv = rinv * damp;
+                    const float gc = cs * gv, gs = sn * gv;
+                    gF[0] = fmaf(gc, wb, gF[0]);
+                    gF[1] = fmaf(gs, wb, gF[1]);
+                    gR[0][0] = fmaf(gc, w0, gR[0][0]);
+                    gR[0][1] = fmaf(gs, w0, gR[0][1]);
+                    gR[1][0] = fmaf(gc, w1, gR[1][0]);
+                    gR[1][1] = fmaf(gs, w1, gR[1][1]);
+                    gR[2][0] = fmaf(gc, w2, gR[2][0]);
+                    gR[2][1] = fmaf(gs, w2, gR[2][1]);
+                }
+            }
+        }
+        const float wa = (float)kRule6W[a] * tr.jac;
+        const float ta[3] = {wa * (float)(1.0 - kRule6X[a] - kRule6Y[a]), wa * (float)kRule6X[a],
+                             wa * (float)kRule6Y[a]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                dF[3 * i + j][0] = fmaf(ta[i], eF[j][0], dF[3 * i + j][0]);
+                dF[3 * i + j][1] = fmaf(ta[i], eF[j][1], dF[3 * i + j][1]);
+                dR[3 * i + j][0] = fmaf(ta[j], eR[i][0], dR[3 * i + j][0]);
+                dR[3 * i + j][1] = fmaf(ta[j], eR[i][1], dR[3 * i + j][1]);
+            }
+            if (kSource) {
+                sF[i][0] = fmaf(ta[i], gF[0], sF[i][0]);
+                sF[i][1] = fmaf(ta[i], gF[1], sF[i][1]);
+                sR[i][0] = fmaf(wa, gR[i][0], sR[i][0]);
+                sR[i][1] = fmaf(wa, gR[i][1], sR[i][1]);
+            }
+        }
+    }
+}
+
 template <bool kComplexK, int kM, bool kPrecise>
-__global__ __launch_bounds__(256) void k_assemble_regular(
+__global__ __launch_bounds__(256, kPrecise ? 1 : 2) void k_assemble_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, int n_src, const float *__restrict__ beta,
     float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
@@ -563,14 +685,23 @@ __global__ __launch_bounds__(256) void k_assemble_regular(
         for (int i = 0; i < 3; ++i) sF[i][0] = sF[i][1] = sR[i][0] = sR[i][1] = 0.0f;
         const float rf = rev ? 1.0f : 0.0f;
         const bool src = wave_has_source || test_source;
-        if (src)
-            regular_images<kComplexK, true, kM, kPrecise>(cur, g.n_masks, packed_masks, skip, yp,
+        if constexpr (kPrecise) {
+            if (src)
+                regular_images<kComplexK, true, kM, true>(cur, g.n_masks, packed_masks, skip, yp,
                                                           v0s, ns, k, krev, krev64, kim, rf, dF,
                                                           dR, sF, sR);
-        else
-            regular_images<kComplexK, false, kM, kPrecise>(cur, g.n_masks, packed_masks, skip, yp,
+            else
+                regular_images<kComplexK, false, kM, true>(cur, g.n_masks, packed_masks, skip, yp,
                                                            v0s, ns, k, krev, krev64, kim, rf, dF,
                                                            dR, sF, sR);
+        } else {
+            if (src)
+                regular_pair_fp32<kComplexK, true, kM>(cur, g.n_masks, packed_masks, skip, yp, v0s,
+                                                       ns, k, krev, kim, rf, dF, dR, sF, sR);
+            else
+                regular_pair_fp32<kComplexK, false, kM>(cur, g.n_masks, packed_masks, skip, yp,
+                                                        v0s, ns, k, krev, kim, rf, dF, dR, sF, sR);
+        }
         if (src) {
             // Robin fold in the units of dF / dR (scaled by dscale when they
             // reach the tables): -(i k beta / 3) SLP = dscale * (+(i k beta / 3) s)
