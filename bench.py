@@ -35,7 +35,7 @@ REFERENCE_FREQ_PER_S = 20.0  # reference README: 40 freqs in ~2 s on Apple M-ser
 # HBM bytes per k_assemble_regular dispatch on the 40-frequency quarter-horn
 # sweep: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a
 # 512 MiB copy kernel), profiles/pmc_traffic_c2_regular_assembly.md
-REGULAR_TRAFFIC_BYTES = {"c2": (8734.56 * 2.0 + 2896140.0) * 1024.0}
+REGULAR_TRAFFIC_BYTES = {"c2": (10640.3 * 2.0 + 2900240.0) * 1024.0}
 
 
 WORKLOADS = {
