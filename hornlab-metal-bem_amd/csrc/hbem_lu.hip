@@ -72,28 +72,44 @@ __global__ __launch_bounds__(256) void k_pad_identity(float *A, int64_t lda, int
 // both must be zero on entry.
 __global__ __launch_bounds__(256) void k_norm1(const float *A, int64_t lda, int64_t a_stride, int n,
                                                double *norms, double *amax) {
-    const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // 16 columns per block (4 per wave), one atomic pair per block: the
+    // per-matrix maxima are contended by every block of that matrix
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.y;
-    if (c >= n) return;
     const float *Ab = A + (size_t)b * a_stride * 2;
-    double s = 0.0;
+    double colmax = 0.0;
     float m = 0.0f;
-    for (int r = lane; r < n; r += 64) {
-        const c32 v = ld(Ab, (size_t)c * lda + r);
-        const float a = __builtin_sqrtf(fmaf(v.re, v.re, v.im * v.im));
-        s += (double)a;
-        m = fmaxf(m, a);
+    for (int q = 0; q < 4; ++q) {
+        const int c = blockIdx.x * 16 + wave * 4 + q;
+        if (c >= n) break;
+        double s = 0.0;
+        // two rows per lane and load (16 bytes; lda is even, so aligned),
+        // loads of several iterations in flight
+#pragma unroll 4
+        for (int r = 2 * lane; r < n; r += 128) {
+            const float4 v = *reinterpret_cast<const float4 *>(Ab + 2 * ((size_t)c * lda + r));
+            const float a0 = __builtin_sqrtf(fmaf(v.x, v.x, v.y * v.y));
+            const float a1 = r + 1 < n ? __builtin_sqrtf(fmaf(v.z, v.z, v.w * v.w)) : 0.0f;
+            s += (double)a0 + (double)a1;
+            m = fmaxf(m, fmaxf(a0, a1));
+        }
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+        colmax = fmax(colmax, s);
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        s += __shfl_xor(s, o);
-        m = fmaxf(m, __shfl_xor(m, o));
-    }
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    __shared__ double red[2][4];
     if (lane == 0) {
+        red[0][wave] = colmax;
+        red[1][wave] = (double)m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double cm = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+        const double am = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
         atomicMax(reinterpret_cast<unsigned long long *>(norms + b),
-                  static_cast<unsigned long long>(__double_as_longlong(s)));
+                  static_cast<unsigned long long>(__double_as_longlong(cm)));
         atomicMax(reinterpret_cast<unsigned long long *>(amax + b),
-                  static_cast<unsigned long long>(__double_as_longlong((double)m)));
+                  static_cast<unsigned long long>(__double_as_longlong(am)));
     }
 }
 
@@ -1374,7 +1390,7 @@ void launch_norm1(const float *A, int64_t lda, int64_t a_stride, int n, int B, d
                   double *amax, hipStream_t stream) {
     launch_fill_zero_f64(norms, (size_t)B, stream);
     launch_fill_zero_f64(amax, (size_t)B, stream);
-    hipLaunchKernelGGL(k_norm1, dim3((n + 3) / 4, B), dim3(256), 0, stream, A, lda, a_stride, n,
+    hipLaunchKernelGGL(k_norm1, dim3((n + 15) / 16, B), dim3(256), 0, stream, A, lda, a_stride, n,
                        norms, amax);
 }
 
