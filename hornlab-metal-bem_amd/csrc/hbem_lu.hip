@@ -882,6 +882,123 @@ __global__ __launch_bounds__(256, 2) void k_lu_gemm_tr(float *A, int64_t lda, in
             }
 }
 
+// The FP32 trailing update with the same 128 x 128 tiling on
+// v_mfma_f32_16x16x4_f32: stages of 16 k staged through LDS planes [k][row |
+// col] (row stride 144 floats: the four k rows a wave reads per instruction
+// fall in different bank quarters), loaded into registers during the previous
+// stage's MFMAs.
+constexpr int kTr32K = 16;
+constexpr int kTr32S = 144;
+
+struct __attribute__((aligned(16))) TrLds32 {
+    float Lr[kTr32K][kTr32S], Li[kTr32K][kTr32S], Ur[kTr32K][kTr32S], Ui[kTr32K][kTr32S];
+};
+
+__global__ __launch_bounds__(256, 2) void k_lu_gemm_tr32(float *A, int64_t lda, int64_t a_stride,
+                                                         int r_begin, int r_end, int c_begin,
+                                                         int c_end, int k_begin, int k_end) {
+    const int b = blockIdx.z;
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const int r0 = r_begin + blockIdx.x * kTrTile;
+    const int c0 = c_begin + blockIdx.y * kTrTile;
+    __shared__ TrLds32 sm;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    // staging of L: thread owns row (tid & 127) and 8 k ((tid >> 7) * 8); of
+    // U: 8 lanes read the 16 k of one column as 16-byte pairs, a wave 8
+    // columns per load, 4 loads per thread
+    const int sr = tid & (kTrTile - 1), sk = (tid >> 7) * 8;
+    const int uk = 2 * (tid & 7), uc = tid >> 3;
+    const bool lrow_ok = r0 + sr < r_end;
+    float2 lv[8];
+    float4 uv[4];
+    auto load_stage = [&](int k) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            lv[i] = lrow_ok ? *reinterpret_cast<const float2 *>(
+                                  Ab + 2 * ((size_t)(k + sk + i) * lda + r0 + sr))
+                            : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = c0 + uc + 32 * q;
+            uv[q] = c < c_end ? *reinterpret_cast<const float4 *>(
+                                    Ab + 2 * ((size_t)c * lda + k + uk))
+                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    };
+    auto store_stage = [&]() {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            sm.Lr[sk + i][sr] = lv[i].x;
+            sm.Li[sk + i][sr] = lv[i].y;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = uc + 32 * q;
+            sm.Ur[uk][c] = uv[q].x;
+            sm.Ui[uk][c] = uv[q].y;
+            sm.Ur[uk + 1][c] = uv[q].z;
+            sm.Ui[uk + 1][c] = uv[q].w;
+        }
+    };
+    f32x4 accr[4][4], acci[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            accr[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acci[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    load_stage(k_begin);
+    for (int k = k_begin; k < k_end; k += kTr32K) {
+        __syncthreads();  // the previous stage's operand reads are done
+        store_stage();
+        __syncthreads();
+        if (k + kTr32K < k_end) load_stage(k + kTr32K);  // in flight during the MFMAs
+#pragma unroll
+        for (int ks = 0; ks < kTr32K; ks += 4) {
+            const int kk = ks + lk;
+            float lr[4], lim[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                lr[t] = sm.Lr[kk][wr + 16 * t + li];
+                lim[t] = sm.Li[kk][wr + 16 * t + li];
+            }
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti) {
+                const float ur = sm.Ur[kk][wc + 16 * ti + li];
+                const float ui = sm.Ui[kk][wc + 16 * ti + li];
+#pragma unroll
+                for (int tj = 0; tj < 4; ++tj) {
+                    // D^T[col][row]: A operand = U^T (col, k), B operand = L^T (k, row)
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, lr[tj], accr[ti][tj], 0, 0, 0);
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui, lim[tj], accr[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui, lr[tj], acci[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, lim[tj], acci[ti][tj], 0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = c0 + wc + 16 * ti + (lane >> 4) * 4 + v;
+                const int row = r0 + wr + 16 * tj + li;
+                if (row < r_end && col < c_end) {
+                    const size_t idx = (size_t)col * lda + row;
+                    c32 x = ld(Ab, idx);
+                    x.re -= accr[ti][tj][v];
+                    x.im -= acci[ti][tj][v];
+                    st(Ab, idx, x);
+                }
+            }
+}
+
 // Power-of-two scale 2^-e per system with 2^e ~ max |a_ij| (FP16 operand
 // range of the trailing update): scale[b] from the max-abs bits in amax[b].
 __global__ void k_pow2_scale(const double *amax, float *scale, int B) {
@@ -1406,7 +1523,7 @@ void launch_pow2_scale(const double *amax, float *scale, int B, hipStream_t stre
 // rank-kLuOuter update of the trailing matrix in the requested precision.
 void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStream_t stream) {
     static_assert(kLuPad % kNB == 0 && kLuOuter % kNB == 0, "whole panels");
-    static_assert(kLuOuter % kTrK == 0, "whole trailing-update stages");
+    static_assert(kLuOuter % kTrK == 0 && kLuOuter % kTr32K == 0, "whole trailing-update stages");
     if (w.N > kLuMaxOrder) throw std::runtime_error("LU order above the supported maximum");
     // int32 zero has the bit pattern of float zero
     launch_fill_zero(reinterpret_cast<float *>(w.info), (size_t)w.B, stream);
@@ -1464,7 +1581,14 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
                 swap_trsm(k, 1, K1, N, 2, 0, 0, 0);
                 gemm(0, k + kNB, K1, K1, N, k, k + kNB);
             }
-            gemm(precision, K1, N, K1, N, K0, K1);
+            if (precision == 0) {
+                // the rank-kLuOuter trailing update in FP32 on the 128-tile kernel
+                const dim3 grid((N - K1 + kTrTile - 1) / kTrTile, (N - K1 + kTrTile - 1) / kTrTile, B);
+                hipLaunchKernelGGL(k_lu_gemm_tr32, grid, dim3(256), 0, stream, w.A, w.lda,
+                                   w.a_stride, K1, N, K1, N, K0, K1);
+            } else {
+                gemm(precision, K1, N, K1, N, K0, K1);
+            }
         }
     }
 }
