@@ -17,6 +17,7 @@ import oracle as O
 from meshes import (
     horn_buffers,
     horn_quarter_and_full,
+    mesh_buffers,
     rel_l2,
     sphere_buffers,
     tiny_two_triangles,
@@ -570,3 +571,48 @@ def test_precise_geometry_brings_the_solution_to_fp64(require_gpu):
         errs.append(e)
         assert e[True][0] < 1e-6 and e[True][1] < 1e-6, (freqs[i], e)
     assert errs[-1][True][1] < errs[-1][False][1], errs[-1]
+
+
+def _half_horn(edge_length):
+    """The quarter horn mirrored across the yz plane only: a y >= 0 half with
+    the xz plane as its one symmetry plane (two images)."""
+    from hornlab_metal_bem.procedural import HornSpec, horn_body
+
+    Vq, Fq, Tq = horn_body(HornSpec(), edge_length=edge_length, quarter=True, seed=0)
+    keys, verts, tris = {}, [], []
+    for sx in (1.0, -1.0):
+        remap = np.empty(Vq.shape[0], dtype=np.int64)
+        for i, p in enumerate(Vq):
+            q = np.array([sx * p[0], p[1], p[2]])
+            key = tuple(np.round(q / 1e-9).astype(np.int64).tolist())
+            if key not in keys:
+                keys[key] = len(verts)
+                verts.append(q)
+            remap[i] = keys[key]
+        t = remap[Fq]
+        tris.append(t[:, [0, 2, 1]] if sx < 0 else t)
+    return mesh_buffers(np.array(verts), np.vstack(tris).astype(np.int32), np.concatenate([Tq, Tq]))
+
+
+@pytest.mark.parametrize("precise", [False, True])
+def test_single_plane_half_horn_matches_oracle(require_gpu, precise):
+    """One symmetry plane (xz, two images: the kernels' two-image paths) on a
+    half horn, FP32 and precise geometry, against the FP64 oracle solve."""
+    b = _half_horn(0.03)
+    f = 4000.0
+    k = np.array([2 * np.pi * f / C0], np.float32)
+    q = np.where(b.physical_tags_i32 == 2, 1j * 1.2041 * 2 * np.pi * f, 0.0)[None]
+    ang = np.deg2rad(np.linspace(0, 90, 7))
+    obs = np.stack([2.0 * np.sin(ang), np.zeros_like(ang), 2.0 * np.cos(ang)]).astype(np.float32)
+    with _session(b, "xz") as s:
+        res = s.assemble_solve_evaluate_standard_neumann_batch(
+            np.array([f]), k, q.astype(np.complex64), obs, source_tags=[2],
+            impedance_source_tag=2, write_surface_pressure=True, solve_mode="fp32_ir",
+            refine_tolerance=1e-12, refine_max_iterations=12, precise_geometry=precise)
+    g = O.Geometry.from_buffers(b, "xz")
+    A, rhs, _ = O.assemble(g, float(k[0]), q[0])
+    p = O.solve(A, rhs)
+    fo = O.field(g, float(k[0]), p, q[0], obs)
+    tol = 1e-6 if precise else 1e-5
+    assert rel_l2(res[0].pressure, p) < tol, rel_l2(res[0].pressure, p)
+    assert rel_l2(res[0].field, fo) < tol, rel_l2(res[0].field, fo)
