@@ -44,6 +44,11 @@ __device__ __forceinline__ c32 cinv(c32 a) {
     return {a.re / d, -a.im / d};
 }
 
+// v's value in lane j of the wave (v_readlane; j uniform).
+__device__ __forceinline__ float lane_value(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
 __device__ __forceinline__ c32 ld(const float *A, size_t idx) {
     const float2 v = *reinterpret_cast<const float2 *>(A + 2 * idx);
     return {v.x, v.y};
@@ -163,15 +168,42 @@ __device__ __forceinline__ unsigned long long pivot_key(c32 v, int row) {
            static_cast<unsigned long long>(0xffffffffu - static_cast<unsigned>(row));
 }
 
+// One step of a wave maximum over 64-bit keys: each lane takes the larger of
+// its key and the key of the lane the DPP control names (rows outside
+// kRowMask keep their own key).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ unsigned long long key_max_dpp(unsigned long long key) {
+    const int lo = static_cast<int>(static_cast<unsigned>(key));
+    const int hi = static_cast<int>(static_cast<unsigned>(key >> 32));
+    const unsigned olo =
+        static_cast<unsigned>(__builtin_amdgcn_update_dpp(lo, lo, kCtrl, kRowMask, 0xf, false));
+    const unsigned ohi =
+        static_cast<unsigned>(__builtin_amdgcn_update_dpp(hi, hi, kCtrl, kRowMask, 0xf, false));
+    const unsigned long long other = (static_cast<unsigned long long>(ohi) << 32) | olo;
+    return other > key ? other : key;
+}
+
+// Maximum over the 16 lanes of each DPP row, left in every lane of the row:
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror.
+__device__ __forceinline__ unsigned long long row_max_u64(unsigned long long key) {
+    key = key_max_dpp<0xB1, 0xF>(key);
+    key = key_max_dpp<0x4E, 0xF>(key);
+    key = key_max_dpp<0x141, 0xF>(key);
+    return key_max_dpp<0x140, 0xF>(key);
+}
+
+// Maximum over the wave, returned in every lane: the row maximum, then
+// row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3 leave
+// the wave maximum in lane 63, which is read back as a uniform value.
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long key) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        const unsigned lo = __shfl_xor(static_cast<unsigned>(key), o);
-        const unsigned hi = __shfl_xor(static_cast<unsigned>(key >> 32), o);
-        const unsigned long long other = (static_cast<unsigned long long>(hi) << 32) | lo;
-        key = other > key ? other : key;
-    }
-    return key;
+    key = row_max_u64(key);
+    key = key_max_dpp<0x142, 0xA>(key);
+    key = key_max_dpp<0x143, 0xC>(key);
+    const unsigned lo = static_cast<unsigned>(
+        __builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(key)), 63));
+    const unsigned hi = static_cast<unsigned>(
+        __builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(key >> 32)), 63));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
 // Partial-pivoting elimination of the kNB panel columns over the rows held
@@ -381,14 +413,18 @@ __global__ __launch_bounds__(256) void k_lu_panel_apply(float *A, int64_t lda, i
     for (int c = 0; c < kNB; ++c) st(Ab, (size_t)(k + c) * lda + r, x[c]);
 }
 
-// Register panel for N - k <= 1024: thread t holds the kNB panel values of
-// one row, the column loop is unrolled so that every register index is
+// Register panel for N - k <= 1024, one wave per 64 rows (blockDim.x =
+// 64 ceil((N - k) / 64)): thread t holds the kNB panel values of one row,
+// the column loop is unrolled so that every register index is
 // static.  A row exchange swaps only the logical row numbers of the two
-// threads involved (no data moves); the pivot thread publishes its row
-// through LDS and every thread writes its values back to its final logical
-// row.  The pivot is reduced per wave with shuffles and across waves with
-// one LDS atomic max per wave into a double-buffered slot.  Two block
-// barriers per column.
+// threads involved (no data moves); every thread writes its values back to
+// its final logical row.  One block barrier per column: each wave reduces
+// its pivot candidate with DPP, and its candidate lane publishes the key and
+// the row's values from the column on to a double-buffered per-wave slot;
+// after the barrier every wave reduces the 16 keys (one DPP row) and reads
+// the winning slot.  A slot of
+// column j is rewritten at column j + 2, after every reader has passed the
+// barrier of column j + 1.
 __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, int64_t a_stride,
                                                        int N, int k, int32_t *ipiv,
                                                        int32_t *info, int32_t *gl,
@@ -403,45 +439,42 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
         v[c] = own ? ld(Ab, (size_t)(k + c) * lda + myrow) : c32{0.0f, 0.0f};
-    __shared__ unsigned long long best_s[2];
-    __shared__ c32 prow[kNB];
+    __shared__ unsigned long long key_s[2][16];
+    __shared__ c32 row_s[2][16][kNB];
     __shared__ int piv_s[kNB];
-    __shared__ int bad_s;
-    if (tid == 0) {
-        best_s[0] = 0ull;
-        best_s[1] = 0ull;
-        bad_s = 0;
-    }
-    __syncthreads();
+    // the block has one wave per 64 rows; slots of absent waves hold no key
+    if (tid < 32 && (tid & 15) >= (int)(blockDim.x >> 6)) key_s[tid >> 4][tid & 15] = 0ull;
+    int bad = 0;  // thread 0: first column without a nonzero pivot, plus one
 #pragma unroll
     for (int j = 0; j < kNB; ++j) {
-        const int col = k + j;
-        unsigned long long key = (own && myrow >= col) ? pivot_key(v[j], myrow) : 0ull;
-        key = wave_max_u64(key);
-        if (lane == 0) atomicMax(&best_s[j & 1], key);
-        __syncthreads();
-        const unsigned long long best = best_s[j & 1];
-        const int p = static_cast<int>(0xffffffffu - static_cast<unsigned>(best));
-        const bool holder = own && myrow == p;
-        if (holder) {
+        const int col = k + j, slot = j & 1;
+        const bool cand = own && myrow >= col;
+        const unsigned long long key = cand ? pivot_key(v[j], myrow) : 0ull;
+        const unsigned long long wkey = wave_max_u64(key);
+        if (cand && key == wkey) {
 #pragma unroll
-            for (int c = j; c < kNB; ++c) prow[c] = v[c];
+            for (int c = j; c < kNB; ++c) row_s[slot][wave][c] = v[c];
         }
+        if (lane == 0) key_s[slot][wave] = wkey;
+        __syncthreads();
+        const unsigned long long mine = key_s[slot][lane & 15];
+        const unsigned long long best = row_max_u64(mine);
+        const int wbest = __builtin_ctzll(__ballot(mine == best)) & 15;
+        const int p = static_cast<int>(0xffffffffu - static_cast<unsigned>(best));
         if (tid == 0) {
-            best_s[(j + 1) & 1] = 0ull;  // its last readers passed the previous barrier
             piv_s[j] = p;
             ipiv[(size_t)b * N + col] = p;
-            if (!(__uint_as_float(static_cast<unsigned>(best >> 32)) > 0.0f) && bad_s == 0)
-                bad_s = col + 1;
+            if (!(__uint_as_float(static_cast<unsigned>(best >> 32)) > 0.0f) && bad == 0)
+                bad = col + 1;
         }
         // exchange logical rows col <-> p
-        if (holder)
+        if (own && myrow == p)
             myrow = col;
         else if (own && myrow == col)
             myrow = p;
-        __syncthreads();
         // rows below the pivot: l = a / pivot, a_c -= l u_c
         if (own && myrow > col) {
+            const c32 *prow = row_s[slot][wbest];
             const c32 piv = prow[j];
             const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
             c32 l = v[j];
@@ -459,7 +492,8 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
 #pragma unroll
         for (int c = 0; c < kNB; ++c) st(Ab, (size_t)(k + c) * lda + myrow, v[c]);
     }
-    if (tid == 0 && bad_s != 0 && info[b] == 0) info[b] = bad_s;
+    if (tid == 0 && bad != 0 && info[b] == 0) info[b] = bad;
+    __syncthreads();
     if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
 }
 
@@ -1014,8 +1048,8 @@ __global__ void k_pow2_scale(const double *amax, float *scale, int B) {
 // ---------------------------------------------------------------------------
 // Triangular solves with the factors: one 1024-thread block per system,
 // x in LDS, sweeps in blocks of kNB rows.  The diagonal block of each sweep
-// step is staged in LDS and solved by one wave with shuffles; the coupling
-// to the other rows is a block GEMV over all threads.
+// step is staged in LDS and solved by one wave; the coupling to the other
+// rows is a block GEMV over all threads.
 //   trans 0: x <- A^{-1} x = U^{-1} L~_{S-1}^{-1} P_{S-1}^T ... L~_0^{-1} P_0^T x
 //   trans 2: x <- A^{-H} x = P_0 L~_0^{-H} ... P_{S-1} L~_{S-1}^{-H} U^{-H} x
 // where s runs over the outer blocks of kLuOuter columns, P_s^T applies the
@@ -1055,9 +1089,43 @@ __device__ __forceinline__ void block_gemv_h(const float *Ab, int64_t lda, int k
     }
 }
 
+// Applies the composed row exchanges of panel steps [q0, q1) to xs, in
+// order (position trow[m] takes the old entry tsrc[m]).  Run by one wave:
+// the records are loaded at once, and each step's reads precede its writes
+// in the wave's own order.
+__device__ __forceinline__ void apply_gather_lists(const int32_t *glb, int q0, int q1, c32 *xs) {
+    const int lane = threadIdx.x & 63;
+    constexpr int kMaxSteps = kLuOuter / kNB;
+    int cnt[kMaxSteps], trow[kMaxSteps], tsrc[kMaxSteps];
+#pragma unroll
+    for (int i = 0; i < kMaxSteps; ++i) {
+        cnt[i] = 0;
+        if (q0 + i < q1) {
+            const int32_t *g = glb + (size_t)(q0 + i) * kLuGather;
+            cnt[i] = g[0];
+            trow[i] = g[1 + lane];
+            tsrc[i] = g[1 + 2 * kNB + lane];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxSteps; ++i) {
+        const bool on = lane < cnt[i];
+        const c32 t = on ? xs[tsrc[i]] : c32{0.0f, 0.0f};
+        if (on) xs[trow[i]] = t;
+    }
+}
+
+// trans 0, forward: at the first step of each outer block the block's row
+// exchanges (its composed gather lists), then per kNB-row step the
+// unit-lower diagonal solve (one wave, lane = row, values broadcast with
+// v_readlane) and the update of the rows below; backward: the upper
+// diagonal solve and the update of the rows above.  Each step issues the
+// loads of its coupling block (one row per thread) and of the next step's
+// diagonal block before it waits, so they overlap the diagonal solve.
 __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, int64_t a_stride,
-                                                   int N, const int32_t *ipiv, float *X,
-                                                   int64_t ldx, int trans, int vpm) {
+                                                   int N, const int32_t *ipiv, const int32_t *gl,
+                                                   int64_t gl_stride, float *X, int64_t ldx,
+                                                   int trans, int vpm) {
     extern __shared__ c32 xs[];
     __shared__ c32 D[kNB][kNB + 1];
     __shared__ c32 part[kNB][kNB + 1];
@@ -1070,26 +1138,29 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
     float *xb = X + (size_t)b * ldx * 2;
     for (int r = tid; r < N; r += 1024) xs[r] = ld(xb, r);
     if (trans == 0) {
+        const int32_t *glb = gl + (size_t)(b / vpm) * gl_stride;
+        const int di = tid % kNB, dj = tid / kNB;  // the thread's diagonal-block entry
+        c32 dnext = ld(Ab, (size_t)dj * lda + di);
+        __syncthreads();  // xs loaded
         for (int kb = 0; kb < N; kb += kNB) {
-            const int xw = min(kLuOuter, N - kb);  // exchanges of an outer block
-            const bool xstart = (kb % kLuOuter) == 0;
-            if (xstart && tid < xw) pv_s[tid] = pv[kb + tid];
-            stage_diag(Ab, lda, kb, D);
-            __syncthreads();
-            if (xstart && tid == 0)
-                for (int j = 0; j < xw; ++j) {
-                    const int p = pv_s[j];
-                    if (p != kb + j) {
-                        const c32 t = xs[kb + j];
-                        xs[kb + j] = xs[p];
-                        xs[p] = t;
-                    }
-                }
+            const c32 dcur = dnext;
+            D[di][dj] = dcur;
+            dnext = kb + kNB < N ? ld(Ab, (size_t)(kb + kNB + dj) * lda + kb + kNB + di) : dcur;
+            const int r = kb + kNB + tid;
+            c32 lr[kNB];
+            if (r < N) {
+#pragma unroll
+                for (int j = 0; j < kNB; ++j) lr[j] = ld(Ab, (size_t)(kb + j) * lda + r);
+            }
+            if (kb % kLuOuter == 0 && tid < 64)
+                apply_gather_lists(glb, kb / kNB, min(N, kb + kLuOuter) / kNB, xs);
             __syncthreads();
             if (tid < 64) {
                 c32 x = lane < kNB ? xs[kb + lane] : c32{0.0f, 0.0f};
+#pragma unroll
                 for (int j = 0; j < kNB - 1; ++j) {
-                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    const float xr = lane_value(x.re, j);
+                    const float xi = lane_value(x.im, j);
                     if (lane > j && lane < kNB) {
                         const c32 l = D[lane][j];
                         x.re -= l.re * xr - l.im * xi;
@@ -1099,27 +1170,49 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
                 if (lane < kNB) xs[kb + lane] = x;
             }
             __syncthreads();
-            for (int r = kb + kNB + tid; r < N; r += 1024) {
+            if (r < N) {
                 c32 acc = xs[r];
-#pragma unroll 8
+#pragma unroll
                 for (int j = 0; j < kNB; ++j) {
-                    const c32 l = ld(Ab, (size_t)(kb + j) * lda + r);
+                    const c32 l = lr[j];
                     const c32 xj = xs[kb + j];
                     acc.re -= l.re * xj.re - l.im * xj.im;
                     acc.im -= l.re * xj.im + l.im * xj.re;
                 }
                 xs[r] = acc;
             }
+            for (int r2 = r + 1024; r2 < N; r2 += 1024) {
+                c32 acc = xs[r2];
+#pragma unroll 8
+                for (int j = 0; j < kNB; ++j) {
+                    const c32 l = ld(Ab, (size_t)(kb + j) * lda + r2);
+                    const c32 xj = xs[kb + j];
+                    acc.re -= l.re * xj.re - l.im * xj.im;
+                    acc.im -= l.re * xj.im + l.im * xj.re;
+                }
+                xs[r2] = acc;
+            }
             __syncthreads();
         }
+        // dnext holds the entry of the last diagonal block
         for (int kb = N - kNB; kb >= 0; kb -= kNB) {
-            stage_diag(Ab, lda, kb, D);
+            const c32 dcur = dnext;
+            D[di][dj] = dcur;
+            if (kb > 0) dnext = ld(Ab, (size_t)(kb - kNB + dj) * lda + kb - kNB + di);
+            const int r = tid;
+            c32 ur[kNB];
+            if (r < kb) {
+#pragma unroll
+                for (int j = 0; j < kNB; ++j) ur[j] = ld(Ab, (size_t)(kb + j) * lda + r);
+            }
             __syncthreads();
             if (tid < 64) {
                 c32 x = lane < kNB ? xs[kb + lane] : c32{0.0f, 0.0f};
+#pragma unroll
                 for (int j = kNB - 1; j >= 0; --j) {
                     if (lane == j) x = cmul(x, cinv(D[j][j]));
-                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    const float xr = lane_value(x.re, j);
+                    const float xi = lane_value(x.im, j);
                     if (lane < j) {
                         const c32 u = D[lane][j];
                         x.re -= u.re * xr - u.im * xi;
@@ -1129,16 +1222,27 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
                 if (lane < kNB) xs[kb + lane] = x;
             }
             __syncthreads();
-            for (int r = tid; r < kb; r += 1024) {
+            if (r < kb) {
                 c32 acc = xs[r];
-#pragma unroll 8
+#pragma unroll
                 for (int j = 0; j < kNB; ++j) {
-                    const c32 u = ld(Ab, (size_t)(kb + j) * lda + r);
+                    const c32 u = ur[j];
                     const c32 xj = xs[kb + j];
                     acc.re -= u.re * xj.re - u.im * xj.im;
                     acc.im -= u.re * xj.im + u.im * xj.re;
                 }
                 xs[r] = acc;
+            }
+            for (int r2 = r + 1024; r2 < kb; r2 += 1024) {
+                c32 acc = xs[r2];
+#pragma unroll 8
+                for (int j = 0; j < kNB; ++j) {
+                    const c32 u = ld(Ab, (size_t)(kb + j) * lda + r2);
+                    const c32 xj = xs[kb + j];
+                    acc.re -= u.re * xj.re - u.im * xj.im;
+                    acc.im -= u.re * xj.im + u.im * xj.re;
+                }
+                xs[r2] = acc;
             }
             __syncthreads();
         }
@@ -1559,7 +1663,8 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
         const int K1 = std::min(N, K0 + kLuOuter);
         for (int k = K0; k < K1; k += kNB) {
             if (N - k <= 1024) {
-                hipLaunchKernelGGL(k_lu_panel_reg, dim3(B), dim3(1024), 0, stream, w.A, w.lda,
+                const int threads = (N - k + 63) / 64 * 64;
+                hipLaunchKernelGGL(k_lu_panel_reg, dim3(B), dim3(threads), 0, stream, w.A, w.lda,
                                    w.a_stride, N, k, w.ipiv, w.info, gl_of(k), gls);
             } else {
                 const int G = (N - k + 1023) / 1024;
@@ -1599,7 +1704,7 @@ void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_
     if (trans != 0 || N <= 2048) {
         const size_t shm = sizeof(float) * 2 * (size_t)N;
         hipLaunchKernelGGL(k_lu_solve, dim3(V), dim3(1024), shm, stream, w.A, w.lda, w.a_stride,
-                           N, w.ipiv, x, ldx, trans, vpm);
+                           N, w.ipiv, w.gl, (int64_t)(N / kNB) * kLuGather, x, ldx, trans, vpm);
         return;
     }
     for (int kb = 0; kb < N; kb += kNB) {
