@@ -153,7 +153,7 @@ struct hbem_session {
     int adj_F = 0, adj_nobs = 0;
     bool adj_complex = false, adj_robin = false;
     // per-call workspaces
-    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, lu_scr_i, lu_scr_f, amax,
+    DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, lu_scr_i, lu_scr_f, lu_pack, amax,
         lu_scale, kv, kh, kn, kf,
         q, kre, kim, fk, obs, field, tags,
         tagout, norms, anorm, active, rowsplit, rcx;
@@ -166,7 +166,7 @@ struct hbem_session {
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_t_f, &tri_t_i, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &chief_C, &chief_d, &chief_pts, &chief_z, &chief_e, &chief_z64, &chief_r, &chief_ainf, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
+                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &chief_C, &chief_d, &chief_pts, &chief_z, &chief_e, &chief_z64, &chief_r, &chief_ainf, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &lu_pack, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag, &io_beta, &qt, &l2g})
@@ -1010,6 +1010,10 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
     hbem::LuWorkspace w{s->A.as<float>(), lda, a_stride, N, n, B, s->ipiv.as<int32_t>(),
                         s->info.as<int32_t>(), s->gl.as<int32_t>(), s->lu_scr_i.as<int32_t>(),
                         s->lu_scr_f.as<float>()};
+    if (precision != 0 && N > hbem::kLuOuter) {
+        s->lu_pack.reserve(sizeof(uint16_t) * (size_t)B * hbem::lu_pack_u16(N));
+        w.pack = s->lu_pack.as<uint16_t>();
+    }
     // padding, norms, FP16 operand scale, LU, first solve: kernels and
     // device copies only (no host synchronisation, no memset nodes)
     auto enqueue = [&]() {
@@ -1037,7 +1041,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
             (uintptr_t)s->rhs.ptr, (uintptr_t)s->x.ptr, (uintptr_t)s->ipiv.ptr,
             (uintptr_t)s->info.ptr, (uintptr_t)s->gl.ptr, (uintptr_t)s->lu_scr_i.ptr,
             (uintptr_t)s->lu_scr_f.ptr, (uintptr_t)s->anorm.ptr, (uintptr_t)s->amax.ptr,
-            (uintptr_t)s->lu_scale.ptr};
+            (uintptr_t)s->lu_scale.ptr, (uintptr_t)w.pack};
         if (!s->lu_graph || s->lu_graph_key != key) {
             if (s->lu_graph) {
                 (void)hipGraphExecDestroy(s->lu_graph);

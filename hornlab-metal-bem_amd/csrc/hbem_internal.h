@@ -321,6 +321,7 @@ struct LuWorkspace {
     int32_t *gl;       // [B][N / 32][kLuGather] composed row exchanges per step
     int32_t *scratch_i;  // [B][kLuMaxOrder / 32] tournament candidates
     float *scratch_f;    // [B][32][32] + [B][64][32] complex: top block, gathered rows
+    uint16_t *pack = nullptr;  // [B][lu_pack_u16(N)]: 16-bit trailing operands (or null)
 };
 
 constexpr int kLuPad = 64;
@@ -331,6 +332,9 @@ constexpr int kLuMaxOrder = 32768;  // tournament panels nominate from <= 32 chu
 // sizes of LuWorkspace::scratch_i (ints) and scratch_f (floats) per system
 constexpr size_t kLuScratchI = kLuMaxOrder / 32;
 constexpr size_t kLuScratchF = (32 * 32 + 64 * 32) * 2;
+
+// 16-bit values of LuWorkspace::pack per system (four planes of N x kLuOuter)
+inline size_t lu_pack_u16(int N) { return (size_t)N * kLuOuter * 4; }
 
 // ints of LuWorkspace::gl per system
 inline size_t lu_gather_ints(int N) { return (size_t)(N / 32) * kLuGather; }

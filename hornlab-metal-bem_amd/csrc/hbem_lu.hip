@@ -916,6 +916,175 @@ __global__ __launch_bounds__(256, 2) void k_lu_gemm_tr(float *A, int64_t lda, in
             }
 }
 
+// Packed 16-bit operands of a rank-kLuOuter trailing update, written once
+// per outer block [K0, K0 + kLuOuter) so that the trailing tiles read 4 bytes
+// per complex operand instead of 8 and convert nothing.  Per system, four
+// planes of [N][kLuOuter] 16-bit values (re/im of L, re/im of U), k
+// contiguous: L plane row i = A[K1 + i][K0 + k]; U plane row j =
+// scale[b] * A[K0 + k][K1 + j] (FP16) or A[K0 + k][K1 + j] (BF16).
+template <int kPrec>
+__global__ __launch_bounds__(256) void k_lu_pack_l(const float *A, int64_t lda, int64_t a_stride,
+                                                   int K0, int K1, int M, uint16_t *pack,
+                                                   int64_t pstride, int64_t plane) {
+    // 64 rows x 32 k per block: thread = (row, 8 consecutive k)
+    const int b = blockIdx.z;
+    const float *Ab = A + (size_t)b * a_stride * 2;
+    const int row = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int k = blockIdx.y * 32 + (threadIdx.x >> 6) * 8;
+    if (row >= M) return;
+    typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+    u16x8 re, im;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const c32 v = ld(Ab, (size_t)(K0 + k + e) * lda + K1 + row);
+        re[e] = tr_round<kPrec>(v.re);
+        im[e] = tr_round<kPrec>(v.im);
+    }
+    uint16_t *pb = pack + (size_t)b * pstride + (size_t)row * kLuOuter + k;
+    *reinterpret_cast<u16x8 *>(pb) = re;
+    *reinterpret_cast<u16x8 *>(pb + plane) = im;
+}
+
+template <int kPrec>
+__global__ __launch_bounds__(256) void k_lu_pack_u(const float *A, int64_t lda, int64_t a_stride,
+                                                   int K0, int K1, int M, uint16_t *pack,
+                                                   int64_t pstride, int64_t plane,
+                                                   const float *scale) {
+    // 8 columns x 256 k per block: thread = (column, 8 consecutive k)
+    const int b = blockIdx.y;
+    const float *Ab = A + (size_t)b * a_stride * 2;
+    const int col = blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int k = (threadIdx.x & 31) * 8;
+    if (col >= M) return;
+    const float us = (kPrec == 1) ? scale[b] : 1.0f;
+    typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+    u16x8 re, im;
+    const float4 *src = reinterpret_cast<const float4 *>(Ab + 2 * ((size_t)(K1 + col) * lda + K0 + k));
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const float4 v = src[h];
+        re[2 * h] = tr_round<kPrec>(v.x * us);
+        im[2 * h] = tr_round<kPrec>(v.y * us);
+        re[2 * h + 1] = tr_round<kPrec>(v.z * us);
+        im[2 * h + 1] = tr_round<kPrec>(v.w * us);
+    }
+    uint16_t *pb = pack + (size_t)b * pstride + 2 * plane + (size_t)col * kLuOuter + k;
+    *reinterpret_cast<u16x8 *>(pb) = re;
+    *reinterpret_cast<u16x8 *>(pb + plane) = im;
+}
+
+// k_lu_gemm_tr on the packed operands (same tiles, LDS planes and MFMAs):
+// a stage of 32 k is 64 bytes per row and plane, loaded as 16-byte vectors
+// (thread = row or column, half of the k range) and stored to LDS as is.
+// Rows and columns >= M of the last tiles load zeros and store nothing.
+template <int kPrec>
+__global__ __launch_bounds__(256, 2) void k_lu_gemm_trp(float *A, int64_t lda, int64_t a_stride,
+                                                     int K1, int M, const uint16_t *pack,
+                                                     int64_t pstride, int64_t plane,
+                                                     const float *scale) {
+    const int b = blockIdx.z;
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const int r0 = blockIdx.x * kTrTile;  // rows / columns relative to K1
+    const int c0 = blockIdx.y * kTrTile;
+    __shared__ TrLds sm;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    const float us = (kPrec == 1) ? scale[b] : 1.0f;
+    const int sr = tid & (kTrTile - 1), sk = (tid >> 7) * 16;
+    const bool l_ok = r0 + sr < M, u_ok = c0 + sr < M;
+    const uint16_t *pb = pack + (size_t)b * pstride;
+    typedef unsigned w4 __attribute__((ext_vector_type(4)));
+    const w4 *lre = reinterpret_cast<const w4 *>(pb + (size_t)(r0 + sr) * kLuOuter + sk);
+    const w4 *lim = reinterpret_cast<const w4 *>(pb + plane + (size_t)(r0 + sr) * kLuOuter + sk);
+    const w4 *ure = reinterpret_cast<const w4 *>(pb + 2 * plane + (size_t)(c0 + sr) * kLuOuter + sk);
+    const w4 *uim = reinterpret_cast<const w4 *>(pb + 3 * plane + (size_t)(c0 + sr) * kLuOuter + sk);
+    w4 st_[8];
+    const w4 z = {0u, 0u, 0u, 0u};
+    auto load_stage = [&](int kk) {  // kk: stage start within the outer block
+        const int q = kk / 8;            // uint4 index
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            st_[h] = l_ok ? lre[q + h] : z;
+            st_[2 + h] = l_ok ? lim[q + h] : z;
+            st_[4 + h] = u_ok ? ure[q + h] : z;
+            st_[6 + h] = u_ok ? uim[q + h] : z;
+        }
+    };
+    auto store_stage = [&]() {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            *reinterpret_cast<w4 *>(&sm.Lr[sr][sk + 8 * h]) = st_[h];
+            *reinterpret_cast<w4 *>(&sm.Li[sr][sk + 8 * h]) = st_[2 + h];
+            *reinterpret_cast<w4 *>(&sm.Ur[sr][sk + 8 * h]) = st_[4 + h];
+            *reinterpret_cast<w4 *>(&sm.Ui[sr][sk + 8 * h]) = st_[6 + h];
+        }
+    };
+    f32x4 accr[4][4], acci[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            accr[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acci[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    using V = typename std::conditional<kPrec == 1, f16x8, bf16x8>::type;
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    load_stage(0);
+    for (int kk = 0; kk < kLuOuter; kk += kTrK) {
+        __syncthreads();  // the previous stage's operand reads are done
+        store_stage();
+        __syncthreads();
+        if (kk + kTrK < kLuOuter) load_stage(kk + kTrK);  // in flight during the MFMAs
+        V lr[4], lm[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            lr[t] = *reinterpret_cast<const V *>(&sm.Lr[wr + 16 * t + li][lk]);
+            lm[t] = *reinterpret_cast<const V *>(&sm.Li[wr + 16 * t + li][lk]);
+        }
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) {
+            const V ur = *reinterpret_cast<const V *>(&sm.Ur[wc + 16 * ti + li][lk]);
+            const V ui = *reinterpret_cast<const V *>(&sm.Ui[wc + 16 * ti + li][lk]);
+            u32x4 bits = *reinterpret_cast<const u32x4 *>(&ui);
+            bits ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};  // -Ui
+            const V uin = *reinterpret_cast<const V *>(&bits);
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                if constexpr (kPrec == 1) {
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ur, lr[tj], accr[ti][tj], 0, 0, 0);
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(uin, lm[tj], accr[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ui, lr[tj], acci[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ur, lm[tj], acci[ti][tj], 0, 0, 0);
+                } else {
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ur, lr[tj], accr[ti][tj], 0, 0, 0);
+                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uin, lm[tj], accr[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ui, lr[tj], acci[ti][tj], 0, 0, 0);
+                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ur, lm[tj], acci[ti][tj], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const float inv_us = 1.0f / us;  // exact: us is a power of two
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = c0 + wc + 16 * ti + (lane >> 4) * 4 + v;
+                const int row = r0 + wr + 16 * tj + li;
+                if (row < M && col < M) {
+                    const size_t idx = (size_t)(K1 + col) * lda + K1 + row;
+                    c32 x = ld(Ab, idx);
+                    x.re -= accr[ti][tj][v] * inv_us;
+                    x.im -= acci[ti][tj][v] * inv_us;
+                    st(Ab, idx, x);
+                }
+            }
+}
+
 // The FP32 trailing update with the same 128 x 128 tiling on
 // v_mfma_f32_16x16x4_f32: stages of 16 k staged through LDS planes [k][row |
 // col] (row stride 144 floats: the four k rows a wave reads per instruction
@@ -1686,7 +1855,33 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
                 swap_trsm(k, 1, K1, N, 2, 0, 0, 0);
                 gemm(0, k + kNB, K1, K1, N, k, k + kNB);
             }
-            if (precision == 0) {
+            if (precision != 0 && w.pack) {
+                // 16-bit operands packed once, then the 128-tile MFMA update
+                const int M = N - K1;
+                const int64_t plane = (int64_t)N * kLuOuter, pstride = 4 * plane;
+                if (precision == 1) {
+                    hipLaunchKernelGGL(k_lu_pack_l<1>, dim3((M + 63) / 64, kLuOuter / 32, B),
+                                       dim3(256), 0, stream, w.A, w.lda, w.a_stride, K0, K1, M,
+                                       w.pack, pstride, plane);
+                    hipLaunchKernelGGL(k_lu_pack_u<1>, dim3((M + 7) / 8, B), dim3(256), 0, stream,
+                                       w.A, w.lda, w.a_stride, K0, K1, M, w.pack, pstride, plane,
+                                       scale);
+                } else {
+                    hipLaunchKernelGGL(k_lu_pack_l<2>, dim3((M + 63) / 64, kLuOuter / 32, B),
+                                       dim3(256), 0, stream, w.A, w.lda, w.a_stride, K0, K1, M,
+                                       w.pack, pstride, plane);
+                    hipLaunchKernelGGL(k_lu_pack_u<2>, dim3((M + 7) / 8, B), dim3(256), 0, stream,
+                                       w.A, w.lda, w.a_stride, K0, K1, M, w.pack, pstride, plane,
+                                       scale);
+                }
+                const dim3 grid((M + kTrTile - 1) / kTrTile, (M + kTrTile - 1) / kTrTile, B);
+                if (precision == 1)
+                    hipLaunchKernelGGL(k_lu_gemm_trp<1>, grid, dim3(256), 0, stream, w.A, w.lda,
+                                       w.a_stride, K1, M, w.pack, pstride, plane, scale);
+                else
+                    hipLaunchKernelGGL(k_lu_gemm_trp<2>, grid, dim3(256), 0, stream, w.A, w.lda,
+                                       w.a_stride, K1, M, w.pack, pstride, plane, scale);
+            } else if (precision == 0) {
                 // the rank-kLuOuter trailing update in FP32 on the 128-tile kernel
                 const dim3 grid((N - K1 + kTrTile - 1) / kTrTile, (N - K1 + kTrTile - 1) / kTrTile, B);
                 hipLaunchKernelGGL(k_lu_gemm_tr32, grid, dim3(256), 0, stream, w.A, w.lda,
