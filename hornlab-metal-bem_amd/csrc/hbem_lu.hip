@@ -148,7 +148,8 @@ __device__ void build_gather_list(const int32_t *piv, int k, int32_t *out) {
                 tsrc = p;
             }
         }
-        const int sj = __shfl(tsrc, j), sip = __shfl(tsrc, ip);
+        const int sj = __builtin_amdgcn_readlane(tsrc, j);  // j, ip are wave-uniform
+        const int sip = __builtin_amdgcn_readlane(tsrc, ip);
         if (lane == j) tsrc = sip;
         if (lane == ip) tsrc = sj;
     }
@@ -416,7 +417,10 @@ __global__ __launch_bounds__(256) void k_lu_panel_apply(float *A, int64_t lda, i
 // Register panel for N - k <= 1024, one wave per 64 rows (blockDim.x =
 // 64 ceil((N - k) / 64)): thread t holds the kNB panel values of one row,
 // the column loop is unrolled so that every register index is
-// static.  A row exchange swaps only the logical row numbers of the two
+// static (a rolled loop selecting the column by predicates issues about
+// twice the instructions per column and runs twice as long; the kernel is
+// bound by the dependent instruction stream of a wave per column, not by
+// its barriers or its wave count).  A row exchange swaps only the logical row numbers of the two
 // threads involved (no data moves); every thread writes its values back to
 // its final logical row.  One block barrier per column: each wave reduces
 // its pivot candidate with DPP, and its candidate lane publishes the key and
@@ -463,7 +467,6 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
         const int p = static_cast<int>(0xffffffffu - static_cast<unsigned>(best));
         if (tid == 0) {
             piv_s[j] = p;
-            ipiv[(size_t)b * N + col] = p;
             if (!(__uint_as_float(static_cast<unsigned>(best >> 32)) > 0.0f) && bad == 0)
                 bad = col + 1;
         }
@@ -494,6 +497,8 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
     }
     if (tid == 0 && bad != 0 && info[b] == 0) info[b] = bad;
     __syncthreads();
+    // the step's pivots, once after the loop
+    if (tid < kNB) ipiv[(size_t)b * N + k + tid] = piv_s[tid];
     if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
 }
 
