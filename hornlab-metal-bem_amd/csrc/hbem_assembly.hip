@@ -1170,9 +1170,13 @@ __global__ void k_robin_neumann(const int32_t *__restrict__ l2g, int T, int vpk,
 // ---------------------------------------------------------------------------
 // Exterior field.  Lanes own observation points; a wave walks a chunk of
 // triangles (wave-uniform records).  grid = (obs blocks, triangle chunks, B).
+// Chunks of 16 triangles give the few observation waves of a frequency
+// enough siblings to hide the records' scalar-load latency (64: 0.63 ms,
+// 16: 0.55 ms, 8: 0.56 ms for the 3 x 37 points, 1856 triangles and 40
+// frequencies of a quarter-horn sweep).
 // ---------------------------------------------------------------------------
 
-constexpr int kFieldChunk = 64;
+constexpr int kFieldChunk = 16;
 
 template <int kMasks>
 __global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__restrict__ field_k,
@@ -1182,6 +1186,9 @@ __global__ __launch_bounds__(256) void k_field(DeviceGeometry g, const float *__
                                                float *__restrict__ out) {
     const int T = g.n_triangles;
     const int o = blockIdx.x * 256 + threadIdx.x;
+    // waves wholly past the last observation point have nothing to add (no
+    // barrier or LDS below)
+    if (blockIdx.x * 256 + (threadIdx.x & ~63) >= n_obs) return;
     const int b = blockIdx.z;  // vector; its wavenumber is field_k[b / vpk]
     const bool ok = o < n_obs;
     const int oo = ok ? o : 0;
