@@ -768,15 +768,17 @@ __global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t 
 // Rank-kLuOuter trailing update in FP16 / BF16 on the gfx950
 // v_mfma_f32_16x16x32_{f16,bf16}: 128 x 128 complex outputs per block of
 // 4 waves (2 x 2, 64 x 64 each = 4 x 4 MFMA tiles with separate real and
-// imaginary FP32 accumulators).  As in k_lu_gemm the MFMA computes the
-// transposed tile C^T = U^T L^T, so 16 lanes of an output row group hold 16
-// consecutive rows of one column (coalesced column-major stores).  The inner
-// range is staged 32 k at a time: each thread loads its share of the next
-// stage into registers while the MFMAs of the current stage run, then
-// converts it into the LDS planes [row | col][k] (row stride 40 elements,
-// so the 16-byte operand reads of a wave are conflict-free).  kPrec 1: U is
-// scaled by the power of two scale[b] before rounding to FP16 and the
-// product by 1/scale[b]; kPrec 2: BF16, round to nearest even.
+// imaginary FP32 accumulators), k_lu_gemm_trp below.  As in k_lu_gemm the
+// MFMA computes the transposed tile C^T = U^T L^T, so 16 lanes of an output
+// row group hold 16 consecutive rows of one column (coalesced column-major
+// stores).  The operands are rounded once per outer block into packed
+// 16-bit planes (k_lu_pack_l / k_lu_pack_u); the inner range is staged 32 k
+// at a time: each thread loads its share of the next stage into registers
+// while the MFMAs of the current stage run, then stores it into the LDS
+// planes [row | col][k] (row stride 40 elements, so the 16-byte operand
+// reads of a wave are conflict-free).  kPrec 1: U is scaled by the power of
+// two scale[b] before rounding to FP16 and the product by 1/scale[b];
+// kPrec 2: BF16, round to nearest even.
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int kTrTile = 128;  // output rows and columns per block
@@ -795,130 +797,6 @@ __device__ __forceinline__ unsigned short tr_round(float x) {
     } else {
         return f32_to_bf16_rne(x);
     }
-}
-
-template <int kPrec>
-__global__ __launch_bounds__(256, 2) void k_lu_gemm_tr(float *A, int64_t lda, int64_t a_stride,
-                                                    int r_begin, int r_end, int c_begin, int c_end,
-                                                    int k_begin, int k_end, const float *scale) {
-    const int b = blockIdx.z;
-    float *Ab = A + (size_t)b * a_stride * 2;
-    const int r0 = r_begin + blockIdx.x * kTrTile;
-    const int c0 = c_begin + blockIdx.y * kTrTile;
-    __shared__ TrLds sm;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
-    const int li = lane & 15, lk = (lane >> 4) * 8;
-    const float us = (kPrec == 1) ? scale[b] : 1.0f;
-    // staging of L: thread owns row (tid & 127) and k half (tid >> 7) * 16
-    // (a wave's loads cover 64 consecutive rows of one column); of U: 16
-    // lanes read the 32 k of one column as 16-byte pairs, a wave 4 columns
-    // per load, 8 loads per thread.
-    const int sr = tid & (kTrTile - 1), sk = (tid >> 7) * 16;
-    const int uk = 2 * (tid & 15), uc = tid >> 4;  // k pair, first column
-    const bool lrow_ok = r0 + sr < r_end;
-    float2 lv[16];
-    float4 uv[8];
-    auto load_stage = [&](int k) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-            lv[i] = lrow_ok ? *reinterpret_cast<const float2 *>(
-                                  Ab + 2 * ((size_t)(k + sk + i) * lda + r0 + sr))
-                            : make_float2(0.0f, 0.0f);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = c0 + uc + 16 * q;
-            uv[q] = c < c_end ? *reinterpret_cast<const float4 *>(
-                                    Ab + 2 * ((size_t)c * lda + k + uk))
-                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-    };
-    auto store_stage = [&]() {
-        typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-        typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            u16x8 lr, lim;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                lr[e] = tr_round<kPrec>(lv[8 * h + e].x);
-                lim[e] = tr_round<kPrec>(lv[8 * h + e].y);
-            }
-            *reinterpret_cast<u16x8 *>(&sm.Lr[sr][sk + 8 * h]) = lr;
-            *reinterpret_cast<u16x8 *>(&sm.Li[sr][sk + 8 * h]) = lim;
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = uc + 16 * q;
-            const u16x2 ur = {tr_round<kPrec>(uv[q].x * us), tr_round<kPrec>(uv[q].z * us)};
-            const u16x2 ui = {tr_round<kPrec>(uv[q].y * us), tr_round<kPrec>(uv[q].w * us)};
-            *reinterpret_cast<u16x2 *>(&sm.Ur[c][uk]) = ur;
-            *reinterpret_cast<u16x2 *>(&sm.Ui[c][uk]) = ui;
-        }
-    };
-    f32x4 accr[4][4], acci[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            accr[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-            acci[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
-    using V = typename std::conditional<kPrec == 1, f16x8, bf16x8>::type;
-    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    load_stage(k_begin);
-    for (int k = k_begin; k < k_end; k += kTrK) {
-        __syncthreads();  // the previous stage's operand reads are done
-        store_stage();
-        __syncthreads();
-        if (k + kTrK < k_end) load_stage(k + kTrK);  // in flight during the MFMAs
-        V lr[4], lim[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            lr[t] = *reinterpret_cast<const V *>(&sm.Lr[wr + 16 * t + li][lk]);
-            lim[t] = *reinterpret_cast<const V *>(&sm.Li[wr + 16 * t + li][lk]);
-        }
-#pragma unroll
-        for (int ti = 0; ti < 4; ++ti) {
-            const V ur = *reinterpret_cast<const V *>(&sm.Ur[wc + 16 * ti + li][lk]);
-            const V ui = *reinterpret_cast<const V *>(&sm.Ui[wc + 16 * ti + li][lk]);
-            u32x4 bits = *reinterpret_cast<const u32x4 *>(&ui);
-            bits ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};  // -Ui
-            const V uin = *reinterpret_cast<const V *>(&bits);
-#pragma unroll
-            for (int tj = 0; tj < 4; ++tj) {
-                if constexpr (kPrec == 1) {
-                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ur, lr[tj], accr[ti][tj], 0, 0, 0);
-                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(uin, lim[tj], accr[ti][tj], 0, 0, 0);
-                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ui, lr[tj], acci[ti][tj], 0, 0, 0);
-                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ur, lim[tj], acci[ti][tj], 0, 0, 0);
-                } else {
-                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ur, lr[tj], accr[ti][tj], 0, 0, 0);
-                    accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uin, lim[tj], accr[ti][tj], 0, 0, 0);
-                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ui, lr[tj], acci[ti][tj], 0, 0, 0);
-                    acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ur, lim[tj], acci[ti][tj], 0, 0, 0);
-                }
-            }
-        }
-    }
-    const float inv_us = 1.0f / us;  // exact: us is a power of two
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int col = c0 + wc + 16 * ti + (lane >> 4) * 4 + v;
-                const int row = r0 + wr + 16 * tj + li;
-                if (row < r_end && col < c_end) {
-                    const size_t idx = (size_t)col * lda + row;
-                    c32 x = ld(Ab, idx);
-                    x.re -= accr[ti][tj][v] * inv_us;
-                    x.im -= acci[ti][tj][v] * inv_us;
-                    st(Ab, idx, x);
-                }
-            }
 }
 
 // Packed 16-bit operands of a rank-kLuOuter trailing update, written once
@@ -978,10 +856,11 @@ __global__ __launch_bounds__(256) void k_lu_pack_u(const float *A, int64_t lda, 
     *reinterpret_cast<u16x8 *>(pb + plane) = im;
 }
 
-// k_lu_gemm_tr on the packed operands (same tiles, LDS planes and MFMAs):
-// a stage of 32 k is 64 bytes per row and plane, loaded as 16-byte vectors
-// (thread = row or column, half of the k range) and stored to LDS as is.
-// Rows and columns >= M of the last tiles load zeros and store nothing.
+// The trailing update on the packed operands: a stage of 32 k is 64 bytes
+// per row and plane, loaded as 16-byte vectors (thread = row or column, half
+// of the k range) and stored to LDS as is.  Rows and columns >= M of the
+// last tiles load zeros and store nothing.  Rows and columns are relative
+// to K1 (the update covers A[K1:N, K1:N], M = N - K1).
 template <int kPrec>
 __global__ __launch_bounds__(256, 2) void k_lu_gemm_trp(float *A, int64_t lda, int64_t a_stride,
                                                      int K1, int M, const uint16_t *pack,
@@ -1803,6 +1682,8 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
     static_assert(kLuPad % kNB == 0 && kLuOuter % kNB == 0, "whole panels");
     static_assert(kLuOuter % kTrK == 0 && kLuOuter % kTr32K == 0, "whole trailing-update stages");
     if (w.N > kLuMaxOrder) throw std::runtime_error("LU order above the supported maximum");
+    if (precision != 0 && w.N > kLuOuter && !w.pack)
+        throw std::runtime_error("16-bit LU needs the packed-operand workspace");
     // int32 zero has the bit pattern of float zero
     launch_fill_zero(reinterpret_cast<float *>(w.info), (size_t)w.B, stream);
     const int N = w.N, B = w.B;
@@ -1819,19 +1700,11 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
         hipLaunchKernelGGL(k_lu_swap_trsm, dim3(blocks, B), dim3(256), 0, stream, w.A, w.lda,
                            w.a_stride, k, n_steps, a0, a1, mode_a, b0, b1, mode_b, gl_of(k), gls);
     };
-    auto gemm = [&](int prec, int r0, int r1, int c0, int c1, int k0, int k1) {
+    auto gemm = [&](int r0, int r1, int c0, int c1, int k0, int k1) {
         if (r1 <= r0 || c1 <= c0) return;
         const dim3 grid((r1 - r0 + 63) / 64, (c1 - c0 + 63) / 64, B);
-        const dim3 tgrid((r1 - r0 + kTrTile - 1) / kTrTile, (c1 - c0 + kTrTile - 1) / kTrTile, B);
-        if (prec == 1)
-            hipLaunchKernelGGL(k_lu_gemm_tr<1>, tgrid, dim3(256), 0, stream, w.A, w.lda,
-                               w.a_stride, r0, r1, c0, c1, k0, k1, scale);
-        else if (prec == 2)
-            hipLaunchKernelGGL(k_lu_gemm_tr<2>, tgrid, dim3(256), 0, stream, w.A, w.lda,
-                               w.a_stride, r0, r1, c0, c1, k0, k1, scale);
-        else
-            hipLaunchKernelGGL(k_lu_gemm<0>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride,
-                               r0, r1, c0, c1, k0, k1, scale);
+        hipLaunchKernelGGL(k_lu_gemm<0>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride, r0,
+                           r1, c0, c1, k0, k1, scale);
     };
     for (int K0 = 0; K0 < N; K0 += kLuOuter) {
         const int K1 = std::min(N, K0 + kLuOuter);
@@ -1852,15 +1725,15 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
                                    w.scratch_f, w.scratch_f + (size_t)B * kNB * kNB * 2);
             }
             swap_trsm(k, 1, K0, k, 0, k + kNB, K1, 1);
-            gemm(0, k + kNB, N, k + kNB, K1, k, k + kNB);
+            gemm(k + kNB, N, k + kNB, K1, k, k + kNB);
         }
         if (K1 < N) {
             swap_trsm(K0, (K1 - K0) / kNB, K1, N, 0, 0, 0, 0);
             for (int k = K0; k < K1; k += kNB) {
                 swap_trsm(k, 1, K1, N, 2, 0, 0, 0);
-                gemm(0, k + kNB, K1, K1, N, k, k + kNB);
+                gemm(k + kNB, K1, K1, N, k, k + kNB);
             }
-            if (precision != 0 && w.pack) {
+            if (precision != 0) {
                 // 16-bit operands packed once, then the 128-tile MFMA update
                 const int M = N - K1;
                 const int64_t plane = (int64_t)N * kLuOuter, pstride = 4 * plane;
@@ -1886,13 +1759,11 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
                 else
                     hipLaunchKernelGGL(k_lu_gemm_trp<2>, grid, dim3(256), 0, stream, w.A, w.lda,
                                        w.a_stride, K1, M, w.pack, pstride, plane, scale);
-            } else if (precision == 0) {
+            } else {
                 // the rank-kLuOuter trailing update in FP32 on the 128-tile kernel
                 const dim3 grid((N - K1 + kTrTile - 1) / kTrTile, (N - K1 + kTrTile - 1) / kTrTile, B);
                 hipLaunchKernelGGL(k_lu_gemm_tr32, grid, dim3(256), 0, stream, w.A, w.lda,
                                    w.a_stride, K1, N, K1, N, K0, K1);
-            } else {
-                gemm(precision, K1, N, K1, N, K0, K1);
             }
         }
     }
