@@ -523,8 +523,6 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int
     const int ncols = min(64, (in_a ? a1 : b1) - cb);
     __shared__ c32 L11[kNB][kNB + 1];
     __shared__ c32 tile[2 * kNB][64 + 1];
-    __shared__ int trow[2 * kNB], tsrc[2 * kNB];
-    __shared__ int cnt_s;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int steps = (mode == 0) ? n_steps : 1;
     if (mode != 0)
@@ -535,19 +533,15 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int
     for (int st_i = 0; st_i < steps; ++st_i) {
         const int ks = k + kNB * st_i;
         const int32_t *g = gl + (size_t)b * gl_stride + (size_t)st_i * kLuGather;
-        if (tid == 0) cnt_s = (mode == 2) ? kNB : g[0];
-        __syncthreads();
-        const int cnt = cnt_s;
-        if (tid < cnt) {
-            trow[tid] = (mode == 2) ? ks + tid : g[1 + tid];
-            tsrc[tid] = (mode == 2) ? ks + tid : g[1 + 2 * kNB + tid];
-        }
-        __syncthreads();
+        // the step's gather record, lane m = entry m (wave-uniform count)
+        const int cnt = (mode == 2) ? kNB : g[0];
+        const bool has = lane < cnt;
+        const int my_row = (mode == 2) ? ks + lane : (has ? g[1 + lane] : 0);
+        const int my_src = (mode == 2) ? ks + lane : (has ? g[1 + 2 * kNB + lane] : 0);
 #pragma unroll
         for (int it = 0; it < 16; ++it) {
             const int cc = wave + 4 * it;
-            if (cc < ncols && lane < cnt)
-                tile[lane][cc] = ld(Ab, (size_t)(cb + cc) * lda + tsrc[lane]);
+            if (cc < ncols && has) tile[lane][cc] = ld(Ab, (size_t)(cb + cc) * lda + my_src);
         }
         __syncthreads();
         if (mode != 0 && tid < ncols) {
@@ -572,8 +566,7 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int
 #pragma unroll
         for (int it = 0; it < 16; ++it) {
             const int cc = wave + 4 * it;
-            if (cc < ncols && lane < cnt)
-                st(Ab, (size_t)(cb + cc) * lda + trow[lane], tile[lane][cc]);
+            if (cc < ncols && has) st(Ab, (size_t)(cb + cc) * lda + my_row, tile[lane][cc]);
         }
         __syncthreads();
     }
