@@ -572,31 +572,20 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(float *A, int64_t lda, int
     }
 }
 
-// Trailing update C -= L * U on MFMA (FP32 accumulation) over rows
-// [r_begin, r_end), columns [c_begin, c_end) and inner range
-// [k_begin, k_end) (a multiple of kNB): L = A[rows][k range],
-// U = A[k range][cols].  64 x 64 complex output tile per block; each of the
-// 4 waves owns a 32 x 32 sub-tile as 2 x 2 MFMA tiles with separate real
-// and imaginary accumulators (re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).
+// Rank-kNB update C -= L U on MFMA (FP32 operands, v_mfma_f32_16x16x4_f32,
+// FP32 accumulation) over rows [r_begin, r_end) and columns [c_begin,
+// c_end), with L = A[rows][k0, k0 + kNB) and U = A[k0, k0 + kNB)[cols]
+// (both outside the C block).  64 x 64 complex output tile per block; each
+// of the 4 waves owns a 32 x 32 sub-tile as 2 x 2 MFMA tiles with separate
+// real and imaginary accumulators (re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).
 // The MFMA computes the transposed tile C^T = U^T L^T so that the 16 lanes
 // of an output row group cover 16 consecutive matrix rows (coalesced
-// column-major stores).  The inner range is staged through LDS kNB at a
-// time.
-//   kPrec = 0: FP32 operands, v_mfma_f32_16x16x4_f32, planes [k][row|col]
-//              with an 80-float stride (conflict-free reads of the 4 k-rows
-//              a wave touches per instruction);
-//   kPrec = 1: FP16 operands, v_mfma_f32_16x16x16_f16; U is multiplied by
-//              the power of two scale[b] before rounding (FP16 range) and the
-//              product by 1/scale[b];
-//   kPrec = 2: BF16 operands (round to nearest even),
-//              v_mfma_f32_16x16x16_bf16.
-//   FP16/BF16 planes are [row|col][k] so that the 4 consecutive k of a lane
-//   are one 8-byte LDS read.
+// column-major stores).  L and U are staged in LDS planes [k][row | col]
+// with an 80-float stride (conflict-free reads of the 4 k-rows a wave
+// touches per instruction); the thread's 16 C entries are loaded first, so
+// their latency overlaps the staging and the MFMAs.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-typedef short i16x4 __attribute__((ext_vector_type(4)));
-constexpr int kGS = 80;   // FP32 plane stride (floats)
-constexpr int kHS = 40;   // 16-bit plane stride (elements)
+constexpr int kGS = 80;  // FP32 plane stride (floats)
 
 __device__ __forceinline__ unsigned short f32_to_bf16_rne(float x) {
     const unsigned u = __float_as_uint(x);
@@ -605,35 +594,34 @@ __device__ __forceinline__ unsigned short f32_to_bf16_rne(float x) {
     return static_cast<unsigned short>(r >> 16);
 }
 
-template <int kPrec>
-struct GemmLds;
-template <>
-struct GemmLds<0> {
+struct GemmLds {
     float Lr[kNB][kGS], Li[kNB][kGS], Ur[kNB][kGS], Ui[kNB][kGS];
 };
-template <>
-struct GemmLds<1> {
-    _Float16 Lr[64][kHS], Li[64][kHS], Ur[64][kHS], Ui[64][kHS];
-};
-template <>
-struct GemmLds<2> {
-    unsigned short Lr[64][kHS], Li[64][kHS], Ur[64][kHS], Ui[64][kHS];
-};
 
-template <int kPrec>
 __global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t a_stride,
                                                  int r_begin, int r_end, int c_begin, int c_end,
-                                                 int k_begin, int k_end, const float *scale) {
+                                                 int k0) {
     const int b = blockIdx.z;
     float *Ab = A + (size_t)b * a_stride * 2;
     const int r0 = r_begin + blockIdx.x * 64;
     const int c0 = c_begin + blockIdx.y * 64;
-    __shared__ GemmLds<kPrec> sm;
+    __shared__ GemmLds sm;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
-    const float us = (kPrec == 1) ? scale[b] : 1.0f;
+    c32 cv[2][2][4];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = c0 + wc + 16 * ti + 4 * lk + v;
+                const int row = r0 + wr + 16 * tj + li;
+                cv[ti][tj][v] = (row < r_end && col < c_end) ? ld(Ab, (size_t)col * lda + row)
+                                                             : c32{0.f, 0.f};
+            }
     f32x4 accr[2][2], acci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -642,104 +630,45 @@ __global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t 
             accr[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
             acci[a][c] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
-    for (int k = k_begin; k < k_end; k += kNB) {
-        __syncthreads();
 #pragma unroll
-        for (int it = 0; it < (kNB * 64) / 256; ++it) {
-            const int e = tid + 256 * it;
-            const int kk = e / 64, rr = e % 64;
-            const c32 v =
-                (r0 + rr < r_end) ? ld(Ab, (size_t)(k + kk) * lda + r0 + rr) : c32{0.f, 0.f};
-            if constexpr (kPrec == 0) {
-                sm.Lr[kk][rr] = v.re;
-                sm.Li[kk][rr] = v.im;
-            } else if constexpr (kPrec == 1) {
-                sm.Lr[rr][kk] = static_cast<_Float16>(v.re);
-                sm.Li[rr][kk] = static_cast<_Float16>(v.im);
-            } else {
-                sm.Lr[rr][kk] = f32_to_bf16_rne(v.re);
-                sm.Li[rr][kk] = f32_to_bf16_rne(v.im);
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < (kNB * 64) / 256; ++it) {
-            const int e = tid + 256 * it;
-            const int cc = e / kNB, kk = e % kNB;
-            const c32 v =
-                (c0 + cc < c_end) ? ld(Ab, (size_t)(c0 + cc) * lda + k + kk) : c32{0.f, 0.f};
-            if constexpr (kPrec == 0) {
-                sm.Ur[kk][cc] = v.re;
-                sm.Ui[kk][cc] = v.im;
-            } else if constexpr (kPrec == 1) {
-                sm.Ur[cc][kk] = static_cast<_Float16>(v.re * us);
-                sm.Ui[cc][kk] = static_cast<_Float16>(v.im * us);
-            } else {
-                sm.Ur[cc][kk] = f32_to_bf16_rne(v.re);
-                sm.Ui[cc][kk] = f32_to_bf16_rne(v.im);
-            }
-        }
-        __syncthreads();
-        if constexpr (kPrec == 0) {
-#pragma unroll
-            for (int ks = 0; ks < kNB; ks += 4) {
-                const int kk = ks + lk;
-                float ur[2], ui[2], lr[2], lim[2];
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    ur[t] = sm.Ur[kk][wc + 16 * t + li];
-                    ui[t] = sm.Ui[kk][wc + 16 * t + li];
-                    lr[t] = sm.Lr[kk][wr + 16 * t + li];
-                    lim[t] = sm.Li[kk][wr + 16 * t + li];
-                }
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-                    for (int tj = 0; tj < 2; ++tj) {
-                        // D^T[col][row]: A operand = U^T (col, k), B operand = L^T (k, row)
-                        accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
-                        accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[ti], lim[tj], accr[ti][tj], 0, 0, 0);
-                        acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
-                        acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
-                    }
-            }
-        } else {
-#pragma unroll
-            for (int ks = 0; ks < kNB; ks += 16) {
-                const int kk = ks + 4 * lk;
-                using V = typename std::conditional<kPrec == 1, f16x4, i16x4>::type;
-                V ur[2], ui[2], uin[2], lr[2], lim[2];
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    ur[t] = *reinterpret_cast<const V *>(&sm.Ur[wc + 16 * t + li][kk]);
-                    ui[t] = *reinterpret_cast<const V *>(&sm.Ui[wc + 16 * t + li][kk]);
-                    lr[t] = *reinterpret_cast<const V *>(&sm.Lr[wr + 16 * t + li][kk]);
-                    lim[t] = *reinterpret_cast<const V *>(&sm.Li[wr + 16 * t + li][kk]);
-                    // -Ui: flip the sign bits of the four 16-bit values
-                    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-                    u32x2 bits = *reinterpret_cast<const u32x2 *>(&ui[t]);
-                    bits ^= u32x2{0x80008000u, 0x80008000u};
-                    uin[t] = *reinterpret_cast<const V *>(&bits);
-                }
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-                    for (int tj = 0; tj < 2; ++tj) {
-                        if constexpr (kPrec == 1) {
-                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
-                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(uin[ti], lim[tj], accr[ti][tj], 0, 0, 0);
-                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
-                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
-                        } else {
-                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
-                            accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(uin[ti], lim[tj], accr[ti][tj], 0, 0, 0);
-                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
-                            acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
-                        }
-                    }
-            }
-        }
+    for (int it = 0; it < (kNB * 64) / 256; ++it) {
+        const int e = tid + 256 * it;
+        const int kk = e / 64, rr = e % 64;
+        const c32 v = (r0 + rr < r_end) ? ld(Ab, (size_t)(k0 + kk) * lda + r0 + rr) : c32{0.f, 0.f};
+        sm.Lr[kk][rr] = v.re;
+        sm.Li[kk][rr] = v.im;
     }
-    const float inv_us = 1.0f / us;  // exact: us is a power of two
+#pragma unroll
+    for (int it = 0; it < (kNB * 64) / 256; ++it) {
+        const int e = tid + 256 * it;
+        const int cc = e / kNB, kk = e % kNB;
+        const c32 v = (c0 + cc < c_end) ? ld(Ab, (size_t)(c0 + cc) * lda + k0 + kk) : c32{0.f, 0.f};
+        sm.Ur[kk][cc] = v.re;
+        sm.Ui[kk][cc] = v.im;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kNB; ks += 4) {
+        const int kk = ks + lk;
+        float ur[2], ui[2], lr[2], lim[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            ur[t] = sm.Ur[kk][wc + 16 * t + li];
+            ui[t] = sm.Ui[kk][wc + 16 * t + li];
+            lr[t] = sm.Lr[kk][wr + 16 * t + li];
+            lim[t] = sm.Li[kk][wr + 16 * t + li];
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                // D^T[col][row]: A operand = U^T (col, k), B operand = L^T (k, row)
+                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lr[tj], accr[ti][tj], 0, 0, 0);
+                accr[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[ti], lim[tj], accr[ti][tj], 0, 0, 0);
+                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[ti], lr[tj], acci[ti][tj], 0, 0, 0);
+                acci[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[ti], lim[tj], acci[ti][tj], 0, 0, 0);
+            }
+    }
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -749,11 +678,10 @@ __global__ __launch_bounds__(256) void k_lu_gemm(float *A, int64_t lda, int64_t 
                 const int col = c0 + wc + 16 * ti + 4 * lk + v;
                 const int row = r0 + wr + 16 * tj + li;
                 if (row < r_end && col < c_end) {
-                    const size_t idx = (size_t)col * lda + row;
-                    c32 x = ld(Ab, idx);
-                    x.re -= accr[ti][tj][v] * inv_us;
-                    x.im -= acci[ti][tj][v] * inv_us;
-                    st(Ab, idx, x);
+                    c32 x = cv[ti][tj][v];
+                    x.re -= accr[ti][tj][v];
+                    x.im -= acci[ti][tj][v];
+                    st(Ab, (size_t)col * lda + row, x);
                 }
             }
 }
@@ -1693,11 +1621,11 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
         hipLaunchKernelGGL(k_lu_swap_trsm, dim3(blocks, B), dim3(256), 0, stream, w.A, w.lda,
                            w.a_stride, k, n_steps, a0, a1, mode_a, b0, b1, mode_b, gl_of(k), gls);
     };
-    auto gemm = [&](int r0, int r1, int c0, int c1, int k0, int k1) {
+    auto gemm = [&](int r0, int r1, int c0, int c1, int k0) {
         if (r1 <= r0 || c1 <= c0) return;
         const dim3 grid((r1 - r0 + 63) / 64, (c1 - c0 + 63) / 64, B);
-        hipLaunchKernelGGL(k_lu_gemm<0>, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride, r0,
-                           r1, c0, c1, k0, k1, scale);
+        hipLaunchKernelGGL(k_lu_gemm, grid, dim3(256), 0, stream, w.A, w.lda, w.a_stride, r0, r1,
+                           c0, c1, k0);
     };
     for (int K0 = 0; K0 < N; K0 += kLuOuter) {
         const int K1 = std::min(N, K0 + kLuOuter);
@@ -1718,13 +1646,13 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
                                    w.scratch_f, w.scratch_f + (size_t)B * kNB * kNB * 2);
             }
             swap_trsm(k, 1, K0, k, 0, k + kNB, K1, 1);
-            gemm(k + kNB, N, k + kNB, K1, k, k + kNB);
+            gemm(k + kNB, N, k + kNB, K1, k);
         }
         if (K1 < N) {
             swap_trsm(K0, (K1 - K0) / kNB, K1, N, 0, 0, 0, 0);
             for (int k = K0; k < K1; k += kNB) {
                 swap_trsm(k, 1, K1, N, 2, 0, 0, 0);
-                gemm(k + kNB, K1, K1, N, k, k + kNB);
+                gemm(k + kNB, K1, K1, N, k);
             }
             if (precision != 0) {
                 // 16-bit operands packed once, then the 128-tile MFMA update
