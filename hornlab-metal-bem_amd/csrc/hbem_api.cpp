@@ -1187,7 +1187,8 @@ int batch_size_for(const hbem_session *s, int F, int N, const hbem_solve_options
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)16 << 30;
     const bool refine = opt.solve_mode != HBEM_SOLVE_F32;
-    const double per = (double)N * N * 8.0 * (refine ? 2.0 : 1.0) + (double)N * 64.0;
+    const double per = (double)N * N * 8.0 * (refine ? 2.0 : 1.0) + (double)N * 64.0 +
+                       (refine ? 2.0 * (double)hbem::lu_pack_u16(N) : 0.0);  // 16-bit LU operands
     const double budget = std::min(0.5 * (double)free_b, 96.0 * (1ull << 30));
     int b = (int)std::max(1.0, std::floor(budget / per));
     return std::max(1, std::min({F, b, 64}));
