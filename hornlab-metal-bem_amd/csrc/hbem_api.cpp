@@ -134,7 +134,7 @@ struct hbem_session {
     hipStream_t stream = nullptr;
     hbem::Plan plan;
     hbem::DeviceGeometry dg{};
-    DevBuf tri_f_aos, tri_i_aos, tri_t_f, tri_t_i, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, tile_start,
+    DevBuf tri_f_aos, tri_i_aos, tri_t_f, tri_t_i, tri_f_soa, tri_i_soa, sg_f, sg_i, sg_rule, sg_rule_reg, tile_start,
         tile_count, tile_ndofs, tile_dofs, tile_pairs, nc_f, nc_i, nc_rule, sg_adj, vtx_d, triv_d,
         normals_d, jac_d;
     // CHIEF rows and solves of the current resident batch
@@ -156,19 +156,20 @@ struct hbem_session {
     DevBuf A, A_orig, rhs, x, x64, r64, xprev, ipiv, info, gl, lu_scr_i, lu_scr_f, lu_pack, amax,
         lu_scale, kv, kh, kn, kf,
         q, kre, kim, fk, obs, field, tags,
-        tagout, norms, anorm, active, rowsplit, rcx;
+        tagout, norms, anorm, active, rowsplit, rcx, rcy, rc_state, rc_est;
     // staging of host-input batches
     DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag, io_beta, qt, l2g;
     double setup_seconds = 0.0;
     int64_t near_pairs = 0;
+    int assembly_mode = HBEM_ASSEMBLY_CORRECTED;
     std::string device_name;
     int cu_count = 0;
 
     ~hbem_session() {
         for (DevBuf *b : {&tri_f_aos, &tri_i_aos, &tri_t_f, &tri_t_i, &tri_f_soa, &tri_i_soa, &sg_f, &sg_i,
-                          &sg_rule, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &chief_C, &chief_d, &chief_pts, &chief_z, &chief_e, &chief_z64, &chief_r, &chief_ainf, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &lu_pack, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
+                          &sg_rule, &sg_rule_reg, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &chief_C, &chief_d, &chief_pts, &chief_z, &chief_e, &chief_z64, &chief_r, &chief_ainf, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &lu_pack, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
-                          &rowsplit, &rcx, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
+                          &rowsplit, &rcx, &rcy, &rc_state, &rc_est, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
                           &io_p, &io_tag, &io_beta, &qt, &l2g})
             b->release();
         if (lu_graph) (void)hipGraphExecDestroy(lu_graph);
@@ -254,148 +255,50 @@ void assemble_batch(hbem_session *s, int B, const float *k_real_h, const float *
 }
 
 
-// LAPACK clacn2-style 1-norm estimate of ||A^{-1}||_1 for each system of a
-// factored batch (Higham's algorithm, as in cgecon).  Returns rcond per
-// system given anorm.
+// Reciprocal condition number of each system of a factored batch, as the
+// reference's cgecon call (estimateReciprocalCondition, main.swift:475-503):
+// ||A^{-1}||_1 by the clacn2 iteration (Hager / Higham), one state machine per
+// system on the device, every round one batched A^{-1} and/or A^{-H} solve of
+// the systems that need it; the host only reads the B solve flags per round.
+// rcond = 1 / (||A||_1 ||A^{-1}||_1), 0 for a zero estimate or norm.
 void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, const std::vector<double> &anorm,
                     std::vector<double> *rcond) {
     const int B = w.B, n = w.n, N = w.N;
-    using cd = std::complex<double>;
-    std::vector<std::vector<cd>> X(B, std::vector<cd>(n));
-    std::vector<int> state(B, 0), iter(B, 0), jidx(B, 0);
-    std::vector<double> est(B, 0.0);
-    std::vector<double> altsgn(B, 1.0);
-    // state: 0 start(A^-1 of 1/n), 1 after first A^-H, 2 loop A^-1 of e_j,
-    // 3 loop A^-H, 4 alt A^-1, 5 done
-    for (int b = 0; b < B; ++b)
-        for (int i = 0; i < n; ++i) X[b][i] = cd(1.0 / n, 0.0);
-    std::vector<int> op(B);  // 0 = A^{-1}, 2 = A^{-H}, -1 none
-    for (int b = 0; b < B; ++b) op[b] = 0;
-    // own scratch: the solution in s->x must survive the estimate
+    hipStream_t st = s->stream;
     s->rcx.reserve((size_t)B * N * 2 * sizeof(float));
-    std::vector<float> hx((size_t)B * N * 2);
-    const double safmin = 1.1754943508222875e-38;
-    auto norm_sign = [&](std::vector<cd> &v) {
-        for (auto &z : v) {
-            const double a = std::abs(z);
-            z = a > safmin ? z / a : cd(1.0, 0.0);
-        }
-    };
-    auto argmax = [&](const std::vector<cd> &v) {
-        int j = 0;
-        double m = -1.0;
-        for (int i = 0; i < (int)v.size(); ++i)
-            if (std::abs(v[i]) > m) {
-                m = std::abs(v[i]);
-                j = i;
-            }
-        return j;
-    };
-    auto l1 = [&](const std::vector<cd> &v) {
-        double acc = 0.0;
-        for (const auto &z : v) acc += std::abs(z);
-        return acc;
-    };
-    for (int round = 0; round < 32; ++round) {
-        bool any = false;
+    s->rcy.reserve((size_t)B * N * 2 * sizeof(float));
+    s->rc_state.reserve(sizeof(int32_t) * 4 * (size_t)B);
+    s->rc_est.reserve(sizeof(double) * (size_t)B);
+    float *X = s->rcx.as<float>(), *Y = s->rcy.as<float>();
+    int32_t *state = s->rc_state.as<int32_t>();
+    double *est_d = s->rc_est.as<double>();
+    hbem::launch_rcond_init(X, N, n, N, B, state, est_d, st);
+    std::vector<int32_t> sw(4 * (size_t)B);
+    // every system needs an A^{-1} solve first; the iteration takes at most
+    // 2 + 2 * 4 + 1 solves per system (clacn2's itmax = 5)
+    bool need[3] = {true, false, false};
+    for (int round = 0; round < 16 && (need[0] || need[2]); ++round) {
         for (int trans : {0, 2}) {
-            bool need = false;
-            for (int b = 0; b < B; ++b) need |= (op[b] == trans);
-            if (!need) continue;
-            any = true;
-            std::fill(hx.begin(), hx.end(), 0.0f);
-            for (int b = 0; b < B; ++b)
-                if (op[b] == trans)
-                    for (int i = 0; i < n; ++i) {
-                        hx[((size_t)b * N + i) * 2] = (float)X[b][i].real();
-                        hx[((size_t)b * N + i) * 2 + 1] = (float)X[b][i].imag();
-                    }
-            hip_check(hipMemcpyAsync(s->rcx.ptr, hx.data(), hx.size() * sizeof(float),
-                                     hipMemcpyHostToDevice, s->stream),
-                      "rcond upload");
-            hbem::lu_solve(w, s->rcx.as<float>(), N, trans, s->stream);
-            hip_check(hipMemcpyAsync(hx.data(), s->rcx.ptr, hx.size() * sizeof(float),
-                                     hipMemcpyDeviceToHost, s->stream),
-                      "rcond download");
-            hip_check(hipStreamSynchronize(s->stream), "rcond sync");
-            for (int b = 0; b < B; ++b) {
-                if (op[b] != trans) continue;
-                for (int i = 0; i < n; ++i)
-                    X[b][i] = cd(hx[((size_t)b * N + i) * 2], hx[((size_t)b * N + i) * 2 + 1]);
-                op[b] = -1;
-                std::vector<cd> &v = X[b];
-                switch (state[b]) {
-                case 0:  // after A^{-1} (1/n)
-                    if (n == 1) {
-                        est[b] = std::abs(v[0]);
-                        state[b] = 5;
-                        break;
-                    }
-                    est[b] = l1(v);
-                    norm_sign(v);
-                    state[b] = 1;
-                    op[b] = 2;
-                    break;
-                case 1:  // after A^{-H}
-                    jidx[b] = argmax(v);
-                    iter[b] = 2;
-                    std::fill(v.begin(), v.end(), cd(0.0, 0.0));
-                    v[jidx[b]] = cd(1.0, 0.0);
-                    state[b] = 2;
-                    op[b] = 0;
-                    break;
-                case 2: {  // after A^{-1} e_j
-                    const double estold = est[b];
-                    est[b] = l1(v);
-                    if (est[b] <= estold) {
-                        est[b] = std::max(est[b], estold);
-                        state[b] = 4;
-                    } else {
-                        norm_sign(v);
-                        state[b] = 3;
-                        op[b] = 2;
-                        break;
-                    }
-                    // fallthrough to the alternating-sign vector
-                    for (int i = 0; i < n; ++i)
-                        v[i] = cd((i % 2 == 0 ? 1.0 : -1.0) *
-                                      (1.0 + (n > 1 ? (double)i / (n - 1) : 0.0)),
-                                  0.0);
-                    op[b] = 0;
-                    break;
-                }
-                case 3: {  // after A^{-H} in the loop
-                    const int jlast = jidx[b];
-                    jidx[b] = argmax(v);
-                    if (std::abs(v[jlast]) != std::abs(v[jidx[b]]) && iter[b] < 5) {
-                        iter[b]++;
-                        std::fill(v.begin(), v.end(), cd(0.0, 0.0));
-                        v[jidx[b]] = cd(1.0, 0.0);
-                        state[b] = 2;
-                        op[b] = 0;
-                    } else {
-                        for (int i = 0; i < n; ++i)
-                            v[i] = cd((i % 2 == 0 ? 1.0 : -1.0) *
-                                          (1.0 + (n > 1 ? (double)i / (n - 1) : 0.0)),
-                                      0.0);
-                        state[b] = 4;
-                        op[b] = 0;
-                    }
-                    break;
-                }
-                case 4: {  // after A^{-1} of the alternating vector
-                    const double temp = 2.0 * l1(v) / (3.0 * n);
-                    if (temp > est[b]) est[b] = temp;
-                    state[b] = 5;
-                    break;
-                }
-                default:
-                    break;
-                }
-            }
+            if (!need[trans]) continue;
+            hbem::launch_rcond_copy(X, Y, N, N, B, st);
+            hbem::lu_solve(w, Y, N, trans, st);
+            hbem::launch_rcond_step(Y, X, N, n, N, B, trans, state, est_d, st);
         }
-        if (!any) break;
+        hip_check(hipGetLastError(), "rcond launch");
+        hip_check(hipMemcpyAsync(sw.data(), state, sizeof(int32_t) * sw.size(),
+                                 hipMemcpyDeviceToHost, st),
+                  "rcond state D2H");
+        hip_check(hipStreamSynchronize(st), "rcond sync");
+        need[0] = need[2] = false;
+        for (int b = 0; b < B; ++b) {
+            const int op = sw[4 * (size_t)b + 1];
+            if (op == 0 || op == 2) need[op] = true;
+        }
     }
+    std::vector<double> est(B);
+    hip_check(hipMemcpyAsync(est.data(), est_d, sizeof(double) * B, hipMemcpyDeviceToHost, st),
+              "rcond D2H");
+    hip_check(hipStreamSynchronize(st), "rcond sync");
     rcond->assign(B, 0.0);
     for (int b = 0; b < B; ++b)
         (*rcond)[b] = (anorm[b] > 0.0 && est[b] > 0.0) ? (1.0 / est[b]) / anorm[b] : 0.0;
@@ -454,6 +357,7 @@ int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
         upload(s->sg_f, P.sg_f, s->stream);
         upload(s->sg_i, P.sg_i, s->stream);
         upload(s->sg_rule, P.sg_rule, s->stream);
+        upload(s->sg_rule_reg, P.sg_rule_reg, s->stream);
         upload(s->tile_start, P.tile_start, s->stream);
         upload(s->tile_count, P.tile_count, s->stream);
         upload(s->tile_ndofs, P.tile_ndofs, s->stream);
@@ -551,6 +455,24 @@ int32_t hbem_session_set_near_field(hbem_session *s, int32_t level, double thres
     });
 }
 
+int32_t hbem_session_set_assembly_mode(hbem_session *s, int32_t mode) {
+    return guarded([&] {
+        if (!s) fail(HBEM_EINVAL, "hbem_session_set_assembly_mode: null session");
+        if (mode != HBEM_ASSEMBLY_CORRECTED && mode != HBEM_ASSEMBLY_REGULAR)
+            fail(HBEM_EINVAL, "hbem_session_set_assembly_mode: unknown mode");
+        std::lock_guard<std::mutex> lock(s->mu);
+        const hbem::Plan &P = s->plan;
+        const bool reg = mode == HBEM_ASSEMBLY_REGULAR;
+        s->dg.sg_rule = (reg ? s->sg_rule_reg : s->sg_rule).as<float>();
+        for (int i = 0; i < 3; ++i) {
+            s->dg.sg_rule_offset[i] = reg ? P.sg_rule_reg_offset[i] : P.sg_rule_offset[i];
+            s->dg.sg_rule_count[i] = reg ? P.sg_rule_reg_count[i] : P.sg_rule_count[i];
+        }
+        s->assembly_mode = mode;
+        s->adj_valid = false;
+    });
+}
+
 int32_t hbem_session_destroy(hbem_session *session) {
     return guarded([&] {
         if (!session) return;
@@ -621,6 +543,24 @@ int32_t hbem_assemble(hbem_session *s, float k_real, float k_imag, const float *
 
 namespace {
 
+// Target relative residual ||b - A x||_inf / ||b||_inf of the refinement:
+// the caller's tolerance; for the FP64-accurate mode at most 1e-13, so the
+// iteration runs to the FP64 residual floor (it stops earlier when a restart
+// brings no progress, as LAPACK's refinement does).
+double refine_tolerance(const hbem_solve_options &o) {
+    if (o.solve_mode != HBEM_SOLVE_F64) return o.refine_tolerance;
+    return o.refine_tolerance > 0.0 ? std::min(o.refine_tolerance, 1e-13) : 1e-13;
+}
+
+// Residual above which a refined system reports refine_stalled: the target,
+// and for the FP64-accurate mode 1e-10 (a thousand times below what the FP32
+// LU alone leaves, with the FP64 floor out of reach only for systems near
+// FP32 singularity).
+double stall_threshold(const hbem_solve_options &o) {
+    const double tol = refine_tolerance(o);
+    return o.solve_mode == HBEM_SOLVE_F64 ? std::max(tol, 1e-10) : tol;
+}
+
 // GMRES-based iterative refinement (Carson-Higham GMRES-IR): residuals
 // r = b - A x in FP64 with the FP32 operator, correction equations
 // M^{-1} A d = M^{-1} r solved by GMRES with M the (low-precision) LU applied
@@ -630,14 +570,14 @@ namespace {
 // residual on entry and exit; best[b] the relative residual ||r||/||b||
 // (inf-norms).  Returns the Arnoldi steps taken per system.
 void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int vpm, int n, int N,
-                  const hbem_solve_options &opt, const std::function<void()> &residual,
-                  std::vector<double> &nrm, std::vector<double> &best,
-                  std::vector<int32_t> &steps) {
+                  const hbem_solve_options &opt, double tol, int max_restarts,
+                  const std::function<void()> &residual, std::vector<double> &nrm,
+                  std::vector<double> &best, std::vector<int32_t> &steps,
+                  std::vector<char> &stalled) {
     using cd = std::complex<double>;
     hipStream_t st = s->stream;
     const int64_t lda = N, a_stride = (int64_t)N * N;
     const int m = std::max(1, std::min(opt.refine_max_iterations, 20));
-    const int max_restarts = 4;
     const int64_t vstride = (int64_t)(m + 1) * N;
     s->kv.reserve(sizeof(double) * 2 * (size_t)B * vstride);
     s->kh.reserve(sizeof(double) * 2 * (size_t)B * (m + 1) * 2);
@@ -672,7 +612,7 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int vpm, i
     for (int restart = 0; restart < max_restarts; ++restart) {
         bool any = false;
         for (int b = 0; b < B; ++b) {
-            active[b] = best[b] > opt.refine_tolerance ? 1 : 0;
+            active[b] = (best[b] > tol && !stalled[b]) ? 1 : 0;
             any |= active[b] != 0;
         }
         if (!any) break;
@@ -697,7 +637,7 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int vpm, i
         for (int b = 0; b < B; ++b) {
             g[b][0] = cd(beta[b], 0.0);
             if (!active[b] || beta[b] == 0.0) done[b] = 1;
-            inner[b] = std::max(1e-14, 0.1 * opt.refine_tolerance / std::max(best[b], 1e-300));
+            inner[b] = std::max(1e-14, 0.1 * tol / std::max(best[b], 1e-300));
         }
         int j = 0;
         for (; j < m; ++j) {
@@ -779,12 +719,27 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int vpm, i
         hip_check(hipMemcpyAsync(s->active.ptr, active.data(), sizeof(int32_t) * B,
                                  hipMemcpyHostToDevice, st),
                   "active H2D");
+        hip_check(hipMemcpyAsync(s->xprev.ptr, s->x64.ptr, sizeof(double) * 2 * (size_t)B * N,
+                                 hipMemcpyDeviceToDevice, st),
+                  "xprev copy");
         hbem::launch_combine_f64(V, N, vstride, m + 1, h1, m + 1, 1.0, s->x64.as<double>(), N, n,
                                  B, s->active.as<int32_t>(), st);
         residual();
         for (int b = 0; b < B; ++b) {
+            if (!active[b]) continue;
             const double rel = nrm[2 * b + 1] > 0.0 ? nrm[2 * b] / nrm[2 * b + 1] : nrm[2 * b];
-            if (active[b]) best[b] = rel;
+            if (rel >= best[b]) {
+                // no progress: keep the previous iterate and stop refining this
+                // system (as the classical loop, main.swift:7051-7058)
+                hip_check(hipMemcpyAsync(s->x64.as<double>() + (size_t)b * N * 2,
+                                         s->xprev.as<double>() + (size_t)b * N * 2,
+                                         sizeof(double) * 2 * (size_t)N, hipMemcpyDeviceToDevice,
+                                         st),
+                          "revert");
+                stalled[b] = 1;
+                continue;
+            }
+            best[b] = rel;
         }
     }
 }
@@ -988,9 +943,10 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
     const int V = B * S;
     // CHIEF solves in the FP64-refined mode, as the reference's least squares
     // runs in FP64 (sweep.py:123)
-    const bool refine = opt.solve_mode == HBEM_SOLVE_F32_IR ||
-                        opt.solve_mode == HBEM_SOLVE_MIXED_F16 ||
-                        opt.solve_mode == HBEM_SOLVE_MIXED_BF16 || opt.n_chief > 0;
+    const bool refine = opt.solve_mode != HBEM_SOLVE_F32 || opt.n_chief > 0;
+    const bool gmres = opt.solve_mode == HBEM_SOLVE_MIXED_F16 ||
+                       opt.solve_mode == HBEM_SOLVE_MIXED_BF16 ||
+                       opt.solve_mode == HBEM_SOLVE_F64;
     const int precision = opt.solve_mode == HBEM_SOLVE_MIXED_F16    ? 1
                           : opt.solve_mode == HBEM_SOLVE_MIXED_BF16 ? 2
                                                                     : 0;
@@ -1030,7 +986,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)V * N,
                                  hipMemcpyDeviceToDevice, st),
                   "rhs copy");
-        hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
+        hbem::lu_solve(w, s->x.as<float>(), N, opt.solve_trans, st, S);
     };
     if (opt.lu_graph) {
         // One hipGraph per batch shape and buffer set, replayed while they
@@ -1077,8 +1033,9 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
     hip_check(hipStreamSynchronize(st), "solve sync");
     std::vector<int32_t> iters(V, 0);
     std::vector<double> resid(V, -1.0);
+    std::vector<char> stalled(V, 0);
     auto refine_all = [&]() {
-    if (refine && opt.refine_max_iterations > 0) {
+        if (!refine || opt.refine_max_iterations <= 0) return;
         s->x64.reserve(sizeof(double) * 2 * (size_t)V * N);
         s->r64.reserve(sizeof(double) * 2 * (size_t)V * N);
         s->xprev.reserve(sizeof(double) * 2 * (size_t)V * N);
@@ -1088,6 +1045,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         hbem::launch_copy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, V, st);
         std::vector<double> best(V, 0.0), nrm(2 * V);
         std::vector<int32_t> active(V, 1);
+        std::fill(stalled.begin(), stalled.end(), 0);
         auto residual = [&]() {
             hbem::launch_residual_f64(s->A_orig.as<float>(), lda, a_stride, n, V,
                                       s->rhs.as<float>(), N, s->x64.as<double>(), N,
@@ -1098,55 +1056,66 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
             hip_check(hipStreamSynchronize(st), "residual sync");
         };
         residual();
+        const double tol = refine_tolerance(opt);
         for (int v = 0; v < V; ++v) {
             best[v] = nrm[2 * v + 1] > 0.0 ? nrm[2 * v] / nrm[2 * v + 1] : nrm[2 * v];
-            if (best[v] <= opt.refine_tolerance) active[v] = 0;
+            if (best[v] <= tol) active[v] = 0;
         }
-        // mixed precision: GMRES-IR; FP32 LU: classical refinement as the
-        // reference's FP64 dense mode (main.swift:6964-7070)
-        if (precision != 0) {
-            gmres_refine(s, w, V, S, n, N, opt, residual, nrm, best, iters);
-            std::fill(active.begin(), active.end(), 0);
-        }
-        for (int it = 0; it < opt.refine_max_iterations && precision == 0; ++it) {
-            bool any = false;
-            for (int v = 0; v < V; ++v) any |= active[v] != 0;
-            if (!any) break;
-            hip_check(hipMemcpyAsync(s->active.ptr, active.data(), sizeof(int32_t) * V,
-                                     hipMemcpyHostToDevice, st),
-                      "active H2D");
-            hip_check(hipMemcpyAsync(s->xprev.ptr, s->x64.ptr, sizeof(double) * 2 * (size_t)V * N,
-                                     hipMemcpyDeviceToDevice, st),
-                      "xprev copy");
-            hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * V * N, st);
-            hbem::launch_copy_f64_to_f32(s->r64.as<double>(), s->x.as<float>(), N, N, n, V, st);
-            hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
-            hbem::launch_axpy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, V,
-                                         s->active.as<int32_t>(), st);
-            residual();
-            for (int v = 0; v < V; ++v) {
-                if (!active[v]) continue;
-                const double rel =
-                    nrm[2 * v + 1] > 0.0 ? nrm[2 * v] / nrm[2 * v + 1] : nrm[2 * v];
-                if (rel >= best[v]) {
-                    // stalled: keep the previous iterate (main.swift:7051-7058)
-                    hip_check(hipMemcpyAsync(s->x64.as<double>() + (size_t)v * N * 2,
-                                             s->xprev.as<double>() + (size_t)v * N * 2,
-                                             sizeof(double) * 2 * (size_t)N,
-                                             hipMemcpyDeviceToDevice, st),
-                              "revert");
-                    active[v] = 0;
-                    continue;
+        if (gmres) {
+            // mixed precision and the FP64-accurate mode: GMRES-IR with the
+            // FP32-stored LU as preconditioner
+            gmres_refine(s, w, V, S, n, N, opt, tol, opt.solve_mode == HBEM_SOLVE_F64 ? 8 : 4,
+                         residual, nrm, best, iters, stalled);
+        } else {
+            // FP32 LU: classical refinement as the reference's
+            // refineDenseSolveSolution (main.swift:6964-7070)
+            for (int it = 0; it < opt.refine_max_iterations; ++it) {
+                bool any = false;
+                for (int v = 0; v < V; ++v) any |= active[v] != 0;
+                if (!any) break;
+                hip_check(hipMemcpyAsync(s->active.ptr, active.data(), sizeof(int32_t) * V,
+                                         hipMemcpyHostToDevice, st),
+                          "active H2D");
+                hip_check(hipMemcpyAsync(s->xprev.ptr, s->x64.ptr,
+                                         sizeof(double) * 2 * (size_t)V * N,
+                                         hipMemcpyDeviceToDevice, st),
+                          "xprev copy");
+                hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * V * N, st);
+                hbem::launch_copy_f64_to_f32(s->r64.as<double>(), s->x.as<float>(), N, N, n, V,
+                                             st);
+                hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
+                hbem::launch_axpy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, V,
+                                             s->active.as<int32_t>(), st);
+                residual();
+                for (int v = 0; v < V; ++v) {
+                    if (!active[v]) continue;
+                    const double rel =
+                        nrm[2 * v + 1] > 0.0 ? nrm[2 * v] / nrm[2 * v + 1] : nrm[2 * v];
+                    if (rel >= best[v]) {
+                        // stalled: keep the previous iterate (main.swift:7051-7058)
+                        hip_check(hipMemcpyAsync(s->x64.as<double>() + (size_t)v * N * 2,
+                                                 s->xprev.as<double>() + (size_t)v * N * 2,
+                                                 sizeof(double) * 2 * (size_t)N,
+                                                 hipMemcpyDeviceToDevice, st),
+                                  "revert");
+                        stalled[v] = 1;
+                        active[v] = 0;
+                        continue;
+                    }
+                    best[v] = rel;
+                    iters[v]++;
+                    if (rel <= tol) active[v] = 0;
                 }
-                best[v] = rel;
-                iters[v]++;
-                if (rel <= opt.refine_tolerance) active[v] = 0;
             }
         }
         hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * V * N, st);
         hbem::launch_copy_f64_to_f32(s->x64.as<double>(), s->x.as<float>(), N, N, n, V, st);
-        for (int v = 0; v < V; ++v) resid[v] = best[v];
-    }
+        const double stall_at = stall_threshold(opt);
+        for (int v = 0; v < V; ++v) {
+            resid[v] = best[v];
+            // stopped above the target (no progress, or out of iterations)
+            stalled[v] = best[v] > stall_at ? 1 : 0;
+        }
     };
     refine_all();
     std::vector<double> chief_res(V, -1.0);
@@ -1160,7 +1129,14 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         chief_residuals(s, B, S, n, N, opt, &chief_res);
     }
     std::vector<double> rcond(B, -1.0);
-    if (opt.estimate_rcond) estimate_rcond(s, w, anorm, &rcond);
+    double rcond_s = 0.0;
+    if (opt.estimate_rcond) {
+        Timer tr;
+        tr.start(st);
+        estimate_rcond(s, w, anorm, &rcond);
+        tr.stop(st);
+        rcond_s = tr.seconds();
+    }
     t.stop(st);
     *solve_seconds = t.seconds();
     diag->resize(B);
@@ -1169,7 +1145,10 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         d.lapack_info = info[b];
         d.refine_iterations = 0;
         d.refine_residual_rel = -1.0;
+        d.refine_stalled = 0;
+        d.rcond_s = rcond_s / B;
         for (int src = 0; src < S; ++src) {
+            d.refine_stalled |= stalled[(size_t)b * S + src];
             d.refine_iterations = std::max(d.refine_iterations, iters[(size_t)b * S + src]);
             d.refine_residual_rel = std::max(d.refine_residual_rel, resid[(size_t)b * S + src]);
         }
@@ -1196,8 +1175,13 @@ int batch_size_for(const hbem_session *s, int F, int N, const hbem_solve_options
 
 
 void check_options(const hbem_solve_options &o) {
-    if (o.solve_mode < HBEM_SOLVE_F32 || o.solve_mode > HBEM_SOLVE_MIXED_BF16)
+    if (o.solve_mode < HBEM_SOLVE_F32 || o.solve_mode > HBEM_SOLVE_F64)
         fail(HBEM_EINVAL, "solve_mode out of range");
+    if (o.solve_mode == HBEM_SOLVE_F64 && o.refine_max_iterations <= 0)
+        fail(HBEM_EINVAL, "the FP64-accurate solve needs refine_max_iterations > 0");
+    if (o.solve_trans != 0 && o.solve_trans != 2) fail(HBEM_EINVAL, "solve_trans must be 0 or 2");
+    if (o.solve_trans == 2 && (o.solve_mode != HBEM_SOLVE_F32 || o.n_chief > 0))
+        fail(HBEM_EINVAL, "conjugate-transpose solves take solve_mode FP32 without CHIEF rows");
     if (o.refine_max_iterations < 0) fail(HBEM_EINVAL, "refine_max_iterations must be >= 0");
     if (!(o.refine_tolerance >= 0.0)) fail(HBEM_EINVAL, "refine_tolerance must be >= 0");
     if (o.n_tags < 0 || (o.n_tags > 0 && !o.tags)) fail(HBEM_EINVAL, "tags invalid");
@@ -1317,6 +1301,7 @@ int32_t hbem_solve_batch(hbem_session *s, int32_t F, const float *k_real, const 
         if (n_obs < 0 || (n_obs > 0 && (!obs_3xn || !field_re || !field_im)))
             fail(HBEM_EINVAL, "hbem_solve_batch: observation arrays invalid");
         check_options(*opts);
+        if (opts->solve_trans != 0) fail(HBEM_EINVAL, "hbem_solve_batch: solve_trans must be 0");
         for (int f = 0; f < F; ++f) {
             if (!std::isfinite(k_real[f]) || !std::isfinite(field_k[f]))
                 fail(HBEM_EINVAL, "hbem_solve_batch: wavenumbers must be finite");
@@ -1421,6 +1406,8 @@ int32_t hbem_solve_batch_device(hbem_session *s, int32_t F, const float *k_real,
         if (n_obs < 0 || (n_obs > 0 && (!obs_3xn || !field)))
             fail(HBEM_EINVAL, "hbem_solve_batch_device: observation arrays invalid");
         check_options(*opts);
+        if (opts->solve_trans != 0)
+            fail(HBEM_EINVAL, "hbem_solve_batch_device: solve_trans must be 0");
         std::lock_guard<std::mutex> lock(s->mu);
         s->adj_valid = false;
         set_device(s);
@@ -1447,12 +1434,15 @@ int32_t hbem_adjoint_forward(hbem_session *s, int32_t F, const float *k_real,
         if (!k_real || !field_k || !q || !tag_avg || n_obs <= 0 || !obs_3xn || !field)
             fail(HBEM_EINVAL, "hbem_adjoint_forward: k_real, field_k, q, tag_avg, obs, field required");
         check_options(*opts);
+        if (opts->solve_trans != 0) fail(HBEM_EINVAL, "hbem_adjoint_forward: solve_trans must be 0");
         if (opts->n_sources > 1)
             fail(HBEM_EINVAL, "hbem_adjoint_forward: one source per frequency");
         std::lock_guard<std::mutex> lock(s->mu);
         s->adj_valid = false;
         if (s->dg.nc_np > 0)
             fail(HBEM_EINVAL, "hbem_adjoint_forward: near-field quadrature must be off");
+        if (s->assembly_mode != HBEM_ASSEMBLY_CORRECTED)
+            fail(HBEM_EUNSUPPORTED, "hbem_adjoint_forward: needs the corrected assembly mode");
         set_device(s);
         hipStream_t st = s->stream;
         const int T = s->plan.n_triangles;
@@ -1613,6 +1603,13 @@ int32_t hbem_dense_solve(int32_t n_sys, int32_t n, const float *A_re, const floa
         if (n_sys <= 0 || n <= 0) fail(HBEM_EINVAL, "hbem_dense_solve: empty problem");
         if (!A_re || !A_im || !b_re || !b_im || !x_re || !x_im || !opts)
             fail(HBEM_EINVAL, "hbem_dense_solve: null array");
+        check_options(*opts);
+        if (opts->n_chief > 0)
+            fail(HBEM_EINVAL, "hbem_dense_solve: CHIEF rows need an assembled session");
+        if (opts->n_sources > 1)
+            fail(HBEM_EINVAL, "hbem_dense_solve: one right-hand side per system");
+        if ((int64_t)n > hbem::kLuMaxOrder)
+            fail(HBEM_EINVAL, "hbem_dense_solve: order above the supported maximum (32768)");
         const int dev = gfx950_device();
         if (dev < 0) fail(HBEM_ENODEV, "no gfx950 (MI355X) device is visible");
         // A transient session without geometry carries the workspaces.

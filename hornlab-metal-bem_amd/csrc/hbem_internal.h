@@ -121,6 +121,14 @@ struct Plan {
     std::vector<float> sg_rule;
     int sg_rule_offset[3] = {0, 0, 0};  // in points
     int sg_rule_count[3] = {0, 0, 0};
+    // The same classes under the plain 6 x 6 rule (regular quadrature on the
+    // touching pairs too, the reference's "optimized" assembly): per kind the
+    // 36 tensor points in the layout above, padded with zero-weight points to
+    // a whole number of FP32 runs.  The 6-point rule maps onto itself under
+    // every vertex permutation, so the class maps leave it unchanged.
+    std::vector<float> sg_rule_reg;
+    int sg_rule_reg_offset[3] = {0, 0, 0};
+    int sg_rule_reg_count[3] = {0, 0, 0};
 
     std::vector<float> areas;        // [T]
     std::vector<int32_t> tags;       // [T]
@@ -328,6 +336,9 @@ constexpr int kLuPad = 64;
 constexpr int kLuOuter = 256;   // outer block of the two-level LU (exchange granularity)
 constexpr int kLuGather = 132;  // ints per step of LuWorkspace::gl
 constexpr int kLuMaxOrder = 32768;  // tournament panels nominate from <= 32 chunks
+// orders up to this solve with one 1024-thread block per vector (x in LDS);
+// larger ones in per-outer-block steps
+constexpr int kLuSolveOneBlock = 2048;
 
 // sizes of LuWorkspace::scratch_i (ints) and scratch_f (floats) per system
 constexpr size_t kLuScratchI = kLuMaxOrder / 32;
@@ -360,6 +371,16 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
 // the LU factors and pivots.
 void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream,
               int vpm = 1);
+
+// Reciprocal condition estimate (cgecon / clacn2) of B factored systems,
+// driven by the host: init X, then rounds of {copy X -> Y, lu_solve(Y,
+// trans), step(trans)} for trans 0 and 2 until every system's state word
+// state[4 b + 1] is -1; est[b] ends as the estimate of ||A_b^{-1}||_1.
+void launch_rcond_init(float *X, int64_t ldx, int n, int N, int B, int32_t *state, double *est,
+                       hipStream_t stream);
+void launch_rcond_copy(const float *X, float *Y, int64_t ld, int N, int B, hipStream_t stream);
+void launch_rcond_step(const float *Y, float *X, int64_t ldx, int n, int N, int B, int trans,
+                       int32_t *state, double *est, hipStream_t stream);
 
 // Residual r = b - A x in FP64 with A the FP32 operator (column-major
 // interleaved), x and r FP64 interleaved.  Also writes per-system

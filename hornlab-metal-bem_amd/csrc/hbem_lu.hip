@@ -1281,87 +1281,390 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
 }
 
 // ---------------------------------------------------------------------------
-// A x = b for large orders: the same sweeps as k_lu_solve, one launch per
-// step so that the coupling updates run on many workgroups: exchanges of an
-// outer block (one thread per system), the kNB x kNB diagonal solve (one
-// wave per system), the update of the remaining rows (256 rows per block).
+// Stepped solves for orders above 2048 (x does not fit one block's LDS
+// budget, and one CU per vector cannot stream the factors fast enough): per
+// outer block of kLuOuter rows, one in-block kernel (one 256-thread block
+// per vector: the block's exchanges, its kNB-row sub-block sweeps with the
+// coupling inside the outer block, the diagonal solves) and one coupling
+// kernel that carries the block's contribution to, or gathers the
+// contribution of, the rest of the vector on many workgroups.
+//   A x = b:   per block s (forward)  P_s^T, L~_s in-block, rows below -= L x_s;
+//              per block s (backward) U_ss in-block, rows above -= U x_s.
+//   A^H x = b: per block s (forward)  x_s -= (U above)^H x, U_ss^H in-block;
+//              per block s (backward) x_s -= (L~ below)^H x, L~_s^H in-block,
+//              then P_s.
 // ---------------------------------------------------------------------------
 
-__global__ void k_solve_exchange(float *X, int64_t ldx, const int32_t *ipiv, int N, int kb,
-                                 int w, int vpm) {
-    const int b = blockIdx.x;
-    if (threadIdx.x != 0) return;
-    float *xb = X + (size_t)b * ldx * 2;
-    const int32_t *pv = ipiv + (size_t)(b / vpm) * N;
-    for (int j = 0; j < w; ++j) {
-        const int p = pv[kb + j];
-        if (p != kb + j) {
-            const c32 t = ld(xb, kb + j);
-            st(xb, kb + j, ld(xb, p));
-            st(xb, p, t);
+// Applies the composed exchanges of panel steps [q0, q1) of one system to x
+// in global memory: forward (position trow takes the old entry tsrc, steps
+// in order) or inverse (position tsrc takes the entry at trow, steps in
+// reverse order).  Every thread of the block calls it; wave 0 moves the
+// entries, one step per barrier (a step touches at most 64 positions, and
+// all its loads complete before its stores: the stores consume the loads).
+__device__ void gather_steps_global(float *xb, const int32_t *glb, int q0, int q1,
+                                    bool inverse) {
+    const int lane = threadIdx.x & 63;
+    for (int i = 0; i < q1 - q0; ++i) {
+        const int q = inverse ? q1 - 1 - i : q0 + i;
+        if (threadIdx.x < 64) {
+            const int32_t *g = glb + (size_t)q * kLuGather;
+            const int cnt = g[0];
+            if (lane < cnt) {
+                const int trow = g[1 + lane], tsrc = g[1 + 2 * kNB + lane];
+                const c32 t = ld(xb, inverse ? trow : tsrc);
+                st(xb, inverse ? tsrc : trow, t);
+            }
         }
+        __syncthreads();
     }
 }
 
-// upper == 0: unit lower block; 1: upper block with its diagonal.
-__global__ __launch_bounds__(64) void k_solve_diag(const float *A, int64_t lda, int64_t a_stride,
-                                                   float *X, int64_t ldx, int kb, int upper,
-                                                   int vpm) {
+// In-block sweep of outer block [K0, K0 + w) for vector blockIdx.x.
+// mode 0: P_s^T then the unit lower L~_ss (forward); 1: the upper U_ss with
+// its diagonal (backward); 2: U_ss^H (forward); 3: L~_ss^H (backward) then
+// P_s.  Coupling to rows outside the block is done by k_solve_upd_*.
+__global__ __launch_bounds__(256) void k_solve_block(const float *A, int64_t lda,
+                                                     int64_t a_stride, int N,
+                                                     const int32_t *gl, int64_t gl_stride,
+                                                     float *X, int64_t ldx, int K0, int mode,
+                                                     int vpm) {
+    __shared__ c32 xs[kLuOuter];
+    __shared__ c32 D[kNB][kNB + 1];
+    __shared__ c32 part[8][kNB + 1];
     const int b = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
     float *xb = X + (size_t)b * ldx * 2;
-    __shared__ c32 D[kNB][kNB + 1];
-    for (int e = lane; e < kNB * kNB; e += 64) {
-        const int i = e % kNB, j = e / kNB;
-        D[i][j] = ld(Ab, (size_t)(kb + j) * lda + kb + i);
+    const int32_t *glb = gl + (size_t)(b / vpm) * gl_stride;
+    const int w = min(kLuOuter, N - K0);
+    const int q0 = K0 / kNB, q1 = (K0 + w) / kNB;
+    if (mode == 0) gather_steps_global(xb, glb, q0, q1, false);
+    if (tid < w) xs[tid] = ld(xb, K0 + tid);
+    const bool fwd = (mode == 0 || mode == 2);
+    const int nsub = w / kNB;
+    for (int s = 0; s < nsub; ++s) {
+        const int o = fwd ? s * kNB : (nsub - 1 - s) * kNB;  // sub-block offset in the block
+        const int c0 = fwd ? 0 : o + kNB, c1 = fwd ? o : w;  // solved entries it couples to
+        __syncthreads();  // xs complete, D and part free
+        for (int e = tid; e < kNB * kNB; e += 256) {
+            const int i = e % kNB, j = e / kNB;
+            D[i][j] = ld(Ab, (size_t)(K0 + o + j) * lda + K0 + o + i);
+        }
+        if (c1 > c0) {
+            c32 acc{0.0f, 0.0f};
+            if (mode <= 1) {
+                // sum_c A(K0+o+i, K0+c) x[c]: lanes along rows i, 8 column chunks
+                const int i = tid % kNB, ch = tid / kNB;
+                for (int c = c0 + ch; c < c1; c += 8) {
+                    const c32 a = ld(Ab, (size_t)(K0 + c) * lda + K0 + o + i);
+                    const c32 xv = xs[c];
+                    acc.re += a.re * xv.re - a.im * xv.im;
+                    acc.im += a.re * xv.im + a.im * xv.re;
+                }
+                part[ch][i] = acc;
+            } else {
+                // sum_r conj(A(K0+r, K0+o+j)) x[r]: 8 lanes along rows r per column j
+                const int ch = tid % 8, j = tid / 8;
+                for (int r = c0 + ch; r < c1; r += 8) {
+                    const c32 a = ld(Ab, (size_t)(K0 + o + j) * lda + K0 + r);
+                    const c32 xv = xs[r];
+                    acc.re += a.re * xv.re + a.im * xv.im;
+                    acc.im += a.re * xv.im - a.im * xv.re;
+                }
+                part[ch][j] = acc;
+            }
+        }
+        __syncthreads();
+        if (tid < 64) {
+            c32 x = lane < kNB ? xs[o + lane] : c32{0.0f, 0.0f};
+            if (c1 > c0 && lane < kNB) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    x.re -= part[q][lane].re;
+                    x.im -= part[q][lane].im;
+                }
+            }
+            if (mode == 0) {
+                for (int j = 0; j < kNB - 1; ++j) {
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane > j && lane < kNB) {
+                        const c32 l = D[lane][j];
+                        x.re -= l.re * xr - l.im * xi;
+                        x.im -= l.re * xi + l.im * xr;
+                    }
+                }
+            } else if (mode == 1) {
+                for (int j = kNB - 1; j >= 0; --j) {
+                    if (lane == j) x = cmul(x, cinv(D[j][j]));
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane < j) {
+                        const c32 u = D[lane][j];
+                        x.re -= u.re * xr - u.im * xi;
+                        x.im -= u.re * xi + u.im * xr;
+                    }
+                }
+            } else if (mode == 2) {
+                for (int j = 0; j < kNB; ++j) {
+                    if (lane == j) x = cmul(x, cinv(cconj(D[j][j])));
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane > j && lane < kNB) {
+                        const c32 u = cconj(D[j][lane]);
+                        x.re -= u.re * xr - u.im * xi;
+                        x.im -= u.re * xi + u.im * xr;
+                    }
+                }
+            } else {
+                for (int j = kNB - 1; j >= 0; --j) {
+                    const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
+                    if (lane < j) {
+                        const c32 l = cconj(D[j][lane]);
+                        x.re -= l.re * xr - l.im * xi;
+                        x.im -= l.re * xi + l.im * xr;
+                    }
+                }
+            }
+            if (lane < kNB) xs[o + lane] = x;
+        }
     }
     __syncthreads();
-    c32 x = lane < kNB ? ld(xb, kb + lane) : c32{0.0f, 0.0f};
-    if (!upper) {
-        for (int j = 0; j < kNB - 1; ++j) {
-            const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
-            if (lane > j && lane < kNB) {
-                const c32 l = D[lane][j];
-                x.re -= l.re * xr - l.im * xi;
-                x.im -= l.re * xi + l.im * xr;
-            }
-        }
-    } else {
-        for (int j = kNB - 1; j >= 0; --j) {
-            if (lane == j) x = cmul(x, cinv(D[j][j]));
-            const float xr = __shfl(x.re, j), xi = __shfl(x.im, j);
-            if (lane < j) {
-                const c32 u = D[lane][j];
-                x.re -= u.re * xr - u.im * xi;
-                x.im -= u.re * xi + u.im * xr;
-            }
-        }
+    if (tid < w) st(xb, K0 + tid, xs[tid]);
+    if (mode == 3) {
+        __syncthreads();  // the block's entries are in x before the exchanges move them
+        gather_steps_global(xb, glb, q0, q1, true);
     }
-    if (lane < kNB) st(xb, kb + lane, x);
 }
 
-// x[r] -= sum_j A[r][kb + j] x[kb + j] for r in [r0, r1).
-__global__ __launch_bounds__(256) void k_solve_update(const float *A, int64_t lda,
-                                                      int64_t a_stride, float *X, int64_t ldx,
-                                                      int kb, int r0, int r1, int vpm) {
+// x[r] -= sum_{c in [K0, K1)} A(r, c) x[c] for rows r in [r0, r1), one
+// thread per row (loads coalesced along each column), grid.y = vector.
+__global__ __launch_bounds__(256) void k_solve_upd_n(const float *A, int64_t lda,
+                                                     int64_t a_stride, float *X, int64_t ldx,
+                                                     int K0, int K1, int r0, int r1, int vpm) {
     const int b = blockIdx.y;
     const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
     float *xb = X + (size_t)b * ldx * 2;
-    __shared__ c32 xk[kNB];
-    if (threadIdx.x < kNB) xk[threadIdx.x] = ld(xb, kb + threadIdx.x);
+    __shared__ c32 xk[kLuOuter];
+    const int w = K1 - K0;
+    if (threadIdx.x < w) xk[threadIdx.x] = ld(xb, K0 + threadIdx.x);
     __syncthreads();
     const int r = r0 + blockIdx.x * 256 + threadIdx.x;
     if (r >= r1) return;
-    c32 acc = ld(xb, r);
+    c32 acc{0.0f, 0.0f};
 #pragma unroll 8
-    for (int j = 0; j < kNB; ++j) {
-        const c32 a = ld(Ab, (size_t)(kb + j) * lda + r);
-        const c32 xj = xk[j];
-        acc.re -= a.re * xj.re - a.im * xj.im;
-        acc.im -= a.re * xj.im + a.im * xj.re;
+    for (int c = 0; c < w; ++c) {
+        const c32 a = ld(Ab, (size_t)(K0 + c) * lda + r);
+        const c32 xj = xk[c];
+        acc.re += a.re * xj.re - a.im * xj.im;
+        acc.im += a.re * xj.im + a.im * xj.re;
     }
-    st(xb, r, acc);
+    c32 v = ld(xb, r);
+    v.re -= acc.re;
+    v.im -= acc.im;
+    st(xb, r, v);
+}
+
+// x[c] -= sum_{r in [r0, r1)} conj(A(r, c)) x[r] for c in [K0, K1), one wave
+// per column (lanes along the column), grid.y = vector.
+__global__ __launch_bounds__(256) void k_solve_upd_h(const float *A, int64_t lda,
+                                                     int64_t a_stride, float *X, int64_t ldx,
+                                                     int K0, int K1, int r0, int r1, int vpm) {
+    const int b = blockIdx.y;
+    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
+    float *xb = X + (size_t)b * ldx * 2;
+    const int c = K0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= K1) return;
+    c32 acc{0.0f, 0.0f};
+#pragma unroll 4
+    for (int r = r0 + lane; r < r1; r += 64) {
+        const c32 a = ld(Ab, (size_t)c * lda + r);
+        const c32 xv = ld(xb, r);
+        acc.re += a.re * xv.re + a.im * xv.im;
+        acc.im += a.re * xv.im - a.im * xv.re;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        acc.re += __shfl_xor(acc.re, off);
+        acc.im += __shfl_xor(acc.im, off);
+    }
+    if (lane == 0) {
+        c32 v = ld(xb, c);
+        v.re -= acc.re;
+        v.im -= acc.im;
+        st(xb, c, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reciprocal condition estimate of every factored system of a batch, as
+// cgecon does it: ||A^{-1}||_1 by Hager's method with Higham's refinements
+// (LAPACK clacn2; one state machine per system, driven by batched solves),
+// rcond = 1 / (||A||_1 ||A^{-1}||_1).  Per system the state word holds
+// [0] the step whose result the next solve returns (kRc*), [1] the solve it
+// needs (0: A^{-1}, 2: A^{-H}, -1: done), [2] iter, [3] j.
+// ---------------------------------------------------------------------------
+
+enum : int { kRcInit = 0, kRcSign1 = 1, kRcEj = 2, kRcSign2 = 3, kRcAlt = 4, kRcDone = 5 };
+
+__global__ __launch_bounds__(256) void k_rcond_init(float *X, int64_t ldx, int n, int N,
+                                                    int32_t *state, double *est) {
+    const int b = blockIdx.x;
+    float *xb = X + (size_t)b * ldx * 2;
+    for (int i = threadIdx.x; i < N; i += 256)
+        st(xb, i, c32{i < n ? 1.0f / (float)n : 0.0f, 0.0f});
+    if (threadIdx.x == 0) {
+        state[4 * b + 0] = kRcInit;
+        state[4 * b + 1] = 0;
+        state[4 * b + 2] = 0;
+        state[4 * b + 3] = 0;
+        est[b] = 0.0;
+    }
+}
+
+// Y = X for the solve of this round (every system; the step ignores the
+// systems that did not ask for it).
+__global__ __launch_bounds__(256) void k_rcond_copy(const float *X, float *Y, int64_t ldx, int N) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < N) st(Y + (size_t)b * ldx * 2, i, ld(X + (size_t)b * ldx * 2, i));
+}
+
+// Consumes the solve result Y of the systems whose pending solve is
+// `trans` and prepares their next input in X (clacn2 jumps 1-5).
+__global__ __launch_bounds__(256) void k_rcond_step(const float *Y, float *X, int64_t ldx, int n,
+                                                    int N, int trans, int32_t *state,
+                                                    double *est) {
+    const int b = blockIdx.x;
+    int32_t *sw = state + 4 * b;
+    if (sw[1] != trans) return;  // uniform over the block
+    const float *yb = Y + (size_t)b * ldx * 2;
+    float *xb = X + (size_t)b * ldx * 2;
+    const int tid = threadIdx.x;
+    __shared__ double sum_s[256];
+    __shared__ float max_s[256];
+    __shared__ int arg_s[256];
+    __shared__ int next_s[2];  // next state, next solve
+    // sum |y_i| (scsum1) and the first index of the largest |y_i| (icmax1)
+    double sum = 0.0;
+    float mx = -1.0f;
+    int arg = 0;
+    for (int i = tid; i < n; i += 256) {
+        const c32 y = ld(yb, i);
+        const float a = sqrtf(y.re * y.re + y.im * y.im);
+        sum += (double)a;
+        if (a > mx) {
+            mx = a;
+            arg = i;
+        }
+    }
+    sum_s[tid] = sum;
+    max_s[tid] = mx;
+    arg_s[tid] = arg;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            sum_s[tid] += sum_s[tid + off];
+            const float m2 = max_s[tid + off];
+            const int a2 = arg_s[tid + off];
+            if (m2 > max_s[tid] || (m2 == max_s[tid] && a2 < arg_s[tid])) {
+                max_s[tid] = m2;
+                arg_s[tid] = a2;
+            }
+        }
+        __syncthreads();
+    }
+    const double l1 = sum_s[0];
+    const int jmax = arg_s[0];
+    // next input: 0 sign(y), 1 e_j, 2 alternating vector, -1 none
+    int fill = -1, jsel = 0;
+    if (tid == 0) {
+        int stt = sw[0], op = -1;
+        switch (stt) {
+        case kRcInit:
+            if (n == 1) {
+                const c32 y = ld(yb, 0);
+                est[b] = sqrt((double)y.re * y.re + (double)y.im * y.im);
+                stt = kRcDone;
+            } else {
+                est[b] = l1;
+                fill = 0;
+                op = 2;
+                stt = kRcSign1;
+            }
+            break;
+        case kRcSign1:
+            sw[3] = jmax;
+            sw[2] = 2;
+            fill = 1;
+            op = 0;
+            stt = kRcEj;
+            break;
+        case kRcEj: {
+            const double old = est[b];
+            est[b] = l1;
+            if (l1 <= old) {
+                fill = 2;
+                op = 0;
+                stt = kRcAlt;
+            } else {
+                fill = 0;
+                op = 2;
+                stt = kRcSign2;
+            }
+            break;
+        }
+        case kRcSign2: {
+            const int jlast = sw[3];
+            const c32 ya = ld(yb, jlast), yb2 = ld(yb, jmax);
+            const float al = sqrtf(ya.re * ya.re + ya.im * ya.im);
+            const float aj = sqrtf(yb2.re * yb2.re + yb2.im * yb2.im);
+            sw[3] = jmax;
+            if (al != aj && sw[2] < 5) {
+                sw[2] += 1;
+                fill = 1;
+                op = 0;
+                stt = kRcEj;
+            } else {
+                fill = 2;
+                op = 0;
+                stt = kRcAlt;
+            }
+            break;
+        }
+        case kRcAlt: {
+            const double temp = 2.0 * (l1 / (3.0 * n));
+            if (temp > est[b]) est[b] = temp;
+            stt = kRcDone;
+            break;
+        }
+        default:
+            break;
+        }
+        sw[0] = stt;
+        sw[1] = op;
+        next_s[0] = fill;
+        next_s[1] = sw[3];
+    }
+    __syncthreads();
+    fill = next_s[0];
+    jsel = next_s[1];
+    if (fill < 0) return;
+    const float safmin = 1.17549435e-38f;
+    for (int i = tid; i < N; i += 256) {
+        c32 v{0.0f, 0.0f};
+        if (i < n) {
+            if (fill == 0) {
+                const c32 y = ld(yb, i);
+                const float a = sqrtf(y.re * y.re + y.im * y.im);
+                v = a > safmin ? c32{y.re / a, y.im / a} : c32{1.0f, 0.0f};
+            } else if (fill == 1) {
+                v.re = (i == jsel) ? 1.0f : 0.0f;
+            } else {
+                const float mag = 1.0f + (n > 1 ? (float)i / (float)(n - 1) : 0.0f);
+                v.re = (i % 2 == 0) ? mag : -mag;
+            }
+        }
+        st(xb, i, v);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1693,30 +1996,69 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
 void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream,
               int vpm) {
     const int N = w.N, V = w.B * vpm;
-    if (trans != 0 || N <= 2048) {
+    if (trans != 0 && trans != 2) throw std::runtime_error("lu_solve: trans must be 0 or 2");
+    const int64_t gls = (int64_t)(N / kNB) * kLuGather;
+    if (N <= kLuSolveOneBlock) {
         const size_t shm = sizeof(float) * 2 * (size_t)N;
         hipLaunchKernelGGL(k_lu_solve, dim3(V), dim3(1024), shm, stream, w.A, w.lda, w.a_stride,
-                           N, w.ipiv, w.gl, (int64_t)(N / kNB) * kLuGather, x, ldx, trans, vpm);
-        return;
+                           N, w.ipiv, w.gl, gls, x, ldx, trans, vpm);
+    } else {
+        auto blk = [&](int K0, int mode) {
+            hipLaunchKernelGGL(k_solve_block, dim3(V), dim3(256), 0, stream, w.A, w.lda,
+                               w.a_stride, N, w.gl, gls, x, ldx, K0, mode, vpm);
+        };
+        auto upd_n = [&](int K0, int K1, int r0, int r1) {
+            if (r1 > r0)
+                hipLaunchKernelGGL(k_solve_upd_n, dim3((r1 - r0 + 255) / 256, V), dim3(256), 0,
+                                   stream, w.A, w.lda, w.a_stride, x, ldx, K0, K1, r0, r1, vpm);
+        };
+        auto upd_h = [&](int K0, int K1, int r0, int r1) {
+            if (r1 > r0)
+                hipLaunchKernelGGL(k_solve_upd_h, dim3((K1 - K0 + 3) / 4, V), dim3(256), 0,
+                                   stream, w.A, w.lda, w.a_stride, x, ldx, K0, K1, r0, r1, vpm);
+        };
+        const int last = (N - 1) / kLuOuter * kLuOuter;
+        if (trans == 0) {
+            for (int K0 = 0; K0 < N; K0 += kLuOuter) {
+                const int K1 = std::min(N, K0 + kLuOuter);
+                blk(K0, 0);
+                upd_n(K0, K1, K1, N);
+            }
+            for (int K0 = last; K0 >= 0; K0 -= kLuOuter) {
+                const int K1 = std::min(N, K0 + kLuOuter);
+                blk(K0, 1);
+                upd_n(K0, K1, 0, K0);
+            }
+        } else {
+            for (int K0 = 0; K0 < N; K0 += kLuOuter) {
+                const int K1 = std::min(N, K0 + kLuOuter);
+                upd_h(K0, K1, 0, K0);
+                blk(K0, 2);
+            }
+            for (int K0 = last; K0 >= 0; K0 -= kLuOuter) {
+                const int K1 = std::min(N, K0 + kLuOuter);
+                upd_h(K0, K1, K1, N);
+                blk(K0, 3);
+            }
+        }
     }
-    for (int kb = 0; kb < N; kb += kNB) {
-        if (kb % kLuOuter == 0)
-            hipLaunchKernelGGL(k_solve_exchange, dim3(V), dim3(64), 0, stream, x, ldx, w.ipiv, N,
-                               kb, std::min(kLuOuter, N - kb), vpm);
-        hipLaunchKernelGGL(k_solve_diag, dim3(V), dim3(64), 0, stream, w.A, w.lda, w.a_stride, x,
-                           ldx, kb, 0, vpm);
-        const int r0 = kb + kNB;
-        if (r0 < N)
-            hipLaunchKernelGGL(k_solve_update, dim3((N - r0 + 255) / 256, V), dim3(256), 0,
-                               stream, w.A, w.lda, w.a_stride, x, ldx, kb, r0, N, vpm);
-    }
-    for (int kb = N - kNB; kb >= 0; kb -= kNB) {
-        hipLaunchKernelGGL(k_solve_diag, dim3(V), dim3(64), 0, stream, w.A, w.lda, w.a_stride, x,
-                           ldx, kb, 1, vpm);
-        if (kb > 0)
-            hipLaunchKernelGGL(k_solve_update, dim3((kb + 255) / 256, V), dim3(256), 0, stream,
-                               w.A, w.lda, w.a_stride, x, ldx, kb, 0, kb, vpm);
-    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("LU solve launch: ") + hipGetErrorString(e));
+}
+
+void launch_rcond_init(float *X, int64_t ldx, int n, int N, int B, int32_t *state, double *est,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(k_rcond_init, dim3(B), dim3(256), 0, stream, X, ldx, n, N, state, est);
+}
+
+void launch_rcond_copy(const float *X, float *Y, int64_t ld, int N, int B, hipStream_t stream) {
+    hipLaunchKernelGGL(k_rcond_copy, dim3((N + 255) / 256, B), dim3(256), 0, stream, X, Y, ld, N);
+}
+
+void launch_rcond_step(const float *Y, float *X, int64_t ldx, int n, int N, int B, int trans,
+                       int32_t *state, double *est, hipStream_t stream) {
+    hipLaunchKernelGGL(k_rcond_step, dim3(B), dim3(256), 0, stream, Y, X, ldx, n, N, trans, state,
+                       est);
 }
 
 void launch_residual_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B,

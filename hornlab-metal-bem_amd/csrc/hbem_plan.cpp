@@ -303,6 +303,30 @@ void build_singular_classes(int nv, int T, const float *vtx, const int32_t *tri,
             P->sg_rule.insert(P->sg_rule.end(), row, row + kSgRule);
         }
     }
+
+    // 6 x 6 tensor rule per kind, 64 rows (36 points, 28 zero-weight copies
+    // of the first point)
+    P->sg_rule_reg.clear();
+    for (int kind = 0; kind < 3; ++kind) {
+        P->sg_rule_reg_offset[kind] = 64 * kind;
+        P->sg_rule_reg_count[kind] = 64;
+        for (int p = 0; p < 64; ++p) {
+            const int a6 = p < 36 ? p / 6 : 0, b6 = p < 36 ? p % 6 : 0;
+            const double x = kRule6X[a6], y = kRule6Y[a6], xs = kRule6X[b6], ys = kRule6Y[b6];
+            const double w = p < 36 ? kRule6W[a6] * kRule6W[b6] : 0.0;
+            const double rho[3] = {1.0 - x - y, x, y}, rhos[3] = {1.0 - xs - ys, xs, ys};
+            float row[kSgRule];
+            row[0] = static_cast<float>(xs - x);
+            row[1] = static_cast<float>(ys - y);
+            row[2] = static_cast<float>(xs);
+            row[3] = static_cast<float>(ys);
+            for (int a = 0; a < 3; ++a) {
+                for (int b = 0; b < 3; ++b) row[4 + 3 * a + b] = static_cast<float>(w * rho[a] * rhos[b]);
+                row[13 + a] = static_cast<float>(w * rho[a]);
+            }
+            P->sg_rule_reg.insert(P->sg_rule_reg.end(), row, row + kSgRule);
+        }
+    }
 }
 
 }  // namespace

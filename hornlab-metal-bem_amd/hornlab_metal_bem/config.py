@@ -89,7 +89,7 @@ MetalNativeAssemblyMode = Literal["corrected", "optimized", "reference", "parity
 NATIVE_SYMMETRY_PLANES: tuple[str, ...] = ("yz", "xz", "xy", "yz+xz")
 
 # Dense solve modes of the HIP path.
-HIP_SOLVE_MODES: tuple[str, ...] = ("fp32", "fp32_ir", "mixed_fp16", "mixed_bf16")
+HIP_SOLVE_MODES: tuple[str, ...] = ("fp32", "fp32_ir", "mixed_fp16", "mixed_bf16", "fp64")
 
 
 @dataclass
@@ -143,7 +143,9 @@ class SolveConfig:
     impedance_sources: dict[int, complex] = field(default_factory=dict)
     impedance_source_callback: Callable[[float], dict[int, complex]] | None = None
 
-    # CHIEF interior points (not built on the HIP path; rejected at solve time)
+    # CHIEF interior points: each frequency becomes the least-squares problem
+    # min |A p - b|^2 + |s (C p - d)|^2, solved on the GPU through the square
+    # system's LU with FP64-residual refinement
     chief_points: NDArray[np.float64] | None = None
     chief_weight: float = 1.0
 
@@ -168,9 +170,9 @@ class SolveConfig:
     metal_native_field_threads_per_group: int | None = None
 
     # HIP path controls.  hip_solve_mode None derives the mode from
-    # dense_solve_dtype: "float32" -> "fp32", "float64" -> "fp32_ir"
-    # (FP32 LU refined against an FP64 residual on the GPU).
-    hip_solve_mode: Literal["fp32", "fp32_ir", "mixed_fp16", "mixed_bf16"] | None = None
+    # dense_solve_dtype: "float32" -> "fp32", "float64" -> "fp64" (FP32 LU +
+    # GMRES-IR on the FP64 residual to FP64 accuracy, in place of zgesv).
+    hip_solve_mode: Literal["fp32", "fp32_ir", "mixed_fp16", "mixed_bf16", "fp64"] | None = None
     hip_refine_max_iterations: int = 8
     hip_refine_tolerance: float = 1.0e-7
     hip_batch_frequencies: int | None = None
@@ -288,7 +290,7 @@ class SolveConfig:
     def effective_hip_solve_mode(self) -> str:
         if self.hip_solve_mode is not None:
             return self.hip_solve_mode
-        return "fp32_ir" if self.dense_solve_dtype == "float64" else "fp32"
+        return "fp64" if self.dense_solve_dtype == "float64" else "fp32"
 
 
 def _validate_source_profile(profile: SourceProfile) -> None:

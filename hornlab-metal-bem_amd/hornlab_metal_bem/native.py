@@ -24,10 +24,29 @@ from numpy.typing import NDArray
 from .backends import AssemblyBackendUnavailable
 from .geometry import SYMMETRY_CODES, GeometryBuffers, validate_native_symmetry_plane
 
-HBEM_ABI_VERSION = 5
+HBEM_ABI_VERSION = 6
 _LIB_NAME = "libhbem.so"
 
-SOLVE_MODES = {"fp32": 0, "fp32_ir": 1, "mixed_fp16": 2, "mixed_bf16": 3}
+SOLVE_MODES = {"fp32": 0, "fp32_ir": 1, "mixed_fp16": 2, "mixed_bf16": 3, "fp64": 4}
+
+_SOLVE_METHODS = {
+    "fp32": "fp32_lu",
+    "fp32_ir": "fp32_lu_fp64_residual_refinement",
+    "mixed_fp16": "fp16_trailing_update_lu_gmres_ir",
+    "mixed_bf16": "bf16_trailing_update_lu_gmres_ir",
+    "fp64": "fp32_lu_gmres_ir_to_fp64_accuracy",
+}
+
+# metal_native_assembly_mode -> quadrature rule of the HIP kernels
+# (include/hbem.h HBEM_ASSEMBLY_*).  "reference" is the reference's CPU
+# quadrature: the 6x6 rule on every pair, plus Sauter-Schwab corrections when
+# the case has complex k or Robin walls (main.swift:8139-8186); it runs here
+# on the same GPU kernels.  "parity" compares the CPU reference with the GPU
+# kernels inside the helper and has no counterpart without that CPU path.
+ASSEMBLY_CORRECTED, ASSEMBLY_REGULAR = 0, 1
+ASSEMBLY_MODES = ("corrected", "optimized", "reference")
+ASSEMBLY_MODE_ENV = "HORNLAB_METAL_BEM_NATIVE_ASSEMBLY_MODE"
+DENSE_SOLVE_DTYPE_ENV = "HORNLAB_METAL_BEM_NATIVE_DENSE_SOLVE_DTYPE"
 
 
 class _Geometry(C.Structure):
@@ -78,7 +97,8 @@ class _SolveOptions(C.Structure):
         ("n_chief", C.c_int32),
         ("precise_geometry", C.c_int32),
         ("chief_weight", C.c_double),
-        ("reserved", C.c_int32 * 4),
+        ("solve_trans", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -99,6 +119,9 @@ class _CaseDiag(C.Structure):
         ("near_assembly_s", C.c_double),
         ("near_pairs", C.c_int64),
         ("chief_residual_rel", C.c_double),
+        ("refine_stalled", C.c_int32),
+        ("reserved_i", C.c_int32),
+        ("rcond_s", C.c_double),
     ]
 
 
@@ -111,6 +134,7 @@ EXPORTED_SYMBOLS = (
     "hbem_session_destroy",
     "hbem_session_get_info",
     "hbem_session_set_near_field",
+    "hbem_session_set_assembly_mode",
     "hbem_assemble",
     "hbem_solve_batch",
     "hbem_solve_batch_device",
@@ -136,6 +160,8 @@ def diag_to_dict(d: "_CaseDiag") -> dict[str, Any]:
         "batch_index": int(d.batch_index),
         "batch_size": int(d.batch_size),
         "near_assembly_seconds": float(d.near_assembly_s),
+        "dense_solve_refine_stalled": bool(d.refine_stalled),
+        "dense_solve_rcond_seconds": float(d.rcond_s),
     }
 
 _lock = threading.Lock()
@@ -193,6 +219,8 @@ def load_library() -> C.CDLL:
         lib.hbem_session_set_near_field.argtypes = [
             C.c_void_p, C.c_int32, C.c_double, C.POINTER(C.c_int64)]
         lib.hbem_session_set_near_field.restype = C.c_int32
+        lib.hbem_session_set_assembly_mode.argtypes = [C.c_void_p, C.c_int32]
+        lib.hbem_session_set_assembly_mode.restype = C.c_int32
         fp = C.POINTER(C.c_float)
         lib.hbem_assemble.argtypes = [C.c_void_p, C.c_float, C.c_float, fp, fp, fp, fp, fp, fp, fp, fp]
         lib.hbem_assemble.restype = C.c_int32
@@ -349,29 +377,82 @@ class HipBemSession:
     """Resident GPU session: geometry, singular-pair plan and workspaces stay
     on the device across calls."""
 
-    def __init__(self, handle: C.c_void_p, buffers: GeometryBuffers, symmetry_plane: str | None):
+    def __init__(self, handle: C.c_void_p, buffers: GeometryBuffers, symmetry_plane: str | None,
+                 extra_env: dict[str, str] | None = None):
         self._handle = handle
         self.geometry_buffers = buffers
         self.symmetry_plane = symmetry_plane
         self._closed = False
         self._near: tuple[int, float] | None = None
         self._near_pairs = 0
+        self._env = dict(extra_env or {})
+        self.assembly_mode = "corrected"
+        self._rule = ASSEMBLY_CORRECTED
+        mode = self._setting(ASSEMBLY_MODE_ENV)
+        if mode is not None:
+            self.set_assembly_mode(mode)
+
+    def _setting(self, key: str) -> str | None:
+        """A helper setting: the session's extra_env first, then the process
+        environment (the reference passes both to its helper subprocess)."""
+        if key in self._env:
+            return self._env[key]
+        return os.environ.get(key)
 
     @classmethod
     def create_session(
         cls,
-        geometry_buffers: GeometryBuffers,
+        grid: Any | None = None,
+        physical_tags: Any | None = None,
+        p1_space: Any | None = None,
+        dp0_space: Any | None = None,
         *,
+        geometry_buffers: GeometryBuffers | None = None,
+        runtime_config: Any | None = None,
+        work_dir: Any | None = None,
+        session_id: str | None = None,
+        keep_artifacts: bool = False,
+        regular_triangle_order: int = 4,
+        duffy_1d_order: int = 4,
+        precision: str = "complex64",
         symmetry_plane: str | None = None,
-        check_open_edges: bool = True,
         aperture_tag: int | None = None,
         velocity_source_tags: Any | None = None,
-        **_ignored: Any,
+        check_open_edges: bool = True,
+        runtime_status: Any | None = None,
+        extra_env: dict[str, str] | None = None,
     ) -> "HipBemSession":
+        """Resident session for one mesh (reference create_session,
+        native.py:252-377).  ``geometry_buffers`` may be passed positionally
+        in place of ``grid``.  work_dir, session_id, keep_artifacts,
+        runtime_config and runtime_status belong to the reference's file IPC
+        and helper discovery and change nothing here; ``extra_env`` carries
+        the helper settings this path honours (assembly mode, dense-solve
+        dtype, near-field quadrature)."""
+        if isinstance(grid, GeometryBuffers):
+            if geometry_buffers is not None:
+                raise ValueError("pass geometry buffers once")
+            geometry_buffers, grid = grid, None
+        if regular_triangle_order != 4 or duffy_1d_order != 4:
+            raise ValueError(
+                "regular_triangle_order and duffy_1d_order are fixed at 4; "
+                "the HIP kernels do not take quadrature overrides"
+            )
+        if precision != "complex64":
+            raise ValueError("precision must be 'complex64' (the FP32 assembly of the HIP path)")
         if aperture_tag is not None:
             raise AssemblyBackendUnavailable(
                 "aperture_tag (coupled infinite baffle) is not supported by the HIP path"
             )
+        if geometry_buffers is None:
+            if grid is None or physical_tags is None:
+                raise ValueError("create_session needs geometry_buffers or (grid, physical_tags)")
+            from .geometry import build_geometry_buffers
+            from .mesh import make_pure_function_spaces
+
+            if p1_space is None or dp0_space is None:
+                p1_space, dp0_space = make_pure_function_spaces(grid)
+            geometry_buffers = build_geometry_buffers(grid, physical_tags, p1_space, dp0_space)
         plane = validate_native_symmetry_plane(
             geometry_buffers, symmetry_plane, check_open_edges=check_open_edges
         )
@@ -392,7 +473,46 @@ class HipBemSession:
         )
         handle = C.c_void_p()
         _check(lib.hbem_session_create(C.byref(g), C.byref(handle)), "create_session")
-        return cls(handle, b, plane)
+        try:
+            return cls(handle, b, plane, extra_env)
+        except BaseException:
+            lib.hbem_session_destroy(handle)
+            raise
+
+    def set_assembly_mode(self, mode: str) -> None:
+        """metal_native_assembly_mode of later operations: "corrected" (6x6
+        rule + Sauter-Schwab on touching pairs), "optimized" (6x6 rule on
+        every pair) or "reference" (the 6x6 rule, with the Sauter-Schwab
+        corrections for complex-k and Robin cases, main.swift:8139-8186).
+        "parity" needs the reference's CPU assembly and is refused."""
+        if mode == "parity":
+            raise AssemblyBackendUnavailable(
+                "metal_native_assembly_mode='parity' cross-checks the reference's CPU assembly "
+                "inside its helper; the HIP path has no CPU assembly to compare against"
+            )
+        if mode not in ASSEMBLY_MODES:
+            raise ValueError(
+                "metal_native_assembly_mode must be 'corrected', 'optimized', 'reference' or 'parity'"
+            )
+        self.assembly_mode = mode
+
+    def _apply_assembly_rule(self, corrections: bool) -> str:
+        """Selects the kernels' quadrature for an operation; ``corrections``:
+        the operation has complex k or Robin walls.  Returns the rule name
+        reported in the diagnostics."""
+        self._ensure_open()
+        mode = self._setting(ASSEMBLY_MODE_ENV)
+        if mode is not None and mode != self.assembly_mode:
+            self.set_assembly_mode(mode)
+        if self.assembly_mode == "corrected" or (self.assembly_mode == "reference" and corrections):
+            rule, name = ASSEMBLY_CORRECTED, "corrected"
+        else:
+            rule, name = ASSEMBLY_REGULAR, "regular_6x6"
+        if rule != self._rule:
+            _check(load_library().hbem_session_set_assembly_mode(self._handle, rule),
+                   "set_assembly_mode")
+            self._rule = rule
+        return name
 
     # -- lifecycle ---------------------------------------------------------
     def close(self) -> None:
@@ -457,10 +577,14 @@ class HipBemSession:
     def _sync_near_quadrature(self) -> None:
         """Applies the environment's near-field setting, read per operation as
         the reference helper does; with the variable unset the session keeps
-        the setting of set_near_quadrature (off by default)."""
-        if NEAR_QUADRATURE_ENV not in os.environ:
+        the setting of set_near_quadrature (off by default).  The session's
+        extra_env takes precedence over the process environment."""
+        if NEAR_QUADRATURE_ENV in self._env:
+            want = near_quadrature_from_env(self._env)
+        elif NEAR_QUADRATURE_ENV in os.environ:
+            want = near_quadrature_from_env()
+        else:
             return
-        want = near_quadrature_from_env()
         if want != self._near:
             if want is None:
                 self.set_near_quadrature(0)
@@ -514,6 +638,7 @@ class HipBemSession:
         qre = np.ascontiguousarray(q.real, dtype=np.float32)
         qim = np.ascontiguousarray(q.imag, dtype=np.float32)
         beta = self._beta_rows([impedance_sources])
+        self._apply_assembly_rule(float(k_imag) != 0.0 or beta is not None)
         bre = np.ascontiguousarray(beta[0].real, np.float32) if beta is not None else None
         bim = np.ascontiguousarray(beta[0].imag, np.float32) if beta is not None else None
         are = np.empty((n, n), np.float32)
@@ -555,14 +680,22 @@ class HipBemSession:
         chief_points: NDArray[Any] | None = None,
         chief_weight: float = 1.0,
         precise_geometry: bool = False,
-        **_ignored: Any,
+        batch_id: str = "batch",
+        operation_id: str = "assembly-solve-field-batch",
+        write_batched_field: bool = False,
     ) -> list[DenseSolveFieldResult]:
         """Assemble, solve and evaluate every frequency; returns one result per
         frequency in order.  ``on_case_result(i, result)`` is called in order;
         returning exactly False stops after that case.  ``impedance_sources``
         ({tag: beta}, or one dict per frequency) adds Robin walls;
         ``extra_neumann_dp0`` (n_extra, F, T) adds right-hand sides that share
-        each case's operator and LU (multi-RHS, native.py:775-783)."""
+        each case's operator and LU (multi-RHS, native.py:775-783).
+        ``dense_solve_dtype="float64"`` (or the session's
+        HORNLAB_METAL_BEM_NATIVE_DENSE_SOLVE_DTYPE) solves each system to FP64
+        accuracy (FP32 LU + GMRES-IR on the FP64 residual) in place of the
+        reference's zgesv.  batch_id, operation_id and write_batched_field name
+        the reference's IPC files and do not change results here (fields are
+        always returned as arrays)."""
         self._ensure_open()
         self._sync_near_quadrature()
         freqs = np.asarray(frequency_hz, dtype=np.float64).reshape(-1)
@@ -625,10 +758,17 @@ class HipBemSession:
             obs = obs.T
         obs = np.ascontiguousarray(obs, dtype=np.float32)
         n_obs = int(obs.shape[1])
-        if dense_solve_dtype == "float64" and solve_mode == "fp32":
-            solve_mode = "fp32_ir"
+        if dense_solve_dtype is None:
+            dense_solve_dtype = self._setting(DENSE_SOLVE_DTYPE_ENV) or "float32"
+        if dense_solve_dtype not in ("float32", "float64"):
+            raise ValueError("dense_solve_dtype must be 'float32' or 'float64'")
+        if dense_solve_dtype == "float64" and solve_mode in ("fp32", "fp32_ir"):
+            solve_mode = "fp64"
         if solve_mode not in SOLVE_MODES:
             raise ValueError(f"solve_mode must be one of {sorted(SOLVE_MODES)}")
+        if solve_mode == "fp64" and int(refine_max_iterations) <= 0:
+            raise ValueError("the float64 dense solve needs refine_max_iterations > 0")
+        rule = self._apply_assembly_rule(bool(np.any(kim != 0)) or beta is not None)
         user_tags = [int(t) for t in (source_tags or [])]
         imp_tag = int(impedance_source_tag) if impedance_source_tag is not None else (
             int(min(user_tags)) if user_tags else 2
@@ -682,6 +822,7 @@ class HipBemSession:
         # callback sees each chunk's cases before the next chunk is solved
         # (per-case streaming and early stop, native.py:1362-1447).
         chunk = F if on_case_result is None else max(1, int(batch_frequencies or 8))
+        self._last_rule = rule
         results: list[DenseSolveFieldResult] = []
         for f0 in range(0, F, chunk):
             f1 = min(F, f0 + chunk)
@@ -716,10 +857,13 @@ class HipBemSession:
             d = diag[i - f0]
             diagnostics: dict[str, Any] = {
                 "implementation": "hip_gfx950_resident_assembly_lu_field_batch",
-                "assembly_mode": "corrected",
+                "assembly_mode": self.assembly_mode,
+                "assembly_rule": self._last_rule,
                 "lapack_info": int(d.lapack_info),
                 "dense_solve_condition_1norm": float(d.dense_solve_condition_1norm),
-                "dense_solve_dtype": "float32" if solve_mode == "fp32" else solve_mode,
+                "dense_solve_dtype": ("float32" if solve_mode == "fp32" else
+                                      "float64" if solve_mode == "fp64" else solve_mode),
+                "dense_solve_method": _SOLVE_METHODS[solve_mode],
                 "dense_solve_refine_iterations": int(d.refine_iterations),
                 "field_k_real_f32": float(fk[i]),
                 "assembly_k_imag_f32": float(kim[i]),
@@ -745,6 +889,9 @@ class HipBemSession:
                 diagnostics["dense_solve_rcond"] = float(d.dense_solve_rcond)
             if d.refine_residual_rel >= 0:
                 diagnostics["dense_solve_refine_residual_rel"] = float(d.refine_residual_rel)
+                diagnostics["dense_solve_refine_stalled"] = bool(d.refine_stalled)
+            if d.dense_solve_rcond >= 0:
+                diagnostics["dense_solve_rcond_seconds"] = float(d.rcond_s)
             def avg(src: int) -> dict[int, complex]:
                 return {int(t): complex(pav[i, src, j, 0], pav[i, src, j, 1]) for j, t in enumerate(tags)}
 
@@ -917,9 +1064,12 @@ class HipBemSession:
         pressure_p1: NDArray[Any],
         neumann_dp0: NDArray[Any],
         observation_points: NDArray[Any],
-        **_ignored: Any,
+        *,
+        batch_id: str = "batch",
+        operation_id: str = "field-batch",
     ) -> NDArray[np.complex64]:
-        """Field (F, n_obs) from given surface pressure and Neumann data."""
+        """Field (F, n_obs) from given surface pressure and Neumann data
+        (batch_id and operation_id name the reference's IPC files only)."""
         self._ensure_open()
         k = np.ascontiguousarray(np.asarray(k_real, dtype=np.float32).reshape(-1))
         F = int(k.size)
@@ -959,8 +1109,12 @@ def dense_solve(
     refine_max_iterations: int = 8,
     refine_tolerance: float = 1e-7,
     estimate_rcond: bool = True,
+    trans: str = "N",
 ) -> tuple[NDArray[np.complex64], list[dict[str, Any]]]:
-    """GPU LU solve of stacked systems A (S, n, n) x = b (S, n)."""
+    """GPU LU solve of stacked systems A (S, n, n) x = b (S, n); trans "C"
+    solves A^H x = b (FP32 LU, no refinement)."""
+    if trans not in ("N", "C"):
+        raise ValueError("trans must be 'N' or 'C'")
     A = np.asarray(A)
     b = np.asarray(b)
     if A.ndim == 2:
@@ -973,6 +1127,7 @@ def dense_solve(
     opts.refine_max_iterations = int(refine_max_iterations)
     opts.refine_tolerance = float(refine_tolerance)
     opts.estimate_rcond = 1 if estimate_rcond else 0
+    opts.solve_trans = 2 if trans == "C" else 0
     xr = np.zeros((S, n), np.float32)
     xi = np.zeros((S, n), np.float32)
     diag = (_CaseDiag * S)()
@@ -994,7 +1149,9 @@ def dense_solve(
             "dense_solve_condition_1norm": float(d.dense_solve_condition_1norm),
             "dense_solve_refine_iterations": int(d.refine_iterations),
             "dense_solve_refine_residual_rel": float(d.refine_residual_rel),
+            "dense_solve_refine_stalled": bool(d.refine_stalled),
             "dense_solve_s": float(d.dense_solve_s),
+            "dense_solve_rcond_seconds": float(d.rcond_s),
         }
         for d in diag
     ]

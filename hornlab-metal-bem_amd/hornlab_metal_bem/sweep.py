@@ -42,6 +42,11 @@ def should_route_native_metal(config: SolveConfig) -> bool:
         )
     if config.aperture_tag is not None:
         raise AssemblyBackendUnavailable("aperture_tag (coupled infinite baffle) is not built on the HIP path")
+    if config.metal_native_assembly_mode == "parity":
+        raise AssemblyBackendUnavailable(
+            "metal_native_assembly_mode='parity' cross-checks the reference's CPU assembly inside "
+            "its helper; the HIP path has no CPU assembly to compare against"
+        )
     return True
 
 
@@ -147,6 +152,14 @@ def _apply_dense_solve_policy(diagnostics: dict, *, threshold: float) -> None:
     sweep.py:383-412)."""
     diagnostics["dense_solve_rcond_warning_threshold"] = float(threshold)
     diagnostics["dense_solve_suspect"] = False
+    if diagnostics.get("dense_solve_dtype") == "float64" and diagnostics.get("dense_solve_refine_stalled"):
+        # the FP64-accurate solve stopped above its residual target: the
+        # answer is not the FP64 solution the setting asks for
+        diagnostics["dense_solve_suspect"] = True
+        diagnostics["dense_solve_recommendation"] = (
+            "The float64 dense solve did not reach its residual target (refinement stalled); "
+            "treat this frequency as suspect and compare against nearby points."
+        )
     if threshold <= 0.0:
         return
     raw = diagnostics.get("dense_solve_rcond")
@@ -253,6 +266,7 @@ def _native_sweep(
         buffers, symmetry_plane=config.native_symmetry_plane,
         check_open_edges=config.native_check_open_edges,
     ) as session:
+        session.set_assembly_mode(config.metal_native_assembly_mode)
         for c0 in range(0, freqs.size, chunk):
             sl = slice(c0, min(freqs.size, c0 + chunk))
             imp_sl = impedance[sl] if isinstance(impedance, list) else impedance

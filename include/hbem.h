@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HBEM_ABI_VERSION 5
+#define HBEM_ABI_VERSION 6
 
 #define HBEM_OK 0
 #define HBEM_EINVAL -1      /* bad argument / contract violation */
@@ -82,6 +82,12 @@ typedef struct hbem_session_info {
 #define HBEM_SOLVE_F32_IR 1     /* FP32 LU + FP64-residual refinement */
 #define HBEM_SOLVE_MIXED_F16 2  /* FP16 MFMA trailing update + refinement */
 #define HBEM_SOLVE_MIXED_BF16 3 /* BF16 MFMA trailing update + refinement */
+/* dense_solve_dtype "float64" (the reference's zgesv of the FP32-assembled
+ * system, main.swift:7101-7185): FP32 LU + GMRES-IR against the FP64
+ * residual, run to ||b - A x||_inf / ||b||_inf <= min(refine_tolerance,
+ * 1e-13) or until a restart brings no progress; a system left above 1e-10
+ * reports refine_stalled. */
+#define HBEM_SOLVE_F64 4
 
 typedef struct hbem_solve_options {
     int32_t solve_mode;            /* HBEM_SOLVE_* */
@@ -120,7 +126,11 @@ typedef struct hbem_solve_options {
      * adjoint's forward always uses 1.  No reference counterpart. */
     int32_t precise_geometry;
     double chief_weight;  /* > 0 (0 means 1) */
-    int32_t reserved[4];
+    /* hbem_dense_solve only: 0 solves A x = b, 2 solves A^H x = b (cgetrs
+     * with trans 'C'; solve_mode HBEM_SOLVE_F32, no refinement); the batch
+     * entry points need 0. */
+    int32_t solve_trans;
+    int32_t reserved[3];
 } hbem_solve_options;
 
 /* Per-frequency diagnostics: the case fields the reference returns
@@ -141,6 +151,9 @@ typedef struct hbem_case_diag {
     double near_assembly_s;              /* near-field kernel alone (0 when off) */
     int64_t near_pairs;                  /* near (test image, trial image) pairs corrected */
     double chief_residual_rel;           /* max over sources |s (C p - d)|_2 / |b|_2; -1 without CHIEF */
+    int32_t refine_stalled;              /* 1: refinement stopped above its tolerance */
+    int32_t reserved_i;
+    double rcond_s;                      /* rcond estimate, per case (HIP events) */
 } hbem_case_diag;
 
 /* ABI version of the loaded library. */
@@ -177,6 +190,19 @@ int32_t hbem_session_destroy(hbem_session *session);
  * NULL) receives the number of near pairs. */
 int32_t hbem_session_set_near_field(hbem_session *session, int32_t level, double threshold,
                                     int64_t *pair_count);
+
+/* Quadrature of every later assembly of the session; stands in for the
+ * reference's HORNLAB_METAL_BEM_NATIVE_ASSEMBLY_MODE setting, read by
+ * assembleRegular (main.swift:8128-8310):
+ *   HBEM_ASSEMBLY_CORRECTED (default, "corrected"): the 6x6 rule on
+ *     non-touching pairs, Sauter-Schwab quadrature on touching pairs;
+ *   HBEM_ASSEMBLY_REGULAR ("optimized"): the 6x6 rule on every pair,
+ *     touching pairs included, quadrature points that coincide excluded.
+ * Near-field quadrature applies in both.  hbem_adjoint_forward needs
+ * HBEM_ASSEMBLY_CORRECTED. */
+#define HBEM_ASSEMBLY_CORRECTED 0
+#define HBEM_ASSEMBLY_REGULAR 1
+int32_t hbem_session_set_assembly_mode(hbem_session *session, int32_t mode);
 
 int32_t hbem_session_get_info(hbem_session *session, hbem_session_info *info);
 

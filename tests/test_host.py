@@ -47,7 +47,7 @@ def test_native_config_defaults():
     assert c.air_density == pytest.approx(1.2041)
     assert c.velocity_sources == {2: 1.0}
     assert c.effective_hip_solve_mode == "fp32"
-    assert native_config(dense_solve_dtype="float64").effective_hip_solve_mode == "fp32_ir"
+    assert native_config(dense_solve_dtype="float64").effective_hip_solve_mode == "fp64"
 
 
 @pytest.mark.parametrize(
