@@ -34,13 +34,14 @@ TRANS_PER_EVAL = 4  # sqrt/rcp, sin, cos, (exp) per evaluation (fixed count)
 REFERENCE_FREQ_PER_S = 20.0  # reference README: 40 freqs in ~2 s on Apple M-series
 # HBM bytes per k_assemble_regular dispatch on the 40-frequency quarter-horn
 # sweep: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a
-# 512 MiB copy kernel), profiles/pmc_traffic_c2_regular_assembly.md
-REGULAR_TRAFFIC_BYTES = {"c2": (10640.3 * 2.0 + 2900240.0) * 1024.0}
+# 512 MiB copy kernel), profiles/pmc_traffic_c2_regular_assembly.md; empty
+# until measured on the current mesh
+REGULAR_TRAFFIC_BYTES: dict[str, float] = {}
 
 
 WORKLOADS = {
     # quarter horn, yz+xz, ~1000 dofs, 40 log frequencies 500 Hz - 20 kHz, 3 x 37 points
-    "c2": dict(quarter=True, symmetry="yz+xz", dofs=1000, fmin=500.0, fmax=20000.0, nf=40,
+    "c2": dict(quarter=True, symmetry="yz+xz", dofs=1080, fmin=500.0, fmax=20000.0, nf=40,
                angles=37),
     # full horn, no symmetry, ~8000 dofs, 64 log frequencies 200 Hz - 16 kHz, 3 x 181 points
     "c3": dict(quarter=False, symmetry=None, dofs=8000, fmin=200.0, fmax=16000.0, nf=64,
@@ -315,7 +316,21 @@ def prepare_inputs(mesh, config):
     return freqs, k, q, pts, buffers
 
 
-def cpu_baseline(buffers, sym, freqs, k, q, pts, sample_idx, threads):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(buffers, sym, freqs, k, q, pts, sample_idx, threads, near_level=1):
+    """The oracle's FP32 port of the reference CPU path on host threads:
+    regular + Sauter-Schwab (+ level-1 subdivided near-field) assembly,
+    scipy cgesv, field.  Returns (frequencies/s, seconds)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
@@ -324,7 +339,8 @@ def cpu_baseline(buffers, sym, freqs, k, q, pts, sample_idx, threads):
     g = O.Geometry.from_buffers(buffers, sym)
     t0 = time.perf_counter()
     for i in sample_idx:
-        A, rhs, _ = O.assemble(g, float(k[i]), q[i], near_level=1, near_threshold=1.5, bits=32)
+        A, rhs, _ = O.assemble(g, float(k[i]), q[i], near_level=near_level, near_threshold=1.5,
+                               bits=32)
         p = O.solve(A, rhs, bits=32)
         O.field(g, float(k[i]), p, q[i], pts, bits=32)
     dt = time.perf_counter() - t0
@@ -349,12 +365,22 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--dofs", type=int, default=None)
-    ap.add_argument("--solve-mode", default=None, choices=["fp32", "fp32_ir", "mixed_fp16", "mixed_bf16"],
-                    help="default: fp32 for c2, mixed_fp16 (GMRES-IR to 1e-6) for c3")
+    ap.add_argument("--solve-mode", default=None,
+                    choices=["fp32", "fp32_ir", "mixed_fp16", "mixed_bf16", "fp64"],
+                    help="default: fp32 for c2 (the solve() default), mixed_fp16 (GMRES-IR to "
+                         "1e-6) for c3")
     ap.add_argument("--refine-tol", type=float, default=None)
+    ap.add_argument("--mixed-tol", type=float, default=1e-6,
+                    help="c2 mode comparison: GMRES-IR residual target of the mixed modes")
+    ap.add_argument("--no-rcond", action="store_true",
+                    help="skip the rcond estimate (solve() runs it by default)")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="c2: skip the per-frequency plain FP32 / graph / mixed-mode lines")
+    ap.add_argument("--no-solve-wall", action="store_true",
+                    help="skip the end-to-end solve() wall time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--lu-graph", type=int, default=1, choices=[0, 1],
-                    help="replay the LU launch sequence from a captured hipGraph (c2/c3)")
+    ap.add_argument("--lu-graph", type=int, default=0, choices=[0, 1],
+                    help="replay the LU launch sequence from a captured hipGraph (solve() does not)")
     ap.add_argument("--save-gradient", default=None, help="c5: write the sampled gradient (.npz)")
     args = ap.parse_args()
     if args.gpus == 1 and args.workload == "c5":
@@ -381,6 +407,7 @@ def main() -> int:
     sym = wl["symmetry"]
     solve_mode = args.solve_mode or ("fp32" if args.workload == "c2" else "mixed_fp16")
     refine_tol = args.refine_tol or (1e-7 if args.workload == "c2" else 1e-6)
+    rcond = not args.no_rcond
     spec, h, mesh, config = workload(args.dofs, name=args.workload)
     freqs, k, q, pts, buffers = prepare_inputs(mesh, config)
     F, T, n, n_obs = freqs.size, buffers.n_triangles, buffers.p1_dof_count, pts.shape[1]
@@ -392,29 +419,75 @@ def main() -> int:
     p_d = torch.zeros(F * n * 2, dtype=torch.float32, device=dev)
     tag_d = torch.zeros(F * 2 * 2, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    print(f"bench: {args.workload} n={n} T={T} F={F} mode={solve_mode} rcond={rcond}",
+          file=sys.stderr, flush=True)
 
-    with HipBemSession.create_session(buffers, symmetry_plane=sym) as s:
-        info = s.info()
-
-        def step():
-            return s.solve_batch_device(
-                F, k_d.data_ptr(), k_d.data_ptr(), q_d.data_ptr(), n_obs, obs_d.data_ptr(),
-                field_d.data_ptr(), tag_d.data_ptr(), p_ptr=p_d.data_ptr(), source_tags=[2],
-                impedance_source_tag=2, solve_mode=solve_mode, estimate_rcond=False,
-                refine_tolerance=refine_tol, lu_graph=bool(args.lu_graph),
-            )
-
-        for _ in range(args.warmup):
+    def timed(step, steps, warmup):
+        for _ in range(warmup):
             step()
         torch.cuda.synchronize()
         diags = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             diags.append(step())
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
+        return time.perf_counter() - t0, diags
+
+    modes = {}
+    with HipBemSession.create_session(buffers, symmetry_plane=sym) as s:
+        info = s.info()
+
+        def make_step(mode, tol, batch=None, graph=False):
+            def step():
+                return s.solve_batch_device(
+                    F, k_d.data_ptr(), k_d.data_ptr(), q_d.data_ptr(), n_obs, obs_d.data_ptr(),
+                    field_d.data_ptr(), tag_d.data_ptr(), p_ptr=p_d.data_ptr(), source_tags=[2],
+                    impedance_source_tag=2, solve_mode=mode, estimate_rcond=rcond,
+                    refine_tolerance=tol, refine_max_iterations=20, batch_frequencies=batch,
+                    lu_graph=graph,
+                )
+            return step
+
+        dt, diags = timed(make_step(solve_mode, refine_tol, graph=bool(args.lu_graph)), args.steps,
+                          args.warmup)
+        p_gpu = (p_d.cpu().numpy().view(np.complex64)).reshape(F, n).copy()
+        f_gpu = (field_d.cpu().numpy().view(np.complex64)).reshape(F, n_obs).copy()
+        print(f"bench: main {1e3 * dt / args.steps:.3f} ms per step", file=sys.stderr, flush=True)
+        mode_fields = {}
+        if args.workload == "c2" and not args.no_modes:
+            # the plain FP32 GPU path (one frequency per launch sequence), the
+            # captured-graph FP32 path and the mixed-precision modes, each over
+            # a few sweeps of the same workload
+            variants = [
+                ("plain_fp32_per_frequency", "fp32", refine_tol, 1, False),
+                ("fp32_lu_graph", "fp32", refine_tol, None, True),
+                ("mixed_fp16", "mixed_fp16", args.mixed_tol, None, False),
+                ("mixed_bf16", "mixed_bf16", args.mixed_tol, None, False),
+            ]
+            for name, mode, tol, batch, graph in variants:
+                msteps = max(2, args.steps // 3)
+                mdt, mdiags = timed(make_step(mode, tol, batch, graph), msteps, 1)
+                modes[name] = {
+                    "value": round(F * msteps / mdt, 3), "ms_per_step": round(1e3 * mdt / msteps, 3),
+                    "solve_mode": mode, "batch_frequencies": batch or F, "lu_graph": graph,
+                    "refine_tolerance": tol if mode != "fp32" else None,
+                    "dense_solve_ms_per_step": round(1e3 * float(np.mean(
+                        [sum(d["dense_solve_seconds"] for d in ds) for ds in mdiags])), 3),
+                }
+                if mode != "fp32":
+                    modes[name]["worst_refine_residual"] = float(max(
+                        d["dense_solve_refine_residual_rel"] for d in mdiags[-1]))
+                    mode_fields[name] = (
+                        (p_d.cpu().numpy().view(np.complex64)).reshape(F, n).copy(),
+                        (field_d.cpu().numpy().view(np.complex64)).reshape(F, n_obs).copy())
+                print(f"bench: {name} {modes[name]['ms_per_step']} ms per step", file=sys.stderr,
+                      flush=True)
+            plain = modes["plain_fp32_per_frequency"]["value"]
+            for name in modes:
+                modes[name]["vs_plain_fp32"] = round(modes[name]["value"] / plain, 3)
     value = F * args.steps / dt
     ms_per_step = 1e3 * dt / args.steps
+
     # stage timings (HIP events on the library stream), per step
     def per_step(key):
         return 1e3 * float(np.mean([sum(d[key] for d in ds) for ds in diags]))
@@ -425,30 +498,57 @@ def main() -> int:
     n_img = 4 if sym == "yz+xz" else (1 if sym is None else 2)
     evals_per_step = float(T) ** 2 * 36 * n_img * F
     achieved = evals_per_step / launches_per_step * FLOPS_PER_EVAL / reg_launch_s / 1e12
-    p_gpu = (p_d.cpu().numpy().view(np.complex64)).reshape(F, n)
-    f_gpu = (field_d.cpu().numpy().view(np.complex64)).reshape(F, n_obs)
+
+    solve_wall = None
+    if not args.no_solve_wall:
+        # the reference's published number: solve() wall time on the same mesh
+        # (host arrays in and out, session setup, result assembly)
+        from hornlab_metal_bem import solve
+
+        solve(mesh, config)  # first call pays one-time library and kernel loading
+        t0 = time.perf_counter()
+        res = solve(mesh, config)
+        wall = time.perf_counter() - t0
+        solve_wall = {"seconds": round(wall, 4), "frequencies_per_s": round(F / wall, 3),
+                      "solve_mode": config.effective_hip_solve_mode,
+                      "estimate_rcond": config.hip_estimate_rcond,
+                      "frequencies": int(res.frequencies_hz.size)}
+        print(f"bench: solve() wall {wall:.3f} s", file=sys.stderr, flush=True)
 
     threads = min(16, os.cpu_count() or 1)
-    cpu = None
+    cpu = cpu_plain = None
     acc = {}
     if not args.no_cpu_baseline:
         sample = list(range(0, F, 5)) if args.workload == "c2" else [F // 2]
-        cps, cdt = cpu_baseline(buffers, sym, freqs, k, q, pts, sample, threads)
+        cps, cdt = cpu_baseline(buffers, sym, freqs, k, q, pts, sample, threads, near_level=1)
+        hz = ", ".join(f"{freqs[i]:.0f}" for i in sample)
         cpu = {
             "value": round(cps, 4),
             "unit": "frequencies/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"{len(sample)} of the {F} frequencies ({', '.join(f'{freqs[i]:.0f}' for i in sample)} Hz), "
-            "FP32 regular + Sauter-Schwab + level-1 near-field assembly, scipy cgesv, field; "
-            f"{cdt:.1f} s",
+            "sample": f"{len(sample)} of the {F} frequencies ({hz} Hz), FP32 regular + "
+            f"Sauter-Schwab + level-1 near-field assembly, scipy cgesv, field; {cdt:.1f} s",
+            "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(),
         }
+        cps2, cdt2 = cpu_baseline(buffers, sym, freqs, k, q, pts, sample, threads, near_level=0)
+        cpu_plain = {"value": round(cps2, 4), "unit": "frequencies/s", "cores": threads,
+                     "kind": "port", "sample": f"the same {len(sample)} frequencies without the "
+                     f"near-field pass; {cdt2:.1f} s"}
+        print("bench: cpu baselines done", file=sys.stderr, flush=True)
         for i in ((0, F - 1) if args.workload == "c2" else (F - 1,)):
             p64, f64 = fp64_reference(buffers, sym, k, q, pts, i, threads)
-            acc[f"{freqs[i]:.0f}Hz"] = {
+            key = f"{freqs[i]:.0f}Hz"
+            acc[key] = {
                 "surface_p_rel_l2": float(np.linalg.norm(p_gpu[i] - p64) / np.linalg.norm(p64)),
                 "field_rel_l2": float(np.linalg.norm(f_gpu[i] - f64) / np.linalg.norm(f64)),
             }
+            for name, (pm, fm) in mode_fields.items():
+                modes[name].setdefault("accuracy_vs_fp64", {})[key] = {
+                    "surface_p_rel_l2": float(np.linalg.norm(pm[i] - p64) / np.linalg.norm(p64)),
+                    "field_rel_l2": float(np.linalg.norm(fm[i] - f64) / np.linalg.norm(f64)),
+                }
     out = {
         "metric": "frequencies solved/sec (assemble+LU+field) at 1k/8k dofs; rel-L2 vs FP64",
         "value": round(value, 3),
@@ -472,14 +572,17 @@ def main() -> int:
             "edge_length_m": round(h, 5),
             "solve_mode": solve_mode,
             "refine_tolerance": refine_tol if solve_mode != "fp32" else None,
+            "estimate_rcond": rcond,
             "lu_graph": bool(args.lu_graph),
             "duffy_pairs": info["planned_pairs"],
         },
+        "setup_s": round(info["resident_context_seconds"], 4),
         "stages_ms_per_step": {
             "assembly": round(per_step("assembly_seconds"), 3),
             "regular_assembly": round(reg_ms, 3),
             "singular_assembly": round(per_step("singular_assembly_seconds"), 3),
             "dense_solve": round(per_step("dense_solve_seconds"), 3),
+            "rcond": round(per_step("dense_solve_rcond_seconds"), 3),
             "field": round(per_step("field_seconds"), 3),
         },
         "roofline": {
@@ -494,7 +597,10 @@ def main() -> int:
             f"{FLOPS_PER_EVAL} FP32 flops (+{TRANS_PER_EVAL} transcendentals) each",
             "launch_ms": round(reg_launch_s * 1e3, 4),
         },
+        "modes": modes or None,
+        "solve_wall": solve_wall,
         "cpu_baseline": cpu,
+        "cpu_baseline_plain": cpu_plain,
         "accuracy_vs_fp64": acc,
     }
     print(json.dumps(out))

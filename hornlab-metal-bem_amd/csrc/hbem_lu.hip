@@ -710,6 +710,48 @@ struct __attribute__((aligned(16))) TrLds {
     unsigned short Lr[kTrTile][kTrS], Li[kTrTile][kTrS], Ur[kTrTile][kTrS], Ui[kTrTile][kTrS];
 };
 
+// C -= scale * (accr + i acci) on a wave's 64 x 64 complex block of a
+// 128-tile (column base cb, row base rb; entries at or beyond c_end / r_end
+// untouched).  MFMA accumulator [ti][tj][v] holds column cb + 16 ti +
+// 4 (lane / 16) + v, row rb + 16 tj + lane % 16.  The loads of 32 entries are
+// issued before any of their stores: the compiler cannot reorder a load past
+// a store it cannot prove disjoint, so a load-modify-store per entry would
+// wait one memory latency per entry.
+__device__ __forceinline__ void tile_epilogue(float *Ab, int64_t lda, int cb, int rb, int c_end,
+                                              int r_end, int lane, const f32x4 (&accr)[4][4],
+                                              const f32x4 (&acci)[4][4], float scale) {
+    const int li = lane & 15, lc = (lane >> 4) * 4;
+#pragma unroll
+    for (int t2 = 0; t2 < 4; t2 += 2) {
+        c32 cv[2][4][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int col = cb + 16 * (t2 + h) + lc + v, row = rb + 16 * tj + li;
+                    cv[h][tj][v] = (row < r_end && col < c_end)
+                                       ? ld(Ab, (size_t)col * lda + row)
+                                       : c32{0.0f, 0.0f};
+                }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int col = cb + 16 * (t2 + h) + lc + v, row = rb + 16 * tj + li;
+                    if (row < r_end && col < c_end) {
+                        c32 x = cv[h][tj][v];
+                        x.re -= accr[t2 + h][tj][v] * scale;
+                        x.im -= acci[t2 + h][tj][v] * scale;
+                        st(Ab, (size_t)col * lda + row, x);
+                    }
+                }
+    }
+}
+
 template <int kPrec>
 __device__ __forceinline__ unsigned short tr_round(float x) {
     if constexpr (kPrec == 1) {
@@ -872,22 +914,7 @@ __global__ __launch_bounds__(256, 2) void k_lu_gemm_trp(float *A, int64_t lda, i
         }
     }
     const float inv_us = 1.0f / us;  // exact: us is a power of two
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int col = c0 + wc + 16 * ti + (lane >> 4) * 4 + v;
-                const int row = r0 + wr + 16 * tj + li;
-                if (row < M && col < M) {
-                    const size_t idx = (size_t)(K1 + col) * lda + K1 + row;
-                    c32 x = ld(Ab, idx);
-                    x.re -= accr[ti][tj][v] * inv_us;
-                    x.im -= acci[ti][tj][v] * inv_us;
-                    st(Ab, idx, x);
-                }
-            }
+    tile_epilogue(Ab, lda, K1 + c0 + wc, K1 + r0 + wr, K1 + M, K1 + M, lane, accr, acci, inv_us);
 }
 
 // The FP32 trailing update with the same 128 x 128 tiling on
@@ -989,22 +1016,7 @@ __global__ __launch_bounds__(256, 2) void k_lu_gemm_tr32(float *A, int64_t lda, 
             }
         }
     }
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int col = c0 + wc + 16 * ti + (lane >> 4) * 4 + v;
-                const int row = r0 + wr + 16 * tj + li;
-                if (row < r_end && col < c_end) {
-                    const size_t idx = (size_t)col * lda + row;
-                    c32 x = ld(Ab, idx);
-                    x.re -= accr[ti][tj][v];
-                    x.im -= acci[ti][tj][v];
-                    st(Ab, idx, x);
-                }
-            }
+    tile_epilogue(Ab, lda, c0 + wc, r0 + wr, c_end, r_end, lane, accr, acci, 1.0f);
 }
 
 // Power-of-two scale 2^-e per system with 2^e ~ max |a_ij| (FP16 operand

@@ -147,6 +147,57 @@ def _directivity_from_pressure(pressure: NDArray[np.complex128], on_axis_idx: in
     return spl - spl[:, on_axis_idx][:, None]
 
 
+def _system_field(system, n_planes: int, n_angles: int, n_sphere: int):
+    """One source's field row split into the arc pressure (n_planes,
+    n_angles) and the observation-sphere values (or None)."""
+    flat = np.asarray(system.field, np.complex128).reshape(-1)
+    arc = flat[: n_planes * n_angles].reshape(n_planes, n_angles)
+    sphere = flat[n_planes * n_angles : n_planes * n_angles + n_sphere] if n_sphere else None
+    return arc, sphere
+
+
+def new_result_rows(source_tags) -> dict:
+    """Per-source accumulators of a sweep (filled by _append_system_result)."""
+    return dict(pressure=[], spl=[], imp=[], surf=[], diag=[], log=[],
+                pavg={int(t): [] for t in source_tags})
+
+
+def _append_system_result(rows: dict, *, frequency_hz: float, system, case, diagnostics: dict,
+                          source_tags, n_planes: int, n_angles: int, n_sphere: int,
+                          on_axis_idx: int, keep_surface_pressure: bool):
+    """Appends one source's result at one frequency to ``rows`` and returns
+    (solver-log entry, arc pressure, directivity) (reference
+    _append_system_result, sweep.py:535-609).  ``case`` is the frequency's
+    result (operator-level times, lapack_info), ``system`` the source's own
+    (field, impedance, tag averages, surface pressure; the case itself for
+    the first source)."""
+    pressure, sphere = _system_field(system, n_planes, n_angles, n_sphere)
+    directivity = _directivity_from_pressure(pressure, on_axis_idx)
+    entry = {
+        "frequency_hz": float(frequency_hz),
+        "iterations": None,
+        "timing_s": case.assembly_s + case.dense_solve_s + case.field_s,
+        "backend": "hip_gfx950_resident_assembly_lu_field_batch",
+        "assembly_s": case.assembly_s,
+        "dense_solve_s": case.dense_solve_s,
+        "field_s": system.field_s,
+        "lapack_info": case.lapack_info,
+        "impedance": system.impedance,
+        "native_diagnostics": diagnostics,
+        "observation_sphere_pressure_complex": sphere,
+    }
+    for t in source_tags:
+        rows["pavg"][int(t)].append((system.surface_pressure_avg or {}).get(int(t), 0j))
+    rows["pressure"].append(pressure)
+    rows["spl"].append(directivity)
+    rows["imp"].append(system.impedance)
+    if keep_surface_pressure:
+        rows["surf"].append(np.asarray(system.pressure, np.complex128))
+    rows["diag"].append(diagnostics)
+    rows["log"].append(entry)
+    return entry, pressure, directivity
+
+
 def _apply_dense_solve_policy(diagnostics: dict, *, threshold: float) -> None:
     """rcond below the threshold marks the frequency suspect (reference:
     sweep.py:383-412)."""
@@ -254,10 +305,7 @@ def _native_sweep(
                             face_scale, impedance)
         for d in src_dicts
     ]
-    per_src = [
-        dict(pressure=[], spl=[], imp=[], surf=[], diag=[], log=[], pavg={t: [] for t in source_tags})
-        for _ in range(S)
-    ]
+    per_src = [new_result_rows(source_tags) for _ in range(S)]
     done: list[float] = []
     stream = config.on_frequency_result is not None
     chunk = (config.hip_batch_frequencies or 8) if stream else freqs.size
@@ -300,34 +348,11 @@ def _native_sweep(
                 entries = []
                 for sidx in range(S):
                     rr = r if sidx == 0 else r.extra_sources[sidx - 1]
-                    flat = np.asarray(rr.field, np.complex128)
-                    pressure = flat[: P * N].reshape(P, N)
-                    sphere = flat[P * N : P * N + n_sphere] if n_sphere else None
-                    directivity = _directivity_from_pressure(pressure, on_axis)
-                    acc = per_src[sidx]
-                    entry = {
-                        "frequency_hz": f,
-                        "iterations": None,
-                        "timing_s": r.assembly_s + r.dense_solve_s + r.field_s,
-                        "backend": "hip_gfx950_resident_assembly_lu_field_batch",
-                        "assembly_s": r.assembly_s,
-                        "dense_solve_s": r.dense_solve_s,
-                        "field_s": rr.field_s,
-                        "lapack_info": r.lapack_info,
-                        "impedance": rr.impedance,
-                        "native_diagnostics": diag,
-                        "observation_sphere_pressure_complex": sphere,
-                    }
-                    for t in source_tags:
-                        acc["pavg"][t].append((rr.surface_pressure_avg or {}).get(int(t), 0j))
-                    acc["pressure"].append(pressure)
-                    acc["spl"].append(directivity)
-                    acc["imp"].append(rr.impedance)
-                    if config.return_surface_pressure:
-                        acc["surf"].append(np.asarray(rr.pressure, np.complex128))
-                    acc["diag"].append(diag)
-                    acc["log"].append(entry)
-                    entries.append((entry, pressure, directivity))
+                    entries.append(_append_system_result(
+                        per_src[sidx], frequency_hz=f, system=rr, case=r, diagnostics=diag,
+                        source_tags=source_tags, n_planes=P, n_angles=N, n_sphere=n_sphere,
+                        on_axis_idx=on_axis, keep_surface_pressure=config.return_surface_pressure,
+                    ))
                 done.append(f)
                 if config.progress_callback is not None:
                     config.progress_callback(i, freqs.size, f)

@@ -1,0 +1,464 @@
+"""GPU tests at the benchmark workloads and through the public API, against
+the FP64 oracle (FP64 assembly + zgesv) and closed forms:
+
+- C2: the benchmarked quarter horn (n >= 1000, ~2000 triangles, yz+xz,
+  40 log frequencies 500 Hz - 20 kHz, 3 x 37 points): assembly at both ends
+  of the band, the fp32, mixed_fp16 (GMRES-IR) and float64 sweeps;
+- C3: the full horn (n ~ 8000, 64 log frequencies 200 Hz - 16 kHz,
+  3 x 181 points) in mixed_fp16 and mixed_bf16 with GMRES-IR to 1e-6;
+- C4: the 16 uneven horn variants through concurrent sessions;
+- C5: the adjoint at ~3000 dofs, directional derivatives against central
+  differences of the GPU forward;
+- solve() / solve_multi_source() on the pulsating sphere and a horn;
+- yz-only and xy-only symmetric solves against the full domain
+  (reference tests/test_metal_native.py:1982-2108, 2354-2460).
+"""
+from __future__ import annotations
+
+import math
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import bench
+import oracle as O
+from meshes import buffers_from_arrays, mesh_buffers, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+C0 = 343.0
+
+
+def _log(msg):
+    print(f"[workloads] {msg}", flush=True)
+
+
+def _session(buffers, plane=None, check_open_edges=True):
+    from hornlab_metal_bem.native import HipBemSession
+
+    return HipBemSession.create_session(buffers, symmetry_plane=plane,
+                                        check_open_edges=check_open_edges)
+
+
+def _oracle_threads():
+    import os
+
+    O.set_threads(min(16, os.cpu_count() or 1))
+
+
+# -- C2 ---------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c2():
+    spec, h, mesh, config = bench.workload(name="c2")
+    freqs, k, q, pts, buffers = bench.prepare_inputs(mesh, config)
+    _oracle_threads()
+    g = O.Geometry.from_buffers(buffers, "yz+xz")
+    ref = {}
+    for i in (0, freqs.size - 1):
+        A, rhs, _ = O.assemble(g, float(k[i]), q[i])
+        p = O.solve(A, rhs)
+        ref[i] = (A, rhs, p, O.field(g, float(k[i]), p, q[i], pts))
+    return freqs, k, q, pts, buffers, ref
+
+
+def test_c2_workload_shape(c2):
+    freqs, k, q, pts, b, _ = c2
+    assert b.p1_dof_count >= 1000 and 1900 <= b.n_triangles <= 2200
+    assert freqs.size == 40 and freqs[0] == pytest.approx(500.0) and freqs[-1] == pytest.approx(20000.0)
+    assert pts.shape == (3, 111)
+
+
+def test_c2_assembly_matches_fp64_oracle(require_gpu, c2):
+    freqs, k, q, pts, b, ref = c2
+    with _session(b, "yz+xz") as s:
+        for i, (A64, r64, _, _) in ref.items():
+            A, rhs = s.assemble_standard_neumann(float(freqs[i]), float(k[i]), q[i])
+            _log(f"c2 assembly {freqs[i]:.0f} Hz: A {rel_l2(A, A64):.2e} rhs {rel_l2(rhs, r64):.2e}")
+            assert rel_l2(A, A64) < 1e-5, (freqs[i], rel_l2(A, A64))
+            assert rel_l2(rhs, r64) < 1e-5, (freqs[i], rel_l2(rhs, r64))
+
+
+@pytest.mark.parametrize("mode,tol", [("fp32", None), ("mixed_fp16", 1e-6), ("mixed_bf16", 1e-6),
+                                      ("fp64", None)])
+def test_c2_sweep_matches_fp64_oracle(require_gpu, c2, mode, tol):
+    """The 40-frequency sweep in each solve mode: surface pressure and field
+    within 1e-5 of FP64 assembly + zgesv at 500 Hz and 20 kHz, refinement
+    residuals at their targets, rcond estimated for every frequency."""
+    freqs, k, q, pts, b, ref = c2
+    kw = {} if tol is None else {"refine_tolerance": tol, "refine_max_iterations": 20}
+    with _session(b, "yz+xz") as s:
+        res = s.assemble_solve_evaluate_standard_neumann_batch(
+            freqs, k, q, pts, source_tags=[2], impedance_source_tag=2,
+            write_surface_pressure=True, solve_mode=mode, estimate_rcond=True, **kw)
+    assert len(res) == freqs.size
+    for r in res:
+        d = r.diagnostics
+        assert r.lapack_info == 0
+        assert 0.0 < d["dense_solve_rcond"] <= 1.0
+        if mode != "fp32":
+            assert d["dense_solve_refine_stalled"] is False, (r.frequency_hz, d)
+            assert d["dense_solve_refine_residual_rel"] <= (tol or 1e-10), (r.frequency_hz, d)
+    for i, (_, _, p64, f64) in ref.items():
+        ep, ef = rel_l2(res[i].pressure, p64), rel_l2(res[i].field, f64)
+        _log(f"c2 {mode} {freqs[i]:.0f} Hz: surface p {ep:.2e} field {ef:.2e}")
+        assert ep < 1e-5 and ef < 1e-5, (mode, freqs[i], ep, ef)
+
+
+# -- C3 ---------------------------------------------------------------------
+
+def test_c3_mixed_precision_sweeps_match_fp64_oracle(require_gpu):
+    """Full horn, n ~ 8000: the 64-frequency sweep in mixed_fp16 and
+    mixed_bf16 reaches the 1e-6 GMRES-IR residual at every frequency; at
+    16 kHz the assembled matrix is within 1e-5 of the FP64 oracle and both
+    solutions within 1e-4 of FP64 assembly + zgesv (field too); at 1 kHz
+    twelve random rows of the matrix match the oracle's rows."""
+    _oracle_threads()
+    spec, h, mesh, config = bench.workload(name="c3")
+    freqs, k, q, pts, b = bench.prepare_inputs(mesh, config)
+    n, F = b.p1_dof_count, freqs.size
+    _log(f"c3 n={n} T={b.n_triangles} F={F}")
+    assert 7500 <= n <= 8500 and F == 64
+    top = F - 1
+    out = {}
+    with _session(b) as s:
+        for mode in ("mixed_fp16", "mixed_bf16"):
+            res = s.assemble_solve_evaluate_standard_neumann_batch(
+                freqs, k, q, pts, source_tags=[2], impedance_source_tag=2,
+                write_surface_pressure=True, solve_mode=mode, refine_tolerance=1e-6,
+                refine_max_iterations=20, estimate_rcond=False)
+            worst = max(r.diagnostics["dense_solve_refine_residual_rel"] for r in res)
+            _log(f"c3 {mode}: worst GMRES-IR residual {worst:.2e}")
+            for r in res:
+                assert r.lapack_info == 0
+                assert r.diagnostics["dense_solve_refine_residual_rel"] <= 1e-6, (mode, r.frequency_hz)
+                assert r.diagnostics["dense_solve_refine_stalled"] is False
+            out[mode] = (res[top].pressure, res[top].field)
+        A, rhs = s.assemble_standard_neumann(float(freqs[top]), float(k[top]), q[top])
+        i1 = int(np.argmin(np.abs(freqs - 1000.0)))
+        A1, _ = s.assemble_standard_neumann(float(freqs[i1]), float(k[i1]), q[i1])
+    g = O.Geometry.from_buffers(b)
+    rows = np.random.default_rng(3).choice(n, 12, replace=False)
+    l2g = np.asarray(b.p1_local2global_i32).reshape(-1, 3)
+    sel = np.isin(l2g, rows).any(axis=1)
+    A1o, _ = O.assemble_subset(g, sel, float(k[i1]), q[i1])
+    e_rows = rel_l2(A1[rows], A1o[rows])
+    _log(f"c3 rows at {freqs[i1]:.0f} Hz: {e_rows:.2e}")
+    assert e_rows < 1e-5
+    del A1, A1o
+    A64, r64, _ = O.assemble(g, float(k[top]), q[top])
+    eA, er = rel_l2(A, A64), rel_l2(rhs, r64)
+    _log(f"c3 assembly {freqs[top]:.0f} Hz: A {eA:.2e} rhs {er:.2e}")
+    assert eA < 1e-5 and er < 1e-5
+    del A
+    p64 = O.solve(A64, r64)
+    del A64
+    f64 = O.field(g, float(k[top]), p64, q[top], pts)
+    for mode, (p, f) in out.items():
+        ep, ef = rel_l2(p, p64), rel_l2(f, f64)
+        _log(f"c3 {mode} {freqs[top]:.0f} Hz: surface p {ep:.2e} field {ef:.2e}")
+        assert ep < 1e-4 and ef < 1e-4, (mode, ep, ef)
+
+
+# -- C4 ---------------------------------------------------------------------
+
+def test_c4_concurrent_sessions_match_eager_runs_and_oracle(require_gpu):
+    """The 16 uneven variants (512 systems) solved by concurrent sessions on
+    8 host threads equal each variant's eager single-session run (up to the
+    order of float atomics), and the two smallest match the FP64 oracle at
+    the top frequency."""
+    _oracle_threads()
+    variants = bench.c4_variants()
+    inputs = [bench.prepare_inputs(mesh, config) for _, _, mesh, config in variants]
+    ns = [b.p1_dof_count for *_, b in inputs]
+    _log(f"c4 n = {ns}")
+    assert len(inputs) == 16 and sum(x[0].size for x in inputs) == 512
+    assert min(ns) >= 550 and max(ns) <= 4200
+
+    def run(inp):
+        freqs, k, q, pts, b = inp
+        with _session(b, "yz+xz") as s:
+            res = s.assemble_solve_evaluate_standard_neumann_batch(
+                freqs, k, q, pts, source_tags=[2], impedance_source_tag=2,
+                write_surface_pressure=True, estimate_rcond=False, batch_frequencies=16)
+        return [(r.pressure, r.field) for r in res]
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        conc = list(ex.map(run, inputs))
+    for v, inp in enumerate(inputs):
+        eager = run(inp)
+        for i, ((pc, fc), (pe, fe)) in enumerate(zip(conc[v], eager)):
+            assert rel_l2(pc, pe) < 1e-5 and rel_l2(fc, fe) < 1e-5, (v, i)
+    for v in np.argsort(ns)[:2]:
+        freqs, k, q, pts, b = inputs[v]
+        i = freqs.size - 1
+        g = O.Geometry.from_buffers(b, "yz+xz")
+        A, rhs, _ = O.assemble(g, float(k[i]), q[i])
+        p64 = O.solve(A, rhs)
+        f64 = O.field(g, float(k[i]), p64, q[i], pts)
+        ep, ef = rel_l2(conc[v][i][0], p64), rel_l2(conc[v][i][1], f64)
+        _log(f"c4 variant {v} (n={ns[v]}) {freqs[i]:.0f} Hz: surface p {ep:.2e} field {ef:.2e}")
+        assert ep < 1e-4 and ef < 1e-4
+
+
+# -- C5 ---------------------------------------------------------------------
+
+def test_c5_adjoint_directional_derivatives_match_gpu_forward(require_gpu):
+    """~3000-dof quarter horn, 24 frequencies: the hand-written adjoint's
+    derivative of the directivity/ripple loss along the gradient and along a
+    random direction of the off-plane vertices against central differences
+    (step 1e-4 m) of the same loss through the GPU forward."""
+    import torch
+
+    from hornlab_metal_bem.adjoint import FieldProblem, directivity_ripple_loss, exterior_field
+
+    freqs, k, q, pts, b, _ = bench.c5_problem()
+    n, F = b.p1_dof_count, freqs.size
+    _log(f"c5 n={n} F={F}")
+    assert 2700 <= n <= 3300 and F == 24
+    prob = FieldProblem.from_buffers(b, k.astype(np.float32), pts, symmetry_plane="yz+xz",
+                                     solve_mode="fp32_ir", refine_tolerance=1e-9,
+                                     source_tags=(2,), impedance_source_tag=2)
+    dev = torch.device("cuda:0")
+    V0 = np.asarray(b.vertices_3xn_f32, np.float64)
+    qd = torch.tensor(q.astype(np.complex64), device=dev)
+
+    def loss_at(V, grad=False):
+        Vt = torch.tensor(V, device=dev, requires_grad=grad)
+        L = directivity_ripple_loss(exterior_field(Vt, qd, prob), 3, pts.shape[1] // 3)
+        if grad:
+            L.backward()
+            return float(L.detach()), Vt.grad.cpu().numpy()
+        return float(L.detach()), None
+
+    L0, G = loss_at(V0, grad=True)
+    off = (np.abs(V0[0]) > 1e-3) & (np.abs(V0[1]) > 1e-3)
+    rng = np.random.default_rng(9)
+    dirs = {"gradient": np.where(off[None], G, 0.0),
+            "random": np.where(off[None], rng.standard_normal(V0.shape), 0.0)}
+    eps = 1e-4
+    for name, d in dirs.items():
+        d = d / np.linalg.norm(d)
+        Lp, _ = loss_at(V0 + eps * d)
+        Lm, _ = loss_at(V0 - eps * d)
+        fd = (Lp - Lm) / (2 * eps)
+        an = float(np.sum(G * d))
+        err = abs(fd - an) / max(abs(fd), 1e-30)
+        _log(f"c5 {name}: adjoint {an:.6e} central difference {fd:.6e} rel {err:.2e}")
+        scale = abs(float(np.sum(G * dirs['gradient'])) / np.linalg.norm(dirs["gradient"]))
+        assert abs(fd - an) <= 1e-3 * scale, (name, an, fd)
+
+
+# -- public API ---------------------------------------------------------------
+
+def test_solve_on_pulsating_sphere_fills_every_result_field(require_gpu):
+    """C1 through solve(): every SolveResult field against the FP64 oracle
+    and the closed forms (flat directivity of a monopole, specific impedance
+    conj(ika/(1+ika)) rho c, the analytic far field)."""
+    from hornlab_metal_bem import ObservationConfig, native_config, solve
+    from hornlab_metal_bem.procedural import sphere_mesh
+
+    a, rho, U = 0.1, 1.21, 1.0
+    mesh = sphere_mesh(a, 3)
+    cfg = native_config(
+        freq_min_hz=100.0, freq_max_hz=2000.0, freq_count=10, freq_spacing="linear",
+        velocity_mode="velocity", air_density=rho, velocity_sources={2: U},
+        return_surface_pressure=True,
+        observation=ObservationConfig(planes=["horizontal"], distance_m=2.0, angle_count=37),
+    )
+    res = solve(mesh, cfg)
+    F = 10
+    np.testing.assert_allclose(res.frequencies_hz, np.linspace(100.0, 2000.0, F))
+    assert res.pressure_complex.shape == (F, 1, 37)
+    assert res.directivity_db.shape == (F, 1, 37)
+    assert res.surface_pressure_complex.shape == (F, mesh.info.n_vertices)
+    assert len(res.solver_log) == F and len(res.native_diagnostics) == F
+    assert set(res.timings) >= {"solve_s", "assembly_s", "dense_solve_s", "total_s"}
+    pts = res.observation_points.reshape(-1, 3).T
+    from hornlab_metal_bem.geometry import build_geometry_buffers
+    from hornlab_metal_bem.mesh import make_pure_function_spaces
+
+    p1, dp0 = make_pure_function_spaces(mesh.grid)
+    g = O.Geometry.from_buffers(build_geometry_buffers(mesh.grid, mesh.physical_tags, p1, dp0))
+    for i, f in enumerate(res.frequencies_hz):
+        kk = float(np.float32(2 * np.pi * f / C0))
+        ka = 2 * np.pi * f / C0 * a
+        d = res.native_diagnostics[i]
+        assert 0.0 < d["dense_solve_rcond"] <= 1.0 and d["lapack_info"] == 0
+        assert res.solver_log[i]["frequency_hz"] == pytest.approx(f)
+        # monopole: flat directivity
+        assert np.max(np.abs(res.directivity_db[i])) < 0.05
+        # specific impedance of the pulsating sphere (q = i rho omega U)
+        z = rho * C0 * (1j * ka) / (1j * ka - 1) / (rho * C0)
+        if abs(ka - math.pi) > 0.4:
+            assert abs(res.impedance[i] / (rho * C0 * U) - z) < 0.03 * abs(z), (f, res.impedance[i])
+        if i in (0, 4, 9):
+            q = np.full(mesh.info.n_triangles, 1j * rho * 2 * np.pi * f * U)
+            A, rhs, _ = O.assemble(g, kk, q)
+            p = O.solve(A, rhs)
+            fo = O.field(g, kk, p, q, pts)
+            imp, _ = O.surface_average(g, p, 2)
+            assert rel_l2(res.surface_pressure_complex[i], p) < 1e-4
+            assert rel_l2(res.pressure_complex[i].reshape(-1), fo) < 1e-4
+            assert abs(res.impedance[i] - imp) < 1e-4 * abs(imp)
+            assert abs(res.surface_pressure_avg[2][i] - imp) < 1e-4 * abs(imp)
+
+
+def test_solve_multi_source_matches_oracle_per_source(require_gpu):
+    """Two velocity sources sharing each frequency's operator and LU: every
+    source's field and surface pressure against its own FP64 oracle solve
+    (q rows from the same host builder the sweep uses)."""
+    from dataclasses import replace
+
+    from hornlab_metal_bem import ObservationConfig, loaded_mesh_from_arrays, native_config
+    from hornlab_metal_bem import solve_multi_source
+    from hornlab_metal_bem.procedural import HornSpec, horn_body
+    from hornlab_metal_bem.sweep import _build_neumann_rows
+
+    V, Fc, T = horn_body(HornSpec(), edge_length=0.03, quarter=True, seed=0)
+    mesh = loaded_mesh_from_arrays(V, Fc, T, native_symmetry_plane="yz+xz")
+    cfg = native_config(
+        freq_min_hz=1000.0, freq_max_hz=6000.0, freq_count=2, freq_spacing="log",
+        native_symmetry_plane="yz+xz", return_surface_pressure=True,
+        observation=ObservationConfig(planes=["horizontal", "vertical"], distance_m=2.0,
+                                      angle_count=9),
+    )
+    sources = [{2: 1.0}, {2: 0.5, 1: 0.05}]
+    out = solve_multi_source(mesh, sources, cfg)
+    assert len(out) == 2
+    from hornlab_metal_bem.geometry import build_geometry_buffers
+    from hornlab_metal_bem.mesh import make_pure_function_spaces
+
+    p1, dp0 = make_pure_function_spaces(mesh.grid)
+    b = build_geometry_buffers(mesh.grid, mesh.physical_tags, p1, dp0)
+    g = O.Geometry.from_buffers(b, "yz+xz")
+    freqs = out[0].frequencies_hz
+    for sidx, src in enumerate(sources):
+        r = out[sidx]
+        q = _build_neumann_rows(dp0, mesh.physical_tags, freqs, replace(cfg, velocity_sources=src))
+        pts = r.observation_points.reshape(-1, 3).T
+        for i, f in enumerate(freqs):
+            kk = float(np.float32(2 * np.pi * f / C0))
+            A, rhs, _ = O.assemble(g, kk, q[i])
+            p = O.solve(A, rhs)
+            fo = O.field(g, kk, p, q[i], pts)
+            ep = rel_l2(r.surface_pressure_complex[i], p)
+            ef = rel_l2(r.pressure_complex[i].reshape(-1), fo)
+            _log(f"multi-source {sidx} {f:.0f} Hz: surface p {ep:.2e} field {ef:.2e}")
+            assert ep < 1e-4 and ef < 1e-4, (sidx, f, ep, ef)
+
+
+# -- single-plane symmetry against the full domain ---------------------------
+
+def _tiny(plane):
+    """Single-triangle half with its one image plane and the two-triangle
+    full mesh it reconstructs (reference test_metal_native.py:416-445 and
+    :475-498 for yz, :623-677 for xy)."""
+    if plane == "yz":
+        half = buffers_from_arrays([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [0.0, 0.0, 0.0]],
+                                   [[0], [1], [2]], [[0, 1, 2]], 3, [2])
+        full = buffers_from_arrays(
+            [[0.0, 1.0, 0.0, 0.0, -1.0, 0.0], [0.0, 0.0, 1.0, 0.0, 0.0, 1.0],
+             [0.0, 0.0, 0.0, 0.0, 0.0, 0.0]],
+            [[0, 3], [1, 5], [2, 4]], [[0, 1, 2], [3, 5, 4]], 6, [2, 2])
+        obs = np.array([[0.25, -0.25], [0.2, 0.2], [1.0, 1.0]], np.float32)
+    else:
+        half = buffers_from_arrays([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [0.0, 0.0, 1.0]],
+                                   [[0], [1], [2]], [[0, 1, 2]], 3, [2])
+        full = buffers_from_arrays(
+            [[0.0, 1.0, 0.0, 0.0, 1.0, 0.0], [0.0, 0.0, 1.0, 0.0, 0.0, 1.0],
+             [0.0, 0.0, 1.0, 0.0, 0.0, -1.0]],
+            [[0, 3], [1, 5], [2, 4]], [[0, 1, 2], [3, 5, 4]], 6, [2, 2])
+        obs = np.array([[0.25, 0.25], [0.2, 0.2], [1.0, -1.0]], np.float32)
+    return half, full, obs
+
+
+@pytest.mark.parametrize("plane", ["yz", "xy"])
+@pytest.mark.parametrize("mode", ["optimized", "corrected"])
+def test_single_plane_tiny_half_matches_even_full_solve(require_gpu, plane, mode):
+    """Half mesh with one symmetry plane against the even-mode reduction of
+    the full two-triangle mesh (rtol 5e-4, atol 5e-5 as the reference)."""
+    half, full, obs = _tiny(plane)
+    f, k = 100.0, 1.8318326
+    with _session(full) as s:
+        s.set_assembly_mode(mode)
+        Af, rf = s.assemble_standard_neumann(f, k, np.array([1.0, 1.0], np.complex64))
+    with _session(half, plane, check_open_edges=False) as s:
+        s.set_assembly_mode(mode)
+        Ah, rh = s.assemble_standard_neumann(f, k, np.array([1.0], np.complex64))
+        ph = np.linalg.solve(Ah.astype(np.complex128), rh.astype(np.complex128)).astype(np.complex64)
+        fh = s.evaluate_standard_exterior_batch(np.array([f]), np.array([k], np.float32), ph[None],
+                                                np.array([[1.0]], np.complex64), obs)[0]
+    real, mirror = np.array([0, 1, 2]), np.array([3, 4, 5])
+    Aeven = Af[np.ix_(real, real)] + Af[np.ix_(real, mirror)]
+    peven = np.linalg.solve(Aeven.astype(np.complex128), rf[real].astype(np.complex128))
+    np.testing.assert_allclose(ph, peven, rtol=5e-4, atol=5e-5)
+    pfull = np.concatenate([peven, peven]).astype(np.complex64)
+    with _session(full) as s:
+        ff = s.evaluate_standard_exterior_batch(np.array([f]), np.array([k], np.float32), pfull[None],
+                                                np.array([[1.0, 1.0]], np.complex64), obs)[0]
+    np.testing.assert_allclose(fh, ff, rtol=5e-4, atol=5e-5)
+
+
+def _mirror(V, F, T, axis):
+    """Mesh and its mirror image across the plane normal to ``axis`` (shared
+    vertices on the plane merged, mirrored triangles re-oriented)."""
+    keys, verts, tris = {}, [], []
+    for sgn in (1.0, -1.0):
+        remap = np.empty(V.shape[0], dtype=np.int64)
+        for i, p in enumerate(V):
+            qv = p.copy()
+            qv[axis] *= sgn
+            key = tuple(np.round(qv / 1e-9).astype(np.int64).tolist())
+            if key not in keys:
+                keys[key] = len(verts)
+                verts.append(qv)
+            remap[i] = keys[key]
+        t = remap[F]
+        tris.append(t[:, [0, 2, 1]] if sgn < 0 else t)
+    return np.array(verts), np.vstack(tris).astype(np.int32), np.concatenate([T, T])
+
+
+@pytest.mark.parametrize("plane", ["yz", "xy"])
+def test_single_plane_horn_half_matches_full_domain(require_gpu, plane):
+    """A horn half with only the yz plane (x >= 0) or only the xy plane
+    (the horn turned so its remaining mirror plane is z = 0) against the full
+    mesh without symmetry: surface pressure on the half's vertices and the
+    field at points on both sides of the plane."""
+    from hornlab_metal_bem.procedural import HornSpec, horn_body
+
+    Vq, Fq, Tq = horn_body(HornSpec(), edge_length=0.03, quarter=True, seed=0)
+    # quarter (x, y >= 0) -> half x >= 0 (yz plane) by mirroring across y = 0
+    Vh, Fh, Th = _mirror(Vq, Fq, Tq, axis=1)
+    Vf, Ff, Tf = _mirror(Vh, Fh, Th, axis=0)
+    if plane == "xy":
+        # cyclic axis permutation (a rotation): (x', y', z') = (y, z, x) puts
+        # the half's mirror plane x = 0 at z' = 0
+        Vh, Vf = Vh[:, [1, 2, 0]], Vf[:, [1, 2, 0]]
+    bh = mesh_buffers(Vh, Fh, Th)
+    bf = mesh_buffers(Vf, Ff, Tf)
+    f = 3000.0
+    k = np.array([2 * np.pi * f / C0], np.float32)
+    ang = np.deg2rad(np.linspace(-80, 80, 9))
+    base = np.stack([1.5 * np.sin(ang), 0.3 + 0 * ang, 1.5 * np.cos(ang)])  # horn frame
+    if plane == "xy":
+        base = base[[1, 2, 0]]
+    obs = base.astype(np.float32)
+
+    def run(buffers, sym):
+        q = np.where(np.asarray(buffers.physical_tags_i32) == 2, 1j * 1.2041 * 2 * np.pi * f, 0.0)
+        with _session(buffers, sym) as s:
+            r = s.assemble_solve_evaluate_standard_neumann_batch(
+                np.array([f]), k, q[None].astype(np.complex64), obs, source_tags=[2],
+                impedance_source_tag=2, write_surface_pressure=True)[0]
+        return r
+
+    rh, rf = run(bh, plane), run(bf, None)
+    key = lambda p: tuple(np.round(p * 1e7).astype(np.int64).tolist())
+    index = {key(p): j for j, p in enumerate(Vf)}
+    match = np.array([index[key(p)] for p in Vh])
+    np.testing.assert_allclose(rh.pressure, rf.pressure[match], rtol=5e-4,
+                               atol=5e-5 * np.abs(rf.pressure).max())
+    np.testing.assert_allclose(rh.field, rf.field, rtol=5e-4, atol=5e-5 * np.abs(rf.field).max())
+    assert rel_l2(rh.pressure, rf.pressure[match]) < 1e-4
