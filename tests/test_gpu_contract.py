@@ -77,8 +77,8 @@ def test_rcond_flags_near_singular_system(require_gpu):
 def test_fp64_solve_reaches_zgesv_accuracy(require_gpu):
     """cond = 1e6: the FP32 LU alone leaves ~1e-2 error; the float64 mode
     (FP32 LU + GMRES-IR on the FP64 residual) matches zgesv of the same FP32
-    matrix to 1e-8 and reports no stall, while classical refinement at
-    cond = 1e9 stalls and says so."""
+    matrix to the FP32 rounding of the returned vector and reports no stall,
+    while classical refinement at cond = 1e9 stalls and says so."""
     from hornlab_metal_bem.native import dense_solve
 
     rng = np.random.default_rng(12)
@@ -89,7 +89,8 @@ def test_fp64_solve_reaches_zgesv_accuracy(require_gpu):
     x32, _ = dense_solve(A, b, solve_mode="fp32")
     x, info = dense_solve(A, b, solve_mode="fp64", refine_max_iterations=12)
     assert rel_l2(x32[0], ref) > 1e-5
-    assert rel_l2(x[0], ref) < 1e-8, (rel_l2(x[0], ref), info[0])
+    # complex64 output: its rounding (eps32 / 2 ~ 6e-8 per entry) is the floor
+    assert rel_l2(x[0], ref) < 1e-7, (rel_l2(x[0], ref), info[0])
     assert info[0]["dense_solve_refine_residual_rel"] < 1e-10, info[0]
     assert info[0]["dense_solve_refine_stalled"] is False
 
