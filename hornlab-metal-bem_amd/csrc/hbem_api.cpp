@@ -28,7 +28,7 @@ struct HbemError : std::runtime_error {
     HbemError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
-void fail(int code, const std::string &msg) { throw HbemError(code, msg); }
+[[noreturn]] void fail(int code, const std::string &msg) { throw HbemError(code, msg); }
 
 template <typename F>
 int32_t guarded(F &&f) {
@@ -127,6 +127,12 @@ struct Timer {
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 }  // namespace
+
+namespace hbem {
+int32_t run_guarded(const std::function<void()> &f) { return guarded(f); }
+void fail_with(int code, const std::string &msg) { fail(code, msg); }
+void check_hip(hipError_t e, const char *what) { hip_check(e, what); }
+}  // namespace hbem
 
 struct hbem_session {
     std::mutex mu;

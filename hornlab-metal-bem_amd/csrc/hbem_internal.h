@@ -5,10 +5,18 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
 namespace hbem {
+
+// The C ABI's error convention for entry points defined outside hbem_api.cpp:
+// run_guarded returns HBEM_OK or the code of the failure and records its
+// message for hbem_last_error(); fail_with / check_hip raise such failures.
+int32_t run_guarded(const std::function<void()> &f);
+[[noreturn]] void fail_with(int code, const std::string &msg);
+void check_hip(hipError_t e, const char *what);
 
 // ---------------------------------------------------------------------------
 // Quadrature constants shared by host and device.

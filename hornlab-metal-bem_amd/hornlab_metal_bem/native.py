@@ -140,6 +140,7 @@ EXPORTED_SYMBOLS = (
     "hbem_solve_batch_device",
     "hbem_evaluate_field",
     "hbem_dense_solve",
+    "hbem_dense_solve_noalloc",
     "hbem_adjoint_forward",
     "hbem_adjoint_backward",
 )
@@ -252,6 +253,10 @@ def load_library() -> C.CDLL:
             C.POINTER(_CaseDiag),
         ]
         lib.hbem_dense_solve.restype = C.c_int32
+        lib.hbem_dense_solve_noalloc.argtypes = [
+            C.c_int32, C.c_int32, vp, vp, vp, vp, C.POINTER(_SolveOptions), vp, vp, vp, vp,
+        ]
+        lib.hbem_dense_solve_noalloc.restype = C.c_int32
         if lib.hbem_abi_version() != HBEM_ABI_VERSION:
             raise AssemblyBackendUnavailable(
                 f"libhbem ABI {lib.hbem_abi_version()} != expected {HBEM_ABI_VERSION}; rebuild"

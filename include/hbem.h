@@ -293,6 +293,31 @@ int32_t hbem_dense_solve(int32_t n_sys, int32_t n, const float *A_re, const floa
                          const float *b_re, const float *b_im, const hbem_solve_options *opts,
                          float *x_re, float *x_im, hbem_case_diag *diag);
 
+/* hbem_dense_solve on device arrays without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its working storage is a fixed device array of the library,
+ * HBEM_NOALLOC_SCRATCH_BYTES on each GPU it runs on, held until the process
+ * ends and shared by every call on that GPU: calls that overlap on one GPU
+ * race (each call overwrites what the previous one left there).
+ *   A_re, A_im [n_sys][n][n] row-major float, b_re, b_im [n_sys][n],
+ *   x_re, x_im [n_sys][n] (written), info [n_sys] int32 or NULL (LAPACK
+ *   info: 0, or the first column without a nonzero pivot): device memory of
+ *   the current device;
+ *   opts: NULL, or solve_mode HBEM_SOLVE_F32 with no rcond, refinement,
+ *   CHIEF rows or extra sources (solve_trans 0 or 2 is honoured);
+ *   stream: the hipStream_t to enqueue on (NULL: the null stream).
+ * Limits: n_sys <= 64, n <= 2048, n_sys * N^2 <= 2^22 with N = n rounded up
+ * to 64; larger problems are refused (HBEM_EINVAL) and take hbem_dense_solve.
+ * The call only enqueues work (no synchronisation).  The first call on a
+ * device looks up the array's address there (hipGetSymbolAddress): make it
+ * outside graph capture. */
+#define HBEM_NOALLOC_SCRATCH_BYTES 37257472
+int32_t hbem_dense_solve_noalloc(int32_t n_sys, int32_t n, const float *A_re, const float *A_im,
+                                 const float *b_re, const float *b_im,
+                                 const hbem_solve_options *opts, float *x_re, float *x_im,
+                                 int32_t *info, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
