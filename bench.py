@@ -172,79 +172,59 @@ def run_c5(args) -> int:
 
 
 def run_c4(args) -> int:
-    """Batched uneven sweep: every variant's session on its own HIP stream,
-    driven from its own host thread, all 512 systems per step; per-system
-    latency is the GPU time of the resident batch the system ran in."""
+    """Batched uneven sweep: the 16 variants' sessions in one session group
+    (one assembly launch over all meshes, size-classed batched LU on parallel
+    streams, the whole sweep replayed from one hipGraph), size-class step
+    autotuned; per-system latency = time from the sweep's start until the
+    system's mesh has its results."""
     import torch
-    from concurrent.futures import ThreadPoolExecutor
 
-    from hornlab_metal_bem.native import HipBemSession
+    from hornlab_metal_bem.native import GroupCase, HipBemGroup, HipBemSession
 
     variants = c4_variants()
     print(f"c4: {len(variants)} variants meshed", file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
-    jobs = []
+    jobs, sessions, cases = [], [], []
     for spec, target, mesh, config in variants:
         freqs, k, q, pts, buffers = prepare_inputs(mesh, config)
         F, n, n_obs = freqs.size, buffers.p1_dof_count, pts.shape[1]
         job = dict(
-            spec=spec, n=n, T=buffers.n_triangles, F=F, n_obs=n_obs,
+            n=n, T=buffers.n_triangles, F=F, n_obs=n_obs,
             k=torch.from_numpy(k.astype(np.float32)).to(dev),
             q=torch.from_numpy(np.ascontiguousarray(q.astype(np.complex64)).view(np.float32)).to(dev),
             obs=torch.from_numpy(np.ascontiguousarray(pts, np.float32)).to(dev),
             field=torch.zeros(F * n_obs * 2, dtype=torch.float32, device=dev),
             tag=torch.zeros(F * 2 * 2, dtype=torch.float64, device=dev),
-            session=HipBemSession.create_session(buffers, symmetry_plane="yz+xz"),
         )
         jobs.append(job)
+        sessions.append(HipBemSession.create_session(buffers, symmetry_plane="yz+xz"))
+        cases.append(GroupCase(F, job["k"].data_ptr(), job["k"].data_ptr(), job["q"].data_ptr(),
+                               n_obs, job["obs"].data_ptr(), job["field"].data_ptr(),
+                               job["tag"].data_ptr()))
     torch.cuda.synchronize()
-    solve_mode = args.solve_mode or "fp32"
+    group = HipBemGroup(sessions)
+    tune = group.autotune(cases, source_tags=[2], impedance_source_tag=2)
+    print(f"c4 autotune {tune}", file=sys.stderr, flush=True)
 
-    def run(job, batch):
-        return job["session"].solve_batch_device(
-            job["F"], job["k"].data_ptr(), job["k"].data_ptr(), job["q"].data_ptr(), job["n_obs"],
-            job["obs"].data_ptr(), job["field"].data_ptr(), job["tag"].data_ptr(), source_tags=[2],
-            impedance_source_tag=2, solve_mode=solve_mode, estimate_rcond=False,
-            batch_frequencies=batch)
+    def step():
+        return group.solve_device(cases, source_tags=[2], impedance_source_tag=2)
 
-    def sweep(batch, workers):
-        with ThreadPoolExecutor(max_workers=workers) as ex:
-            return list(ex.map(lambda j: run(j, batch), jobs))
-
-    # autotune: resident batch size and host concurrency, one timed sweep each
-    best, trials = None, []
-    for batch in (8, 16, 32):
-        for workers in (4, 8, 16):
-            sweep(batch, workers)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            sweep(batch, workers)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            trials.append({"batch_frequencies": batch, "host_threads": workers, "s": round(dt, 4)})
-            print(f"c4 autotune {trials[-1]}", file=sys.stderr, flush=True)
-            if best is None or dt < best[0]:
-                best = (dt, batch, workers)
-    _, batch, workers = best
     for _ in range(args.warmup):
-        sweep(batch, workers)
+        step()
     torch.cuda.synchronize()
-    diags = []
+    lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        diags.append(sweep(batch, workers))
+        _, member_s, _ = step()
+        for j, ls in zip(jobs, member_s):
+            lat.extend([ls] * j["F"])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    group.close()
+    for s in sessions:
+        s.close()
     systems = sum(j["F"] for j in jobs)
-    lat = []
-    for step in diags:
-        for per_job in step:
-            for d in per_job:
-                lat.append(d["batch_size"] * (d["assembly_seconds"] + d["dense_solve_seconds"]
-                                              + d["field_seconds"]))
     lat = np.asarray(lat) * 1e3
-    for j in jobs:
-        j["session"].close()
     ns = [j["n"] for j in jobs]
     out = {
         "metric": "frequencies solved/sec (assemble+LU+field) at 1k/8k dofs; rel-L2 vs FP64",
@@ -261,10 +241,13 @@ def run_c4(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": f"batched uneven sweep: {len(jobs)} quarter-horn variants, n {min(ns)}-{max(ns)} "
-            f"P1 dofs, {WORKLOADS['c4']['nf']} log frequencies each = {systems} systems",
+            f"P1 dofs, {WORKLOADS['c4']['nf']} log frequencies each = {systems} systems, one "
+            "session group (one assembly launch, size-classed batched LU, one hipGraph)",
             "n_dofs": ns,
-            "solve_mode": solve_mode,
-            "autotuned": {"batch_frequencies": batch, "host_threads": workers, "trials": trials},
+            "solve_mode": "fp32",
+            "autotuned": {"size_class_step": tune["granularity"],
+                          "assembly_split_per_class": tune["split_assembly"],
+                          "sweep_seconds_per_candidate": tune["seconds"]},
         },
         "latency_ms_per_system": {"p50": round(float(np.percentile(lat, 50)), 3),
                                   "p99": round(float(np.percentile(lat, 99)), 3)},
@@ -273,7 +256,6 @@ def run_c4(args) -> int:
     }
     print(json.dumps(out))
     return 0
-
 
 
 def workload(target_dofs: int | None = None, seed: int = 0, name: str = "c2"):
@@ -361,8 +343,8 @@ def fp64_reference(buffers, sym, k, q, pts, idx, threads):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--dofs", type=int, default=None)
     ap.add_argument("--solve-mode", default=None,
@@ -465,7 +447,7 @@ def main() -> int:
                 ("mixed_bf16", "mixed_bf16", args.mixed_tol, None, False),
             ]
             for name, mode, tol, batch, graph in variants:
-                msteps = max(2, args.steps // 3)
+                msteps = max(2, min(10, args.steps // 3))
                 mdt, mdiags = timed(make_step(mode, tol, batch, graph), msteps, 1)
                 modes[name] = {
                     "value": round(F * msteps / mdt, 3), "ms_per_step": round(1e3 * mdt / msteps, 3),
@@ -498,6 +480,19 @@ def main() -> int:
     n_img = 4 if sym == "yz+xz" else (1 if sym is None else 2)
     evals_per_step = float(T) ** 2 * 36 * n_img * F
     achieved = evals_per_step / launches_per_step * FLOPS_PER_EVAL / reg_launch_s / 1e12
+
+    # transcendental side of the assembly roofline: rsq + sin + cos per
+    # computed pair-point evaluation, each serving both directed blocks
+    # (reciprocity), against the device's measured transcendental rate
+    from hornlab_metal_bem.native import probe_transcendental_rate
+
+    trans_peak = probe_transcendental_rate()
+    trans_launch = 3.0 * evals_per_step / 2.0 / launches_per_step
+    trans = {"achieved_ops_per_s": trans_launch / reg_launch_s, "peak_ops_per_s": trans_peak,
+             "frac": round(trans_launch / reg_launch_s / trans_peak, 4),
+             "ops_per_launch": trans_launch,
+             "peak": "hbem_probe_transcendental: v_sin/v_cos/v_rsq in equal parts, 8 chains per "
+                     "lane, 8 blocks of 256 per CU"}
 
     solve_wall = None
     if not args.no_solve_wall:
@@ -597,6 +592,7 @@ def main() -> int:
             f"{FLOPS_PER_EVAL} FP32 flops (+{TRANS_PER_EVAL} transcendentals) each",
             "launch_ms": round(reg_launch_s * 1e3, 4),
         },
+        "roofline_transcendental": trans,
         "modes": modes or None,
         "solve_wall": solve_wall,
         "cpu_baseline": cpu,

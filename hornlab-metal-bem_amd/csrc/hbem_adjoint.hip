@@ -675,6 +675,23 @@ __global__ __launch_bounds__(256) void k_adjoint_singular(
     const float k = k_real[b], krev = k * kInv2Pi, kim = kComplexK ? k_imag[b] : 0.0f;
     const float nhx = gf[15], nhy = gf[16], nhz = gf[17];
 
+    // A corner shared by the test and trial triangles (the same caller vertex
+    // at the same rule-local index on both sides) moves both points of every
+    // rule pair: along the components where the two image signs agree its
+    // derivative is wc sum_q (rho'_a(y) - rho_a(x)) grad_r, accumulated with
+    // the exact weight difference (linear in the stored (x'-x, y'-y)) instead
+    // of as two large cancelling sums.
+    const int packed0 = live ? g.sg_adj[(size_t)6 * np + c] : 0;
+    const int tm0 = packed0 & 15, sm0 = packed0 >> 4;
+    bool comb[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const bool same_v =
+            live && g.sg_adj[(size_t)j * np + c] == g.sg_adj[(size_t)(3 + j) * np + c];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            comb[j][d] = same_v && (((tm0 >> d) & 1) == ((sm0 >> d) & 1));
+    }
     const float *__restrict__ rp = g.sg_rule + (size_t)g.sg_rule_offset[kind] * kSgRule;
     const int cnt = g.sg_rule_count[kind];
     double dx[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -711,15 +728,16 @@ __global__ __launch_bounds__(256) void k_adjoint_singular(
             const float iw = 1.0f / wq;
             const float ra[3] = {w0 * iw, w1v * iw, w2v * iw};
             const float sb[3] = {s0, xs, ys};
+            // rho'_a(x', y') - rho_a(x, y) of the rule-local basis
+            const float dr[3] = {-(ddx + ddy), ddx, ddy};
+            const float gg[3] = {gx, gy, gz};
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                ax[j][0] = fmaf(-ra[j], gx, ax[j][0]);
-                ax[j][1] = fmaf(-ra[j], gy, ax[j][1]);
-                ax[j][2] = fmaf(-ra[j], gz, ax[j][2]);
-                ay[j][0] = fmaf(sb[j], gx, ay[j][0]);
-                ay[j][1] = fmaf(sb[j], gy, ay[j][1]);
-                ay[j][2] = fmaf(sb[j], gz, ay[j][2]);
-            }
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    ax[j][d] = fmaf(comb[j][d] ? dr[j] : -ra[j], gg[d], ax[j][d]);
+                    ay[j][d] = fmaf(comb[j][d] ? 0.0f : sb[j], gg[d], ay[j][d]);
+                }
             an[0] = fmaf(-a3, rx, an[0]);
             an[1] = fmaf(-a3, ry, an[1]);
             an[2] = fmaf(-a3, rz, an[2]);

@@ -330,6 +330,43 @@ int32_t hbem_device_count(void) {
     return n;
 }
 
+int32_t hbem_probe_transcendental(double *ops_per_second) {
+    return guarded([&] {
+        if (!ops_per_second) fail(HBEM_EINVAL, "hbem_probe_transcendental: null argument");
+        const int dev = gfx950_device();
+        if (dev < 0) fail(HBEM_ENODEV, "no gfx950 (MI355X) device is visible");
+        hip_check(hipSetDevice(dev), "hipSetDevice");
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        // 8 blocks of 256 per CU (8 waves per SIMD), long enough to time
+        const int blocks = prop.multiProcessorCount * 8, iters = 4096;
+        hipStream_t st = nullptr;
+        hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        DevBuf out;
+        double best = 0.0;
+        try {
+            out.reserve(sizeof(float) * (size_t)blocks * 256);
+            hbem::launch_probe_transcendental(out.as<float>(), blocks, 64, st);  // warm-up
+            for (int rep = 0; rep < 3; ++rep) {
+                Timer t;
+                t.start(st);
+                hbem::launch_probe_transcendental(out.as<float>(), blocks, iters, st);
+                t.stop(st);
+                hip_check(hipGetLastError(), "probe launch");
+                const double ops = (double)blocks * 256.0 * iters * 16.0;
+                best = std::max(best, ops / t.seconds());
+            }
+        } catch (...) {
+            out.release();
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        out.release();
+        (void)hipStreamDestroy(st);
+        *ops_per_second = best;
+    });
+}
+
 int32_t hbem_session_create(const hbem_geometry *geom, hbem_session **out) {
     return guarded([&] {
         if (!geom || !out) fail(HBEM_EINVAL, "hbem_session_create: null argument");
@@ -1668,6 +1705,458 @@ int32_t hbem_dense_solve(int32_t n_sys, int32_t n, const float *A_re, const floa
                 diag[b].dense_solve_s = solve_s / B;
             }
         }
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Session groups: several meshes' sweeps solved together (the batched uneven
+// sweep).  One regular and one singular assembly launch cover every member;
+// the members' systems are padded into size classes of a common order so
+// that each class factors and solves as one batch; each class runs on its own
+// stream, forked from and joined to the group stream; the whole sequence is
+// captured once per shape and buffer set and replayed as one hipGraph.
+// ---------------------------------------------------------------------------
+
+struct hbem_group {
+    std::vector<hbem_session *> members;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::vector<hipStream_t> side;  // one per size class
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;  // per class: solves joined, assembly done (split)
+    struct SizeClass {
+        int N = 0;
+        int B = 0;
+        std::vector<int> members, offset;
+        DevBuf A, rhs, x, ipiv, info, gl, scr_i, scr_f, nsys;
+    };
+    std::vector<SizeClass> classes;
+    int granularity = 512;   // padded-order step of the size classes (autotuned)
+    int classes_gran = 0;    // the step the current classes were built with
+    bool split = false;      // one assembly launch per size class (autotuned)
+    bool tuned = false;
+    DevBuf jobs_d, tags_d, stamps;  // stamps: [0] sweep start, [1 + m] member m done
+    double clock_hz = 0.0;
+    hipGraphExec_t graph = nullptr;
+    std::vector<uintptr_t> graph_key;
+
+    ~hbem_group() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (SizeClass &c : classes)
+            for (DevBuf *b : {&c.A, &c.rhs, &c.x, &c.ipiv, &c.info, &c.gl, &c.scr_i, &c.scr_f, &c.nsys})
+                b->release();
+        jobs_d.release();
+        tags_d.release();
+        stamps.release();
+        for (hipEvent_t e : ev_join) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+constexpr int kGroupGranularities[4] = {64, 256, 512, 1024};
+constexpr int kGroupCandidates = 8;  // granularities x {one assembly launch, one per class}
+
+void check_group_case(const hbem_session *s, const hbem_group_case &c) {
+    if (c.n_freq <= 0) fail(HBEM_EINVAL, "hbem_group: n_freq must be positive");
+    if (!c.k_real || !c.field_k || !c.q || !c.tag_avg)
+        fail(HBEM_EINVAL, "hbem_group: k_real, field_k, q and tag_avg are required");
+    if (c.n_obs < 0 || (c.n_obs > 0 && (!c.obs_3xn || !c.field)))
+        fail(HBEM_EINVAL, "hbem_group: observation arrays invalid");
+    (void)s;
+}
+
+// Size classes of padded order round_up(N_m, granularity), largest first.
+void build_classes(hbem_group *g, const hbem_group_case *cases, int gran) {
+    std::vector<std::pair<int, int>> order;  // (class order, member)
+    for (size_t m = 0; m < g->members.size(); ++m) {
+        const int N = (int)round_up(g->members[m]->plan.n_dofs, hbem::kLuPad);
+        order.push_back({(int)round_up(N, gran), (int)m});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const auto &a, const auto &b) { return a.first > b.first; });
+    std::vector<hbem_group::SizeClass> keep = std::move(g->classes);
+    g->classes.clear();
+    for (const auto &om : order) {
+        if (g->classes.empty() || g->classes.back().N != om.first) {
+            g->classes.emplace_back();
+            g->classes.back().N = om.first;
+        }
+        hbem_group::SizeClass &c = g->classes.back();
+        c.members.push_back(om.second);
+        c.offset.push_back(c.B);
+        c.B += cases[om.second].n_freq;
+    }
+    // reuse the previous classes' buffers where they exist
+    for (size_t i = 0; i < g->classes.size() && i < keep.size(); ++i) {
+        hbem_group::SizeClass &c = g->classes[i];
+        c.A = keep[i].A; c.rhs = keep[i].rhs; c.x = keep[i].x; c.ipiv = keep[i].ipiv;
+        c.info = keep[i].info; c.gl = keep[i].gl; c.scr_i = keep[i].scr_i;
+        c.scr_f = keep[i].scr_f; c.nsys = keep[i].nsys;
+        keep[i].A = keep[i].rhs = keep[i].x = keep[i].ipiv = keep[i].info = keep[i].gl =
+            keep[i].scr_i = keep[i].scr_f = keep[i].nsys = DevBuf{};
+    }
+    for (hbem_group::SizeClass &c : keep)
+        for (DevBuf *b : {&c.A, &c.rhs, &c.x, &c.ipiv, &c.info, &c.gl, &c.scr_i, &c.scr_f, &c.nsys})
+            b->release();
+    for (hbem_group::SizeClass &c : g->classes) {
+        const size_t N = (size_t)c.N, B = (size_t)c.B;
+        c.A.reserve(sizeof(float) * 2 * B * N * N);
+        c.rhs.reserve(sizeof(float) * 2 * B * N);
+        c.x.reserve(sizeof(float) * 2 * B * N);
+        c.ipiv.reserve(sizeof(int32_t) * B * N);
+        c.info.reserve(sizeof(int32_t) * B);
+        c.gl.reserve(sizeof(int32_t) * B * hbem::lu_gather_ints(c.N));
+        c.scr_i.reserve(sizeof(int32_t) * B * hbem::kLuScratchI);
+        c.scr_f.reserve(sizeof(float) * B * hbem::kLuScratchF);
+        std::vector<int32_t> nsys;
+        for (size_t i = 0; i < c.members.size(); ++i)
+            for (int f = 0; f < cases[c.members[i]].n_freq; ++f)
+                nsys.push_back(g->members[c.members[i]]->plan.n_dofs);
+        upload(c.nsys, nsys, g->stream);
+    }
+}
+
+// Enqueues the whole group sweep on g->stream (forking one stream per size
+// class); kernels and device copies only.
+// jobs: per size class, the class's members (heaviest first), classes in
+// order; class_job[c] = index of the class's first job (class_job[C] = jobs).
+void enqueue_group(hbem_group *g, const hbem_group_case *cases, const std::vector<hbem::AssemblyJob> &jobs,
+                   const std::vector<int> &class_job, int ntag) {
+    hipStream_t st = g->stream;
+    long long *stamps = g->stamps.as<long long>();
+    hbem::launch_timestamp(stamps, st);
+    for (size_t m = 0; m < g->members.size(); ++m)
+        if (cases[m].n_obs > 0)
+            hbem::launch_fill_zero(cases[m].field, (size_t)2 * cases[m].n_freq * cases[m].n_obs, st);
+    for (hbem_group::SizeClass &c : g->classes) {
+        hbem::launch_fill_zero(c.A.as<float>(), (size_t)2 * c.B * c.N * c.N, st);
+        hbem::launch_fill_zero(c.rhs.as<float>(), (size_t)2 * c.B * c.N, st);
+    }
+    if (!g->split) {
+        // one launch over every member's tile pairs, one over every member's
+        // singular classes; the size classes' solves fork after it
+        hbem::launch_assemble_multi(g->jobs_d.as<hbem::AssemblyJob>(), jobs, false, st);
+        hip_check(hipEventRecord(g->ev_fork, st), "event record");
+    }
+    for (size_t ci = 0; ci < g->classes.size(); ++ci) {
+        hbem_group::SizeClass &c = g->classes[ci];
+        hipStream_t ss = g->side[ci];
+        if (g->split) {
+            // the class's members in one launch each; its solves fork while
+            // the next class assembles
+            const std::vector<hbem::AssemblyJob> cj(jobs.begin() + class_job[ci],
+                                                    jobs.begin() + class_job[ci + 1]);
+            hbem::launch_assemble_multi(g->jobs_d.as<hbem::AssemblyJob>() + class_job[ci], cj, false,
+                                        st);
+            hip_check(hipEventRecord(g->ev_join[ci + g->classes.size()], st), "event record");
+            hip_check(hipStreamWaitEvent(ss, g->ev_join[ci + g->classes.size()], 0), "fork");
+        } else {
+            hip_check(hipStreamWaitEvent(ss, g->ev_fork, 0), "fork");
+        }
+        const int64_t N = c.N, a_stride = N * N;
+        hbem::launch_pad_identity_var(c.A.as<float>(), N, a_stride, c.nsys.as<int32_t>(), c.N, c.B,
+                                      ss);
+        hbem::LuWorkspace w{c.A.as<float>(), N, a_stride, c.N, c.N, c.B, c.ipiv.as<int32_t>(),
+                            c.info.as<int32_t>(), c.gl.as<int32_t>(), c.scr_i.as<int32_t>(),
+                            c.scr_f.as<float>()};
+        hbem::lu_factor(w, 0, nullptr, ss);
+        hip_check(hipMemcpyAsync(c.x.ptr, c.rhs.ptr, sizeof(float) * 2 * (size_t)c.B * c.N,
+                                 hipMemcpyDeviceToDevice, ss),
+                  "rhs copy");
+        hbem::lu_solve(w, c.x.as<float>(), N, 0, ss);
+        for (size_t i = 0; i < c.members.size(); ++i) {
+            const int m = c.members[i];
+            const hbem_session *s = g->members[m];
+            const hbem_group_case &cs = cases[m];
+            const float *xm = c.x.as<float>() + (size_t)c.offset[i] * N * 2;
+            if (cs.n_obs > 0)
+                hbem::launch_field(s->dg, cs.n_freq, cs.field_k, 1, xm, N, cs.q, cs.n_obs,
+                                   cs.obs_3xn, cs.field, ss);
+            hbem::launch_tag_average(s->dg, cs.n_freq, xm, N, ntag, g->tags_d.as<int32_t>(),
+                                     cs.tag_avg, ss);
+            if (cs.p)
+                hip_check(hipMemcpy2DAsync(cs.p, sizeof(float) * 2 * s->plan.n_dofs, xm,
+                                           sizeof(float) * 2 * N,
+                                           sizeof(float) * 2 * s->plan.n_dofs, cs.n_freq,
+                                           hipMemcpyDeviceToDevice, ss),
+                          "p copy");
+            hbem::launch_timestamp(stamps + 1 + m, ss);
+        }
+        hip_check(hipEventRecord(g->ev_join[ci], ss), "event record");
+        hip_check(hipStreamWaitEvent(st, g->ev_join[ci], 0), "join");
+    }
+    hip_check(hipGetLastError(), "group launch");
+}
+
+// One group sweep at the given granularity.  Fills diag (member order,
+// frequency order within a member) and member_seconds (completion time of
+// each member's results after the sweep started).
+void group_solve(hbem_group *g, const hbem_group_case *cases, const hbem_solve_options &opts,
+                 int gran, hbem_case_diag *diag, double *member_seconds, double *sweep_seconds) {
+    const size_t M = g->members.size();
+    hipStream_t st = g->stream;
+    bool rebuild = gran != g->classes_gran || g->classes.empty();
+    for (const hbem_group::SizeClass &c : g->classes) {
+        int B = 0;
+        for (int m : c.members) B += cases[m].n_freq;
+        rebuild |= B != c.B;  // member batch sizes changed
+    }
+    if (rebuild) {
+        build_classes(g, cases, gran);
+        g->classes_gran = gran;
+    }
+    while (g->side.size() < g->classes.size()) {
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        g->side.push_back(s);
+    }
+    while (g->ev_join.size() < 2 * g->classes.size()) {
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        g->ev_join.push_back(e);
+    }
+    g->stamps.reserve(sizeof(long long) * (1 + M));
+    if (g->clock_hz == 0.0) {
+        int khz = 0;
+        hip_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device),
+                  "wall clock rate");
+        g->clock_hz = 1e3 * (double)khz;
+    }
+    // assembly jobs: by size class, within a class the members with the most
+    // blocks first; block offsets restart at every class when the assembly
+    // is split per class
+    std::vector<hbem::AssemblyJob> jobs;
+    std::vector<int> class_job;
+    int reg = 0, sing = 0;
+    for (size_t ci = 0; ci < g->classes.size(); ++ci) {
+        const hbem_group::SizeClass &c = g->classes[ci];
+        class_job.push_back((int)jobs.size());
+        if (g->split) reg = sing = 0;
+        std::vector<int> by_work(c.members.size());
+        for (size_t i = 0; i < by_work.size(); ++i) by_work[i] = (int)i;
+        std::stable_sort(by_work.begin(), by_work.end(), [&](int a, int b) {
+            const int ma = c.members[a], mb = c.members[b];
+            return (int64_t)g->members[ma]->dg.n_tile_pairs * cases[ma].n_freq >
+                   (int64_t)g->members[mb]->dg.n_tile_pairs * cases[mb].n_freq;
+        });
+        for (int i : by_work) {
+        const int m = c.members[i], oi = c.offset[i];
+        const hbem_session *s = g->members[m];
+        hbem::AssemblyJob j{};
+        j.g = s->dg;
+        j.k_real = cases[m].k_real;
+        j.k_imag = nullptr;
+        j.q = cases[m].q;
+        j.beta = nullptr;
+        j.n_src = 1;
+        j.n_batch = cases[m].n_freq;
+        j.lda = c.N;
+        j.a_stride = (int64_t)c.N * c.N;
+        j.A = c.A.as<float>() + (size_t)oi * j.a_stride * 2;
+        j.rhs_stride = c.N;
+        j.rhs = c.rhs.as<float>() + (size_t)oi * c.N * 2;
+        j.block_begin_reg = reg;
+        j.block_begin_sing = sing;
+        reg += hbem::regular_blocks(s->dg, j.n_batch);
+        sing += hbem::singular_blocks(s->dg, j.n_batch);
+        jobs.push_back(j);
+        }
+    }
+    class_job.push_back((int)jobs.size());
+    upload(g->jobs_d, jobs, st);
+    std::vector<int32_t> tag_list;
+    tag_list.push_back(opts.impedance_tag);
+    for (int i = 0; i < opts.n_tags; ++i) tag_list.push_back(opts.tags[i]);
+    upload(g->tags_d, tag_list, st);
+    const int ntag = (int)tag_list.size();
+    std::vector<uintptr_t> key = {(uintptr_t)gran, (uintptr_t)g->split, (uintptr_t)g->jobs_d.ptr,
+                                  (uintptr_t)g->tags_d.ptr, (uintptr_t)g->stamps.ptr,
+                                  (uintptr_t)ntag};
+    for (size_t m = 0; m < M; ++m) {
+        const hbem_group_case &c = cases[m];
+        for (uintptr_t v : {(uintptr_t)c.n_freq, (uintptr_t)c.k_real, (uintptr_t)c.field_k,
+                            (uintptr_t)c.q, (uintptr_t)c.n_obs, (uintptr_t)c.obs_3xn,
+                            (uintptr_t)c.field, (uintptr_t)c.p, (uintptr_t)c.tag_avg})
+            key.push_back(v);
+    }
+    for (const hbem_group::SizeClass &c : g->classes)
+        for (const DevBuf *b : {&c.A, &c.rhs, &c.x, &c.ipiv, &c.info, &c.gl, &c.nsys})
+            key.push_back((uintptr_t)b->ptr);
+    if (!g->graph || g->graph_key != key) {
+        if (g->graph) {
+            (void)hipGraphExecDestroy(g->graph);
+            g->graph = nullptr;
+        }
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture begin");
+        hipGraph_t graph = nullptr;
+        try {
+            enqueue_group(g, cases, jobs, class_job, ntag);
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(st, &graph), "capture end");
+        const hipError_t e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hip_check(e, "graph instantiate");
+        g->graph_key = key;
+    }
+    Timer tw;
+    tw.start(st);
+    hip_check(hipGraphLaunch(g->graph, st), "graph launch");
+    tw.stop(st);
+    const double sweep_s = tw.seconds();
+    if (sweep_seconds) *sweep_seconds = sweep_s;
+    // per-class pivot info; member completion times from the device stamps
+    std::vector<long long> stamp(1 + M);
+    hip_check(hipMemcpy(stamp.data(), g->stamps.ptr, sizeof(long long) * (1 + M),
+                        hipMemcpyDeviceToHost),
+              "stamps D2H");
+    size_t d0 = 0;
+    std::vector<size_t> first(M);
+    for (size_t m = 0; m < M; ++m) {
+        first[m] = d0;
+        d0 += cases[m].n_freq;
+    }
+    for (size_t ci = 0; ci < g->classes.size(); ++ci) {
+        const hbem_group::SizeClass &c = g->classes[ci];
+        std::vector<int32_t> info(c.B);
+        hip_check(hipMemcpy(info.data(), c.info.ptr, sizeof(int32_t) * c.B, hipMemcpyDeviceToHost),
+                  "info D2H");
+        for (size_t i = 0; i < c.members.size(); ++i) {
+            const int m = c.members[i];
+            const double done = (double)(stamp[1 + m] - stamp[0]) / g->clock_hz;
+            if (member_seconds) member_seconds[m] = done;
+            const double per = done / cases[m].n_freq;
+            for (int f = 0; f < cases[m].n_freq; ++f) {
+                hbem_case_diag &d = diag[first[m] + f];
+                d = hbem_case_diag{};
+                d.lapack_info = info[c.offset[i] + f];
+                d.dense_solve_rcond = -1.0;
+                d.refine_residual_rel = -1.0;
+                d.chief_residual_rel = -1.0;
+                d.dense_solve_s = per;
+                d.batch_index = (int)ci;
+                d.batch_size = c.B;
+            }
+        }
+    }
+}
+
+// Locks every member session (address order) for the duration of a call.
+struct GroupLock {
+    std::vector<std::unique_lock<std::mutex>> locks;
+    explicit GroupLock(hbem_group *g) {
+        std::vector<hbem_session *> m = g->members;
+        std::sort(m.begin(), m.end());
+        for (hbem_session *s : m) locks.emplace_back(s->mu);
+    }
+};
+
+void check_group_options(const hbem_solve_options &o) {
+    check_options(o);
+    if (o.solve_mode != HBEM_SOLVE_F32 || o.estimate_rcond || o.robin_beta || o.n_chief > 0 ||
+        o.n_sources > 1 || o.precise_geometry || o.solve_trans != 0)
+        fail(HBEM_EUNSUPPORTED,
+             "hbem_group: FP32 LU without rcond, Robin walls, CHIEF rows, extra sources or "
+             "precise geometry");
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hbem_group_create(int32_t n_sessions, hbem_session *const *sessions, hbem_group **out) {
+    return guarded([&] {
+        if (!out || !sessions || n_sessions <= 0 || n_sessions > 1024)
+            fail(HBEM_EINVAL, "hbem_group_create: invalid arguments");
+        *out = nullptr;
+        auto g = std::make_unique<hbem_group>();
+        for (int i = 0; i < n_sessions; ++i) {
+            hbem_session *s = sessions[i];
+            if (!s) fail(HBEM_EINVAL, "hbem_group_create: null session");
+            for (hbem_session *t : g->members)
+                if (t == s) fail(HBEM_EINVAL, "hbem_group_create: a session appears twice");
+            if (i > 0 && s->device != sessions[0]->device)
+                fail(HBEM_EINVAL, "hbem_group_create: sessions on different devices");
+            if (s->dg.n_masks != sessions[0]->dg.n_masks)
+                fail(HBEM_EUNSUPPORTED, "hbem_group_create: members need the same symmetry image count");
+            g->members.push_back(s);
+        }
+        g->device = sessions[0]->device;
+        hip_check(hipSetDevice(g->device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming), "hipEventCreate");
+        *out = g.release();
+    });
+}
+
+int32_t hbem_group_destroy(hbem_group *group) {
+    return guarded([&] {
+        if (!group) return;
+        (void)hipSetDevice(group->device);
+        delete group;
+    });
+}
+
+int32_t hbem_group_solve_device(hbem_group *group, const hbem_group_case *cases,
+                                const hbem_solve_options *opts, hbem_case_diag *diag,
+                                double *member_seconds, double *sweep_seconds) {
+    return guarded([&] {
+        if (!group || !cases || !opts || !diag) fail(HBEM_EINVAL, "hbem_group_solve_device: null argument");
+        check_group_options(*opts);
+        for (size_t m = 0; m < group->members.size(); ++m) {
+            check_group_case(group->members[m], cases[m]);
+            if (group->members[m]->dg.nc_np > 0)
+                fail(HBEM_EUNSUPPORTED, "hbem_group_solve_device: near-field quadrature must be off");
+            if (group->members[m]->assembly_mode != HBEM_ASSEMBLY_CORRECTED)
+                fail(HBEM_EUNSUPPORTED, "hbem_group_solve_device: needs the corrected assembly mode");
+        }
+        GroupLock lock(group);
+        for (hbem_session *s : group->members) s->adj_valid = false;
+        hip_check(hipSetDevice(group->device), "hipSetDevice");
+        group_solve(group, cases, *opts, group->granularity, diag, member_seconds, sweep_seconds);
+    });
+}
+
+int32_t hbem_group_autotune(hbem_group *group, const hbem_group_case *cases,
+                            const hbem_solve_options *opts, int32_t *granularity,
+                            double *seconds) {
+    return guarded([&] {
+        if (!group || !cases || !opts) fail(HBEM_EINVAL, "hbem_group_autotune: null argument");
+        check_group_options(*opts);
+        for (size_t m = 0; m < group->members.size(); ++m) check_group_case(group->members[m], cases[m]);
+        GroupLock lock(group);
+        hip_check(hipSetDevice(group->device), "hipSetDevice");
+        size_t total = 0;
+        for (size_t m = 0; m < group->members.size(); ++m) total += cases[m].n_freq;
+        std::vector<hbem_case_diag> diag(total);
+        int best = 0;
+        double best_s = 1e300;
+        for (int i = 0; i < kGroupCandidates; ++i) {
+            const int gran = kGroupGranularities[i % 4];
+            group->split = i >= 4;
+            double t = 0.0, t2 = 0.0;
+            // one sweep captures and warms the graph, the next is timed
+            group_solve(group, cases, *opts, gran, diag.data(), nullptr, &t);
+            group_solve(group, cases, *opts, gran, diag.data(), nullptr, &t2);
+            if (seconds) seconds[i] = t2;
+            if (t2 < best_s) {
+                best_s = t2;
+                best = i;
+            }
+        }
+        group->granularity = kGroupGranularities[best % 4];
+        group->split = best >= 4;
+        group->tuned = true;
+        if (granularity) *granularity = group->granularity * (group->split ? -1 : 1);
     });
 }
 

@@ -501,12 +501,15 @@ __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks
     }
 }
 
+// One tile pair (pair index tp) at one frequency (b) of the regular
+// assembly: the block body shared by the single-mesh kernel and the
+// multi-mesh launch of a session group.
 template <bool kComplexK, int kM, bool kPrecise>
-__global__ __launch_bounds__(256, kPrecise ? 1 : 2) void k_assemble_regular(
-    DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
-    const float *__restrict__ q, int n_src, const float *__restrict__ beta,
-    float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
-    int64_t rhs_stride) {
+__device__ __forceinline__ void regular_tile_pair(
+    const DeviceGeometry &g, int tp, int b, const float *__restrict__ k_real,
+    const float *__restrict__ k_imag, const float *__restrict__ q, int n_src,
+    const float *__restrict__ beta, float *__restrict__ A, int64_t lda, int64_t a_stride,
+    float *__restrict__ rhs, int64_t rhs_stride) {
     // LDS tables, each [T dofs][S dofs] with real and imaginary planes:
     // F (rows of t, columns of s) and R (rows of s, columns of t), indexed
     // (test slot) * snd + (trial slot), so a wave's adds for one entry hit
@@ -515,8 +518,7 @@ __global__ __launch_bounds__(256, kPrecise ? 1 : 2) void k_assemble_regular(
     const int T = g.n_triangles;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int s_tile = g.tile_pairs[2 * blockIdx.x], t_tile = g.tile_pairs[2 * blockIdx.x + 1];
-    const int b = blockIdx.y;
+    const int s_tile = g.tile_pairs[2 * tp], t_tile = g.tile_pairs[2 * tp + 1];
     const bool diag = s_tile == t_tile;
     const int s0 = g.tile_start[s_tile], scount = g.tile_count[s_tile], snd = g.tile_ndofs[s_tile];
     const int t0 = g.tile_start[t_tile], tcount = g.tile_count[t_tile], tnd = g.tile_ndofs[t_tile];
@@ -797,6 +799,41 @@ __global__ __launch_bounds__(256, kPrecise ? 1 : 2) void k_assemble_regular(
     }
 }
 
+template <bool kComplexK, int kM, bool kPrecise>
+__global__ __launch_bounds__(256, kPrecise ? 1 : 2) void k_assemble_regular(
+    DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
+    const float *__restrict__ q, int n_src, const float *__restrict__ beta,
+    float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
+    int64_t rhs_stride) {
+    regular_tile_pair<kComplexK, kM, kPrecise>(g, blockIdx.x, blockIdx.y, k_real, k_imag, q, n_src,
+                                               beta, A, lda, a_stride, rhs, rhs_stride);
+}
+
+// The member of a multi-mesh launch that owns flattened block `blk`: jobs
+// are ordered by their first block (a handful of members: a scalar scan).
+__device__ __forceinline__ int job_of_block(const AssemblyJob *__restrict__ jobs, int n_jobs,
+                                            int blk, bool singular) {
+    int j = 0;
+    for (int i = 1; i < n_jobs; ++i)
+        if ((singular ? jobs[i].block_begin_sing : jobs[i].block_begin_reg) <= blk) j = i;
+    return j;
+}
+
+// Regular assembly of every member of a session group in one launch: block
+// x = (member, frequency, tile pair) flattened; members share the mask count
+// kM and the real / complex wavenumber choice.
+template <bool kComplexK, int kM>
+__global__ __launch_bounds__(256, 2) void k_assemble_regular_multi(
+    const AssemblyJob *__restrict__ jobs, int n_jobs) {
+    const int j = job_of_block(jobs, n_jobs, blockIdx.x, false);
+    const AssemblyJob &jb = jobs[j];
+    const int local = blockIdx.x - jb.block_begin_reg;
+    const int ntp = jb.g.n_tile_pairs;
+    regular_tile_pair<kComplexK, kM, false>(jb.g, local % ntp, local / ntp, jb.k_real, jb.k_imag,
+                                            jb.q, jb.n_src, jb.beta, jb.A, jb.lda, jb.a_stride,
+                                            jb.rhs, jb.rhs_stride);
+}
+
 // ---------------------------------------------------------------------------
 // Singular pair classes.  One lane per class, kFB frequencies per lane; a
 // wave holds classes of one kind, so the Sauter-Schwab points are
@@ -811,15 +848,18 @@ __global__ __launch_bounds__(256, kPrecise ? 1 : 2) void k_assemble_regular(
 
 constexpr int kSgRun = 32;
 
+// Block (bx, by) of the singular-class assembly (4 waves of classes, kFB
+// frequencies): the body shared by the single-mesh kernel and the
+// multi-mesh launch.
 template <bool kComplexK, int kFB>
-__global__ __launch_bounds__(256) void k_assemble_singular(
-    DeviceGeometry g, const float *__restrict__ rule, int n_batch,
+__device__ __forceinline__ void singular_block(
+    const DeviceGeometry &g, int bx, int by, const float *__restrict__ rule, int n_batch,
     const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, int n_src, const float *__restrict__ beta,
     float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
     int64_t rhs_stride) {
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int w = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
     const int w1 = g.sg_waves[0], w2 = w1 + g.sg_waves[1], w3 = w2 + g.sg_waves[2];
     if (w >= w3) return;
     const int kind = w < w1 ? 0 : (w < w2 ? 1 : 2);
@@ -836,7 +876,7 @@ __global__ __launch_bounds__(256) void k_assemble_singular(
     const float g2x = gf[12], g2y = gf[13], g2z = gf[14];
     const float nx = gf[15], ny = gf[16], nz = gf[17];
 
-    const int b0 = blockIdx.y * kFB;
+    const int b0 = by * kFB;
     float kk[kFB], krev[kFB], kim[kFB];
 #pragma unroll
     for (int f = 0; f < kFB; ++f) {
@@ -998,6 +1038,32 @@ __global__ __launch_bounds__(256) void k_assemble_singular(
             }
         }
     }
+}
+
+template <bool kComplexK, int kFB>
+__global__ __launch_bounds__(256) void k_assemble_singular(
+    DeviceGeometry g, const float *__restrict__ rule, int n_batch,
+    const float *__restrict__ k_real, const float *__restrict__ k_imag,
+    const float *__restrict__ q, int n_src, const float *__restrict__ beta,
+    float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
+    int64_t rhs_stride) {
+    singular_block<kComplexK, kFB>(g, blockIdx.x, blockIdx.y, rule, n_batch, k_real, k_imag, q,
+                                   n_src, beta, A, lda, a_stride, rhs, rhs_stride);
+}
+
+// Singular classes of every member of a session group in one launch: block
+// x = (member, frequency group, wave block) flattened.
+template <bool kComplexK, int kFB>
+__global__ __launch_bounds__(256) void k_assemble_singular_multi(
+    const AssemblyJob *__restrict__ jobs, int n_jobs) {
+    const int j = job_of_block(jobs, n_jobs, blockIdx.x, true);
+    const AssemblyJob &jb = jobs[j];
+    const int local = blockIdx.x - jb.block_begin_sing;
+    const int waves = jb.g.sg_waves[0] + jb.g.sg_waves[1] + jb.g.sg_waves[2];
+    const int nbx = (waves + 3) / 4;
+    singular_block<kComplexK, kFB>(jb.g, local % nbx, local / nbx, jb.g.sg_rule, jb.n_batch,
+                                   jb.k_real, jb.k_imag, jb.q, jb.n_src, jb.beta, jb.A, jb.lda,
+                                   jb.a_stride, jb.rhs, jb.rhs_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -1378,6 +1444,57 @@ void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float 
                            rhs_stride);
 }
 
+void launch_assemble_multi(const AssemblyJob *jobs_dev, const std::vector<AssemblyJob> &jobs,
+                           bool complex_k, hipStream_t stream) {
+    if (jobs.empty()) return;
+    const int n = (int)jobs.size();
+    if (n == 1) {
+        // one member: the single-mesh kernels (geometry in the kernel
+        // arguments, fewer registers than the member lookup)
+        const AssemblyJob &j = jobs[0];
+        const float *kim = complex_k ? j.k_imag : nullptr;
+        launch_assemble_regular(j.g, j.n_batch, j.k_real, kim, j.q, j.n_src, j.beta, j.A, j.lda,
+                                j.a_stride, j.rhs, j.rhs_stride, stream, false);
+        launch_assemble_singular(j.g, j.n_batch, j.k_real, kim, j.q, j.n_src, j.beta, j.A, j.lda,
+                                 j.a_stride, j.rhs, j.rhs_stride, stream);
+        return;
+    }
+    int reg = 0, sing = 0, dofs = 0;
+    for (const AssemblyJob &j : jobs) {
+        reg += regular_blocks(j.g, j.n_batch);
+        sing += singular_blocks(j.g, j.n_batch);
+        dofs = std::max(dofs, j.g.max_tile_dofs);
+    }
+    const size_t shm = sizeof(float) * 4 * (size_t)dofs * dofs;
+    constexpr int kFB = 2;
+    auto go = [&](auto complex_k_t) {
+        constexpr bool kC = decltype(complex_k_t)::value;
+        if (reg > 0) {
+            switch (jobs[0].g.n_masks) {
+            case 1:
+                hipLaunchKernelGGL((k_assemble_regular_multi<kC, 1>), dim3(reg), dim3(256), shm,
+                                   stream, jobs_dev, n);
+                break;
+            case 2:
+                hipLaunchKernelGGL((k_assemble_regular_multi<kC, 2>), dim3(reg), dim3(256), shm,
+                                   stream, jobs_dev, n);
+                break;
+            default:
+                hipLaunchKernelGGL((k_assemble_regular_multi<kC, 4>), dim3(reg), dim3(256), shm,
+                                   stream, jobs_dev, n);
+                break;
+            }
+        }
+        if (sing > 0)
+            hipLaunchKernelGGL((k_assemble_singular_multi<kC, kFB>), dim3(sing), dim3(256), 0,
+                               stream, jobs_dev, n);
+    };
+    if (complex_k)
+        go(std::true_type{});
+    else
+        go(std::false_type{});
+}
+
 void launch_assemble_near(const DeviceGeometry &g, int n_batch, const float *k_real,
                           const float *k_imag, const float *q, int n_src, const float *beta,
                           float *A, int64_t lda, int64_t a_stride, float *rhs,
@@ -1428,4 +1545,54 @@ void launch_tag_average(const DeviceGeometry &g, int n_batch, const float *p, in
                        n_tags, tags_dev, out);
 }
 
+}  // namespace hbem
+
+// ---------------------------------------------------------------------------
+// Transcendental-pipe probe: the sustained rate of the hardware sine, cosine
+// and reciprocal square root (v_sin_f32, v_cos_f32, v_rsq_f32, one third
+// each, as in the assembly kernels) over the whole device.  Eight
+// independent chains per lane keep the pipes busy; every block runs the same
+// count, so ops = blocks * 256 * iters * 8.
+// ---------------------------------------------------------------------------
+
+namespace hbem {
+namespace {
+__global__ __launch_bounds__(256) void k_probe_transcendental(float *out, int iters) {
+    float v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = 0.01f * (float)(threadIdx.x + 1) + 0.1f * (float)c;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int c = 0; c < 8; c += 4) {
+            v[c] = __builtin_amdgcn_sinf(v[c]);
+            v[c + 1] = __builtin_amdgcn_cosf(v[c + 1]);
+            v[c + 2] = __builtin_amdgcn_rsqf(v[c + 2] + 1.0f);
+            v[c + 3] = __builtin_amdgcn_sinf(v[c + 3]);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c += 4) {
+            v[c] = __builtin_amdgcn_cosf(v[c]);
+            v[c + 1] = __builtin_amdgcn_rsqf(v[c + 1] + 1.0f);
+            v[c + 2] = __builtin_amdgcn_sinf(v[c + 2]);
+            v[c + 3] = __builtin_amdgcn_cosf(v[c + 3]);
+        }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) s += v[c];
+    out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+}  // namespace
+
+namespace {
+__global__ void k_timestamp(long long *slot) { *slot = wall_clock64(); }
+}  // namespace
+
+void launch_timestamp(long long *slot, hipStream_t stream) {
+    hipLaunchKernelGGL(k_timestamp, dim3(1), dim3(1), 0, stream, slot);
+}
+
+void launch_probe_transcendental(float *out, int blocks, int iters, hipStream_t stream) {
+    hipLaunchKernelGGL(k_probe_transcendental, dim3(blocks), dim3(256), 0, stream, out, iters);
+}
 }  // namespace hbem

@@ -241,6 +241,14 @@ struct DeviceGeometry {
     int max_tile_dofs;
 };
 
+// Transcendental-pipe probe: blocks x 256 threads x iters x 16 ops
+// (sin, cos, rsq in equal parts), out [blocks * 256].
+void launch_probe_transcendental(float *out, int blocks, int iters, hipStream_t stream);
+
+// Writes the device's constant-rate wall clock (wall_clock64) to *slot when
+// the stream reaches it (timestamps inside captured graphs).
+void launch_timestamp(long long *slot, hipStream_t stream);
+
 // Zero n floats.
 void launch_fill_zero(float *ptr, size_t n, hipStream_t stream);
 void launch_fill_zero_f64(double *ptr, size_t n, hipStream_t stream);
@@ -254,6 +262,39 @@ void launch_assemble_regular(const DeviceGeometry &g, int n_batch, const float *
                              const float *k_imag, const float *q, int n_src, const float *beta,
                              float *A, int64_t lda, int64_t a_stride, float *rhs,
                              int64_t rhs_stride, hipStream_t stream, bool precise = false);
+
+// One member of a multi-mesh assembly launch (a session group solved
+// together): its geometry, its batch's inputs, where its systems go, and
+// the first flattened block it owns in the regular and singular launches.
+struct AssemblyJob {
+    DeviceGeometry g;
+    const float *k_real;
+    const float *k_imag;  // null: real k
+    const float *q;       // [n_batch][n_src][T] complex
+    const float *beta;    // null: no Robin walls
+    int n_src;
+    int n_batch;
+    float *A;
+    int64_t lda, a_stride;
+    float *rhs;
+    int64_t rhs_stride;
+    int block_begin_reg;
+    int block_begin_sing;
+};
+
+// Blocks a member needs in the multi-mesh launches.
+inline int regular_blocks(const DeviceGeometry &g, int n_batch) { return g.n_tile_pairs * n_batch; }
+inline int singular_blocks(const DeviceGeometry &g, int n_batch) {
+    const int waves = g.sg_waves[0] + g.sg_waves[1] + g.sg_waves[2];
+    return ((waves + 3) / 4) * ((n_batch + 1) / 2);
+}
+
+// Regular and singular assembly of every job in one launch each (jobs_dev:
+// the jobs in device memory, jobs: the same on the host, with block_begin_*
+// prefix sums of regular_blocks / singular_blocks).  All jobs share
+// n_masks; complex_k selects the damped kernels for all of them.
+void launch_assemble_multi(const AssemblyJob *jobs_dev, const std::vector<AssemblyJob> &jobs,
+                           bool complex_k, hipStream_t stream);
 
 // Singular (Sauter-Schwab) pair blocks plus the -M/2 mass term.
 void launch_assemble_singular(const DeviceGeometry &g, int n_batch, const float *k_real,
@@ -361,6 +402,10 @@ inline size_t lu_gather_ints(int N) { return (size_t)(N / 32) * kLuGather; }
 // Pads rows/cols n..N-1 with the identity (zeros elsewhere).
 void launch_pad_identity(float *A, int64_t lda, int64_t a_stride, int n, int N, int B,
                          hipStream_t stream);
+
+// The same with per-system true orders n_sys[B] (device).
+void launch_pad_identity_var(float *A, int64_t lda, int64_t a_stride, const int32_t *n_sys, int N,
+                             int B, hipStream_t stream);
 
 // 1-norm of each matrix (max column sum of |a_ij|) -> norms[B] and the
 // largest modulus -> amax[B] (double).

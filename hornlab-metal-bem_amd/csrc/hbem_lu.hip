@@ -72,6 +72,20 @@ __global__ __launch_bounds__(256) void k_pad_identity(float *A, int64_t lda, int
     }
 }
 
+// The same with the true order of each system in n_sys[b] (systems of
+// different orders sharing one padded order N).
+__global__ __launch_bounds__(256) void k_pad_identity_var(float *A, int64_t lda, int64_t a_stride,
+                                                          const int32_t *n_sys, int N) {
+    const int c = blockIdx.x;
+    const int b = blockIdx.y;
+    const int n = n_sys[b];
+    float *Ab = A + (size_t)b * a_stride * 2;
+    for (int r = threadIdx.x; r < N; r += 256) {
+        if (r < n && c < n) continue;
+        st(Ab, (size_t)c * lda + r, c32{r == c ? 1.0f : 0.0f, 0.0f});
+    }
+}
+
 // norms[b] = max_c sum_r |a_rc| over the true n x n block (clange '1', FP32
 // moduli summed in FP64) and amax[b] = max |a_rc|.  One wave per column;
 // both must be zero on entry.
@@ -1894,6 +1908,12 @@ __global__ void k_col_to_rowsplit(const float *A, int64_t lda, int64_t a_stride,
 void launch_pad_identity(float *A, int64_t lda, int64_t a_stride, int n, int N, int B,
                          hipStream_t stream) {
     hipLaunchKernelGGL(k_pad_identity, dim3(N, B), dim3(256), 0, stream, A, lda, a_stride, n, N);
+}
+
+void launch_pad_identity_var(float *A, int64_t lda, int64_t a_stride, const int32_t *n_sys, int N,
+                             int B, hipStream_t stream) {
+    hipLaunchKernelGGL(k_pad_identity_var, dim3(N, B), dim3(256), 0, stream, A, lda, a_stride, n_sys,
+                       N);
 }
 
 void launch_norm1(const float *A, int64_t lda, int64_t a_stride, int n, int B, double *norms,

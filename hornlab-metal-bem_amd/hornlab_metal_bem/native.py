@@ -125,11 +125,26 @@ class _CaseDiag(C.Structure):
     ]
 
 
+class _GroupCase(C.Structure):
+    _fields_ = [
+        ("n_freq", C.c_int32),
+        ("k_real", C.c_void_p),
+        ("field_k", C.c_void_p),
+        ("q", C.c_void_p),
+        ("n_obs", C.c_int32),
+        ("obs_3xn", C.c_void_p),
+        ("field", C.c_void_p),
+        ("p", C.c_void_p),
+        ("tag_avg", C.c_void_p),
+    ]
+
+
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED_SYMBOLS = (
     "hbem_abi_version",
     "hbem_last_error",
     "hbem_device_count",
+    "hbem_probe_transcendental",
     "hbem_session_create",
     "hbem_session_destroy",
     "hbem_session_get_info",
@@ -140,6 +155,10 @@ EXPORTED_SYMBOLS = (
     "hbem_solve_batch_device",
     "hbem_evaluate_field",
     "hbem_dense_solve",
+    "hbem_group_create",
+    "hbem_group_destroy",
+    "hbem_group_solve_device",
+    "hbem_group_autotune",
     "hbem_dense_solve_noalloc",
     "hbem_adjoint_forward",
     "hbem_adjoint_backward",
@@ -167,6 +186,14 @@ def diag_to_dict(d: "_CaseDiag") -> dict[str, Any]:
 
 _lock = threading.Lock()
 _lib: C.CDLL | None = None
+
+
+def probe_transcendental_rate() -> float:
+    """Sustained sin/cos/rsq operations per second of the device
+    (hbem_probe_transcendental)."""
+    out = C.c_double(0.0)
+    _check(load_library().hbem_probe_transcendental(C.byref(out)), "probe_transcendental")
+    return float(out.value)
 
 
 def library_path() -> Path:
@@ -211,6 +238,8 @@ def load_library() -> C.CDLL:
         lib.hbem_abi_version.restype = C.c_int32
         lib.hbem_last_error.restype = C.c_char_p
         lib.hbem_device_count.restype = C.c_int32
+        lib.hbem_probe_transcendental.argtypes = [C.POINTER(C.c_double)]
+        lib.hbem_probe_transcendental.restype = C.c_int32
         lib.hbem_session_create.argtypes = [C.POINTER(_Geometry), C.POINTER(C.c_void_p)]
         lib.hbem_session_create.restype = C.c_int32
         lib.hbem_session_destroy.argtypes = [C.c_void_p]
@@ -253,6 +282,20 @@ def load_library() -> C.CDLL:
             C.POINTER(_CaseDiag),
         ]
         lib.hbem_dense_solve.restype = C.c_int32
+        lib.hbem_group_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+        lib.hbem_group_create.restype = C.c_int32
+        lib.hbem_group_destroy.argtypes = [C.c_void_p]
+        lib.hbem_group_destroy.restype = C.c_int32
+        lib.hbem_group_solve_device.argtypes = [
+            C.c_void_p, C.POINTER(_GroupCase), C.POINTER(_SolveOptions), C.POINTER(_CaseDiag),
+            C.POINTER(C.c_double), C.POINTER(C.c_double),
+        ]
+        lib.hbem_group_solve_device.restype = C.c_int32
+        lib.hbem_group_autotune.argtypes = [
+            C.c_void_p, C.POINTER(_GroupCase), C.POINTER(_SolveOptions), C.POINTER(C.c_int32),
+            C.POINTER(C.c_double),
+        ]
+        lib.hbem_group_autotune.restype = C.c_int32
         lib.hbem_dense_solve_noalloc.argtypes = [
             C.c_int32, C.c_int32, vp, vp, vp, vp, C.POINTER(_SolveOptions), vp, vp, vp, vp,
         ]
@@ -1104,6 +1147,104 @@ class HipBemSession:
 
 # The reference's class name for the same role.
 MetalNativeStandardSession = HipBemSession
+
+
+@dataclass
+class GroupCase:
+    """One member's device arrays for HipBemGroup.solve_device (device
+    addresses, e.g. ``tensor.data_ptr()``): k_real, field_k [F]; q [F][T]
+    interleaved complex; obs [3][n_obs]; field [F][n_obs] complex; p
+    [F][n] complex or None; tag_avg [F][1 + n_tags][2] double."""
+
+    n_freq: int
+    k_real: int
+    field_k: int
+    q: int
+    n_obs: int
+    obs: int
+    field: int
+    tag_avg: int
+    p: int | None = None
+
+
+class HipBemGroup:
+    """Several sessions solved together (hbem_group_*, the batched uneven
+    sweep): one assembly launch over all members, size-classed batched LU on
+    parallel streams, the whole sweep replayed from one hipGraph.  Borrows
+    the sessions: close the group before them."""
+
+    def __init__(self, sessions: list[HipBemSession]):
+        if not sessions:
+            raise ValueError("a group needs at least one session")
+        for s in sessions:
+            s._ensure_open()
+        self.sessions = list(sessions)
+        arr = (C.c_void_p * len(sessions))(*[s._handle for s in sessions])
+        h = C.c_void_p()
+        _check(load_library().hbem_group_create(len(sessions), arr, C.byref(h)), "group_create")
+        self._handle = h
+        self.granularity = 512
+        self.tuning: dict[int, float] = {}
+
+    def close(self) -> None:
+        if self._handle:
+            load_library().hbem_group_destroy(self._handle)
+            self._handle = None
+
+    def __enter__(self) -> "HipBemGroup":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def _args(self, cases, source_tags, impedance_source_tag, keep):
+        if len(cases) != len(self.sessions):
+            raise ValueError("one case per member session")
+        arr = (_GroupCase * len(cases))()
+        for i, c in enumerate(cases):
+            arr[i] = _GroupCase(int(c.n_freq), c.k_real, c.field_k, c.q, int(c.n_obs), c.obs, c.field,
+                                c.p, c.tag_avg)
+        tags = np.ascontiguousarray(list(source_tags), dtype=np.int32)
+        keep.append(tags)
+        opts = _SolveOptions()
+        opts.solve_mode = SOLVE_MODES["fp32"]
+        opts.n_tags = int(tags.size)
+        opts.tags = _i32p(tags) if tags.size else None
+        opts.impedance_tag = int(impedance_source_tag)
+        return arr, opts
+
+    def autotune(self, cases: list[GroupCase], *, source_tags=(), impedance_source_tag: int = 2) -> dict:
+        """Times one sweep per size-class step and keeps the fastest."""
+        keep: list = []
+        arr, opts = self._args(cases, source_tags, impedance_source_tag, keep)
+        gran = C.c_int32(0)
+        secs = (C.c_double * 8)()
+        _check(load_library().hbem_group_autotune(self._handle, arr, C.byref(opts), C.byref(gran), secs),
+               "group_autotune")
+        self.granularity = abs(int(gran.value))
+        self.split_assembly = int(gran.value) < 0
+        steps = (64, 256, 512, 1024)
+        self.tuning = {f"{g}{'/split' if i >= 4 else '/one-launch'}": float(secs[i])
+                       for i, g in enumerate(steps + steps)}
+        return {"granularity": self.granularity, "split_assembly": self.split_assembly,
+                "seconds": self.tuning}
+
+    def solve_device(self, cases: list[GroupCase], *, source_tags=(), impedance_source_tag: int = 2):
+        """One sweep of every member.  Returns (per-member case diagnostics,
+        member latencies in s, sweep seconds)."""
+        keep: list = []
+        arr, opts = self._args(cases, source_tags, impedance_source_tag, keep)
+        total = sum(int(c.n_freq) for c in cases)
+        diag = (_CaseDiag * total)()
+        lat = (C.c_double * len(cases))()
+        sweep = C.c_double(0.0)
+        _check(load_library().hbem_group_solve_device(self._handle, arr, C.byref(opts), diag, lat,
+                                                        C.byref(sweep)), "group_solve_device")
+        out, i = [], 0
+        for c in cases:
+            out.append([diag_to_dict(diag[i + f]) for f in range(int(c.n_freq))])
+            i += int(c.n_freq)
+        return out, [float(x) for x in lat], float(sweep.value)
 
 
 def dense_solve(

@@ -166,6 +166,13 @@ const char *hbem_last_error(void);
  * discover_native_runtime (native.py:95-179). */
 int32_t hbem_device_count(void);
 
+/* Sustained rate (operations per second over the whole device) of the
+ * hardware transcendental pipe on the mix the assembly kernels issue (sine,
+ * cosine, reciprocal square root in equal parts): the ceiling the regular
+ * assembly's transcendental side is measured against.  No reference
+ * counterpart (a device probe, like hbem_device_count). */
+int32_t hbem_probe_transcendental(double *ops_per_second);
+
 typedef struct hbem_session hbem_session;
 
 /* Upload geometry and build the resident plan (incidence, singular pairs,
@@ -292,6 +299,58 @@ int32_t hbem_evaluate_field(hbem_session *session, int32_t n_freq, const float *
 int32_t hbem_dense_solve(int32_t n_sys, int32_t n, const float *A_re, const float *A_im,
                          const float *b_re, const float *b_im, const hbem_solve_options *opts,
                          float *x_re, float *x_im, hbem_case_diag *diag);
+
+/* Session groups: the sweeps of several meshes solved together (the
+ * batched uneven sweep; the reference runs such cases through its resident
+ * context's double-buffered case slots and solve pool, main.swift:5520-5562,
+ * 9332-9832).  One regular and one singular assembly launch cover every
+ * member's tile pairs and singular classes; the members' systems are padded
+ * (identity) into size classes of a common order, each class factored and
+ * solved as one batch on its own stream; field and reductions per member;
+ * the whole sequence is captured once per shape and buffer set and replayed
+ * as one hipGraph.  Members must share the device and the symmetry image
+ * count, use the corrected assembly and have near-field quadrature off; the
+ * group borrows the sessions (destroy the group first). */
+typedef struct hbem_group hbem_group;
+
+/* One member's arrays for hbem_group_solve_device: device memory, the
+ * meanings of hbem_solve_batch_device's arguments with one source. */
+typedef struct hbem_group_case {
+    int32_t n_freq;
+    const float *k_real;   /* [F] */
+    const float *field_k;  /* [F] */
+    const float *q;        /* [F][n_triangles] interleaved complex */
+    int32_t n_obs;
+    const float *obs_3xn;  /* [3][n_obs] */
+    float *field;          /* [F][n_obs] interleaved complex (overwritten) */
+    float *p;              /* [F][p1_dof_count] interleaved complex or NULL */
+    double *tag_avg;       /* [F][1 + n_tags][2] */
+} hbem_group_case;
+
+int32_t hbem_group_create(int32_t n_sessions, hbem_session *const *sessions, hbem_group **out);
+int32_t hbem_group_destroy(hbem_group *group);
+
+/* Solves every member's frequencies (cases[n_sessions]).  opts: solve_mode
+ * HBEM_SOLVE_F32 without rcond, Robin walls, CHIEF rows, extra sources or
+ * precise geometry (HBEM_EUNSUPPORTED otherwise); tags as in the batch calls.
+ * diag [sum of n_freq] (members in order); member_seconds [n_sessions] or
+ * NULL: the time from the sweep's start until the member's results are
+ * complete (its systems' latency); sweep_seconds or NULL: the sweep. */
+int32_t hbem_group_solve_device(hbem_group *group, const hbem_group_case *cases,
+                                const hbem_solve_options *opts, hbem_case_diag *diag,
+                                double *member_seconds, double *sweep_seconds);
+
+/* Times one sweep for each of eight launch shapes -- size-class step
+ * (padded-order granularity 64, 256, 512, 1024: from one class per mesh to
+ * a few large classes with more padding) x assembly as one launch over all
+ * members or one launch per size class (whose solves then overlap the next
+ * class's assembly) -- and keeps the fastest for later
+ * hbem_group_solve_device calls.  granularity (or NULL) receives the chosen
+ * step, negative when the assembly is split per class; seconds[8] (or NULL)
+ * the timings, in the order (64, 256, 512, 1024) one launch, then split. */
+int32_t hbem_group_autotune(hbem_group *group, const hbem_group_case *cases,
+                            const hbem_solve_options *opts, int32_t *granularity,
+                            double *seconds);
 
 /* hbem_dense_solve on device arrays without any device allocation.  No
  * counterpart in the reference: it is for callers that cannot allocate
