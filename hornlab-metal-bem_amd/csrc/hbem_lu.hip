@@ -720,6 +720,32 @@ constexpr int kTrTile = 128;  // output rows and columns per block
 constexpr int kTrK = 32;      // k per stage
 constexpr int kTrS = 40;      // LDS plane row stride (16-bit elements)
 
+// Tile of a trailing-update block, remapped for the L2 of each XCD.  The
+// hardware hands workgroups to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2: linear id g goes to XCD g % 8, so the remap gives XCD j
+// the j-th contiguous eighth of the tile list (a bijection onto all tiles),
+// and that list walks each system in row groups of 8 tiles, column by
+// column, so that the group's 8 L strips and the current U strip stay in the
+// XCD's L2 while the blocks in flight sweep the columns.  Returns the tile's
+// row and column index and its system.
+struct TrTile {
+    int x, y, b;
+};
+__device__ __forceinline__ TrTile trailing_tile() {
+    const int X = gridDim.x, Y = gridDim.y;
+    const int total = X * Y * (int)gridDim.z;
+    const int g = blockIdx.x + X * (blockIdx.y + Y * blockIdx.z);
+    const int per = total / 8, rem = total % 8;
+    const int xcd = g % 8, k = g / 8;
+    const int t = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k;
+    const int b = t / (X * Y);
+    const int u = t - b * (X * Y);
+    const int rg = u / (8 * Y);
+    const int hx = min(8, X - 8 * rg);
+    const int v = u - rg * 8 * Y;
+    return TrTile{8 * rg + v % hx, v / hx, b};
+}
+
 struct __attribute__((aligned(16))) TrLds {
     unsigned short Lr[kTrTile][kTrS], Li[kTrTile][kTrS], Ur[kTrTile][kTrS], Ui[kTrTile][kTrS];
 };
@@ -730,7 +756,9 @@ struct __attribute__((aligned(16))) TrLds {
 // 4 (lane / 16) + v, row rb + 16 tj + lane % 16.  The loads of 32 entries are
 // issued before any of their stores: the compiler cannot reorder a load past
 // a store it cannot prove disjoint, so a load-modify-store per entry would
-// wait one memory latency per entry.
+// wait one memory latency per entry.  The C loads and stores are
+// non-temporal: each trailing entry is touched once per update, and keeping
+// it out of L2 leaves L2 to the L and U strips that neighbouring tiles share.
 __device__ __forceinline__ void tile_epilogue(float *Ab, int64_t lda, int cb, int rb, int c_end,
                                               int r_end, int lane, const f32x4 (&accr)[4][4],
                                               const f32x4 (&acci)[4][4], float scale) {
@@ -745,9 +773,12 @@ __device__ __forceinline__ void tile_epilogue(float *Ab, int64_t lda, int cb, in
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int col = cb + 16 * (t2 + h) + lc + v, row = rb + 16 * tj + li;
-                    cv[h][tj][v] = (row < r_end && col < c_end)
-                                       ? ld(Ab, (size_t)col * lda + row)
-                                       : c32{0.0f, 0.0f};
+                    typedef float f2v __attribute__((ext_vector_type(2)));
+                    f2v t = {0.0f, 0.0f};
+                    if (row < r_end && col < c_end)
+                        t = __builtin_nontemporal_load(
+                            reinterpret_cast<const f2v *>(Ab + 2 * ((size_t)col * lda + row)));
+                    cv[h][tj][v] = c32{t.x, t.y};
                 }
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -760,7 +791,10 @@ __device__ __forceinline__ void tile_epilogue(float *Ab, int64_t lda, int cb, in
                         c32 x = cv[h][tj][v];
                         x.re -= accr[t2 + h][tj][v] * scale;
                         x.im -= acci[t2 + h][tj][v] * scale;
-                        st(Ab, (size_t)col * lda + row, x);
+                        typedef float f2v __attribute__((ext_vector_type(2)));
+                        const f2v o = {x.re, x.im};
+                        __builtin_nontemporal_store(
+                            o, reinterpret_cast<f2v *>(Ab + 2 * ((size_t)col * lda + row)));
                     }
                 }
     }
@@ -843,10 +877,11 @@ __global__ __launch_bounds__(256, 2) void k_lu_gemm_trp(float *A, int64_t lda, i
                                                      int K1, int M, const uint16_t *pack,
                                                      int64_t pstride, int64_t plane,
                                                      const float *scale) {
-    const int b = blockIdx.z;
+    const TrTile tile = trailing_tile();
+    const int b = tile.b;
     float *Ab = A + (size_t)b * a_stride * 2;
-    const int r0 = blockIdx.x * kTrTile;  // rows / columns relative to K1
-    const int c0 = blockIdx.y * kTrTile;
+    const int r0 = tile.x * kTrTile;  // rows / columns relative to K1
+    const int c0 = tile.y * kTrTile;
     __shared__ TrLds sm;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -946,10 +981,11 @@ struct __attribute__((aligned(16))) TrLds32 {
 __global__ __launch_bounds__(256, 2) void k_lu_gemm_tr32(float *A, int64_t lda, int64_t a_stride,
                                                          int r_begin, int r_end, int c_begin,
                                                          int c_end, int k_begin, int k_end) {
-    const int b = blockIdx.z;
+    const TrTile tile = trailing_tile();
+    const int b = tile.b;
     float *Ab = A + (size_t)b * a_stride * 2;
-    const int r0 = r_begin + blockIdx.x * kTrTile;
-    const int c0 = c_begin + blockIdx.y * kTrTile;
+    const int r0 = r_begin + tile.x * kTrTile;
+    const int c0 = c_begin + tile.y * kTrTile;
     __shared__ TrLds32 sm;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -1067,14 +1103,30 @@ __device__ __forceinline__ void stage_diag(const float *Ab, int64_t lda, int kb,
 __device__ __forceinline__ void block_gemv_h(const float *Ab, int64_t lda, int kb, int r0, int r1,
                                              c32 *xs, c32 (*part)[kNB + 1]) {
     const int c = threadIdx.x / kNB, chunk = threadIdx.x % kNB;
-    c32 acc{0.0f, 0.0f};
+    c32 acc{0.0f, 0.0f}, acc2{0.0f, 0.0f};
     const size_t cc = (size_t)(kb + c) * lda;
-    for (int r = r0 + chunk; r < r1; r += kNB) {
+    int r = r0 + chunk;
+    // eight loads in flight per round trip
+    for (; r + 7 * kNB < r1; r += 8 * kNB) {
+        c32 a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = ld(Ab, cc + r + u * kNB);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const c32 x = xs[r + u * kNB];
+            c32 &s = (u & 1) ? acc2 : acc;
+            s.re += a[u].re * x.re + a[u].im * x.im;
+            s.im += a[u].re * x.im - a[u].im * x.re;
+        }
+    }
+    for (; r < r1; r += kNB) {
         const c32 a = ld(Ab, cc + r);
         const c32 x = xs[r];
         acc.re += a.re * x.re + a.im * x.im;
         acc.im += a.re * x.im - a.im * x.re;
     }
+    acc.re += acc2.re;
+    acc.im += acc2.im;
     part[chunk][c] = acc;
     __syncthreads();
     if (threadIdx.x < kNB) {
