@@ -14,6 +14,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hbem.h"
@@ -165,6 +166,9 @@ struct hbem_session {
         tagout, norms, anorm, active, rowsplit, rcx, rcy, rc_state, rc_est;
     // staging of host-input batches
     DevBuf io_k, io_kim, io_fk, io_q, io_obs, io_field, io_p, io_tag, io_beta, qt, l2g;
+    // second stream of the session: the rcond iteration, concurrent with
+    // the refinement and the field on `stream`
+    hipStream_t aux_stream = nullptr;
     double setup_seconds = 0.0;
     int64_t near_pairs = 0;
     int assembly_mode = HBEM_ASSEMBLY_CORRECTED;
@@ -179,6 +183,7 @@ struct hbem_session {
                           &io_p, &io_tag, &io_beta, &qt, &l2g})
             b->release();
         if (lu_graph) (void)hipGraphExecDestroy(lu_graph);
+        if (aux_stream) (void)hipStreamDestroy(aux_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -267,14 +272,18 @@ void assemble_batch(hbem_session *s, int B, const float *k_real_h, const float *
 // system on the device, every round one batched A^{-1} and/or A^{-H} solve of
 // the systems that need it; the host only reads the B solve flags per round.
 // rcond = 1 / (||A||_1 ||A^{-1}||_1), 0 for a zero estimate or norm.
-void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, const std::vector<double> &anorm,
-                    std::vector<double> *rcond) {
-    const int B = w.B, n = w.n, N = w.N;
-    hipStream_t st = s->stream;
+void reserve_rcond(hbem_session *s, int B, int N) {
     s->rcx.reserve((size_t)B * N * 2 * sizeof(float));
     s->rcy.reserve((size_t)B * N * 2 * sizeof(float));
     s->rc_state.reserve(sizeof(int32_t) * 4 * (size_t)B);
     s->rc_est.reserve(sizeof(double) * (size_t)B);
+}
+
+// Runs on `st`; its buffers come from reserve_rcond (no allocation here, so
+// it can run beside other work of the session on another host thread).
+void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, const std::vector<double> &anorm,
+                    std::vector<double> *rcond, hipStream_t st) {
+    const int B = w.B, n = w.n, N = w.N;
     float *X = s->rcx.as<float>(), *Y = s->rcy.as<float>();
     int32_t *state = s->rc_state.as<int32_t>();
     double *est_d = s->rc_est.as<double>();
@@ -980,7 +989,8 @@ void chief_residuals(hbem_session *s, int B, int S, int n, int N, const hbem_sol
 }
 
 void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solve_options &opt,
-                    std::vector<hbem_case_diag> *diag, double *solve_seconds) {
+                    std::vector<hbem_case_diag> *diag, double *solve_seconds,
+                    const std::function<void()> &after_solve = nullptr) {
     hipStream_t st = s->stream;
     const int64_t lda = N, a_stride = (int64_t)N * N;
     const int V = B * S;
@@ -1160,6 +1170,38 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
             stalled[v] = best[v] > stall_at ? 1 : 0;
         }
     };
+    // The rcond iteration needs only the LU factors: it runs on the aux
+    // stream from its own host thread while this thread refines, solves the
+    // CHIEF rows and enqueues the field (the factors are read-only from here
+    // on; the two sides use disjoint buffers).
+    std::vector<double> rcond(B, -1.0);
+    double rcond_s = 0.0;
+    std::exception_ptr rc_error;
+    std::thread rc_thread;
+    if (opt.estimate_rcond) {
+        if (!s->aux_stream)
+            hip_check(hipStreamCreateWithFlags(&s->aux_stream, hipStreamNonBlocking),
+                      "hipStreamCreate");
+        reserve_rcond(s, B, N);
+        rc_thread = std::thread([&] {
+            try {
+                hip_check(hipSetDevice(s->device), "hipSetDevice");
+                Timer tr;
+                tr.start(s->aux_stream);
+                estimate_rcond(s, w, anorm, &rcond, s->aux_stream);
+                tr.stop(s->aux_stream);
+                rcond_s = tr.seconds();
+            } catch (...) {
+                rc_error = std::current_exception();
+            }
+        });
+    }
+    struct Join {
+        std::thread &t;
+        ~Join() {
+            if (t.joinable()) t.join();
+        }
+    } join_rcond{rc_thread};
     refine_all();
     std::vector<double> chief_res(V, -1.0);
     if (opt.n_chief > 0) {
@@ -1171,17 +1213,11 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         refine_all();
         chief_residuals(s, B, S, n, N, opt, &chief_res);
     }
-    std::vector<double> rcond(B, -1.0);
-    double rcond_s = 0.0;
-    if (opt.estimate_rcond) {
-        Timer tr;
-        tr.start(st);
-        estimate_rcond(s, w, anorm, &rcond);
-        tr.stop(st);
-        rcond_s = tr.seconds();
-    }
     t.stop(st);
     *solve_seconds = t.seconds();
+    if (after_solve) after_solve();
+    if (rc_thread.joinable()) rc_thread.join();
+    if (rc_error) std::rethrow_exception(rc_error);
     diag->resize(B);
     for (int b = 0; b < B; ++b) {
         hbem_case_diag &d = (*diag)[b];
@@ -1287,27 +1323,30 @@ void solve_core(hbem_session *s, int F, const float *kre_d, const float *kim_d,
         ta.stop(st);
         std::vector<hbem_case_diag> dg;
         double solve_s = 0.0;
-        solve_resident(s, B, S, n, N, opts, &dg, &solve_s);
-        tf.start(st);
-        const float *qf = qc;
-        if (bc) {
-            s->qt.reserve(sizeof(float) * 2 * (size_t)V * T);
-            hbem::launch_robin_neumann(s->dg, V, S, kre_d + f0, kim_d ? kim_d + f0 : nullptr, qc,
-                                       bc, s->x.as<float>(), N, s->qt.as<float>(), st);
-            qf = s->qt.as<float>();
-        }
-        if (n_obs > 0)
-            hbem::launch_field(s->dg, V, fk_d + f0, S, s->x.as<float>(), N, qf, n_obs, obs_d,
-                               field_d + (size_t)f0 * S * n_obs * 2, st);
-        hbem::launch_tag_average(s->dg, V, s->x.as<float>(), N, ntag, s->tags.as<int32_t>(),
-                                 tag_d + (size_t)f0 * S * ntag * 2, st);
-        if (p_d)
-            hip_check(hipMemcpy2DAsync(p_d + (size_t)f0 * S * n * 2, sizeof(float) * 2 * n,
-                                       s->x.ptr, sizeof(float) * 2 * N, sizeof(float) * 2 * n, V,
-                                       hipMemcpyDeviceToDevice, st),
-                      "p copy");
-        hip_check(hipGetLastError(), "field launch");
-        tf.stop(st);
+        // field and reductions: enqueued while the rcond estimate may still run
+        auto field_stage = [&]() {
+            tf.start(st);
+            const float *qf = qc;
+            if (bc) {
+                s->qt.reserve(sizeof(float) * 2 * (size_t)V * T);
+                hbem::launch_robin_neumann(s->dg, V, S, kre_d + f0, kim_d ? kim_d + f0 : nullptr,
+                                           qc, bc, s->x.as<float>(), N, s->qt.as<float>(), st);
+                qf = s->qt.as<float>();
+            }
+            if (n_obs > 0)
+                hbem::launch_field(s->dg, V, fk_d + f0, S, s->x.as<float>(), N, qf, n_obs, obs_d,
+                                   field_d + (size_t)f0 * S * n_obs * 2, st);
+            hbem::launch_tag_average(s->dg, V, s->x.as<float>(), N, ntag, s->tags.as<int32_t>(),
+                                     tag_d + (size_t)f0 * S * ntag * 2, st);
+            if (p_d)
+                hip_check(hipMemcpy2DAsync(p_d + (size_t)f0 * S * n * 2, sizeof(float) * 2 * n,
+                                           s->x.ptr, sizeof(float) * 2 * N, sizeof(float) * 2 * n,
+                                           V, hipMemcpyDeviceToDevice, st),
+                          "p copy");
+            hip_check(hipGetLastError(), "field launch");
+            tf.stop(st);
+        };
+        solve_resident(s, B, S, n, N, opts, &dg, &solve_s, field_stage);
         const double asm_s = ta.seconds(), reg_s = treg.seconds(), sing_s = tsing.seconds();
         const double near_s = tnear.seconds();
         const double field_s = tf.seconds();
