@@ -504,6 +504,9 @@ __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks
 // One tile pair (pair index tp) at one frequency (b) of the regular
 // assembly: the block body shared by the single-mesh kernel and the
 // multi-mesh launch of a session group.
+// list entries per (trial column, local vertex) of the gathered flush
+constexpr int kGatherCap = 8;
+
 template <bool kComplexK, int kM, bool kPrecise>
 __device__ __forceinline__ void regular_tile_pair(
     const DeviceGeometry &g, int tp, int b, const float *__restrict__ k_real,
@@ -525,7 +528,11 @@ __device__ __forceinline__ void regular_tile_pair(
     const int plane = snd * tnd;
     float *accF = acc;              // re, then im at + plane
     float *accR = acc + 2 * plane;  // re, then im at + plane
-    for (int e = threadIdx.x; e < 4 * snd * tnd; e += 256) acc[e] = 0.0f;
+    // single-image FP32 blocks gather each flush per trial column (below);
+    // their tables hold the column lists first and are zeroed after
+    constexpr bool kGather = kM == 1 && !kPrecise;
+    if constexpr (!kGather)
+        for (int e = threadIdx.x; e < 4 * snd * tnd; e += 256) acc[e] = 0.0f;
 
     const float k = k_real[b];
     const float kim = kComplexK ? k_imag[b] : 0.0f;
@@ -592,6 +599,60 @@ __device__ __forceinline__ void regular_tile_pair(
     // -4 pi and 4 pi folded out of the kernels, jac = (2 A_test)(2 A_trial)
     const float dscale = -kInv4Pi * jac_s * g.row_weight;
     const float gscale = kInv4Pi * jac_s * g.row_weight;
+    // Column lists of the trial side (kGather): for trial dof slot c and
+    // local vertex j, the lanes whose vertex j has slot c (at most kGatherCap;
+    // more and the block keeps per-lane atomics).  A lane owns columns
+    // lane and lane + 64 and holds their lists in registers, packed one byte
+    // per entry.
+    uint32_t glist[2][3][2] = {};
+    int gcnt[2][3] = {};
+    int gmax[3] = {0, 0, 0};
+    bool gather = false;
+    if constexpr (kGather) {
+        int *cnt = reinterpret_cast<int *>(acc);                            // [3][snd]
+        unsigned char *lst = reinterpret_cast<unsigned char *>(cnt + 3 * snd);  // [3][snd][cap]
+        for (int e = threadIdx.x; e < 3 * snd; e += 256) cnt[e] = 0;
+        __syncthreads();
+        if (wave == 0 && trial_ok) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int slot = j * snd + slc[j];
+                const int pos = atomicAdd(cnt + slot, 1);
+                if (pos < kGatherCap) lst[slot * kGatherCap + pos] = (unsigned char)lane;
+            }
+        }
+        __syncthreads();
+        int over = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = lane + 64 * h;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int n = c < snd ? cnt[j * snd + c] : 0;
+                over |= n > kGatherCap;
+                gcnt[h][j] = min(n, kGatherCap);
+                if (n > 0) {
+                    const uint32_t *w = reinterpret_cast<const uint32_t *>(lst) + (j * snd + c) * 2;
+                    glist[h][j][0] = w[0];
+                    glist[h][j][1] = w[1];
+                }
+            }
+        }
+        // every lane has read the lists; the flag is block-uniform
+        gather = __syncthreads_or(over) == 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int m = max(gcnt[0][j], gcnt[1][j]);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) m = max(m, __shfl_xor(m, off));
+            gmax[j] = __builtin_amdgcn_readfirstlane(m);
+        }
+        for (int e = threadIdx.x; e < 4 * snd * tnd; e += 256) acc[e] = 0.0f;
+    }
+    (void)glist;
+    (void)gcnt;
+    (void)gmax;
+    (void)gather;
     __syncthreads();
 
     const int n_tests = max(0, min(kTestsPerWave, tcount - wave * kTestsPerWave));
@@ -622,7 +683,49 @@ __device__ __forceinline__ void regular_tile_pair(
     for (int i = 0; i < 9; ++i) dF[i][0] = dF[i][1] = dR[i][0] = dR[i][1] = 0.0f;
     int held[3] = {-1, -1, -1};  // tile dof slot of each position
     auto flush = [&](int r) {
-        if (trial_ok) {
+        if (kGather && gather) {
+            // lanes own trial columns: each sums the contributions of the
+            // lanes in its lists (cross-lane reads), then one atomic per
+            // table word and owned column
+            const int row = held[r];
+            float v[3][4];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                v[j][0] = trial_ok ? dF[3 * r + j][0] * dscale : 0.0f;
+                v[j][1] = trial_ok ? dF[3 * r + j][1] * dscale : 0.0f;
+                v[j][2] = trial_ok ? dR[3 * j + r][0] * dscale : 0.0f;
+                v[j][3] = trial_ok ? dR[3 * j + r][1] * dscale : 0.0f;
+            }
+            const int nh = snd > 64 ? 2 : 1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (h >= nh) break;
+                float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < gmax[j]; ++k) {
+                        const bool on = k < gcnt[h][j];
+                        const int src =
+                            on ? (int)((glist[h][j][k >> 2] >> (8 * (k & 3))) & 0xffu) : lane;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const float x = __shfl(v[j][q], src);
+                            sum[q] += on ? x : 0.0f;
+                        }
+                    }
+                const int c = lane + 64 * h;
+                if (c < snd && (gcnt[h][0] + gcnt[h][1] + gcnt[h][2]) > 0) {
+                    float *cF = accF + row * snd + c;
+                    float *cR = accR + row * snd + c;
+                    __hip_atomic_fetch_add(cF, sum[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(cF + plane, sum[1], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(cR, sum[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(cR + plane, sum[3], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        } else if (trial_ok) {
             float *cF = accF + held[r] * snd;
             float *cR = accR + held[r] * snd;
 #pragma unroll
