@@ -169,6 +169,7 @@ struct hbem_session {
     // second stream of the session: the rcond iteration, concurrent with
     // the refinement and the field on `stream`
     hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_lu = nullptr;  // end of the factorization, for the aux stream
     double setup_seconds = 0.0;
     int64_t near_pairs = 0;
     int assembly_mode = HBEM_ASSEMBLY_CORRECTED;
@@ -184,6 +185,7 @@ struct hbem_session {
             b->release();
         if (lu_graph) (void)hipGraphExecDestroy(lu_graph);
         if (aux_stream) (void)hipStreamDestroy(aux_stream);
+        if (ev_lu) (void)hipEventDestroy(ev_lu);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -281,8 +283,9 @@ void reserve_rcond(hbem_session *s, int B, int N) {
 
 // Runs on `st`; its buffers come from reserve_rcond (no allocation here, so
 // it can run beside other work of the session on another host thread).
-void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, const std::vector<double> &anorm,
-                    std::vector<double> *rcond, hipStream_t st) {
+// Returns the estimates of ||A^{-1}||_1; rcond = 1 / (||A||_1 est).
+void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, std::vector<double> *est_out,
+                    hipStream_t st) {
     const int B = w.B, n = w.n, N = w.N;
     float *X = s->rcx.as<float>(), *Y = s->rcy.as<float>();
     int32_t *state = s->rc_state.as<int32_t>();
@@ -310,13 +313,11 @@ void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, const std::vect
             if (op == 0 || op == 2) need[op] = true;
         }
     }
-    std::vector<double> est(B);
-    hip_check(hipMemcpyAsync(est.data(), est_d, sizeof(double) * B, hipMemcpyDeviceToHost, st),
+    est_out->assign(B, 0.0);
+    hip_check(hipMemcpyAsync(est_out->data(), est_d, sizeof(double) * B, hipMemcpyDeviceToHost,
+                             st),
               "rcond D2H");
     hip_check(hipStreamSynchronize(st), "rcond sync");
-    rcond->assign(B, 0.0);
-    for (int b = 0; b < B; ++b)
-        (*rcond)[b] = (anorm[b] > 0.0 && est[b] > 0.0) ? (1.0 / est[b]) / anorm[b] : 0.0;
 }
 
 }  // namespace
@@ -1023,9 +1024,24 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         s->lu_pack.reserve(sizeof(uint16_t) * (size_t)B * hbem::lu_pack_u16(N));
         w.pack = s->lu_pack.as<uint16_t>();
     }
+    // The rcond iteration needs only the LU factors: it runs on the aux
+    // stream from its own host thread, from the end of the factorization on,
+    // while this thread solves, refines, solves the CHIEF rows and enqueues
+    // the field (the factors are read-only from there on; the two sides use
+    // disjoint buffers).
+    const bool want_rcond = opt.estimate_rcond != 0;
+    if (want_rcond) {
+        if (!s->aux_stream)
+            hip_check(hipStreamCreateWithFlags(&s->aux_stream, hipStreamNonBlocking),
+                      "hipStreamCreate");
+        if (!s->ev_lu)
+            hip_check(hipEventCreateWithFlags(&s->ev_lu, hipEventDisableTiming), "hipEventCreate");
+        reserve_rcond(s, B, N);
+    }
     // padding, norms, FP16 operand scale, LU, first solve: kernels and
-    // device copies only (no host synchronisation, no memset nodes)
-    auto enqueue = [&]() {
+    // device copies only (no host synchronisation, no memset nodes);
+    // mark_lu records the end of the factorization for the rcond stream
+    auto enqueue = [&](bool mark_lu) {
         hbem::launch_pad_identity(s->A.as<float>(), lda, a_stride, n, N, B, st);
         hbem::launch_norm1(s->A.as<float>(), lda, a_stride, n, B, s->anorm.as<double>(),
                            s->amax.as<double>(), st);
@@ -1036,6 +1052,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
                                      hipMemcpyDeviceToDevice, st),
                       "A copy");
         hbem::lu_factor(w, precision, s->lu_scale.as<float>(), st);
+        if (mark_lu) hip_check(hipEventRecord(s->ev_lu, st), "hipEventRecord");
         hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)V * N,
                                  hipMemcpyDeviceToDevice, st),
                   "rhs copy");
@@ -1059,7 +1076,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
             hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture begin");
             hipGraph_t graph = nullptr;
             try {
-                enqueue();
+                enqueue(false);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1072,9 +1089,35 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
             s->lu_graph_key = key;
         }
         hip_check(hipGraphLaunch(s->lu_graph, st), "graph launch");
+        if (want_rcond) hip_check(hipEventRecord(s->ev_lu, st), "hipEventRecord");
     } else {
-        enqueue();
+        enqueue(want_rcond);
     }
+    std::vector<double> est;
+    double rcond_s = 0.0;
+    std::exception_ptr rc_error;
+    std::thread rc_thread;
+    if (want_rcond) {
+        rc_thread = std::thread([&] {
+            try {
+                hip_check(hipSetDevice(s->device), "hipSetDevice");
+                hip_check(hipStreamWaitEvent(s->aux_stream, s->ev_lu, 0), "hipStreamWaitEvent");
+                Timer tr;
+                tr.start(s->aux_stream);
+                estimate_rcond(s, w, &est, s->aux_stream);
+                tr.stop(s->aux_stream);
+                rcond_s = tr.seconds();
+            } catch (...) {
+                rc_error = std::current_exception();
+            }
+        });
+    }
+    struct Join {
+        std::thread &t;
+        ~Join() {
+            if (t.joinable()) t.join();
+        }
+    } join_rcond{rc_thread};
     std::vector<int32_t> info(B);
     hip_check(hipMemcpyAsync(info.data(), s->info.ptr, sizeof(int32_t) * B, hipMemcpyDeviceToHost,
                              st),
@@ -1170,38 +1213,6 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
             stalled[v] = best[v] > stall_at ? 1 : 0;
         }
     };
-    // The rcond iteration needs only the LU factors: it runs on the aux
-    // stream from its own host thread while this thread refines, solves the
-    // CHIEF rows and enqueues the field (the factors are read-only from here
-    // on; the two sides use disjoint buffers).
-    std::vector<double> rcond(B, -1.0);
-    double rcond_s = 0.0;
-    std::exception_ptr rc_error;
-    std::thread rc_thread;
-    if (opt.estimate_rcond) {
-        if (!s->aux_stream)
-            hip_check(hipStreamCreateWithFlags(&s->aux_stream, hipStreamNonBlocking),
-                      "hipStreamCreate");
-        reserve_rcond(s, B, N);
-        rc_thread = std::thread([&] {
-            try {
-                hip_check(hipSetDevice(s->device), "hipSetDevice");
-                Timer tr;
-                tr.start(s->aux_stream);
-                estimate_rcond(s, w, anorm, &rcond, s->aux_stream);
-                tr.stop(s->aux_stream);
-                rcond_s = tr.seconds();
-            } catch (...) {
-                rc_error = std::current_exception();
-            }
-        });
-    }
-    struct Join {
-        std::thread &t;
-        ~Join() {
-            if (t.joinable()) t.join();
-        }
-    } join_rcond{rc_thread};
     refine_all();
     std::vector<double> chief_res(V, -1.0);
     if (opt.n_chief > 0) {
@@ -1218,6 +1229,10 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
     if (after_solve) after_solve();
     if (rc_thread.joinable()) rc_thread.join();
     if (rc_error) std::rethrow_exception(rc_error);
+    std::vector<double> rcond(B, -1.0);
+    if (want_rcond)
+        for (int b = 0; b < B; ++b)
+            rcond[b] = (anorm[b] > 0.0 && est[b] > 0.0) ? (1.0 / est[b]) / anorm[b] : 0.0;
     diag->resize(B);
     for (int b = 0; b < B; ++b) {
         hbem_case_diag &d = (*diag)[b];
