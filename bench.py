@@ -180,10 +180,12 @@ def run_c5(args) -> int:
 
 def run_c4(args) -> int:
     """Batched uneven sweep: the 16 variants' sessions in one session group
-    (one assembly launch over all meshes, size-classed batched LU on parallel
-    streams, the whole sweep replayed from one hipGraph), size-class step
-    autotuned; per-system latency = time from the sweep's start until the
-    system's mesh has its results."""
+    (size-classed batched LU; the assembly either one launch over all meshes
+    with the whole sweep replayed from one hipGraph, or one launch per size
+    class with each class's LU on a second hardware queue beside the next
+    class's assembly), size-class step and assembly split autotuned;
+    per-system latency = time from the sweep's start until the system's mesh
+    has its results."""
     import torch
 
     from hornlab_metal_bem.native import GroupCase, HipBemGroup, HipBemSession
@@ -249,7 +251,9 @@ def run_c4(args) -> int:
         "config": {
             "workload": f"batched uneven sweep: {len(jobs)} quarter-horn variants, n {min(ns)}-{max(ns)} "
             f"P1 dofs, {WORKLOADS['c4']['nf']} log frequencies each = {systems} systems, one "
-            "session group (one assembly launch, size-classed batched LU, one hipGraph)",
+            "session group (size-classed batched LU; "
+            + ("assembly per size class, each class's LU beside the next class's assembly)"
+               if tune["split_assembly"] else "one assembly launch, replayed hipGraph)"),
             "n_dofs": ns,
             "solve_mode": "fp32",
             "autotuned": {"size_class_step": tune["granularity"],

@@ -1777,7 +1777,9 @@ struct hbem_group {
     std::vector<hbem_session *> members;
     int device = -1;
     hipStream_t stream = nullptr;
-    std::vector<hipStream_t> side;  // one per size class
+    std::vector<hipStream_t> side;   // graph capture: one per size class
+    std::vector<hipStream_t> lanes;  // per-class assembly: streams with hardware queues of their own
+    hipStream_t asm_lane = nullptr;  // per-class assembly: the assembly launches (own queue)
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join;  // per class: solves joined, assembly done (split)
     struct SizeClass {
@@ -1807,6 +1809,8 @@ struct hbem_group {
         for (hipEvent_t e : ev_join) (void)hipEventDestroy(e);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
+        for (hipStream_t s : lanes) (void)hipStreamDestroy(s);
+        if (asm_lane) (void)hipStreamDestroy(asm_lane);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1815,6 +1819,20 @@ namespace {
 
 constexpr int kGroupGranularities[4] = {64, 256, 512, 1024};
 constexpr int kGroupCandidates = 8;  // granularities x {one assembly launch, one per class}
+// Streams the per-class sweep spreads the size classes' solves over.  They and
+// the stream of the assembly launches are created with a CU mask (all CUs),
+// which gives each a hardware queue of its own: streams created without one
+// share the process's few hardware queues, where a solve queued between two
+// assembly launches delays the second.  (Reserving CUs for the solves, or
+// 2, 4 or 6 solve streams, measured slower on the 16-mesh sweep.)
+constexpr int kGroupLanes = 3;
+
+// CU mask selecting all `total` CUs.
+std::vector<uint32_t> full_cu_mask(int total) {
+    std::vector<uint32_t> mask((size_t)(total + 31) / 32, 0u);
+    for (int i = 0; i < total; ++i) mask[(size_t)i / 32] |= 1u << (i % 32);
+    return mask;
+}
 
 void check_group_case(const hbem_session *s, const hbem_group_case &c) {
     if (c.n_freq <= 0) fail(HBEM_EINVAL, "hbem_group: n_freq must be positive");
@@ -1876,8 +1894,11 @@ void build_classes(hbem_group *g, const hbem_group_case *cases, int gran) {
     }
 }
 
-// Enqueues the whole group sweep on g->stream (forking one stream per size
-// class); kernels and device copies only.
+// Enqueues the whole group sweep on g->stream, forking the size classes'
+// solves onto g->side (one stream per class, for graph capture) or, when the
+// assembly is split per class, onto g->lanes (launched directly: a replayed
+// graph runs the per-class sweep's branches one after another on one
+// hardware queue).  Kernels and device copies only.
 // jobs: per size class, the class's members (heaviest first), classes in
 // order; class_job[c] = index of the class's first job (class_job[C] = jobs).
 void enqueue_group(hbem_group *g, const hbem_group_case *cases, const std::vector<hbem::AssemblyJob> &jobs,
@@ -1892,23 +1913,28 @@ void enqueue_group(hbem_group *g, const hbem_group_case *cases, const std::vecto
         hbem::launch_fill_zero(c.A.as<float>(), (size_t)2 * c.B * c.N * c.N, st);
         hbem::launch_fill_zero(c.rhs.as<float>(), (size_t)2 * c.B * c.N, st);
     }
+    hipStream_t as = st;  // the assembly's stream
     if (!g->split) {
         // one launch over every member's tile pairs, one over every member's
         // singular classes; the size classes' solves fork after it
         hbem::launch_assemble_multi(g->jobs_d.as<hbem::AssemblyJob>(), jobs, false, st);
         hip_check(hipEventRecord(g->ev_fork, st), "event record");
+    } else {
+        hip_check(hipEventRecord(g->ev_fork, st), "event record");
+        as = g->asm_lane;
+        hip_check(hipStreamWaitEvent(as, g->ev_fork, 0), "fork");
     }
     for (size_t ci = 0; ci < g->classes.size(); ++ci) {
         hbem_group::SizeClass &c = g->classes[ci];
-        hipStream_t ss = g->side[ci];
+        hipStream_t ss = g->split ? g->lanes[ci % g->lanes.size()] : g->side[ci];
         if (g->split) {
             // the class's members in one launch each; its solves fork while
             // the next class assembles
             const std::vector<hbem::AssemblyJob> cj(jobs.begin() + class_job[ci],
                                                     jobs.begin() + class_job[ci + 1]);
             hbem::launch_assemble_multi(g->jobs_d.as<hbem::AssemblyJob>() + class_job[ci], cj, false,
-                                        st);
-            hip_check(hipEventRecord(g->ev_join[ci + g->classes.size()], st), "event record");
+                                        as);
+            hip_check(hipEventRecord(g->ev_join[ci + g->classes.size()], as), "event record");
             hip_check(hipStreamWaitEvent(ss, g->ev_join[ci + g->classes.size()], 0), "fork");
         } else {
             hip_check(hipStreamWaitEvent(ss, g->ev_fork, 0), "fork");
@@ -1945,6 +1971,10 @@ void enqueue_group(hbem_group *g, const hbem_group_case *cases, const std::vecto
         hip_check(hipEventRecord(g->ev_join[ci], ss), "event record");
         hip_check(hipStreamWaitEvent(st, g->ev_join[ci], 0), "join");
     }
+    if (as != st) {
+        // the last class's assembly event, already recorded on as
+        hip_check(hipStreamWaitEvent(st, g->ev_join[2 * g->classes.size() - 1], 0), "join");
+    }
     hip_check(hipGetLastError(), "group launch");
 }
 
@@ -1965,10 +1995,24 @@ void group_solve(hbem_group *g, const hbem_group_case *cases, const hbem_solve_o
         build_classes(g, cases, gran);
         g->classes_gran = gran;
     }
-    while (g->side.size() < g->classes.size()) {
+    while (!g->split && g->side.size() < g->classes.size()) {
         hipStream_t s = nullptr;
         hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
         g->side.push_back(s);
+    }
+    if (g->split && g->lanes.empty()) {
+        int ncu = 0;
+        hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->device),
+                  "CU count");
+        const std::vector<uint32_t> mask = full_cu_mask(ncu);
+        for (int i = 0; i < kGroupLanes; ++i) {
+            hipStream_t s = nullptr;
+            hip_check(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()),
+                      "hipExtStreamCreateWithCUMask");
+            g->lanes.push_back(s);
+        }
+        hip_check(hipExtStreamCreateWithCUMask(&g->asm_lane, (uint32_t)mask.size(), mask.data()),
+                  "hipExtStreamCreateWithCUMask");
     }
     while (g->ev_join.size() < 2 * g->classes.size()) {
         hipEvent_t e = nullptr;
@@ -2042,7 +2086,7 @@ void group_solve(hbem_group *g, const hbem_group_case *cases, const hbem_solve_o
     for (const hbem_group::SizeClass &c : g->classes)
         for (const DevBuf *b : {&c.A, &c.rhs, &c.x, &c.ipiv, &c.info, &c.gl, &c.nsys})
             key.push_back((uintptr_t)b->ptr);
-    if (!g->graph || g->graph_key != key) {
+    if (!g->split && (!g->graph || g->graph_key != key)) {
         if (g->graph) {
             (void)hipGraphExecDestroy(g->graph);
             g->graph = nullptr;
@@ -2064,7 +2108,10 @@ void group_solve(hbem_group *g, const hbem_group_case *cases, const hbem_solve_o
     }
     Timer tw;
     tw.start(st);
-    hip_check(hipGraphLaunch(g->graph, st), "graph launch");
+    if (g->split)
+        enqueue_group(g, cases, jobs, class_job, ntag);
+    else
+        hip_check(hipGraphLaunch(g->graph, st), "graph launch");
     tw.stop(st);
     const double sweep_s = tw.seconds();
     if (sweep_seconds) *sweep_seconds = sweep_s;
