@@ -170,6 +170,12 @@ struct hbem_session {
     // the refinement and the field on `stream`
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_lu = nullptr;  // end of the factorization, for the aux stream
+    // row maps of the factors and the solve contexts of the two streams
+    // ([0] stream, [1] aux_stream) for the multi-workgroup triangular solves
+    // (orders <= 2048); trsv_on while the current solve call uses them
+    DevBuf lu_perm, lu_pi, lu_lf, lu_dinv, ts_ctx[2], ts_y[2], ts_z[2];
+    hbem::TrsvSync trsv[2] = {};
+    bool trsv_on = false;
     double setup_seconds = 0.0;
     int64_t near_pairs = 0;
     int assembly_mode = HBEM_ASSEMBLY_CORRECTED;
@@ -181,7 +187,8 @@ struct hbem_session {
                           &sg_rule, &sg_rule_reg, &tile_start, &tile_count, &tile_ndofs, &tile_dofs, &tile_pairs, &nc_f, &nc_i, &nc_rule, &sg_adj, &vtx_d, &triv_d, &normals_d, &jac_d, &adj_k, &adj_kim, &adj_fk, &adj_q, &adj_beta, &adj_obs, &adj_c, &adj_ft, &adj_sig, &adj_gN, &adj_l32, &adj_e64, &adj_l64, &adj_r64, &adj_act, &chief_C, &chief_d, &chief_pts, &chief_z, &chief_e, &chief_z64, &chief_r, &chief_ainf, &A, &A_orig, &rhs, &x, &x64, &r64, &xprev, &ipiv, &info, &gl, &lu_scr_i, &lu_scr_f, &lu_pack, &amax, &lu_scale, &kv, &kh, &kn, &kf, &q, &kre,
                           &kim, &fk, &obs, &field, &tags, &tagout, &norms, &anorm, &active,
                           &rowsplit, &rcx, &rcy, &rc_state, &rc_est, &io_k, &io_kim, &io_fk, &io_q, &io_obs, &io_field,
-                          &io_p, &io_tag, &io_beta, &qt, &l2g})
+                          &io_p, &io_tag, &io_beta, &qt, &l2g, &lu_perm, &lu_pi, &lu_lf, &lu_dinv, &ts_ctx[0],
+                          &ts_ctx[1], &ts_y[0], &ts_y[1], &ts_z[0], &ts_z[1]})
             b->release();
         if (lu_graph) (void)hipGraphExecDestroy(lu_graph);
         if (aux_stream) (void)hipStreamDestroy(aux_stream);
@@ -281,6 +288,23 @@ void reserve_rcond(hbem_session *s, int B, int N) {
     s->rc_est.reserve(sizeof(double) * (size_t)B);
 }
 
+// Solve context `which` (0 for the session stream, 1 for the aux stream) with
+// room for V vectors of order N; initialised on `st` whenever its buffers
+// move (outside any graph capture: the context's epoch then advances on the
+// device, launch by launch).
+hbem::TrsvSync trsv_sync(hbem_session *s, int which, int V, int N, hipStream_t st) {
+    DevBuf &c = s->ts_ctx[which], &y = s->ts_y[which], &z = s->ts_z[which];
+    const void *c0 = c.ptr, *y0 = y.ptr, *z0 = z.ptr;
+    const size_t words = hbem::trsv_words(V, N);
+    c.reserve(sizeof(uint32_t) * 4);
+    y.reserve(sizeof(uint64_t) * words);
+    z.reserve(sizeof(uint64_t) * words);
+    const hbem::TrsvSync sync{c.as<uint32_t>(), y.as<uint64_t>(), z.as<uint64_t>()};
+    if (c.ptr != c0 || y.ptr != y0 || z.ptr != z0)
+        hbem::launch_trsv_init(sync, std::min(y.bytes, z.bytes) / sizeof(uint64_t), st);
+    return sync;
+}
+
 // Runs on `st`; its buffers come from reserve_rcond (no allocation here, so
 // it can run beside other work of the session on another host thread).
 // Returns the estimates of ||A^{-1}||_1; rcond = 1 / (||A||_1 est).
@@ -299,7 +323,7 @@ void estimate_rcond(hbem_session *s, const hbem::LuWorkspace &w, std::vector<dou
         for (int trans : {0, 2}) {
             if (!need[trans]) continue;
             hbem::launch_rcond_copy(X, Y, N, N, B, st);
-            hbem::lu_solve(w, Y, N, trans, st);
+            hbem::lu_solve(w, Y, N, trans, st, 1, s->trsv_on ? &s->trsv[1] : nullptr);
             hbem::launch_rcond_step(Y, X, N, n, N, B, trans, state, est_d, st);
         }
         hip_check(hipGetLastError(), "rcond launch");
@@ -646,7 +670,7 @@ void gmres_refine(hbem_session *s, const hbem::LuWorkspace &w, int B, int vpm, i
     auto precond = [&](const double *src, int64_t ld_src, int i) {
         hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * B * N, st);
         hbem::launch_copy_f64_to_f32(src, s->x.as<float>(), ld_src, N, n, B, st);
-        hbem::lu_solve(w, s->x.as<float>(), N, 0, st, vpm);
+        hbem::lu_solve(w, s->x.as<float>(), N, 0, st, vpm, s->trsv_on ? &s->trsv[0] : nullptr);
         hbem::launch_copy_f32_to_f64(s->x.as<float>(), vec(i), N, vstride, n, B, st);
     };
     auto norms_of = [&](int i) {
@@ -1024,6 +1048,21 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         s->lu_pack.reserve(sizeof(uint16_t) * (size_t)B * hbem::lu_pack_u16(N));
         w.pack = s->lu_pack.as<uint16_t>();
     }
+    // multi-workgroup triangular solves: row maps built by the factorization,
+    // one solve context per stream
+    s->trsv_on = N <= hbem::kLuSolveOneBlock;
+    if (s->trsv_on) {
+        s->lu_perm.reserve(sizeof(int32_t) * (size_t)B * N);
+        s->lu_pi.reserve(sizeof(int32_t) * (size_t)B * hbem::lu_solve_blocks(N) * N);
+        s->lu_dinv.reserve(sizeof(float) * (size_t)B * hbem::lu_dinv_floats(N));
+        s->lu_lf.reserve(sizeof(float) * 2 * (size_t)B * a_stride);
+        w.lf = s->lu_lf.as<float>();
+        w.perm = s->lu_perm.as<int32_t>();
+        w.pi = s->lu_pi.as<int32_t>();
+        w.dinv = s->lu_dinv.as<float>();
+        s->trsv[0] = trsv_sync(s, 0, V, N, st);
+        s->trsv[1] = trsv_sync(s, 1, B, N, st);
+    }
     // The rcond iteration needs only the LU factors: it runs on the aux
     // stream from its own host thread, from the end of the factorization on,
     // while this thread solves, refines, solves the CHIEF rows and enqueues
@@ -1056,7 +1095,8 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)V * N,
                                  hipMemcpyDeviceToDevice, st),
                   "rhs copy");
-        hbem::lu_solve(w, s->x.as<float>(), N, opt.solve_trans, st, S);
+        hbem::lu_solve(w, s->x.as<float>(), N, opt.solve_trans, st, S,
+                       s->trsv_on ? &s->trsv[0] : nullptr);
     };
     if (opt.lu_graph) {
         // One hipGraph per batch shape and buffer set, replayed while they
@@ -1067,7 +1107,8 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
             (uintptr_t)s->rhs.ptr, (uintptr_t)s->x.ptr, (uintptr_t)s->ipiv.ptr,
             (uintptr_t)s->info.ptr, (uintptr_t)s->gl.ptr, (uintptr_t)s->lu_scr_i.ptr,
             (uintptr_t)s->lu_scr_f.ptr, (uintptr_t)s->anorm.ptr, (uintptr_t)s->amax.ptr,
-            (uintptr_t)s->lu_scale.ptr, (uintptr_t)w.pack};
+            (uintptr_t)s->lu_scale.ptr, (uintptr_t)w.pack, (uintptr_t)w.perm, (uintptr_t)w.pi, (uintptr_t)w.dinv, (uintptr_t)w.lf,
+            (uintptr_t)s->ts_ctx[0].ptr, (uintptr_t)s->ts_y[0].ptr, (uintptr_t)s->ts_z[0].ptr};
         if (!s->lu_graph || s->lu_graph_key != key) {
             if (s->lu_graph) {
                 (void)hipGraphExecDestroy(s->lu_graph);
@@ -1179,7 +1220,8 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
                 hbem::launch_fill_zero(s->x.as<float>(), (size_t)2 * V * N, st);
                 hbem::launch_copy_f64_to_f32(s->r64.as<double>(), s->x.as<float>(), N, N, n, V,
                                              st);
-                hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
+                hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S,
+                               s->trsv_on ? &s->trsv[0] : nullptr);
                 hbem::launch_axpy_f32_to_f64(s->x.as<float>(), s->x64.as<double>(), N, N, n, V,
                                              s->active.as<int32_t>(), st);
                 residual();
@@ -1220,7 +1262,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
         hip_check(hipMemcpyAsync(s->x.ptr, s->rhs.ptr, sizeof(float) * 2 * (size_t)V * N,
                                  hipMemcpyDeviceToDevice, st),
                   "rhs copy");
-        hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S);
+        hbem::lu_solve(w, s->x.as<float>(), N, 0, st, S, s->trsv_on ? &s->trsv[0] : nullptr);
         refine_all();
         chief_residuals(s, B, S, n, N, opt, &chief_res);
     }

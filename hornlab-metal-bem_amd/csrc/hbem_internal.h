@@ -379,6 +379,17 @@ struct LuWorkspace {
     int32_t *scratch_i;  // [B][kLuMaxOrder / 32] tournament candidates
     float *scratch_f;    // [B][32][32] + [B][64][32] complex: top block, gathered rows
     uint16_t *pack = nullptr;  // [B][lu_pack_u16(N)]: 16-bit trailing operands (or null)
+    // Row maps, final-order L and diagonal-block inverses of the
+    // multi-workgroup solves (or null; then lu_factor skips them): perm[B][N]
+    // original row of each final row, pi[B][lu_solve_blocks(N)][N] row of
+    // each final row in the order at the end of outer block s, lf[B][a_stride]
+    // complex L below the diagonal in final row order (layout of A),
+    // dinv[B][N / 64][2][64][64] complex inverses of the 64 x 64 diagonal
+    // blocks of L and U.
+    int32_t *perm = nullptr;
+    int32_t *pi = nullptr;
+    float *lf = nullptr;
+    float *dinv = nullptr;
 };
 
 constexpr int kLuPad = 64;
@@ -388,6 +399,25 @@ constexpr int kLuMaxOrder = 32768;  // tournament panels nominate from <= 32 chu
 // orders up to this solve with one 1024-thread block per vector (x in LDS);
 // larger ones in per-outer-block steps
 constexpr int kLuSolveOneBlock = 2048;
+
+// outer blocks of order N (row maps per system of LuWorkspace::pi)
+inline int lu_solve_blocks(int N) { return (N + kLuOuter - 1) / kLuOuter; }
+// floats of LuWorkspace::dinv per system
+inline size_t lu_dinv_floats(int N) { return (size_t)N * 64 * 2 * 2; }
+
+// Synchronisation state of the multi-workgroup triangular solves of one
+// stream (orders <= kLuSolveOneBlock): ctx[4] = {ticket, finished blocks,
+// epoch, error}; y and z [V][N][2] epoch-tagged words of the forward and
+// backward values.  Set up once by launch_trsv_init (y and z cleared over
+// n_words words each); a context serves one stream at a time (launches on it
+// are ordered).
+struct TrsvSync {
+    uint32_t *ctx;
+    uint64_t *y;
+    uint64_t *z;
+};
+inline size_t trsv_words(int V, int N) { return (size_t)V * N * 2; }
+void launch_trsv_init(const TrsvSync &sync, size_t n_words, hipStream_t stream);
 
 // sizes of LuWorkspace::scratch_i (ints) and scratch_f (floats) per system
 constexpr size_t kLuScratchI = kLuMaxOrder / 32;
@@ -421,9 +451,11 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
 
 // Solves A x = b (trans = 0) or A^H x = b (trans = 2) in place on x
 // [B * vpm][ldx] interleaved complex (vector v against matrix v / vpm), using
-// the LU factors and pivots.
+// the LU factors and pivots.  With sync (capacity for B * vpm vectors) and
+// the workspace's row maps, orders <= kLuSolveOneBlock run on N / 64
+// workgroups per vector.
 void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream,
-              int vpm = 1);
+              int vpm = 1, const TrsvSync *sync = nullptr);
 
 // Reciprocal condition estimate (cgecon / clacn2) of B factored systems,
 // driven by the host: init X, then rounds of {copy X -> Y, lu_solve(Y,

@@ -1359,6 +1359,353 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
 }
 
 // ---------------------------------------------------------------------------
+// Triangular solves on many workgroups per vector (orders <= 2048, with a
+// TrsvSync).  One CU streams the factors of one system at a few tens of GB/s,
+// so the one-workgroup solve above is bound by that CU; here the order is cut
+// into 64-row panels and each panel is a 256-thread workgroup that streams its
+// own rows of the factor, folds in the values of the panels it depends on as
+// they are published, applies the inverse of its 64 x 64 diagonal block (one
+// matrix-vector product on all threads: the inverses are built once per
+// factorization by k_lu_diag_inverse, since a serial 64-step substitution on
+// one wave is the longest link of the chain) and publishes its values.  A value
+// is published as two 64-bit words (real or imaginary bits | epoch << 32),
+// stored and polled with agent-scope atomics: a reader sees a whole word or
+// none of it, so the tag says the value is there and no flag, fence or store
+// wait sits between producer and consumer.  A workgroup takes its panel from
+// a ticket counter in arrival order and waits only on panels with smaller
+// tickets, which belong to workgroups already running, so every wait ends
+// without co-residency of the grid; each wait is also bounded (kTrsvSpin
+// polls): past it the panel goes on with NaN values and sets the context's
+// error word.
+// Rows are in the final order of the factorization (all exchanges applied).
+// The L columns of outer block s are stored in the row order at the end of
+// block s; k_lu_final_l copies L once per factorization into final row order
+// (lf), so every coupling tile is read contiguously.
+//   mode 0: y = L^{-1} P b      (forward,  b = X read through perm)
+//   mode 1: X = U^{-1} y        (backward)
+//   mode 2: y = U^{-H} b        (forward,  b = X)
+//   mode 3: X = P^T L^{-H} y    (backward, X written through perm)
+// Forward panels publish into sync.y, backward panels into sync.z ([V][N][2]
+// tagged words).
+// ---------------------------------------------------------------------------
+constexpr int kTrsvRows = 64;
+constexpr uint32_t kTrsvSpin = 1u << 20;
+static_assert(kLuOuter % kTrsvRows == 0 && kLuPad % kTrsvRows == 0, "panels inside outer blocks");
+
+__global__ void k_trsv_init(uint32_t *ctx, uint64_t *y, uint64_t *z, size_t n_words) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < 4) ctx[i] = i == 2 ? 1u : 0u;  // ticket, finished, epoch 1, error
+    if (i < n_words) {
+        y[i] = 0ull;  // epoch 0: never current
+        z[i] = 0ull;
+    }
+}
+
+// Row maps of one system per block: the composed exchanges of every panel
+// step are staged in LDS and replayed on an index vector by wave 0, once for
+// perm (all steps) and once per outer block s for pi[s].
+__global__ __launch_bounds__(1024) void k_lu_solve_tables(const int32_t *gl, int64_t gl_stride,
+                                                          int N, int32_t *perm, int32_t *pi) {
+    extern __shared__ int32_t tsm[];
+    int32_t *ord = tsm, *inv = tsm + N, *prm = tsm + 2 * N, *lists = tsm + 3 * N;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int Q = N / kNB, S = (N + kLuOuter - 1) / kLuOuter, steps = kLuOuter / kNB;
+    const int32_t *glb = gl + (size_t)b * gl_stride;
+    for (int i = tid; i < Q * kLuGather; i += 1024) lists[i] = glb[i];
+    for (int r = tid; r < N; r += 1024) ord[r] = r;
+    __syncthreads();
+    // position trow takes the entry at tsrc, steps in order; each step's
+    // reads precede its writes in the wave's order
+    auto replay = [&](int q0, int q1) {
+        for (int q = q0; q < q1; ++q) {
+            const int32_t *g = lists + (size_t)q * kLuGather;
+            const bool on = lane < g[0];
+            const int t = on ? ord[g[1 + 2 * kNB + lane]] : 0;
+            if (on) ord[g[1 + lane]] = t;
+        }
+    };
+    if (tid < 64) replay(0, Q);
+    __syncthreads();
+    for (int r = tid; r < N; r += 1024) {
+        prm[r] = ord[r];
+        perm[(size_t)b * N + r] = ord[r];
+    }
+    __syncthreads();
+    for (int r = tid; r < N; r += 1024) ord[r] = r;
+    __syncthreads();
+    for (int s = 0; s < S; ++s) {
+        if (tid < 64) replay(s * steps, min(Q, (s + 1) * steps));
+        __syncthreads();
+        for (int p = tid; p < N; p += 1024) inv[ord[p]] = p;
+        __syncthreads();
+        for (int r = tid; r < N; r += 1024) pi[((size_t)b * S + s) * N + r] = inv[prm[r]];
+        __syncthreads();
+    }
+}
+
+// Inverses of the 64 x 64 diagonal blocks of the factors: dinv[b][p][0] =
+// L_pp^{-1} (unit lower), dinv[b][p][1] = U_pp^{-1}, element (i, j) at
+// i * 64 + j.  One wave per block and factor (blockIdx.z); lane c builds
+// column c by substitution with the column's entries in LDS.
+__global__ __launch_bounds__(64) void k_lu_diag_inverse(const float *A, int64_t lda,
+                                                        int64_t a_stride, int N, float *dinv) {
+    __shared__ c32 T[kTrsvRows][kTrsvRows];
+    __shared__ c32 col[kTrsvRows][kTrsvRows];  // [row][column]
+    const int p = blockIdx.x, b = blockIdx.y, upper = blockIdx.z, c = threadIdx.x;
+    const int P = N / kTrsvRows, R0 = p * kTrsvRows;
+    const float *Ab = A + (size_t)b * a_stride * 2;
+    for (int j = 0; j < kTrsvRows; ++j) T[c][j] = ld(Ab, (size_t)(R0 + j) * lda + R0 + c);
+    __syncthreads();
+    // the loops run over the same k on every lane (no divergence, unrolled
+    // LDS reads): the entries of column c outside its triangle come out as
+    // exact zeros, since they only sum products with zeros
+    if (!upper) {
+        // column c of L^{-1} by forward substitution
+        for (int i = 0; i < kTrsvRows; ++i) {
+            c32 acc{i == c ? 1.0f : 0.0f, 0.0f};
+#pragma unroll 8
+            for (int k = 0; k < i; ++k) {
+                const c32 l = T[i][k], y = col[k][c];
+                acc.re -= l.re * y.re - l.im * y.im;
+                acc.im -= l.re * y.im + l.im * y.re;
+            }
+            col[i][c] = acc;
+        }
+    } else {
+        // column c of U^{-1} by backward substitution
+        for (int i = kTrsvRows - 1; i >= 0; --i) {
+            c32 acc{i == c ? 1.0f : 0.0f, 0.0f};
+#pragma unroll 8
+            for (int k = i + 1; k < kTrsvRows; ++k) {
+                const c32 u = T[i][k], y = col[k][c];
+                acc.re -= u.re * y.re - u.im * y.im;
+                acc.im -= u.re * y.im + u.im * y.re;
+            }
+            col[i][c] = cmul(acc, cinv(T[i][i]));
+        }
+    }
+    float *out = dinv + (((size_t)b * P + p) * 2 + upper) * kTrsvRows * kTrsvRows * 2;
+    for (int i = 0; i < kTrsvRows; ++i) st(out, (size_t)i * kTrsvRows + c, col[i][c]);
+}
+
+// L below the diagonal in final row order: lf[b][c][r] = L[pi[s(c)][r]][c]
+// for r > c, one block per column (its stored column read with a gather
+// inside the column, the copy written contiguously).
+__global__ __launch_bounds__(256) void k_lu_final_l(const float *A, int64_t lda, int64_t a_stride,
+                                                    int N, const int32_t *pi, float *lf) {
+    const int c = blockIdx.x, b = blockIdx.y, S = (N + kLuOuter - 1) / kLuOuter;
+    const float *col = A + ((size_t)b * a_stride + (size_t)c * lda) * 2;
+    const int32_t *m = pi + ((size_t)b * S + c / kLuOuter) * N;
+    float *out = lf + ((size_t)b * a_stride + (size_t)c * lda) * 2;
+    for (int r = c + 1 + (int)threadIdx.x; r < N; r += 256) st(out, r, ld(col, m[r]));
+}
+
+__device__ __forceinline__ uint64_t tag_word(float v, uint32_t ep) {
+    return ((uint64_t)ep << 32) | (uint64_t)__float_as_uint(v);
+}
+__device__ __forceinline__ float word_value(uint64_t w) { return __uint_as_float((uint32_t)w); }
+__device__ __forceinline__ uint32_t word_epoch(uint64_t w) { return (uint32_t)(w >> 32); }
+
+template <int kMode>
+__global__ __launch_bounds__(256) void k_trsv(const float *A, const float *lf, int64_t lda,
+                                              int64_t a_stride, int N, const int32_t *perm,
+                                              const float *dinv, float *X, int64_t ldx, int vpm,
+                                              TrsvSync sync) {
+    constexpr bool kFwd = kMode == 0 || kMode == 2;
+    // modes 0, 1: lanes = own rows, waves = column quarters of a coupling
+    // panel; modes 2, 3 (conjugate transposes, columns of the stored factor):
+    // lanes = coupling rows, waves = quarters of the own rows
+    constexpr bool kRowLanes = kMode <= 1;
+    __shared__ c32 D[kTrsvRows][kTrsvRows + 1];
+    __shared__ c32 xs[2][kTrsvRows];
+    __shared__ c32 vs[kTrsvRows];
+    __shared__ c32 part[4][kTrsvRows];
+    __shared__ uint32_t role_s, ep_s;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int P = N / kTrsvRows;
+    if (tid == 0) {
+        role_s = atomicAdd(&sync.ctx[0], 1u);
+        ep_s = __hip_atomic_load(&sync.ctx[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int role = (int)role_s;
+    const uint32_t ep = ep_s;
+    const int v = role / P, q = role % P;
+    const int p = kFwd ? q : P - 1 - q;  // panels taken in dependency order
+    const int R0 = p * kTrsvRows;
+    const int b = v / vpm;
+    // the factor the coupling tiles come from: L in final row order (modes
+    // 0, 3) or U
+    const float *Ab = (kMode == 0 || kMode == 3 ? lf : A) + (size_t)b * a_stride * 2;
+    const int32_t *permb = perm + (size_t)b * N;
+    const uint64_t *yv = sync.y + (size_t)v * N * 2;
+    uint64_t *pub = (kFwd ? sync.y : sync.z) + (size_t)v * N * 2;
+    float *xv = X + (size_t)v * ldx * 2;
+    // D[i][j] = entry (i, j) of the inverse of the own diagonal block of the
+    // factor solved with: L^{-1}, U^{-1}, (U^H)^{-1} = (U^{-1})^H, (L^H)^{-1}
+    const float *Dv = dinv + (((size_t)b * P + p) * 2 + (kMode == 1 || kMode == 2 ? 1 : 0)) *
+                                 kTrsvRows * kTrsvRows * 2;
+    for (int e = tid; e < kTrsvRows * kTrsvRows; e += 256) {
+        if (kRowLanes) {
+            const int j = e & 63, i = e >> 6;
+            D[i][j] = ld(Dv, (size_t)i * kTrsvRows + j);
+        } else {
+            const int i = e & 63, j = e >> 6;
+            D[i][j] = cconj(ld(Dv, (size_t)j * kTrsvRows + i));
+        }
+    }
+    c32 rhs{0.0f, 0.0f};
+    int out_row = 0;  // mode 3: the row of X the lane's value goes to
+    if (w == 0) {
+        const int r = R0 + lane;
+        if (kMode == 3) out_row = permb[r];
+        // the backward passes read the forward pass's words (earlier launch)
+        rhs = kMode == 0   ? ld(xv, permb[r])
+              : kMode == 2 ? ld(xv, r)
+                           : c32{word_value(yv[2 * r]), word_value(yv[2 * r + 1])};
+    }
+    __syncthreads();
+    const int n_dep = kFwd ? p : P - 1 - p;
+    auto dep = [&](int k) { return kFwd ? k : P - 1 - k; };
+    // 16 factor values of coupling panel J for this thread: entries (own
+    // row, J column) of L or U (modes 0, 1), entries (J row, own column) of
+    // U or L (modes 2, 3; conjugated in the fold)
+    auto load_tile = [&](int J, c32 (&t)[16]) {
+        if (kRowLanes) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                t[j] = ld(Ab, (size_t)(J * kTrsvRows + w * 16 + j) * lda + R0 + lane);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                t[j] = ld(Ab, (size_t)(R0 + w * 16 + j) * lda + J * kTrsvRows + lane);
+        }
+    };
+    c32 acc[kRowLanes ? 1 : 16];
+#pragma unroll
+    for (int j = 0; j < (kRowLanes ? 1 : 16); ++j) acc[j] = c32{0.0f, 0.0f};
+    bool ok = true;
+    // waits until panel J's values carry this epoch (wave 0, lane = row of
+    // J, polling its two words; relaxed coherent loads, no cache
+    // maintenance) and stages them in xs[buf]
+    auto fetch = [&](int J, int buf) {
+        if (w == 0) {
+            const uint64_t *src = pub + 2 * ((size_t)J * kTrsvRows + lane);
+            uint64_t re = 0ull, im = 0ull;
+            for (uint32_t i = 0;; ++i) {
+                re = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                im = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__all(word_epoch(re) == ep && word_epoch(im) == ep)) break;
+                if (!ok || i + 1 >= kTrsvSpin) {
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            xs[buf][lane] = c32{word_value(re), word_value(im)};
+        }
+        __syncthreads();
+    };
+    auto fold = [&](const c32 (&t)[16], int buf) {
+        if (kRowLanes) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const c32 x = xs[buf][w * 16 + j];
+                acc[0].re += t[j].re * x.re - t[j].im * x.im;
+                acc[0].im += t[j].re * x.im + t[j].im * x.re;
+            }
+        } else {
+            const c32 x = xs[buf][lane];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {  // conj(t) x
+                acc[j].re += t[j].re * x.re + t[j].im * x.im;
+                acc[j].im += t[j].re * x.im - t[j].im * x.re;
+            }
+        }
+    };
+    c32 ta[16], tb[16];
+    if (n_dep > 0) load_tile(dep(0), ta);
+    for (int k = 0; k < n_dep; k += 2) {
+        if (k + 1 < n_dep) load_tile(dep(k + 1), tb);
+        fetch(dep(k), 0);
+        fold(ta, 0);
+        if (k + 1 < n_dep) {
+            if (k + 2 < n_dep) load_tile(dep(k + 2), ta);
+            fetch(dep(k + 1), 1);
+            fold(tb, 1);
+        }
+    }
+    // coupling sums of the own rows into part[0][row]
+    if (kRowLanes) {
+        part[w][lane] = acc[0];
+        __syncthreads();
+        if (w == 0) {
+            c32 s = part[0][lane];
+#pragma unroll
+            for (int u = 1; u < 4; ++u) {
+                s.re += part[u][lane].re;
+                s.im += part[u][lane].im;
+            }
+            part[0][lane] = s;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            float re = acc[j].re, im = acc[j].im;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                re += __shfl_xor(re, off);
+                im += __shfl_xor(im, off);
+            }
+            acc[j] = c32{re, im};
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) part[0][w * 16 + j] = acc[j];
+        }
+    }
+    __syncthreads();
+    if (w == 0) vs[lane] = c32{rhs.re - part[0][lane].re, rhs.im - part[0][lane].im};
+    __syncthreads();
+    {
+        // own values = D vs: thread (row lane, column quarter w)
+        c32 s{0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const c32 d = D[lane][w * 16 + j], u = vs[w * 16 + j];
+            s.re += d.re * u.re - d.im * u.im;
+            s.im += d.re * u.im + d.im * u.re;
+        }
+        part[w][lane] = s;
+    }
+    __syncthreads();
+    if (w == 0) {
+        c32 x = part[0][lane];
+#pragma unroll
+        for (int u = 1; u < 4; ++u) {
+            x.re += part[u][lane].re;
+            x.im += part[u][lane].im;
+        }
+        if (!ok) x = c32{__builtin_nanf(""), __builtin_nanf("")};
+        if (kMode == 1) st(xv, R0 + lane, x);
+        if (kMode == 3) st(xv, out_row, x);
+        uint64_t *dst = pub + 2 * (size_t)(R0 + lane);
+        __hip_atomic_store(dst, tag_word(x.re, ep), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(dst + 1, tag_word(x.im, ep), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (!ok) atomicOr(&sync.ctx[3], 1u);
+        // the last workgroup resets the ticket counter and opens the next epoch
+        if (atomicAdd(&sync.ctx[1], 1u) == gridDim.x - 1) {
+            __hip_atomic_store(&sync.ctx[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sync.ctx[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sync.ctx[2], ep + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Stepped solves for orders above 2048 (x does not fit one block's LDS
 // budget, and one CU per vector cannot stream the factors fast enough): per
 // outer block of kLuOuter rows, one in-block kernel (one 256-thread block
@@ -2075,14 +2422,42 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
             }
         }
     }
+    if (w.perm && w.pi && w.dinv && w.lf && N <= kLuSolveOneBlock) {
+        const size_t shm = sizeof(int32_t) * (3 * (size_t)N + (size_t)(N / kNB) * kLuGather);
+        hipLaunchKernelGGL(k_lu_solve_tables, dim3(B), dim3(1024), shm, stream, w.gl, gls, N,
+                           w.perm, w.pi);
+        hipLaunchKernelGGL(k_lu_final_l, dim3(N, B), dim3(256), 0, stream, w.A, w.lda, w.a_stride,
+                           N, w.pi, w.lf);
+        hipLaunchKernelGGL(k_lu_diag_inverse, dim3(N / kTrsvRows, B, 2), dim3(64), 0, stream, w.A,
+                           w.lda, w.a_stride, N, w.dinv);
+    }
+}
+
+void launch_trsv_init(const TrsvSync &sync, size_t n_words, hipStream_t stream) {
+    const size_t n = std::max<size_t>(4, n_words);
+    hipLaunchKernelGGL(k_trsv_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       sync.ctx, sync.y, sync.z, n_words);
 }
 
 void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream,
-              int vpm) {
+              int vpm, const TrsvSync *sync) {
     const int N = w.N, V = w.B * vpm;
     if (trans != 0 && trans != 2) throw std::runtime_error("lu_solve: trans must be 0 or 2");
     const int64_t gls = (int64_t)(N / kNB) * kLuGather;
-    if (N <= kLuSolveOneBlock) {
+    if (sync && w.perm && w.pi && w.dinv && w.lf && N <= kLuSolveOneBlock) {
+        const dim3 grid((unsigned)(V * (N / kTrsvRows)));
+        if (trans == 0) {
+            hipLaunchKernelGGL(k_trsv<0>, grid, dim3(256), 0, stream, w.A, w.lf, w.lda, w.a_stride,
+                               N, w.perm, w.dinv, x, ldx, vpm, *sync);
+            hipLaunchKernelGGL(k_trsv<1>, grid, dim3(256), 0, stream, w.A, w.lf, w.lda, w.a_stride,
+                               N, w.perm, w.dinv, x, ldx, vpm, *sync);
+        } else {
+            hipLaunchKernelGGL(k_trsv<2>, grid, dim3(256), 0, stream, w.A, w.lf, w.lda, w.a_stride,
+                               N, w.perm, w.dinv, x, ldx, vpm, *sync);
+            hipLaunchKernelGGL(k_trsv<3>, grid, dim3(256), 0, stream, w.A, w.lf, w.lda, w.a_stride,
+                               N, w.perm, w.dinv, x, ldx, vpm, *sync);
+        }
+    } else if (N <= kLuSolveOneBlock) {
         const size_t shm = sizeof(float) * 2 * (size_t)N;
         hipLaunchKernelGGL(k_lu_solve, dim3(V), dim3(1024), shm, stream, w.A, w.lda, w.a_stride,
                            N, w.ipiv, w.gl, gls, x, ldx, trans, vpm);
