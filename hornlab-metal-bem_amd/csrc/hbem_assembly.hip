@@ -383,7 +383,14 @@ __device__ __forceinline__ void regular_images(const TestRec &tr, int n_masks,
 // the images inner: the image sums of a test point share one outer product
 // with t's basis weights (dF / dR / sF / sR are linear in the image sums),
 // so the 3 x 3 block accumulation costs 36 FMAs per test point instead of
-// 36 per test point and image.  hR[m][b] are formed once per pair.
+// 36 per test point and image.  hR[m][b] are formed once per pair.  The
+// kernel is issue-bound, so the per-evaluation work is kept to the terms
+// that change with the trial point: q2 = 1/r^2 is folded into the phase
+// terms (g = (sin k + cos / r) / r^2 ...), the test-side normal factor hF
+// (fixed per test point and image) multiplies the image's sums once, and the
+// RHS point sum is the sum of its three basis sums (the basis weights of a
+// point add up to its weight).  Only non-touching combinations come here
+// (touching ones are skipped or go to the singular kernel), so r > 0.
 template <bool kComplexK, bool kSource, int kM>
 __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks, int packed_masks,
                                                   unsigned skip, const float (&yp)[18],
@@ -410,14 +417,13 @@ __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks
             float h = 0.0f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) h = fmaf(yp[3 * bq + c] - v0t[c], nt[c], h);
-            hR[m][bq] = h * rf;  // rf = 0 drops the (s, t) double-layer block
+            hR[m][bq] = -h * rf;  // rf = 0 drops the (s, t) double-layer block
         }
     }
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
         float eF[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
         float eR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
-        float gF[2] = {0.f, 0.f};
         float gR[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
 #pragma unroll
         for (int m = 0; m < kM; ++m) {
@@ -425,12 +431,13 @@ __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks
             const float xa = sg[m][0] * tr.x[3 * a], ya = sg[m][1] * tr.x[3 * a + 1],
                         za = sg[m][2] * tr.x[3 * a + 2];
             const float hF = fmaf(v0s[0] - xa, ns[0], fmaf(v0s[1] - ya, ns[1], (v0s[2] - za) * ns[2]));
+            float em[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};  // this image's sums before hF
 #pragma unroll
             for (int bq = 0; bq < 6; ++bq) {
                 const float dx = yp[3 * bq] - xa;
                 const float dy = yp[3 * bq + 1] - ya;
                 const float dz = yp[3 * bq + 2] - za;
-                const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), 1.0e-14f);
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 const float rinv = __builtin_amdgcn_rsqf(r2);
                 const float r = r2 * rinv;
                 const float t = __builtin_amdgcn_fractf(r * krev);  // revolutions, [0, 1)
@@ -443,22 +450,21 @@ __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks
                     q2 *= damp;
                 }
                 const float fr = kComplexK ? rinv + kim : rinv;
-                const float p1 = fmaf(sn, k, cs * fr);
-                const float p2 = fmaf(-cs, k, sn * fr);
-                const float uF = hF * q2;
-                const float uR = -hR[m][bq] * q2;
+                // g = p q2 with p1 = sin k + cos fr, p2 = sin fr - cos k
+                const float kq = k * q2, fq = fr * q2;
+                const float g1 = fmaf(sn, kq, cs * fq);
+                const float g2 = fmaf(-cs, kq, sn * fq);
+                const float r1 = g1 * hR[m][bq], r2v = g2 * hR[m][bq];
                 const float wb = (float)kRule6W[bq];
                 const float w1 = wb * (float)kRule6X[bq];
                 const float w2 = wb * (float)kRule6Y[bq];
                 const float w0 = wb - w1 - w2;
-                const float f1 = p1 * uF, f2 = p2 * uF;
-                const float r1 = p1 * uR, r2v = p2 * uR;
-                eF[0][0] = fmaf(f1, w0, eF[0][0]);
-                eF[0][1] = fmaf(f2, w0, eF[0][1]);
-                eF[1][0] = fmaf(f1, w1, eF[1][0]);
-                eF[1][1] = fmaf(f2, w1, eF[1][1]);
-                eF[2][0] = fmaf(f1, w2, eF[2][0]);
-                eF[2][1] = fmaf(f2, w2, eF[2][1]);
+                em[0][0] = fmaf(g1, w0, em[0][0]);
+                em[0][1] = fmaf(g2, w0, em[0][1]);
+                em[1][0] = fmaf(g1, w1, em[1][0]);
+                em[1][1] = fmaf(g2, w1, em[1][1]);
+                em[2][0] = fmaf(g1, w2, em[2][0]);
+                em[2][1] = fmaf(g2, w2, em[2][1]);
                 eR[0][0] = fmaf(r1, w0, eR[0][0]);
                 eR[0][1] = fmaf(r2v, w0, eR[0][1]);
                 eR[1][0] = fmaf(r1, w1, eR[1][0]);
@@ -468,8 +474,6 @@ __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks
                 if (kSource) {
                     const float gv = rinv * damp;
                     const float gc = cs * gv, gs = sn * gv;
-                    gF[0] = fmaf(gc, wb, gF[0]);
-                    gF[1] = fmaf(gs, wb, gF[1]);
                     gR[0][0] = fmaf(gc, w0, gR[0][0]);
                     gR[0][1] = fmaf(gs, w0, gR[0][1]);
                     gR[1][0] = fmaf(gc, w1, gR[1][0]);
@@ -478,10 +482,20 @@ __device__ __forceinline__ void regular_pair_fp32(const TestRec &tr, int n_masks
                     gR[2][1] = fmaf(gs, w2, gR[2][1]);
                 }
             }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                eF[j][0] = fmaf(hF, em[j][0], eF[j][0]);
+                eF[j][1] = fmaf(hF, em[j][1], eF[j][1]);
+            }
         }
         const float wa = (float)kRule6W[a] * tr.jac;
         const float ta[3] = {wa * (float)(1.0 - kRule6X[a] - kRule6Y[a]), wa * (float)kRule6X[a],
                              wa * (float)kRule6Y[a]};
+        float gF[2] = {0.f, 0.f};
+        if (kSource) {
+            gF[0] = gR[0][0] + gR[1][0] + gR[2][0];
+            gF[1] = gR[0][1] + gR[1][1] + gR[2][1];
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
 #pragma unroll
