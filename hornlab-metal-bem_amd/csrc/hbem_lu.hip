@@ -306,7 +306,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel_tournament(
     const float *A, int64_t lda, int64_t a_stride, int N, int k, int G, const int32_t *cand,
     int32_t *ipiv, int32_t *info, int32_t *gl, int64_t gl_stride, float *top, float *gsrc) {
     const int b = blockIdx.x;
-    const int tid = threadIdx.x, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float *Ab = A + (size_t)b * a_stride * 2;
     const int ncand = G * kNB;
     const int id = tid < ncand ? cand[(size_t)b * ncand + tid] : -1;
@@ -319,45 +319,42 @@ __global__ __launch_bounds__(1024) void k_lu_panel_tournament(
     __shared__ int win_s[kNB];
     __shared__ float mag_s[kNB];
     __shared__ int piv_s[kNB];
-    __shared__ int posr[2 * kNB], posc[2 * kNB];
     const int rank = eliminate_rows(v, own, (unsigned)id, best_s, prow, win_s, mag_s);
     float *topb = top + (size_t)b * kNB * kNB * 2;
     if (rank >= 0) {
 #pragma unroll
         for (int c = 0; c < kNB; ++c) st(topb, (size_t)rank * kNB + c, v[c]);
     }
-    if (tid == 0) {
-        // exchanges in order: the winner of column j moves to row k + j
-        int nt = 0;  // touched positions: posr[i] holds row posc[i]
-        auto content = [&](int p) {
-            for (int i = 0; i < nt; ++i)
-                if (posr[i] == p) return posc[i];
-            return p;
-        };
-        auto set = [&](int p, int row) {
-            for (int i = 0; i < nt; ++i)
-                if (posr[i] == p) {
-                    posc[i] = row;
-                    return;
-                }
-            posr[nt] = p;
-            posc[nt] = row;
-            ++nt;
-        };
-        int bad = 0;
+    if (wave == 0) {
+        // exchanges in order: the winner of column j moves to row k + j.
+        // One touched position per lane: lane i < nt holds position tpos,
+        // whose row is tcon; lanes 0..31 are positions k..k+31, and a
+        // position outside them joins the first free lane when a winner
+        // is taken from it (untouched positions hold their own row).
+        int tpos = lane < kNB ? k + lane : -1, tcon = tpos;
+        int nt = kNB, bad = 0;
         for (int j = 0; j < kNB; ++j) {
             const int w = win_s[j] >= 0 ? win_s[j] : k + j;
-            int p = w;  // current position of row w
-            for (int i = 0; i < nt; ++i)
-                if (posc[i] == w) p = posr[i];
-            const int cj = content(k + j), cp = content(p);
-            set(k + j, cp);
-            set(p, cj);
-            piv_s[j] = p;
-            ipiv[(size_t)b * N + k + j] = p;
+            const unsigned long long at = __ballot(lane < nt && tcon == w);
+            int lp;  // lane of the current position of row w
+            if (at != 0ull) {
+                lp = __ffsll(static_cast<unsigned long long>(at)) - 1;
+            } else {
+                lp = nt++;
+                if (lane == lp) tpos = tcon = w;
+            }
+            const int p = __builtin_amdgcn_readlane(tpos, lp);
+            const int cj = __builtin_amdgcn_readlane(tcon, j);  // position k + j is lane j
+            const int cp = __builtin_amdgcn_readlane(tcon, lp);
+            if (lane == j) tcon = cp;
+            if (lane == lp) tcon = cj;
+            if (lane == 0) {
+                piv_s[j] = p;
+                ipiv[(size_t)b * N + k + j] = p;
+            }
             if (!(mag_s[j] > 0.0f) && bad == 0) bad = k + j + 1;
         }
-        if (bad != 0 && info[b] == 0) info[b] = bad;
+        if (lane == 0 && bad != 0 && info[b] == 0) info[b] = bad;
     }
     __syncthreads();
     int32_t *g = gl + (size_t)b * gl_stride;
@@ -365,7 +362,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel_tournament(
     __syncthreads();
     const int cnt = g[0];
     float *gs = gsrc + (size_t)b * 2 * kNB * kNB * 2;
-    for (int e = tid; e < cnt * kNB; e += 1024) {
+    for (int e = tid; e < cnt * kNB; e += (int)blockDim.x) {
         const int m = e / kNB, c = e % kNB;
         st(gs, (size_t)m * kNB + c, ld(Ab, (size_t)(k + c) * lda + g[1 + 2 * kNB + m]));
     }
@@ -2372,7 +2369,9 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
                 const int G = (N - k + 1023) / 1024;
                 hipLaunchKernelGGL(k_lu_panel_nominate, dim3(G, B), dim3(1024), 0, stream, w.A,
                                    w.lda, w.a_stride, N, k, w.scratch_i);
-                hipLaunchKernelGGL(k_lu_panel_tournament, dim3(B), dim3(1024), 0, stream, w.A,
+                // one thread per candidate row (whole waves)
+                const int tthreads = std::min(1024, (G * kNB + 63) / 64 * 64);
+                hipLaunchKernelGGL(k_lu_panel_tournament, dim3(B), dim3(tthreads), 0, stream, w.A,
                                    w.lda, w.a_stride, N, k, G, w.scratch_i, w.ipiv, w.info,
                                    gl_of(k), gls, w.scratch_f, w.scratch_f + (size_t)B * kNB * kNB * 2);
                 hipLaunchKernelGGL(k_lu_panel_apply, dim3((N - k + 255) / 256, B), dim3(256), 0,
