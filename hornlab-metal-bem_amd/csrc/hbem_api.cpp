@@ -172,7 +172,7 @@ struct hbem_session {
     hipEvent_t ev_lu = nullptr;  // end of the factorization, for the aux stream
     // row maps of the factors and the solve contexts of the two streams
     // ([0] stream, [1] aux_stream) for the multi-workgroup triangular solves
-    // (orders <= 2048); trsv_on while the current solve call uses them
+    // (orders <= 8192); trsv_on while the current solve call uses them
     DevBuf lu_perm, lu_pi, lu_lf, lu_dinv, ts_ctx[2], ts_y[2], ts_z[2];
     hbem::TrsvSync trsv[2] = {};
     bool trsv_on = false;
@@ -1050,7 +1050,7 @@ void solve_resident(hbem_session *s, int B, int S, int n, int N, const hbem_solv
     }
     // multi-workgroup triangular solves: row maps built by the factorization,
     // one solve context per stream
-    s->trsv_on = N <= hbem::kLuSolveOneBlock;
+    s->trsv_on = N <= hbem::kTrsvMaxOrder;
     if (s->trsv_on) {
         s->lu_perm.reserve(sizeof(int32_t) * (size_t)B * N);
         s->lu_pi.reserve(sizeof(int32_t) * (size_t)B * hbem::lu_solve_blocks(N) * N);
@@ -1302,9 +1302,12 @@ int batch_size_for(const hbem_session *s, int F, int N, const hbem_solve_options
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)16 << 30;
     const bool refine = opt.solve_mode != HBEM_SOLVE_F32;
-    const double per = (double)N * N * 8.0 * (refine ? 2.0 : 1.0) + (double)N * 64.0 +
-                       (refine ? 2.0 * (double)hbem::lu_pack_u16(N) : 0.0);  // 16-bit LU operands
-    const double budget = std::min(0.5 * (double)free_b, 96.0 * (1ull << 30));
+    // A, the refinement's copy of A, L in final row order for the
+    // multi-workgroup solves, the 16-bit LU operands
+    const double mats = 1.0 + (refine ? 1.0 : 0.0) + (N <= hbem::kTrsvMaxOrder ? 1.0 : 0.0);
+    const double per = (double)N * N * 8.0 * mats + (double)N * 64.0 +
+                       (refine ? 2.0 * (double)hbem::lu_pack_u16(N) : 0.0);
+    const double budget = std::min(0.6 * (double)free_b, 192.0 * (1ull << 30));
     int b = (int)std::max(1.0, std::floor(budget / per));
     return std::max(1, std::min({F, b, 64}));
 }

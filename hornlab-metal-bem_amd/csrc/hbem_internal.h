@@ -400,13 +400,17 @@ constexpr int kLuMaxOrder = 32768;  // tournament panels nominate from <= 32 chu
 // larger ones in per-outer-block steps
 constexpr int kLuSolveOneBlock = 2048;
 
+// orders up to this solve on N / 64 workgroups per vector when the caller
+// gives a TrsvSync (the row-map kernel keeps 2 N ints in LDS)
+constexpr int kTrsvMaxOrder = 8192;
+
 // outer blocks of order N (row maps per system of LuWorkspace::pi)
 inline int lu_solve_blocks(int N) { return (N + kLuOuter - 1) / kLuOuter; }
 // floats of LuWorkspace::dinv per system
 inline size_t lu_dinv_floats(int N) { return (size_t)N * 64 * 2 * 2; }
 
 // Synchronisation state of the multi-workgroup triangular solves of one
-// stream (orders <= kLuSolveOneBlock): ctx[4] = {ticket, finished blocks,
+// stream (orders <= kTrsvMaxOrder): ctx[4] = {ticket, finished blocks,
 // epoch, error}; y and z [V][N][2] epoch-tagged words of the forward and
 // backward values.  Set up once by launch_trsv_init (y and z cleared over
 // n_words words each); a context serves one stream at a time (launches on it
@@ -452,7 +456,7 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
 // Solves A x = b (trans = 0) or A^H x = b (trans = 2) in place on x
 // [B * vpm][ldx] interleaved complex (vector v against matrix v / vpm), using
 // the LU factors and pivots.  With sync (capacity for B * vpm vectors) and
-// the workspace's row maps, orders <= kLuSolveOneBlock run on N / 64
+// the workspace's row maps, orders <= kTrsvMaxOrder run on N / 64
 // workgroups per vector.
 void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_t stream,
               int vpm = 1, const TrsvSync *sync = nullptr);

@@ -1356,8 +1356,8 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const float *A, int64_t lda, 
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves on many workgroups per vector (orders <= 2048, with a
-// TrsvSync).  One CU streams the factors of one system at a few tens of GB/s,
+// Triangular solves on many workgroups per vector (orders <= kTrsvMaxOrder,
+// with a TrsvSync).  One CU streams the factors of one system at a few tens of GB/s,
 // so the one-workgroup solve above is bound by that CU; here the order is cut
 // into 64-row panels and each panel is a 256-thread workgroup that streams its
 // own rows of the factor, folds in the values of the panels it depends on as
@@ -1399,43 +1399,57 @@ __global__ void k_trsv_init(uint32_t *ctx, uint64_t *y, uint64_t *z, size_t n_wo
 }
 
 // Row maps of one system per block: the composed exchanges of every panel
-// step are staged in LDS and replayed on an index vector by wave 0, once for
-// perm (all steps) and once per outer block s for pi[s].
+// step are replayed on an index vector in LDS by wave 0 (the records of an
+// outer block's steps loaded at once), once for perm (all steps) and once per
+// outer block s for pi[s].  LDS: 2 N ints.
 __global__ __launch_bounds__(1024) void k_lu_solve_tables(const int32_t *gl, int64_t gl_stride,
                                                           int N, int32_t *perm, int32_t *pi) {
     extern __shared__ int32_t tsm[];
-    int32_t *ord = tsm, *inv = tsm + N, *prm = tsm + 2 * N, *lists = tsm + 3 * N;
+    int32_t *ord = tsm, *inv = tsm + N;
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
-    const int Q = N / kNB, S = (N + kLuOuter - 1) / kLuOuter, steps = kLuOuter / kNB;
+    constexpr int kSteps = kLuOuter / kNB;
+    const int Q = N / kNB, S = (N + kLuOuter - 1) / kLuOuter;
     const int32_t *glb = gl + (size_t)b * gl_stride;
-    for (int i = tid; i < Q * kLuGather; i += 1024) lists[i] = glb[i];
-    for (int r = tid; r < N; r += 1024) ord[r] = r;
-    __syncthreads();
+    int32_t *permb = perm + (size_t)b * N;
     // position trow takes the entry at tsrc, steps in order; each step's
     // reads precede its writes in the wave's order
     auto replay = [&](int q0, int q1) {
-        for (int q = q0; q < q1; ++q) {
-            const int32_t *g = lists + (size_t)q * kLuGather;
-            const bool on = lane < g[0];
-            const int t = on ? ord[g[1 + 2 * kNB + lane]] : 0;
-            if (on) ord[g[1 + lane]] = t;
+        for (int q = q0; q < q1; q += kSteps) {
+            int cnt[kSteps], trow[kSteps], tsrc[kSteps];
+#pragma unroll
+            for (int i = 0; i < kSteps; ++i) {
+                cnt[i] = 0;
+                trow[i] = tsrc[i] = 0;
+                if (q + i < q1) {
+                    const int32_t *g = glb + (size_t)(q + i) * kLuGather;
+                    cnt[i] = g[0];
+                    trow[i] = g[1 + lane];
+                    tsrc[i] = g[1 + 2 * kNB + lane];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kSteps; ++i) {
+                const bool on = lane < cnt[i];
+                const int t = on ? ord[tsrc[i]] : 0;
+                if (on) ord[trow[i]] = t;
+            }
         }
     };
+    for (int r = tid; r < N; r += 1024) ord[r] = r;
+    __syncthreads();
     if (tid < 64) replay(0, Q);
     __syncthreads();
-    for (int r = tid; r < N; r += 1024) {
-        prm[r] = ord[r];
-        perm[(size_t)b * N + r] = ord[r];
-    }
-    __syncthreads();
+    for (int r = tid; r < N; r += 1024) permb[r] = ord[r];
+    __syncthreads();  // perm (global) is read back below by other threads of the block
     for (int r = tid; r < N; r += 1024) ord[r] = r;
     __syncthreads();
     for (int s = 0; s < S; ++s) {
-        if (tid < 64) replay(s * steps, min(Q, (s + 1) * steps));
+        if (tid < 64) replay(s * kSteps, min(Q, (s + 1) * kSteps));
         __syncthreads();
         for (int p = tid; p < N; p += 1024) inv[ord[p]] = p;
         __syncthreads();
-        for (int r = tid; r < N; r += 1024) pi[((size_t)b * S + s) * N + r] = inv[prm[r]];
+        int32_t *m = pi + ((size_t)b * S + s) * N;
+        for (int r = tid; r < N; r += 1024) m[r] = inv[permb[r]];
         __syncthreads();
     }
 }
@@ -1527,7 +1541,11 @@ __global__ __launch_bounds__(256) void k_trsv(const float *A, const float *lf, i
     __syncthreads();
     const int role = (int)role_s;
     const uint32_t ep = ep_s;
-    const int v = role / P, q = role % P;
+    // tickets panel-major over the vectors: every vector's chain advances at
+    // once (vector-major tickets left most of a large grid waiting for the
+    // first vectors' chains); a panel still depends only on smaller tickets
+    const int V = (int)gridDim.x / P;
+    const int q = role / V, v = role % V;
     const int p = kFwd ? q : P - 1 - q;  // panels taken in dependency order
     const int R0 = p * kTrsvRows;
     const int b = v / vpm;
@@ -2421,8 +2439,8 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
             }
         }
     }
-    if (w.perm && w.pi && w.dinv && w.lf && N <= kLuSolveOneBlock) {
-        const size_t shm = sizeof(int32_t) * (3 * (size_t)N + (size_t)(N / kNB) * kLuGather);
+    if (w.perm && w.pi && w.dinv && w.lf && N <= kTrsvMaxOrder) {
+        const size_t shm = sizeof(int32_t) * 2 * (size_t)N;
         hipLaunchKernelGGL(k_lu_solve_tables, dim3(B), dim3(1024), shm, stream, w.gl, gls, N,
                            w.perm, w.pi);
         hipLaunchKernelGGL(k_lu_final_l, dim3(N, B), dim3(256), 0, stream, w.A, w.lda, w.a_stride,
@@ -2443,7 +2461,7 @@ void lu_solve(const LuWorkspace &w, float *x, int64_t ldx, int trans, hipStream_
     const int N = w.N, V = w.B * vpm;
     if (trans != 0 && trans != 2) throw std::runtime_error("lu_solve: trans must be 0 or 2");
     const int64_t gls = (int64_t)(N / kNB) * kLuGather;
-    if (sync && w.perm && w.pi && w.dinv && w.lf && N <= kLuSolveOneBlock) {
+    if (sync && w.perm && w.pi && w.dinv && w.lf && N <= kTrsvMaxOrder) {
         const dim3 grid((unsigned)(V * (N / kTrsvRows)));
         if (trans == 0) {
             hipLaunchKernelGGL(k_trsv<0>, grid, dim3(256), 0, stream, w.A, w.lf, w.lda, w.a_stride,
