@@ -1457,46 +1457,56 @@ __global__ __launch_bounds__(1024) void k_lu_solve_tables(const int32_t *gl, int
 // Inverses of the 64 x 64 diagonal blocks of the factors: dinv[b][p][0] =
 // L_pp^{-1} (unit lower), dinv[b][p][1] = U_pp^{-1}, element (i, j) at
 // i * 64 + j.  One wave per block and factor (blockIdx.z); lane c builds
-// column c by substitution with the column's entries in LDS.
+// column c by substitution, the column in registers (both loops unrolled,
+// so every index is a constant), the block in LDS (read as broadcasts).  The
+// loops run over the same k on every lane: entries outside the column's
+// triangle come out as exact zeros (sums of products with zeros).
+template <bool kUpper>
+__device__ __forceinline__ void diag_inverse_column(const c32 (*T)[kTrsvRows], int c,
+                                                    c32 (&col)[kTrsvRows]) {
+    if (!kUpper) {
+#pragma unroll
+        for (int i = 0; i < kTrsvRows; ++i) {
+            c32 acc{i == c ? 1.0f : 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < i; ++k) {
+                const c32 l = T[i][k];
+                acc.re -= l.re * col[k].re - l.im * col[k].im;
+                acc.im -= l.re * col[k].im + l.im * col[k].re;
+            }
+            col[i] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int i = kTrsvRows - 1; i >= 0; --i) {
+            c32 acc{i == c ? 1.0f : 0.0f, 0.0f};
+#pragma unroll
+            for (int k = i + 1; k < kTrsvRows; ++k) {
+                const c32 u = T[i][k];
+                acc.re -= u.re * col[k].re - u.im * col[k].im;
+                acc.im -= u.re * col[k].im + u.im * col[k].re;
+            }
+            col[i] = cmul(acc, cinv(T[i][i]));
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void k_lu_diag_inverse(const float *A, int64_t lda,
                                                         int64_t a_stride, int N, float *dinv) {
     __shared__ c32 T[kTrsvRows][kTrsvRows];
-    __shared__ c32 col[kTrsvRows][kTrsvRows];  // [row][column]
     const int p = blockIdx.x, b = blockIdx.y, upper = blockIdx.z, c = threadIdx.x;
     const int P = N / kTrsvRows, R0 = p * kTrsvRows;
     const float *Ab = A + (size_t)b * a_stride * 2;
     for (int j = 0; j < kTrsvRows; ++j) T[c][j] = ld(Ab, (size_t)(R0 + j) * lda + R0 + c);
     __syncthreads();
-    // the loops run over the same k on every lane (no divergence, unrolled
-    // LDS reads): the entries of column c outside its triangle come out as
-    // exact zeros, since they only sum products with zeros
-    if (!upper) {
-        // column c of L^{-1} by forward substitution
-        for (int i = 0; i < kTrsvRows; ++i) {
-            c32 acc{i == c ? 1.0f : 0.0f, 0.0f};
-#pragma unroll 8
-            for (int k = 0; k < i; ++k) {
-                const c32 l = T[i][k], y = col[k][c];
-                acc.re -= l.re * y.re - l.im * y.im;
-                acc.im -= l.re * y.im + l.im * y.re;
-            }
-            col[i][c] = acc;
-        }
-    } else {
-        // column c of U^{-1} by backward substitution
-        for (int i = kTrsvRows - 1; i >= 0; --i) {
-            c32 acc{i == c ? 1.0f : 0.0f, 0.0f};
-#pragma unroll 8
-            for (int k = i + 1; k < kTrsvRows; ++k) {
-                const c32 u = T[i][k], y = col[k][c];
-                acc.re -= u.re * y.re - u.im * y.im;
-                acc.im -= u.re * y.im + u.im * y.re;
-            }
-            col[i][c] = cmul(acc, cinv(T[i][i]));
-        }
-    }
+    c32 col[kTrsvRows];
+    if (upper)
+        diag_inverse_column<true>(T, c, col);
+    else
+        diag_inverse_column<false>(T, c, col);
     float *out = dinv + (((size_t)b * P + p) * 2 + upper) * kTrsvRows * kTrsvRows * 2;
-    for (int i = 0; i < kTrsvRows; ++i) st(out, (size_t)i * kTrsvRows + c, col[i][c]);
+#pragma unroll
+    for (int i = 0; i < kTrsvRows; ++i) st(out, (size_t)i * kTrsvRows + c, col[i]);
 }
 
 // L below the diagonal in final row order: lf[b][c][r] = L[pi[s(c)][r]][c]

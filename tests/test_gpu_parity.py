@@ -478,6 +478,39 @@ def test_streaming_callback_stops_early(require_gpu):
         assert part[i].diagnostics["resident_batch"]["size"] == 2
 
 
+def test_multi_workgroup_solves_across_batch_sizes(require_gpu):
+    """One session solves the same sweep as one resident batch, then in
+    batches of 3, 3 and 1, then as one batch again (solve mode fp32 with the
+    rcond estimate, so both streams' solve contexts see every size).  The
+    panel workgroups of the triangular solves pass values through
+    epoch-tagged words that persist between calls: a word left by a larger
+    batch must never pass for the current one.  Every run equals the first to
+    the run-to-run noise of the float-atomic assembly (1e-4), with the same
+    rcond estimates (1e-3)."""
+    b = horn_buffers(edge_length=0.03, quarter=True)
+    freqs = np.array([500.0, 800.0, 1300.0, 2100.0, 3400.0, 5500.0, 8900.0])
+    k = (2 * np.pi * freqs / C0).astype(np.float32)
+    tags = b.physical_tags_i32
+    q = np.stack([np.where(tags == 2, 1j * 1.2041 * 2 * np.pi * f, 0.0) for f in freqs]).astype(np.complex64)
+    ang = np.deg2rad(np.linspace(0, 180, 7))
+    obs = np.stack([2.0 * np.sin(ang), np.zeros_like(ang), 2.0 * np.cos(ang)]).astype(np.float32)
+    kw = dict(source_tags=[2], impedance_source_tag=2, write_surface_pressure=True)
+    with _session(b, "yz+xz") as s:
+        runs = [s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, **kw),
+                s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, batch_frequencies=3,
+                                                                 **kw),
+                s.assemble_solve_evaluate_standard_neumann_batch(freqs, k, q, obs, **kw)]
+    ref = runs[0]
+    assert [r.diagnostics["resident_batch"]["size"] for r in runs[1]] == [3, 3, 3, 3, 3, 3, 1]
+    for run in runs[1:]:
+        for i in range(freqs.size):
+            assert np.all(np.isfinite(run[i].pressure)) and np.all(np.isfinite(run[i].field))
+            assert rel_l2(run[i].pressure, ref[i].pressure) < 1e-4, i
+            assert rel_l2(run[i].field, ref[i].field) < 1e-4, i
+            rc, rc0 = run[i].diagnostics["dense_solve_rcond"], ref[i].diagnostics["dense_solve_rcond"]
+            assert rc > 0 and abs(rc - rc0) <= 1e-3 * rc0, (i, rc, rc0)
+
+
 @pytest.mark.parametrize("mode", ["fp32", "mixed_fp16"])
 def test_lu_graph_replays_match_eager(require_gpu, mode):
     """The captured pad/norm/LU/solve graph is replayed for new wavenumbers
