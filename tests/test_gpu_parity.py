@@ -284,6 +284,36 @@ def test_dense_solve_tournament_panels_and_stepped_solve(require_gpu):
     assert rel_l2(x2[0], ref) < 1e-7, rel_l2(x2[0], ref)
 
 
+@pytest.mark.parametrize("n", [4200, 8300])
+def test_dense_solve_large_orders_both_directions(require_gpu, n):
+    """A x = b and A^H x = b at n = 4200 (multi-workgroup solves) and
+    n = 8300 (above their order limit: the stepped solves), with the rcond
+    estimate on, against a known solution: b = A x_true, A well conditioned
+    (I + noise), so the FP32 LU must return x_true to ~1e-6.  At n = 4200 the
+    estimate is also checked against the exact 1-norm condition (within the
+    estimator's factor 3)."""
+    from hornlab_metal_bem.native import dense_solve
+
+    rng = np.random.default_rng(23 + n)
+    A = ((rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(2.0 * n))
+    A = (A + 1.5 * np.eye(n)).astype(np.complex64)
+    x_true = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    A64 = A.astype(np.complex128)
+    b = (A64 @ x_true).astype(np.complex64)
+    bh = (A64.conj().T @ x_true).astype(np.complex64)
+    print(f"[large solve] n={n}", flush=True)
+    x, info = dense_solve(A[None], b[None], estimate_rcond=True)
+    assert info[0]["lapack_info"] == 0
+    assert rel_l2(x[0], x_true) < 1e-5, rel_l2(x[0], x_true)
+    xh, _ = dense_solve(A[None], bh[None], estimate_rcond=False, trans="C")
+    assert rel_l2(xh[0], x_true) < 1e-5, rel_l2(xh[0], x_true)
+    est = info[0]["dense_solve_rcond"]
+    assert np.isfinite(est) and est > 0
+    if n <= 5000:
+        rc = 1.0 / (np.linalg.norm(A64, 1) * np.linalg.norm(np.linalg.inv(A64), 1))
+        assert rc <= est * 1.01 and est < 3.0 * rc, (rc, est)
+
+
 def test_robin_assembly_matches_oracle(require_gpu):
     """Robin walls on the horn (tag 1, including images and touching
     pairs): -(i k beta / 3) V folded into the matrix as the oracle does."""
