@@ -389,8 +389,10 @@ __global__ __launch_bounds__(256) void k_residual_h_f64(const float *__restrict_
 // ---------------------------------------------------------------------------
 constexpr int kAdjTestsPerWave = 96;
 
+// Two waves per SIMD (at most 256 VGPRs; unbounded the kernel took 282 and
+// ran one wave per SIMD with its transcendental chains exposed).
 template <bool kComplexK>
-__global__ __launch_bounds__(256) void k_adjoint_regular(
+__global__ __launch_bounds__(256, 2) void k_adjoint_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ lam, const float *__restrict__ p, int64_t ld,
     const float *__restrict__ qt, double *__restrict__ gV, double *__restrict__ gN,
@@ -470,20 +472,15 @@ __global__ __launch_bounds__(256) void k_adjoint_regular(
         const float *tf = tfa + (size_t)t * kTriF;
         const int32_t *ti = tia + (size_t)t * kTriI;
         const float jt = tf[22];
-        float v0t[3], xo[18];
-        {
-            float e1[3], e2[3];
+        // test points as vertex 0 plus X_a e1 + Y_a e2, the offsets formed
+        // where they are used (registers: the kernel's occupancy is set by
+        // its VGPR count)
+        float v0t[3], te1[3], te2[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                v0t[c] = tf[23 + c];
-                e1[c] = tf[26 + c] - v0t[c];
-                e2[c] = tf[29 + c] - v0t[c];
-            }
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    xo[3 * a + c] = fmaf((float)kRule6X[a], e1[c], (float)kRule6Y[a] * e2[c]);
+        for (int c = 0; c < 3; ++c) {
+            v0t[c] = tf[23 + c];
+            te1[c] = tf[26 + c] - v0t[c];
+            te2[c] = tf[29 + c] - v0t[c];
         }
         // L_a = rw w_a |N_t| sum_i conj(lambda_i) phi_i(a)
         float lr[3], li[3];
@@ -518,8 +515,10 @@ __global__ __launch_bounds__(256) void k_adjoint_regular(
                         az = v0s[2] - sz * v0t[2];
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
-                const float bx = ax - sx * xo[3 * a], by = ay - sy * xo[3 * a + 1],
-                            bz = az - sz * xo[3 * a + 2];
+                const float xoa = fmaf((float)kRule6X[a], te1[0], (float)kRule6Y[a] * te2[0]);
+                const float yoa = fmaf((float)kRule6X[a], te1[1], (float)kRule6Y[a] * te2[1]);
+                const float zoa = fmaf((float)kRule6X[a], te1[2], (float)kRule6Y[a] * te2[2]);
+                const float bx = ax - sx * xoa, by = ay - sy * yoa, bz = az - sz * zoa;
                 const float alr = Lr[a] * qr - Li[a] * qi, ali = Lr[a] * qi + Li[a] * qr;
                 float gxa = 0.0f, gya = 0.0f, gza = 0.0f;
                 float gsr = 0.0f, gsi = 0.0f;  // sum_b w_b G

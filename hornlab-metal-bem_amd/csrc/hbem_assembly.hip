@@ -917,7 +917,7 @@ __device__ __forceinline__ void regular_tile_pair(
 }
 
 template <bool kComplexK, int kM, bool kPrecise>
-__global__ __launch_bounds__(256, kPrecise ? 1 : 2) void k_assemble_regular(
+__global__ __launch_bounds__(256, 2) void k_assemble_regular(
     DeviceGeometry g, const float *__restrict__ k_real, const float *__restrict__ k_imag,
     const float *__restrict__ q, int n_src, const float *__restrict__ beta,
     float *__restrict__ A, int64_t lda, int64_t a_stride, float *__restrict__ rhs,
