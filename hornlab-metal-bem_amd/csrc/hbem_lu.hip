@@ -221,6 +221,32 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long ke
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
+// 32-bit pivot search for the register panel: a candidate's key is the bit
+// pattern of its |re| + |im| with the sign bit cleared (non-negative, so
+// integer order is value order; a NaN sorts above every number), -1 for a
+// lane without a candidate.  One DPP step is a single integer max.
+__device__ __forceinline__ int pivot_bits(c32 v) {
+    return static_cast<int>(__float_as_uint(fabsf(v.re) + fabsf(v.im)) & 0x7fffffffu);
+}
+// (lanes outside kRowMask see INT_MIN, the identity of the max, which lets
+// the compiler fold the DPP move into the max)
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int imax_dpp(int x) {
+    return max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ int row_max_i32(int x) {
+    x = imax_dpp<0xB1, 0xF>(x);
+    x = imax_dpp<0x4E, 0xF>(x);
+    x = imax_dpp<0x141, 0xF>(x);
+    return imax_dpp<0x140, 0xF>(x);
+}
+__device__ __forceinline__ int wave_max_i32(int x) {
+    x = row_max_i32(x);
+    x = imax_dpp<0x142, 0xA>(x);
+    x = imax_dpp<0x143, 0xC>(x);
+    return __builtin_amdgcn_readlane(x, 63);
+}
+
 // Partial-pivoting elimination of the kNB panel columns over the rows held
 // by the block, one row per thread: the pivot of column j is the row not yet
 // chosen with the largest |re| + |im| (ties: the smallest id), and every
@@ -434,10 +460,12 @@ __global__ __launch_bounds__(256) void k_lu_panel_apply(float *A, int64_t lda, i
 // its barriers or its wave count).  A row exchange swaps only the logical row numbers of the two
 // threads involved (no data moves); every thread writes its values back to
 // its final logical row.  One block barrier per column: each wave reduces
-// its pivot candidate with DPP, and its candidate lane publishes the key and
-// the row's values from the column on to a double-buffered per-wave slot;
-// after the barrier every wave reduces the 16 keys (one DPP row) and reads
-// the winning slot.  A slot of
+// its pivot candidate with DPP over 32-bit magnitude keys (one integer max
+// per step; the smallest row among equal magnitudes takes a second
+// reduction only when the ballot shows a tie), and its candidate lane
+// publishes the key, its row and the row's values from the column on to a
+// double-buffered per-wave slot; after the barrier every wave reduces the
+// 16 keys (one DPP row) and reads the winning slot.  A slot of
 // column j is rewritten at column j + 2, after every reader has passed the
 // barrier of column j + 1.
 __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, int64_t a_stride,
@@ -454,32 +482,49 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
         v[c] = own ? ld(Ab, (size_t)(k + c) * lda + myrow) : c32{0.0f, 0.0f};
-    __shared__ unsigned long long key_s[2][16];
+    __shared__ int mag_s[2][16];  // a wave's largest candidate key (-1: none)
+    __shared__ int rid_s[2][16];  // the logical row of that wave's candidate
     __shared__ c32 row_s[2][16][kNB];
     __shared__ int piv_s[kNB];
     // the block has one wave per 64 rows; slots of absent waves hold no key
-    if (tid < 32 && (tid & 15) >= (int)(blockDim.x >> 6)) key_s[tid >> 4][tid & 15] = 0ull;
+    if (tid < 32 && (tid & 15) >= (int)(blockDim.x >> 6)) mag_s[tid >> 4][tid & 15] = -1;
     int bad = 0;  // thread 0: first column without a nonzero pivot, plus one
 #pragma unroll
     for (int j = 0; j < kNB; ++j) {
         const int col = k + j, slot = j & 1;
         const bool cand = own && myrow >= col;
-        const unsigned long long key = cand ? pivot_key(v[j], myrow) : 0ull;
-        const unsigned long long wkey = wave_max_u64(key);
-        if (cand && key == wkey) {
+        const int key = cand ? pivot_bits(v[j]) : -1;
+        const int wkey = wave_max_i32(key);
+        // the wave's candidate: its lane of largest key, of those the
+        // smallest row (a second reduction only for a tie)
+        const bool top = cand && key == wkey;
+        const unsigned long long hit = __ballot(top);
+        bool win = top;
+        if (__popcll(hit) > 1) {
+            const int t = wave_max_i32(top ? 0x7fffffff - myrow : -1);
+            win = top && 0x7fffffff - myrow == t;
+        }
+        if (win) {
+            rid_s[slot][wave] = myrow;
 #pragma unroll
             for (int c = j; c < kNB; ++c) row_s[slot][wave][c] = v[c];
         }
-        if (lane == 0) key_s[slot][wave] = wkey;
+        if (lane == 0) mag_s[slot][wave] = wkey;
         __syncthreads();
-        const unsigned long long mine = key_s[slot][lane & 15];
-        const unsigned long long best = row_max_u64(mine);
-        const int wbest = __builtin_ctzll(__ballot(mine == best)) & 15;
-        const int p = static_cast<int>(0xffffffffu - static_cast<unsigned>(best));
+        const int mine = mag_s[slot][lane & 15];
+        const int best = row_max_i32(mine);
+        const unsigned hw = static_cast<unsigned>(__ballot(lane < 16 && mine == best));
+        int wbest = __builtin_ctz(hw);
+        if (__popc(hw) > 1) {
+            // tied waves: the smallest row among their candidates
+            const int rk = (lane < 16 && mine == best) ? 0x7fffffff - rid_s[slot][lane] : -1;
+            const int t = row_max_i32(rk);
+            wbest = __builtin_ctz(static_cast<unsigned>(__ballot(lane < 16 && rk == t)));
+        }
+        const int p = rid_s[slot][wbest];
         if (tid == 0) {
             piv_s[j] = p;
-            if (!(__uint_as_float(static_cast<unsigned>(best >> 32)) > 0.0f) && bad == 0)
-                bad = col + 1;
+            if (!(__int_as_float(best) > 0.0f) && bad == 0) bad = col + 1;
         }
         // exchange logical rows col <-> p
         if (own && myrow == p)
@@ -490,9 +535,11 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
         if (own && myrow > col) {
             const c32 *prow = row_s[slot][wbest];
             const c32 piv = prow[j];
-            const bool nz = (piv.re != 0.0f || piv.im != 0.0f);
             c32 l = v[j];
-            if (nz) l = cmul(l, cinv(piv));
+            if (piv.re != 0.0f || piv.im != 0.0f) {
+                const float s = 1.0f / (piv.re * piv.re + piv.im * piv.im);
+                l = cmul(l, c32{piv.re * s, -piv.im * s});
+            }
             v[j] = l;
 #pragma unroll
             for (int c = j + 1; c < kNB; ++c) {
@@ -511,6 +558,179 @@ __global__ __launch_bounds__(1024) void k_lu_panel_reg(float *A, int64_t lda, in
     // the step's pivots, once after the loop
     if (tid < kNB) ipiv[(size_t)b * N + k + tid] = piv_s[tid];
     if (wave == 0) build_gather_list(piv_s, k, gl + (size_t)b * gl_stride);
+}
+
+// Block row of U for the trailing columns of an outer block [K0, K1 =
+// K0 + kLuOuter): the exchanges of the block's kLuOuter / kNB steps applied
+// to columns [K1, N) (rows [K0, N)), then U12 = L11^{-1} A12 with the
+// block's unit-lower L11, kBrCols columns per workgroup, the block row held
+// in LDS throughout.  One launch in place of the exchange launch and the
+// per-step solve and rank-kNB update launches.  The steps' gather records
+// are composed into one source row per row (map_s); the rows below K1 that
+// the exchanges move are read before, and written after, one barrier.  The
+// workgroups of one system share an XCD (B % 8 == 0), so L11 is read into
+// one L2.
+constexpr int kBrCols = 32;
+constexpr int kBrMaxOrder = 8192;  // map_s: N - K0 ints of dynamic LDS
+// A workgroup's time is set by its serial steps (about 160 us at 256 x 32),
+// so the launch pays when few rounds of workgroups fill the chip; beyond
+// this many workgroups the per-step launches run faster (measured: 40
+// systems of order 1088 gain, 64 of order 8000 lose).
+constexpr int kBrMaxGroups = 1536;
+__global__ __launch_bounds__(256, 2) void k_lu_block_row(float *A, int64_t lda, int64_t a_stride,
+                                                      int N, int K0, int K1, const int32_t *gl,
+                                                      int64_t gl_stride) {
+    extern __shared__ int map_s[];          // [N - K0] source row of row K0 + i
+    __shared__ c32 X[kLuOuter][kBrCols];    // the block row, column c of row r at c ^ (r & 31)
+    __shared__ int moved_s[kLuOuter];
+    __shared__ int nmoved_s;
+    const int per = gridDim.x, B = gridDim.y;
+    int b = blockIdx.y, tile = blockIdx.x;
+    if ((B & 7) == 0) {
+        const int L = blockIdx.x + blockIdx.y * per, k = L >> 3;
+        b = (L & 7) + 8 * (k / per);
+        tile = k % per;
+    }
+    const int tid = threadIdx.x;
+    const int c0 = K1 + tile * kBrCols;
+    const int nc = min(kBrCols, N - c0);
+    float *Ab = A + (size_t)b * a_stride * 2;
+    const int M = N - K0;
+    for (int i = tid; i < M; i += 256) map_s[i] = K0 + i;
+    if (tid == 0) nmoved_s = 0;
+    __syncthreads();
+    const int32_t *g0 = gl + (size_t)b * gl_stride + (size_t)(K0 / kNB) * kLuGather;
+    for (int s = 0; s < kLuOuter / kNB; ++s) {
+        const int32_t *g = g0 + (size_t)s * kLuGather;
+        const int cnt = g[0];
+        int dst = 0, src = 0;
+        if (tid < cnt) {
+            dst = g[1 + tid];
+            src = map_s[g[1 + 2 * kNB + tid] - K0];
+        }
+        __syncthreads();
+        if (tid < cnt) map_s[dst - K0] = src;
+        __syncthreads();
+    }
+    // rows below the block that receive another row (at most kNB per step)
+    for (int i = kLuOuter + tid; i < M; i += 256)
+        if (map_s[i] != K0 + i) {
+            const int q = atomicAdd(&nmoved_s, 1);
+            if (q < kLuOuter) moved_s[q] = i;
+        }
+    // thread = row of the block row (256 threads, kLuOuter rows), all loads
+    // of the block row and of the moved rows in flight together
+    static_assert(kLuOuter == 256, "one block-row row per thread");
+    __syncthreads();
+    const int nm = min(nmoved_s, kLuOuter);
+    {
+        const int src = map_s[tid];
+        const int mrow = tid < nm ? moved_s[tid] : 0;
+        const int msrc = tid < nm ? map_s[mrow] : 0;
+        const float *col0 = Ab + 2 * (size_t)c0 * lda;
+        c32 v[kBrCols], mv[kBrCols];
+#pragma unroll
+        for (int c = 0; c < kBrCols; ++c)
+            v[c] = c < nc ? ld(col0, (size_t)c * lda + src) : c32{0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < kBrCols; ++c) X[tid][c ^ (tid & 31)] = v[c];
+#pragma unroll
+        for (int c = 0; c < kBrCols; ++c)
+            mv[c] = (tid < nm && c < nc) ? ld(col0, (size_t)c * lda + msrc) : c32{0.0f, 0.0f};
+        __syncthreads();  // every old value this workgroup needs has been read
+        if (tid < nm) {
+#pragma unroll
+            for (int c = 0; c < kBrCols; ++c)
+                if (c < nc) st(Ab, (size_t)(c0 + c) * lda + K0 + mrow, mv[c]);
+        }
+    }
+    for (int r0 = 0; r0 < kLuOuter; r0 += kNB) {
+        // the step's rows: unit-lower solve in wave 0, lane = column; lane i
+        // also holds row i of the diagonal block of L (loads coalesced along
+        // the column, all in flight), whose entries reach every lane by
+        // v_readlane
+        if (tid < 64) {
+            c32 lrow[kNB], x[kNB];
+#pragma unroll
+            for (int j = 0; j < kNB; ++j)
+                lrow[j] = (j < tid && tid < kNB)
+                              ? ld(Ab, (size_t)(K0 + r0 + j) * lda + K0 + r0 + tid)
+                              : c32{0.0f, 0.0f};
+#pragma unroll
+            for (int i = 0; i < kNB; ++i)
+                x[i] = tid < kBrCols ? X[r0 + i][tid ^ i] : c32{0.0f, 0.0f};
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) {
+                // an empty asm that ties column j of L to the final x_j: the
+                // step's lane reads cannot be hoisted above it (hoisted
+                // together they exhaust the scalar registers)
+                asm volatile("" : "+v"(lrow[j].re), "+v"(lrow[j].im) : "v"(x[j].re), "v"(x[j].im));
+#pragma unroll
+                for (int i = j + 1; i < kNB; ++i) {
+                    const float lre = lane_value(lrow[j].re, i), lim = lane_value(lrow[j].im, i);
+                    x[i].re = fmaf(-lre, x[j].re, fmaf(lim, x[j].im, x[i].re));
+                    x[i].im = fmaf(-lre, x[j].im, fmaf(-lim, x[j].re, x[i].im));
+                }
+            }
+            if (tid < kBrCols) {
+#pragma unroll
+                for (int i = 0; i < kNB; ++i) X[r0 + i][tid ^ i] = x[i];
+            }
+        }
+        __syncthreads();
+        // rows below the step within the block: X_r -= L_r,step X_step;
+        // thread = 4 columns x one row of each 32-row group
+        const int rb = r0 + kNB, ngrp = (kLuOuter - rb) / 32;
+        if (ngrp > 0) {
+            constexpr int kG = kLuOuter / 32 - 1;
+            const int rr = tid >> 3, cg = tid & 7;
+            c32 acc[kG][4];
+#pragma unroll
+            for (int m = 0; m < kG; ++m)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc[m][q] = m < ngrp ? X[rb + rr + 32 * m][(4 * cg + q) ^ rr] : c32{0.0f, 0.0f};
+            // L of the thread's rows, kJb columns of loads in flight at once
+            constexpr int kJb = 4;
+            const size_t lc = (size_t)(K0 + r0) * lda + K0 + rb + rr;
+            for (int j0 = 0; j0 < kNB; j0 += kJb) {
+                c32 lv[kJb][kG];
+#pragma unroll
+                for (int jj = 0; jj < kJb; ++jj)
+#pragma unroll
+                    for (int m = 0; m < kG; ++m)
+                        lv[jj][m] = m < ngrp ? ld(Ab, lc + (size_t)(j0 + jj) * lda + 32 * m)
+                                             : c32{0.0f, 0.0f};
+#pragma unroll
+                for (int jj = 0; jj < kJb; ++jj) {
+                    const int j = j0 + jj;
+                    c32 u[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) u[q] = X[r0 + j][(4 * cg + q) ^ j];
+#pragma unroll
+                    for (int m = 0; m < kG; ++m) {
+                        const c32 l = lv[jj][m];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            acc[m][q].re =
+                                fmaf(-l.re, u[q].re, fmaf(l.im, u[q].im, acc[m][q].re));
+                            acc[m][q].im =
+                                fmaf(-l.re, u[q].im, fmaf(-l.im, u[q].re, acc[m][q].im));
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < kG; ++m)
+                if (m < ngrp)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) X[rb + rr + 32 * m][(4 * cg + q) ^ rr] = acc[m][q];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < kBrCols; ++c)
+        if (c < nc) st(Ab, (size_t)(c0 + c) * lda + K0 + tid, X[tid][c ^ (tid & 31)]);
 }
 
 // Row exchanges and/or the unit-lower solve U12 = L11^{-1} A12 of panel
@@ -2410,10 +2630,25 @@ void lu_factor(const LuWorkspace &w, int precision, const float *scale, hipStrea
             gemm(k + kNB, N, k + kNB, K1, k);
         }
         if (K1 < N) {
-            swap_trsm(K0, (K1 - K0) / kNB, K1, N, 0, 0, 0, 0);
-            for (int k = K0; k < K1; k += kNB) {
-                swap_trsm(k, 1, K1, N, 2, 0, 0, 0);
-                gemm(k + kNB, K1, K1, N, k);
+            const int br_tiles = (N - K1 + kBrCols - 1) / kBrCols;
+            if (N <= kBrMaxOrder && (int64_t)br_tiles * B <= kBrMaxGroups) {
+                static const bool lds_attr = [] {
+                    // dynamic LDS beside the static block row (above 64 KiB in all)
+                    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lu_block_row),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)(sizeof(int) * kBrMaxOrder));
+                    return true;
+                }();
+                (void)lds_attr;
+                const dim3 grid(br_tiles, B);
+                hipLaunchKernelGGL(k_lu_block_row, grid, dim3(256), sizeof(int) * (size_t)(N - K0),
+                                   stream, w.A, w.lda, w.a_stride, N, K0, K1, w.gl, gls);
+            } else {
+                swap_trsm(K0, (K1 - K0) / kNB, K1, N, 0, 0, 0, 0);
+                for (int k = K0; k < K1; k += kNB) {
+                    swap_trsm(k, 1, K1, N, 2, 0, 0, 0);
+                    gemm(k + kNB, K1, K1, N, k);
+                }
             }
             if (precision != 0) {
                 // 16-bit operands packed once, then the 128-tile MFMA update
