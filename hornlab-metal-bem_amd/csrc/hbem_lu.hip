@@ -2338,19 +2338,18 @@ __global__ __launch_bounds__(256) void k_rcond_step(const float *Y, float *X, in
 }
 
 // ---------------------------------------------------------------------------
-// FP64 residual r = b - A x over the true n x n block.
-// Block = 256 rows of one system; x staged through LDS in chunks.
+// FP64 products with the FP32 operator (true n x n block), batched over
+// systems.  Workgroup = 64 rows of one system; its 4 waves take every 4th
+// column of each 256-column chunk of x (staged through LDS), and their
+// partial sums are added in wave order, so results do not depend on timing.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t lda,
-                                                      int64_t a_stride, int n, const float *bvec,
-                                                      int64_t ldb, const double *x, int64_t ldx,
-                                                      double *r, double *norms, int vpm) {
-    const int b = blockIdx.y;
-    const int row = blockIdx.x * 256 + threadIdx.x;
-    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
-    const double *xb = x + (size_t)b * ldx * 2;
+// (A x)_row for row = blockIdx.x * 64 + lane, valid in wave 0 of the block.
+__device__ __forceinline__ void gemv_rows_f64(const float *Ab, int64_t lda, int n, int row,
+                                              const double *xb, double &yr, double &yi) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __shared__ double xsh[2][256];
+    __shared__ double red[2][4][64];
     double ar = 0.0, ai = 0.0;
     for (int c0 = 0; c0 < n; c0 += 256) {
         __syncthreads();
@@ -2360,7 +2359,8 @@ __global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t ld
         __syncthreads();
         if (row < n) {
             const int cend = min(256, n - c0);
-            for (int cc = 0; cc < cend; ++cc) {
+#pragma unroll 8
+            for (int cc = wave; cc < cend; cc += 4) {
                 const c32 a = ld(Ab, (size_t)(c0 + cc) * lda + row);
                 const double xr = xsh[0][cc], xi = xsh[1][cc];
                 ar = fma((double)a.re, xr, ar);
@@ -2370,6 +2370,25 @@ __global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t ld
             }
         }
     }
+    red[0][wave][lane] = ar;
+    red[1][wave][lane] = ai;
+    __syncthreads();
+    yr = ((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane];
+    yi = ((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane];
+}
+
+// FP64 residual r = b - A x and the per-system maxima of |r| and |b|
+// (norms[2 b], norms[2 b + 1], zero on entry).
+__global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t lda,
+                                                      int64_t a_stride, int n, const float *bvec,
+                                                      int64_t ldb, const double *x, int64_t ldx,
+                                                      double *r, double *norms, int vpm) {
+    const int b = blockIdx.y;
+    const int row = blockIdx.x * 64 + (threadIdx.x & 63);
+    const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
+    double ar, ai;
+    gemv_rows_f64(Ab, lda, n, row, x + (size_t)b * ldx * 2, ar, ai);
+    if (threadIdx.x >= 64) return;
     double rn = 0.0, bn = 0.0;
     if (row < n) {
         const double br = bvec[((size_t)b * ldb + row) * 2], bi = bvec[((size_t)b * ldb + row) * 2 + 1];
@@ -2383,7 +2402,7 @@ __global__ __launch_bounds__(256) void k_residual_f64(const float *A, int64_t ld
         rn = fmax(rn, __shfl_xor(rn, o));
         bn = fmax(bn, __shfl_xor(bn, o));
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (threadIdx.x == 0) {
         atomicMax(reinterpret_cast<unsigned long long *>(norms + 2 * b),
                   static_cast<unsigned long long>(__double_as_longlong(rn)));
         atomicMax(reinterpret_cast<unsigned long long *>(norms + 2 * b + 1),
@@ -2402,30 +2421,11 @@ __global__ __launch_bounds__(256) void k_gemv_f64(const float *A, int64_t lda, i
                                                   int n, const double *x, int64_t ldx, double *y,
                                                   int64_t ldy, int vpm) {
     const int b = blockIdx.y;
-    const int row = blockIdx.x * 256 + threadIdx.x;
+    const int row = blockIdx.x * 64 + (threadIdx.x & 63);
     const float *Ab = A + (size_t)(b / vpm) * a_stride * 2;
-    const double *xb = x + (size_t)b * ldx * 2;
-    __shared__ double xsh[2][256];
-    double ar = 0.0, ai = 0.0;
-    for (int c0 = 0; c0 < n; c0 += 256) {
-        __syncthreads();
-        const int c = c0 + threadIdx.x;
-        xsh[0][threadIdx.x] = c < n ? xb[2 * c] : 0.0;
-        xsh[1][threadIdx.x] = c < n ? xb[2 * c + 1] : 0.0;
-        __syncthreads();
-        if (row < n) {
-            const int cend = min(256, n - c0);
-            for (int cc = 0; cc < cend; ++cc) {
-                const c32 a = ld(Ab, (size_t)(c0 + cc) * lda + row);
-                const double xr = xsh[0][cc], xi = xsh[1][cc];
-                ar = fma((double)a.re, xr, ar);
-                ar = fma(-(double)a.im, xi, ar);
-                ai = fma((double)a.re, xi, ai);
-                ai = fma((double)a.im, xr, ai);
-            }
-        }
-    }
-    if (row < n) {
+    double ar, ai;
+    gemv_rows_f64(Ab, lda, n, row, x + (size_t)b * ldx * 2, ar, ai);
+    if (threadIdx.x < 64 && row < n) {
         y[((size_t)b * ldy + row) * 2] = ar;
         y[((size_t)b * ldy + row) * 2 + 1] = ai;
     }
@@ -2786,13 +2786,13 @@ void launch_residual_f64(const float *A, int64_t lda, int64_t a_stride, int n, i
                          const float *b, int64_t ldb, const double *x, int64_t ldx, double *r,
                          double *norms, hipStream_t stream, int vpm) {
     launch_fill_zero_f64(norms, (size_t)2 * B, stream);
-    hipLaunchKernelGGL(k_residual_f64, dim3((n + 255) / 256, B), dim3(256), 0, stream, A, lda,
+    hipLaunchKernelGGL(k_residual_f64, dim3((n + 63) / 64, B), dim3(256), 0, stream, A, lda,
                        a_stride, n, b, ldb, x, ldx, r, norms, vpm);
 }
 
 void launch_gemv_f64(const float *A, int64_t lda, int64_t a_stride, int n, int B, const double *x,
                      int64_t ldx, double *y, int64_t ldy, hipStream_t stream, int vpm) {
-    hipLaunchKernelGGL(k_gemv_f64, dim3((n + 255) / 256, B), dim3(256), 0, stream, A, lda, a_stride,
+    hipLaunchKernelGGL(k_gemv_f64, dim3((n + 63) / 64, B), dim3(256), 0, stream, A, lda, a_stride,
                        n, x, ldx, y, ldy, vpm);
 }
 
