@@ -314,6 +314,37 @@ def test_dense_solve_large_orders_both_directions(require_gpu, n):
         assert rc <= est * 1.01 and est < 3.0 * rc, (rc, est)
 
 
+@pytest.mark.parametrize("n_sys", [5, 8])
+def test_dense_solve_block_rows_with_far_exchanges(require_gpu, n_sys):
+    """Order 700: the block rows of U right of the outer blocks at columns 0
+    and 256 are each formed in one launch (the block's exchanges composed,
+    then its unit-lower solve).  5 systems keep the workgroups in launch
+    order, 8 group each system's workgroups on one XCD.  Rows are randomly
+    permuted ahead of a dominant diagonal, so the pivots come from far below
+    the block and the exchanges move rows below it.  FP32 and mixed_fp16 +
+    GMRES-IR against a known solution per system."""
+    from hornlab_metal_bem.native import dense_solve
+
+    n = 700
+    rng = np.random.default_rng(1700 + n_sys)
+    A = np.empty((n_sys, n, n), np.complex64)
+    X = rng.standard_normal((n_sys, n)) + 1j * rng.standard_normal((n_sys, n))
+    Bv = np.empty((n_sys, n), np.complex64)
+    for s_ in range(n_sys):
+        M = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(2.0 * n)
+        M = (M + 2.0 * np.eye(n))[rng.permutation(n)]
+        A[s_] = M.astype(np.complex64)
+        Bv[s_] = (A[s_].astype(np.complex128) @ X[s_]).astype(np.complex64)
+    print(f"[block rows] n={n} systems={n_sys}", flush=True)
+    for mode in ("fp32", "mixed_fp16"):
+        kw = ({} if mode == "fp32" else
+              {"solve_mode": "mixed_fp16", "refine_tolerance": 1e-9, "refine_max_iterations": 12})
+        x, info = dense_solve(A, Bv, estimate_rcond=False, **kw)
+        for s_ in range(n_sys):
+            assert info[s_]["lapack_info"] == 0
+            assert rel_l2(x[s_], X[s_]) < 1e-5, (mode, s_, rel_l2(x[s_], X[s_]))
+
+
 def test_robin_assembly_matches_oracle(require_gpu):
     """Robin walls on the horn (tag 1, including images and touching
     pairs): -(i k beta / 3) V folded into the matrix as the oracle does."""
